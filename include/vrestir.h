@@ -1,0 +1,245 @@
+/*
+ * vrestir.h -- C ABI of the MI355X-native VolumetricReSTIR render pass.
+ *
+ * This is the drop-in boundary for the reference's render-pass plugin
+ * (Source/RenderPasses/VolumetricReSTIR). Every entry point below names the
+ * reference interface it replaces. Plain C types only: no HIP, no torch, no
+ * C++ in the signatures. All functions return a vrestir_status; the text of
+ * the last failure on a handle is available from vrestir_last_error().
+ *
+ * Threading: one handle per host thread / HIP stream (the reference executes
+ * its pass serially from one render thread, RenderGraphExe.cpp:33-42).
+ * Ownership: the handle owns its device buffers (volume brick pools,
+ * reservoirs, features); the caller owns the output image and the stream.
+ */
+#ifndef VRESTIR_H
+#define VRESTIR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VRESTIR_ABI_VERSION 1
+
+typedef enum vrestir_status {
+    VRESTIR_OK = 0,
+    VRESTIR_EINVAL = 1,       /* bad argument (replaces C++ exceptions / logError) */
+    VRESTIR_EIO = 2,          /* file could not be read or parsed (.vbx) */
+    VRESTIR_EHIP = 3,         /* a HIP runtime call failed */
+    VRESTIR_ENOMEM = 4,       /* device or host allocation failed */
+    VRESTIR_ESTATE = 5,       /* call out of order (e.g. execute before set_volume) */
+    VRESTIR_EUNSUPPORTED = 6  /* option the pass does not implement (surface scenes) */
+} vrestir_status;
+
+/* Tracking methods, MIS, neighbour sampler, reprojection modes:
+ * same numeric values as VR/HostDeviceSharedConstants.slang:5-15,30-32. */
+enum {
+    VRESTIR_RATIO_TRACKING = 0,
+    VRESTIR_ANALYTIC_TRACKING = 1,
+    VRESTIR_RAY_MARCHING = 2,
+    VRESTIR_RESIDUAL_RATIO_TRACKING = 3,
+    VRESTIR_ANALOG_RESIDUAL_RATIO_TRACKING = 4
+};
+enum { VRESTIR_MIS_NONE = 0, VRESTIR_MIS_TALBOT = 1 };
+enum { VRESTIR_SAMPLER_HAMMERSLEY = 0, VRESTIR_SAMPLER_R2 = 1 };
+enum { VRESTIR_REPROJECTION_LINEAR = 0, VRESTIR_REPROJECTION_NONE = 1, VRESTIR_REPROJECTION_NO_BACKGROUND = 2 };
+
+/* Option surface: one field per VolumetricReSTIRParams member
+ * (VR/VolumetricReSTIR.h:130-206), same meaning, same defaults
+ * (vrestir_default_params). Booleans are int32 0/1. */
+typedef struct vrestir_params {
+    int32_t maxBounces;
+    int32_t enableTemporalReuse;
+    int32_t enableSpatialReuse;
+    int32_t vertexReuse;                 /* only 0 is implemented (VRESTIR_EUNSUPPORTED) */
+    int32_t vertexReuseStartBounce;
+    int32_t useReference;                /* volume path tracer instead of ReSTIR */
+    int32_t useEnvironmentLights;
+    int32_t useAnalyticLights;
+    int32_t useEmissiveLights;           /* emissive triangles: surface scenes only, must be 0 */
+    int32_t baselineSamplePerPixel;
+    int32_t visualizeTotalTransmittance;
+    int32_t useSurfaceScene;             /* must be 0 (VRESTIR_EUNSUPPORTED) */
+    int32_t usePrevVolumeForReproj;
+    /* initial sampling */
+    int32_t initialBaseMipLevel;
+    int32_t initialM;
+    int32_t initialLightSamples;
+    int32_t initialLightingMipLevel;
+    int32_t initialVisibilityUseLinearSampler;
+    int32_t initialLightingUseLinearSampler;
+    uint32_t initialLightingTrackingMethod;
+    float initialVisibilityTStepScale;
+    float initialLightingTStepScale;
+    int32_t initialUseRussianRoulette;
+    int32_t initialUseCoarserGridForIndirectBounce;
+    /* temporal reuse */
+    float temporalReuseMThreshold;       /* multiplicative: cap = threshold * M_current */
+    uint32_t temporalReprojectionMode;
+    uint32_t temporalMISMethod;
+    int32_t temporalReprojectionMipLevel;
+    /* spatial reuse */
+    int32_t spatialReuseRounds;
+    int32_t spatialVisibilityMipLevel;
+    int32_t spatialLightingMipLevel;
+    int32_t spatialVisibilityUseLinearSampler;
+    int32_t spatialLightingUseLinearSampler;
+    float spatialVisibilityTStepScale;
+    float spatialLightingTStepScale;
+    uint32_t spatialVisibilityTrackingMethod;
+    uint32_t spatialLightingTrackingMethod;
+    uint32_t randomSamplerType;
+    float sampleRadius;
+    int32_t spatialSampleCount;          /* neighbours + 1 (the pixel itself) */
+    int32_t enableVisibilitySimilarityRejection; /* kept for ABI parity; never read (as in the reference) */
+    uint32_t spatialMISMethod;
+    /* final shading */
+    int32_t finalLightSamples;
+    int32_t finalVisibilitySamples;
+    uint32_t finalVisibilityTrackingMethod;
+    uint32_t finalLightTrackingMethod;
+    uint32_t finalRandomSamplerType;     /* kept for ABI parity; never read (as in the reference) */
+    float finalTStepScale;
+} vrestir_params;
+
+enum { VRESTIR_BRICK_FP32 = 0, VRESTIR_BRICK_FP16 = 1 };
+
+/* A density volume given as a dense mip-0 grid (the form every synthetic
+ * workload takes); replaces Scene::addGVDBVolume (SC/Scene.h:481,
+ * SC/Scene.cpp:2754-3326) for in-memory grids. Mips 1..numMips-1 are 2x box
+ * reductions, conservative grids (slots 8..15) follow the reference's
+ * 27-neighbour dilation of mip 0 (GL/gvdb_volume_gvdb.cpp:2753-2800). */
+typedef struct vrestir_volume_desc {
+    int32_t nx, ny, nz;          /* mip-0 voxels per axis, each a multiple of 8 << (numMips-1) */
+    const float* density;        /* host memory, nx*ny*nz values, x fastest; values < 0 are clamped to 0 */
+    int32_t numMips;             /* 1..8 */
+    int32_t brickFormat;         /* VRESTIR_BRICK_FP32 / VRESTIR_BRICK_FP16 */
+    float indexToWorld[16];      /* mip-0 index space -> world, row-vector affine (translation in [12..14]) */
+    float sigmaA[3];
+    float sigmaS[3];             /* sigma_a + sigma_s must be grey (SC/Scene.cpp:3247) */
+    float g;                     /* Henyey-Greenstein anisotropy */
+    float densityScale;          /* densityScaleFactorByScaling */
+    const float* temperature;    /* optional emission grid (same dims) or NULL */
+    float LeScale, temperatureCutOff, temperatureScale;
+} vrestir_volume_desc;
+
+/* Point light (LT/LightHelpers.slang:223-244): Li = intensity / dist^2. */
+typedef struct vrestir_point_light {
+    float posW[3];
+    float intensity[3];
+} vrestir_point_light;
+
+/* Pinhole camera (SC/Camera/Camera.cpp:113-189), no jitter. */
+typedef struct vrestir_camera {
+    float posW[3];
+    float target[3];
+    float up[3];
+    float fovY;                  /* radians */
+    float nearZ, farZ;
+} vrestir_camera;
+
+typedef struct vrestir_handle vrestir_handle;
+
+/* ---- lifecycle: getPasses()/VolumetricReSTIR::create (VR/VolumetricReSTIR.cpp:56-70) ---- */
+int32_t vrestir_version(void);
+void vrestir_default_params(vrestir_params* out);                     /* VR/VolumetricReSTIR.h:132-206 */
+int32_t vrestir_create(const vrestir_params* params, vrestir_handle** out); /* binds the current HIP device */
+int32_t vrestir_destroy(vrestir_handle* h);
+const char* vrestir_last_error(const vrestir_handle* h);               /* NULL handle: last create() error */
+
+/* ---- scene binding: setScene (VR/VolumetricReSTIR.cpp:1218-1256) ---- */
+int32_t vrestir_set_volume(vrestir_handle* h, const vrestir_volume_desc* desc);
+int32_t vrestir_load_vbx(vrestir_handle* h, const char* pathPrefix, int32_t numMips, int32_t brickFormat,
+                         const float sigmaA[3], const float sigmaS[3], float g, float densityScale);
+int32_t vrestir_set_lights(vrestir_handle* h, const vrestir_point_light* lights, int32_t count,
+                           const float envRadiance[3]);
+int32_t vrestir_set_camera(vrestir_handle* h, const vrestir_camera* cam);
+
+/* ---- options: updateDict / getScriptingDictionary (VR/VolumetricReSTIR.cpp:1280-1347).
+ *      Any change resets the temporal history and the frame counter, like mOptionsChanged. ---- */
+int32_t vrestir_set_params(vrestir_handle* h, const vrestir_params* params);
+int32_t vrestir_get_params(const vrestir_handle* h, vrestir_params* out);
+int32_t vrestir_reset_history(vrestir_handle* h);
+
+/* ---- per frame: VolumetricReSTIR::execute (VR/VolumetricReSTIR.cpp:303-906).
+ * Enqueues the whole frame on `stream` (a hipStream_t, NULL = default stream).
+ * outRGBA: device float4[width*height] ("accumulated_color", VR/VolumetricReSTIR.cpp:39-43);
+ * outMotionVec: device float2[width*height] or NULL ("mvec"). ---- */
+int32_t vrestir_execute(vrestir_handle* h, void* stream, int32_t width, int32_t height,
+                        float* outRGBA, float* outMotionVec);
+int32_t vrestir_frame_count(const vrestir_handle* h, int32_t* out);
+
+/* ---- stage-level access, the six dispatches of execute (VR/VolumetricReSTIR.cpp:499-762).
+ * Used by parity tests to run one kernel on injected inputs. ---- */
+enum {
+    VRESTIR_STAGE_GENERATE_FEATURES = 0,  /* VR/GenerateFeatures.cs.slang:57-102 */
+    VRESTIR_STAGE_TRACE_RAYS = 1,         /* VR/TraceRays.cs.slang:64-201 */
+    VRESTIR_STAGE_TEMPORAL_REUSE = 2,     /* VR/TemporalReuse.cs.slang:80-377 */
+    VRESTIR_STAGE_SPATIAL_REUSE = 3,      /* VR/SpatialReuse.cs.slang:94-265 (round = argument) */
+    VRESTIR_STAGE_COPY_RESERVOIRS = 4,    /* VR/CopyReservoirs.cs.slang:84-95 */
+    VRESTIR_STAGE_FINAL_SHADING = 5       /* VR/FinalShading.cs.slang:71-141 */
+};
+/* Buffers in the reference's own identities (VR/VolumetricReSTIR.cpp:419-450).
+ * Host exchange format per pixel: features {int32 noReflectiveSurface; float transmittance};
+ * reservoir {float runningSum, M, depth, p_y, lightUV[2]; int32 lightID, sampledPixel, extraBounceStartId};
+ * extra bounce {float wi_dist[3]} x (maxBounces-1); color float4. */
+enum {
+    VRESTIR_BUF_FEATURES = 0,
+    VRESTIR_BUF_TEMPORAL_FEATURES = 1,
+    VRESTIR_BUF_RESERVOIRS0 = 2,
+    VRESTIR_BUF_RESERVOIRS1 = 3,
+    VRESTIR_BUF_TEMPORAL_RESERVOIRS = 4,
+    VRESTIR_BUF_EXTRA0 = 5,
+    VRESTIR_BUF_EXTRA1 = 6,
+    VRESTIR_BUF_TEMPORAL_EXTRA = 7,
+    VRESTIR_BUF_COLOR = 8
+};
+int32_t vrestir_prepare_frame(vrestir_handle* h, int32_t width, int32_t height);
+int32_t vrestir_run_stage(vrestir_handle* h, void* stream, int32_t stage, int32_t round, float* outRGBA);
+int32_t vrestir_end_frame(vrestir_handle* h);
+int32_t vrestir_read_buffer(vrestir_handle* h, int32_t buffer, void* hostDst, size_t bytes);
+int32_t vrestir_write_buffer(vrestir_handle* h, int32_t buffer, const void* hostSrc, size_t bytes);
+
+/* ---- batched tracking primitive (the adapters of VR/VolumeTrackingAdapterGVDB.slang run
+ * over a ray batch; used by parity tests and the tracking micro-benchmark).
+ * rays: device float[8*n] = {origin.xyz, dir.xyz, tMax, unused}; seeds: device uint32[3*n] =
+ * {pixel.x, pixel.y, sampleNumber}; out: device float[8*n] (layout per mode, see DESIGN). ---- */
+enum {
+    VRESTIR_TRACK_TR_ANALYTIC = 0,
+    VRESTIR_TRACK_TR_RAY_MARCHING = 1,
+    VRESTIR_TRACK_TR_RATIO = 2,
+    VRESTIR_TRACK_TR_RESIDUAL_RATIO = 3,
+    VRESTIR_TRACK_TR_ANALOG_RESIDUAL_RATIO = 4,
+    VRESTIR_TRACK_SAMPLE_ANALYTIC = 5,     /* up to 4 free-flight samples in one traversal */
+    VRESTIR_TRACK_SAMPLE_DECOMPOSITION = 6,/* delta tracking against per-brick bounds */
+    VRESTIR_TRACK_SAMPLE_CELL_BY_DENSITY = 7,
+    VRESTIR_TRACK_FEATURE_MARCH = 8,
+    VRESTIR_TRACK_DENSITY_AT_POINT = 9
+};
+int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t mode, int32_t mipLevel,
+                            int32_t useLinearSampler, float tStepScale, int32_t numSamples, int32_t n,
+                            const float* raysDev, const uint32_t* seedsDev, float* outDev);
+
+/* ---- host-side helpers (no GPU needed) ---- */
+/* Seeded fBm density (4-octave Perlin by default) for the synthetic workloads. */
+int32_t vrestir_synth_fbm(uint64_t seed, int32_t nx, int32_t ny, int32_t nz, int32_t octaves,
+                          float baseFrequency, float lacunarity, float gain, int32_t sphereMask,
+                          float threshold, float amplitude, float* out);
+/* Camera orbit about a centre (camera animation mode 2, VR/VolumetricReSTIR.cpp:879-904). */
+int32_t vrestir_orbit_camera(const vrestir_camera* base, const float center[3], int32_t frame,
+                             float degreesPerFrame, vrestir_camera* out);
+/* Brick statistics of a dense grid: number of 8^3 bricks with any non-zero voxel. */
+int32_t vrestir_count_bricks(const float* density, int32_t nx, int32_t ny, int32_t nz, int64_t* out);
+/* Write / read the .vbx container (gvdb-voxel-src/GVDB_FILESPEC.txt, minor version 12). */
+int32_t vrestir_write_vbx(const char* path, const float* density, int32_t nx, int32_t ny, int32_t nz,
+                          const float indexToWorld[16]);
+int32_t vrestir_read_vbx(const char* path, float* densityOrNull, int32_t* nx, int32_t* ny, int32_t* nz,
+                         float indexToWorld[16]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VRESTIR_H */

@@ -1,0 +1,173 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+always as the checker; the product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_float, c_int, c_size_t, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+REF_SPLITMIX = os.path.join(HERE, "_ref", "libref_splitmix64.so")
+REF_XOSHIRO = os.path.join(HERE, "_ref", "libref_xoshiro128ss.so")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        raise RuntimeError(f"{LIB} missing: run `make -C oracle`")
+    from volumetricrestirrelease_amd import abi
+    L = ctypes.CDLL(LIB)
+    sig = {
+        "orc_create": (c_void_p, []),
+        "orc_destroy": (None, [c_void_p]),
+        "vrestir_default_params_oracle": (None, [POINTER(abi.Params)]),
+        "orc_set_params": (c_int, [c_void_p, POINTER(abi.Params)]),
+        "orc_set_volume": (c_int, [c_void_p, POINTER(abi.VolumeDesc)]),
+        "orc_set_lights": (c_int, [c_void_p, POINTER(abi.PointLight), c_int, POINTER(c_float)]),
+        "orc_set_camera": (c_int, [c_void_p, POINTER(abi.Camera), c_int, c_int]),
+        "orc_execute": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_int]),
+        "orc_prepare_frame": (c_int, [c_void_p, c_int, c_int, c_int]),
+        "orc_run_stage": (c_int, [c_void_p, c_int, c_int]),
+        "orc_end_frame": (c_int, [c_void_p]),
+        "orc_read_buffer": (c_int, [c_void_p, c_int, c_void_p, c_size_t]),
+        "orc_write_buffer": (c_int, [c_void_p, c_int, c_void_p, c_size_t]),
+        "orc_frame_count": (c_int, [c_void_p]),
+        "orc_total_round_id": (c_int, [c_void_p]),
+        "orc_stage_seconds": (None, [c_void_p, POINTER(c_double)]),
+        "orc_track_batch": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_int, c_int, c_void_p, c_void_p,
+                                    c_void_p]),
+        "orc_rng_stream": (None, [c_uint32, c_uint32, c_uint32, c_int, c_void_p]),
+        "orc_morton": (c_uint32, [c_uint32, c_uint32]),
+        "orc_splitmix64": (None, [c_uint64, c_int, c_void_p]),
+        "orc_xoshiro128ss": (None, [c_void_p, c_int, c_void_p]),
+        "orc_round_to_half": (c_float, [c_float]),
+        "orc_neighbor_offsets": (None, [c_int, c_float, c_int, c_int, c_void_p]),
+        "orc_grid_info": (c_int, [c_void_p, c_int, POINTER(c_int), POINTER(c_float), POINTER(c_int)]),
+        "orc_brick_bounds": (c_int, [c_void_p, c_int, c_int, c_int, c_int, POINTER(c_float)]),
+        "orc_voxel": (c_float, [c_void_p, c_int, c_int, c_int, c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+class Oracle:
+    """The pass on the CPU: same stage/buffer identities as the product's C ABI."""
+
+    def __init__(self):
+        self.L = lib()
+        self.h = self.L.orc_create()
+
+    def close(self):
+        if self.h:
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        rc = self.L.orc_set_params(self.h, ctypes.byref(p))
+        assert rc == 0, rc
+
+    def set_scene(self, spec, frame=0):
+        d = spec.volume_desc()
+        assert self.L.orc_set_volume(self.h, ctypes.byref(d)) == 0
+        arr, n, env = spec.light_array()
+        assert self.L.orc_set_lights(self.h, arr, n, env) == 0
+        self.set_camera(spec, frame)
+        self.set_params(spec.make_params())
+
+    def set_camera(self, spec, frame=0):
+        c = spec.camera_struct(frame)
+        assert self.L.orc_set_camera(self.h, ctypes.byref(c), spec.width, spec.height) == 0
+
+    def execute(self, W, H, row_begin=0, row_end=0, nthreads=1):
+        out = np.zeros((H, W, 4), dtype=np.float32)
+        rc = self.L.orc_execute(self.h, W, H, out.ctypes.data, row_begin, row_end, nthreads)
+        assert rc == 0, rc
+        return out
+
+    def prepare(self, W, H, nthreads=1):
+        assert self.L.orc_prepare_frame(self.h, W, H, nthreads) == 0
+
+    def run_stage(self, stage, rnd=0):
+        assert self.L.orc_run_stage(self.h, stage, rnd) == 0
+
+    def end_frame(self):
+        self.L.orc_end_frame(self.h)
+
+    def read(self, buf, nbytes):
+        out = np.zeros(nbytes, dtype=np.uint8)
+        rc = self.L.orc_read_buffer(self.h, buf, out.ctypes.data, nbytes)
+        assert rc == 0, (buf, nbytes, rc)
+        return out
+
+    def write(self, buf, data: np.ndarray):
+        data = np.ascontiguousarray(data)
+        rc = self.L.orc_write_buffer(self.h, buf, data.ctypes.data, data.nbytes)
+        assert rc == 0, (buf, data.nbytes, rc)
+
+    def track(self, mode, mip, linear, tstep, nsamples, rays: np.ndarray, seeds: np.ndarray):
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
+        n = rays.shape[0]
+        out = np.zeros((n, 8), dtype=np.float32)
+        rc = self.L.orc_track_batch(self.h, mode, mip, linear, tstep, nsamples, n, rays.ctypes.data,
+                                    seeds.ctypes.data, out.ctypes.data)
+        assert rc == 0, rc
+        return out
+
+    def stage_seconds(self):
+        out = (c_double * 6)()
+        self.L.orc_stage_seconds(self.h, out)
+        return list(out)
+
+    def grid_info(self, slot):
+        dims = (c_int * 3)()
+        mx = c_float()
+        nb = c_int()
+        rc = self.L.orc_grid_info(self.h, slot, dims, ctypes.byref(mx), ctypes.byref(nb))
+        assert rc == 0
+        return tuple(dims), mx.value, nb.value
+
+
+def rng_stream(px, py, sample_number, n):
+    out = np.zeros(n, dtype=np.uint32)
+    lib().orc_rng_stream(px, py, sample_number, n, out.ctypes.data)
+    return out
+
+
+def ref_splitmix64(seed: int, n: int):
+    """The reference's own splitmix64.c (compiled from the reference tree into oracle/_ref)."""
+    L = ctypes.CDLL(REF_SPLITMIX)
+    L.ref_splitmix64_seed.argtypes = [c_uint64]
+    L.ref_splitmix64_next.restype = c_uint64
+    L.ref_splitmix64_seed(seed)
+    return np.array([L.ref_splitmix64_next() for _ in range(n)], dtype=np.uint64)
+
+
+def ref_xoshiro128ss(seed, n: int):
+    """The reference's own xoshiro128starstar.c (compiled into oracle/_ref)."""
+    L = ctypes.CDLL(REF_XOSHIRO)
+    L.ref_xoshiro128ss_seed.argtypes = [c_void_p]
+    L.ref_xoshiro128ss_next.restype = c_uint32
+    s = np.ascontiguousarray(seed, dtype=np.uint32)
+    L.ref_xoshiro128ss_seed(s.ctypes.data)
+    return np.array([L.ref_xoshiro128ss_next() for _ in range(n)], dtype=np.uint32)

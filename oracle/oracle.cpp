@@ -1,0 +1,1136 @@
+// oracle.cpp -- TEST INFRASTRUCTURE ONLY: the CPU oracle of the VolumetricReSTIR pass.
+//
+// A scalar FP32 restatement of the reference's shaders (FP64 only where the
+// reference uses it: the R2 neighbour sequence). It is the checker for the
+// HIP product and the CPU baseline of bench.py; nothing in the product links,
+// loads or calls it. Parity status of each part is listed in the oracle
+// section of the design notes; functions below cite the reference file:line
+// they follow.
+#include "vrestir_oracle.h"
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <thread>
+
+#include "oracle_pass.hpp"
+
+namespace orc {
+
+// IEEE binary16 round-to-nearest-even of a float (storage precision of fp16 bricks).
+float roundToHalf(float v) {
+    if (std::isnan(v) || std::isinf(v)) return v;
+    float a = std::fabs(v);
+    if (a >= 65520.f) return std::copysign(INFINITY, v);
+    if (a < 0x1p-24f * 0.5f) return std::copysign(0.f, v);
+    int e;
+    std::frexp(a, &e);             // a = m * 2^e, m in [0.5,1)
+    int exp2 = e - 1;              // a in [2^exp2, 2^(exp2+1))
+    if (exp2 < -14) exp2 = -14;    // subnormal spacing
+    float q = std::ldexp(1.f, exp2 - 10);
+    float r = std::nearbyint(a / q) * q;  // default rounding mode: nearest even
+    return std::copysign(r, v);
+}
+
+struct Oracle {
+    SceneState S;
+    vrestir_params P;
+    bool volumeSet = false, camSet = false;
+    int W = 0, H = 0;
+    int frameCount = 0;
+    bool temporalAccumulated = false;
+    bool optionsChanged = true;
+    // buffers (VR/VolumetricReSTIR.cpp:419-450)
+    std::vector<Features> features, temporalFeatures;
+    std::vector<Reservoir> res[2], temporalRes;
+    std::vector<f3> extra[2], temporalExtra;
+    std::vector<f4> color[2], output;
+    std::vector<f2> mvec;
+    int rowBegin = 0, rowEnd = 0, nthreads = 1;
+    // per-frame constants
+    int numTotalRounds = 0;
+    SamplingOptions initialOpt, spatialOpt, finalOpt;
+    int totalRoundId = 0;
+    double stageSeconds[6] = {0, 0, 0, 0, 0, 0};
+};
+
+static int maxB(const Oracle& O) { return O.P.maxBounces; }
+
+template <class F>
+static void forRows(Oracle& O, F&& fn) {
+    int y0 = O.rowBegin, y1 = O.rowEnd;
+    int nt = std::max(1, O.nthreads);
+    if (nt == 1 || y1 - y0 < 2) {
+        for (int y = y0; y < y1; y++)
+            for (int x = 0; x < O.W; x++) fn(x, y);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++)
+        th.emplace_back([&, t]() {
+            for (int y = y0 + t; y < y1; y += nt)
+                for (int x = 0; x < O.W; x++) fn(x, y);
+        });
+    for (auto& t : th) t.join();
+}
+
+// ---------------------------------------------------------------------------
+// evaluate_F_ (VR/ReSTIRHelper.slang:91-423), no surface scene, no vertex reuse.
+static f3 evaluateF(const Oracle& O, Reservoir tap, const f3* extraBuf, Ray ray, SampleGen& sg,
+                    const SamplingOptions& opt, bool noReuse, bool isFinalShading) {
+    const Medium& m = O.S.med;
+    int mipOff = 0;  // previous-frame grids are not bound (static volumes)
+    bool isBg = tap.depth == kRayTMax;
+    ray.tMax = tap.depth;
+    f3 F = mk3(1.f);
+    int maxInd = maxB(O) > 1 ? decodeMaxIndirect(tap.sampledPixel) : 0;
+    bool isSelfEmission = maxB(O) > 1 ? (maxInd == 0 && tap.lightID == -3) : tap.lightID == -3;
+    float visibility = 1.f;
+    f3 pW = ray.at(ray.tMax);
+    MediumInteraction mi{ray.at(ray.tMax), -ray.dir, m.g, true};
+    {
+        f3 sigma_s = mk3(1.f);
+        float density = (isBg || noReuse) ? 1.f : densityWorld(m, pW, mipOff);
+        if (density == 0.f) return mk3(0.f);
+        if (!noReuse) {
+            visibility = computeVisibility(m, ray, sg, opt.visibilitySamples, opt.visibilityMipLevel + mipOff,
+                                           opt.visibilityUseLinearSampler, opt.visibilityTrackingMethod,
+                                           opt.visibilityTStepScale);
+        }
+        sigma_s = isBg ? mk3(1.f) : (isSelfEmission ? m.sigma_a : m.sigma_s);
+        if (noReuse && !isBg) sigma_s = sigma_s / m.sigma_t;
+        F = F * (mk3(visibility * density) * sigma_s);
+    }
+    int bounceId = 0;
+    if (anyPositive(F)) {
+        if (isBg) {
+            F = F * O.S.env;
+        } else if (isSelfEmission) {
+            F = F * mk3(0.f);  // emission grids are not bound in this oracle build
+        } else {
+            bool isScatterSelfEmission = false;
+            if (maxB(O) > 1 && maxInd > 0) {
+                Ray scatterRay{};
+                isScatterSelfEmission = decodePathTag(tap.sampledPixel) == 1;
+                int nInd = maxInd;
+                for (; bounceId < nInd; bounceId++) {
+                    bool curEmissive = isScatterSelfEmission && bounceId == nInd - 1;
+                    f4 wd = decodeWiDist(extraBuf[tap.extraBounceStartId + bounceId]);
+                    if (curEmissive) {
+                        f3 ep = decodeEmissivePosition(tap.lightID, tap.lightUV);
+                        wd = {ep.x, ep.y, ep.z, -1.f};
+                    }
+                    if (wd.w == kRayTMax) return mk3(0.f);
+                    float dist = 1.f;
+                    if (wd.w == -1.f) {
+                        f3 disp = mk3(wd.x, wd.y, wd.z) - pW;
+                        dist = length(disp);
+                        f3 dir = normalize(disp);
+                        scatterRay = {pW, dir, 0.f, dist};
+                    } else {
+                        scatterRay = {pW, mk3(wd.x, wd.y, wd.z), 0.f, wd.w};
+                    }
+                    float bsdf = phaseHG(dot(mi.wo, scatterRay.dir), mi.g);
+                    F = F * bsdf;
+                    if (allZero(F)) return mk3(0.f);
+                    if (wd.w == -1.f) pW = mk3(wd.x, wd.y, wd.z);
+                    else pW = scatterRay.at(scatterRay.tMax);
+                    f3 ss = mk3(1.f);
+                    float scatterDensity = 1.f;
+                    if (!noReuse) {
+                        ss = curEmissive ? m.sigma_a : m.sigma_s;
+                        scatterDensity = std::max(0.f, densityWorld(m, pW, mipOff));
+                    } else {
+                        ss = curEmissive ? m.sigma_a / m.sigma_t : m.sigma_s / m.sigma_t;
+                        scatterDensity = 1.f;
+                    }
+                    F = F * (mk3(scatterDensity) * ss);
+                    if (curEmissive) F = F * (1.f / (dist * dist));
+                    if (allZero(F)) return mk3(0.f);
+                    float scatterVis = 1.f;
+                    if (!noReuse)
+                        scatterVis = computeVisibility(m, scatterRay, sg, opt.visibilitySamples,
+                                                       opt.visibilityMipLevel + mipOff, opt.visibilityUseLinearSampler,
+                                                       opt.visibilityTrackingMethod, opt.visibilityTStepScale);
+                    F = F * scatterVis;
+                    mi.wo = -scatterRay.dir;
+                    mi.p = pW;
+                    if (allZero(F)) return mk3(0.f);
+                }
+                if (isScatterSelfEmission) F = F * mk3(0.f);
+                else mi = {scatterRay.at(scatterRay.tMax), -scatterRay.dir, m.g, true};
+            }
+            if (!isScatterSelfEmission && anyPositive(F)) {
+                // evaluate_L_in_volume (VR/ReSTIRHelper.slang:443-496)
+                bool cull = !isFinalShading;
+                Ray sh{};
+                f3 Ld = mk3(0.f);
+                if (tap.lightID < 0) {
+                    f3 wi = decodeLightUV(tap.lightUV, tap.lightID);
+                    sh = {mi.p, wi, 0, kRayTMax};
+                    Ld = O.S.env * phaseHG(dot(mi.wo, wi), mi.g);
+                } else if (tap.lightID < (int)O.S.lights.size()) {
+                    SceneLightSample ls = analyticLightSample(O.S, tap.lightID, mi.p);
+                    sh = {mi.p, ls.rayDir, 0, ls.rayDistance};
+                    Ld = ls.Li * phaseHG(dot(mi.wo, ls.rayDir), mi.g);
+                } else {
+                    return mk3(0.f);  // emissive-triangle lights: surface scenes only
+                }
+                float Tr = computeVisibility(m, sh, sg, opt.lightSamples, cull ? opt.lightingMipLevel + mipOff : 0,
+                                             opt.lightingUseLinearSampler, opt.lightingTrackingMethod,
+                                             opt.lightingTStepScale);
+                F = F * (Ld * Tr);
+            }
+        }
+    }
+    return F;
+}
+
+static float evaluatePHat(const Oracle& O, const Ray& ray, SampleGen& sg, const f3* extraBuf,
+                          const SamplingOptions& opt, const Reservoir& tap) {
+    f3 F = evaluateF(O, tap, extraBuf, ray, sg, opt, false, false);
+    return luminance(F);
+}
+
+// ComputeInitialSample (VR/ComputeInitialSample.slang:4-395), no surface scene, no vertex reuse.
+static Reservoir computeInitialSample(const Oracle& O, Ray primaryRay, float preHit, float prePdf, float preTr,
+                                      int maxBounces, SampleGen& sg, const SamplingOptions& opt,
+                                      bool coarserIndirect, bool useRR, bool noReuse, f3* extra) {
+    const Medium& m = O.S.med;
+    float pathPdf = 1.f, pathPHat = 1.f;
+    Ray ray = primaryRay;
+    Reservoir combined = newReservoir();
+    Reservoir out{};
+    float primaryDepth = 0;
+    for (int bounce = 0; bounce < maxBounces; bounce++) {
+        out = newReservoir();
+        out.M = 1;
+        float curHit;
+        float pdfDist = 0;
+        MediumInteraction mi{};
+        float Tr;
+        if (bounce >= 1 || noReuse) {
+            if (noReuse) {
+                sampleDecomposition(m, ray, sg, mi, 0);
+                pdfDist = 1.f;
+                Tr = 1.f;
+                curHit = mi.valid ? length(mi.p - ray.origin) : kRayTMax;
+            } else {
+                int vm = opt.visibilityMipLevel;
+                if (coarserIndirect) vm = std::min((opt.visibilityMipLevel >= 8 ? 8 : 0) + m.numMips - 1, vm + 1);
+                float h[4], pd[4], tr[4];
+                sampleAnalytic(m, ray, sg, opt.visibilityUseLinearSampler, h, vm, pd, tr, 1);
+                curHit = h[0];
+                pdfDist = pd[0];
+                Tr = tr[0];
+                mi = {ray.at(curHit), -ray.dir, m.g, curHit != kRayTMax};
+            }
+        } else {
+            curHit = preHit;
+            pdfDist = prePdf;
+            Tr = preTr;
+            mi = {ray.at(curHit), -ray.dir, m.g, curHit != kRayTMax};
+        }
+        pathPdf *= pdfDist;
+        bool hitEmpty = false;
+        float actualDensity = 0.f;
+        if (bounce == 0) {
+            if (maxBounces > 1) out.sampledPixel = encodeMaxIndirect(out.sampledPixel, 0);
+            out.depth = mi.valid ? curHit : kRayTMax;
+            primaryDepth = out.depth;
+            out.p_y = pathPdf;
+        } else {
+            out.depth = primaryDepth;
+            out.sampledPixel = encodeMaxIndirect(out.sampledPixel, bounce);
+            extra[bounce - 1] = encodeWiDist({ray.dir.x, ray.dir.y, ray.dir.z, !mi.valid ? kRayTMax : curHit});
+            out.p_y = pathPdf;
+        }
+        if (mi.valid) actualDensity = noReuse ? 1.f : densityWorld(m, mi.p, 0);
+        if ((!mi.valid && bounce > 0) || (mi.valid && actualDensity == 0)) {
+            out.p_y = 0.f;
+            out.runningSum = 0.f;
+            hitEmpty = true;
+        }
+        float pdfDir = 1.f;
+        if (!hitEmpty) {
+            if (mi.valid) {
+                f3 albedo = m.sigma_s / m.sigma_t;
+                out.lightID = -1;
+                out.lightUV = {0, 0};
+                float outLightPdf = 0.f, outVis = 1.f;
+                f3 Ld = mk3(0.f), Le = mk3(0.f);
+                f3 oma = mk3(1.f) - albedo;
+                bool shouldComputeLightVisibility = opt.lightSamples == 0 ? false : true;
+                Ld = sampleDirectLighting(O.S, sg, outLightPdf, mi, opt.useEnv, opt.useAnalytic, opt.useEmissive,
+                                          shouldComputeLightVisibility, opt.lightingMipLevel,
+                                          opt.lightingTrackingMethod, opt.lightingUseLinearSampler, opt.lightSamples,
+                                          opt.lightingTStepScale, out.lightID, out.lightUV, outVis);
+                f3 wo = -ray.dir, wi = mk3(0.f);
+                if (maxBounces > 1) {
+                    f2 u;
+                    u.x = next1D(sg);
+                    u.y = next1D(sg);
+                    pdfDir = samplePhase(mi.g, wo, wi, u);
+                }
+                float p_src = out.p_y;
+                float er = luminance(oma * Le) / (luminance(oma * Le) + luminance(albedo * Ld));
+                if (std::isnan(er)) er = 0.f;
+                if (next1D(sg) < er) {
+                    p_src *= er;
+                    out.lightID = -3;
+                } else {
+                    p_src *= outLightPdf * (1 - er);
+                }
+                out.runningSum = p_src == 0.f ? 0.f : 1.f;
+                out.p_y = p_src;
+                {
+                    float p_y;
+                    pathPHat *= Tr;
+                    pathPHat *= actualDensity;
+                    if (out.lightID == -3) p_y = pathPHat * luminance(m.sigma_a * Le);
+                    else p_y = pathPHat * luminance(m.sigma_s * Ld * outLightPdf);
+                    pathPHat *= luminance(m.sigma_s) * pdfDir;
+                    if (noReuse) {
+                        p_y /= m.sigma_t;
+                        pathPHat /= m.sigma_t;
+                    }
+                    if (out.runningSum > 0.f) {
+                        out.runningSum = out.p_y == 0.f ? 0.f : p_y / out.p_y;
+                        if (out.lightID == -3 && bounce > 0) {
+                            encodeEmissivePosition(mi.p, out.lightID, out.lightUV);
+                            p_y /= (curHit * curHit);
+                            out.sampledPixel = encodePathTag(out.sampledPixel, 1);
+                        }
+                        out.p_y = p_y;
+                    }
+                }
+                pathPdf *= pdfDir;
+                if (maxBounces > 1 && bounce < maxBounces - 1) {
+                    ray = {mi.p, wi, 0, kRayTMax};
+                    if (useRR && bounce >= 2) {
+                        if (next1D(sg) < albedo.x) pathPdf *= albedo.x;
+                        else { hitEmpty = true; combined.M++; }
+                    }
+                }
+            } else {
+                f3 Le = O.S.env;
+                pathPHat *= Tr;
+                float p_y = pathPHat * luminance(Le);
+                out.runningSum = out.p_y == 0.f ? 0.f : p_y / out.p_y;
+                out.p_y = p_y;
+                hitEmpty = true;
+            }
+        }
+        if (maxBounces > 1) resampleStep(out, combined, sg);
+        if (hitEmpty) break;
+    }
+    if (maxBounces > 1) { combined.M = 1; return combined; }
+    return out;
+}
+
+// IntegrateByVolumePathTracing (VR/VolumePathTracingFunctions.slang:3-132), NEE on.
+static f3 pathTrace(const Oracle& O, Ray ray, SampleGen& sg, int lightSamples, bool useEnv, bool useAnalytic,
+                    bool useEmissive, int mip, int maxBounces, bool useRR) {
+    const Medium& m = O.S.med;
+    MediumInteraction mi{};
+    f3 beta = mk3(1.f), L = mk3(0.f);
+    for (int bounce = 0; bounce < maxBounces; bounce++) {
+        mi.valid = false;
+        sampleDecomposition(m, ray, sg, mi, mip);
+        if (mi.valid) {
+            f3 albedo = m.sigma_s / m.sigma_t;
+            beta = beta * albedo;
+            f3 Ld = directLighting(O.S, sg, mi, lightSamples, useEnv, useAnalytic, useEmissive, mip, 3);
+            L = L + beta * Ld;
+            f3 wo = -ray.dir, wi = mk3(0.f);
+            if (maxBounces > 1) {
+                f2 u;
+                u.x = next1D(sg);
+                u.y = next1D(sg);
+                samplePhase(mi.g, wo, wi, u);
+            }
+            if (bounce < maxBounces - 1) {
+                ray = {mi.p, wi, 0, kRayTMax};
+                if (useRR && bounce >= 2) {
+                    if (next1D(sg) < albedo.x) beta = beta / albedo.x;
+                    else bounce = maxBounces;
+                }
+            }
+        } else {
+            f3 Le = O.S.env;
+            if (bounce == 0) L = L + beta * Le;
+            bounce = maxBounces;
+        }
+    }
+    return L;
+}
+
+static inline bool inRange(int x, int y, int W, int H) { return x >= 0 && x < W && y >= 0 && y < H; }
+
+// ---------------------------------------------------------------------------
+// Kernels.
+
+// GenerateFeatures (VR/GenerateFeatures.cs.slang:57-102).
+static void stageFeatures(Oracle& O) {
+    if (O.P.useReference) return;
+    forRows(O, [&](int x, int y) {
+        SampleGen sg{};  // unused by the adapter
+        f3 d = rayDir(O.S.cam, x, y, O.W, O.H);
+        Ray r{O.S.cam.pos, d, 0, kRayTMax};
+        float accT = featureMarch(O.S.med, r, 0, true, O.initialOpt.visibilityTStepScale, sg);
+        O.features[y * O.W + x] = {1, accT};
+    });
+}
+
+// TraceRays (VR/TraceRays.cs.slang:64-201).
+static void stageTrace(Oracle& O) {
+    const int b = maxB(O);
+    forRows(O, [&](int x, int y) {
+        SampleGen sg = makeSampleGen(x, y, (uint32_t)(O.numTotalRounds * O.frameCount));
+        int rid = y * O.W + x;
+        f3 d = rayDir(O.S.cam, x, y, O.W, O.H);
+        Ray ray{O.S.cam.pos, d, 0, kRayTMax};
+        if (O.P.useReference) {
+            f3 avg = mk3(0.f);
+            for (int s = 0; s < O.P.baselineSamplePerPixel; s++)
+                avg = avg + pathTrace(O, ray, sg, std::max(1, O.initialOpt.lightSamples), O.initialOpt.useEnv,
+                                      O.initialOpt.useAnalytic, O.initialOpt.useEmissive, 0, b,
+                                      O.P.initialUseRussianRoulette != 0);
+            avg = avg / (float)O.P.baselineSamplePerPixel;
+            O.color[0][rid] = {avg.x, avg.y, avg.z, 1.f};
+            return;
+        }
+        f3 finalExtra[8] = {};
+        Reservoir fin = newReservoir();
+        int M = O.P.initialM;
+        int rounds = (M + 3) / 4;
+        bool noReuse = !O.P.enableSpatialReuse && !O.P.enableTemporalReuse;
+        for (int r = 0; r < rounds; r++) {
+            float h[4] = {0, 0, 0, 0}, pd[4] = {0, 0, 0, 0}, tr[4] = {0, 0, 0, 0};
+            int rs = r == rounds - 1 ? (M - 4 * (rounds - 1)) : 4;
+            if (!noReuse)
+                sampleAnalytic(O.S.med, ray, sg, O.initialOpt.visibilityUseLinearSampler, h,
+                               O.initialOpt.visibilityMipLevel, pd, tr, rs);
+            for (int s = 0; s < rs; s++) {
+                f3 ex[8] = {};
+                Reservoir o = computeInitialSample(O, ray, h[s], pd[s], tr[s], b, sg, O.initialOpt,
+                                                   O.P.initialUseCoarserGridForIndirectBounce != 0,
+                                                   O.P.initialUseRussianRoulette != 0, noReuse, ex);
+                bool sel = resampleStep(o, fin, sg);
+                if (sel && b > 1) {
+                    int mi = decodeMaxIndirect(fin.sampledPixel);
+                    for (int k = 0; k < mi; k++) finalExtra[k] = ex[k];
+                }
+            }
+        }
+        // p-hat re-evaluation with the spatial options (VR/TraceRays.cs.slang:169-183)
+        fin.extraBounceStartId = 0;
+        float ph = evaluatePHat(O, ray, sg, finalExtra, O.spatialOpt, fin);
+        if (fin.runningSum > 0.f) {
+            fin.runningSum *= fin.p_y == 0.f ? 0.f : ph / fin.p_y;
+            fin.p_y = ph;
+        }
+        fin.extraBounceStartId = b > 1 ? rid * (b - 1) : 0;
+        O.res[0][rid] = fin;
+        if (b > 1) {
+            int mi = decodeMaxIndirect(fin.sampledPixel);
+            for (int k = 0; k < mi; k++) O.extra[0][fin.extraBounceStartId + k] = finalExtra[k];
+        }
+    });
+}
+
+static Features featuresAt(const std::vector<Features>& f, int id) {
+    if (id < 0 || id >= (int)f.size()) return {0, 0.f};  // out-of-range structured-buffer loads read 0
+    return f[id];
+}
+
+// TemporalReuse (VR/TemporalReuse.cs.slang:80-377).
+static void stageTemporal(Oracle& O, int curIdx) {
+    if (!O.temporalAccumulated) return;  // gIsFirstFrame
+    const int b = maxB(O);
+    const Camera& pc = O.S.prevCam;
+    std::vector<Reservoir>& cur = O.res[curIdx];
+    std::vector<f3>& curExtra = O.extra[curIdx];
+    // Threads write only their own pixel of `cur` and read other pixels of the
+    // temporal buffers only, so the row-parallel loop is race-free.
+    forRows(O, [&](int x, int y) {
+        SampleGen sg = makeSampleGen(x, y, (uint32_t)(O.numTotalRounds * O.frameCount + 1));
+        int selectedId = -1;
+        int pixelId = y * O.W + x;
+        int numUsed = 1;
+        Reservoir taps[2];
+        taps[0] = cur[pixelId];
+        f3 d = rayDir(O.S.cam, x, y, O.W, O.H);
+        Ray ray{O.S.cam.pos, d, 0, kRayTMax};
+        bool talbot = O.P.temporalMISMethod == 1;
+        Reservoir output = talbot ? newReservoir() : taps[0];
+        int centerExtra = taps[0].extraBounceStartId;
+        float temporalOriginalDepth = 0.f;
+        int rsx = 0, rsy = 0;
+        Features cf = O.features[pixelId];
+        bool isBgRes = cf.transmittance == 1.f && cf.noReflectiveSurface;
+        bool useFallback = true;
+        if (O.P.temporalReprojectionMode != 1) {
+            float reprojDepth = taps[0].depth;
+            if (reprojDepth == kRayTMax && O.P.temporalReprojectionMode != 2 && !isBgRes)
+                reprojDepth = rejectionSampleDepth(O.S.med, ray, sg, 8 + O.P.temporalReprojectionMipLevel);
+            f3 p = ray.origin + ray.dir * reprojDepth;
+            f4 vp = mulRow({p.x, p.y, p.z, 1.f}, pc.view);
+            f4 cp = mulRow(vp, pc.proj);
+            float sx = cp.x / cp.w, sy = cp.y / cp.w;
+            int sxi, syi;
+            if (reprojDepth == kRayTMax) {
+                sx = (float)x + 0.5f; sy = (float)y + 0.5f;
+                sxi = x; syi = y;
+            } else {
+                sx = 0.5f * sx + 0.5f;
+                sy = -0.5f * sy + 0.5f;
+                sx *= (float)O.W; sy *= (float)O.H;
+                sxi = (int)sx; syi = (int)sy;
+            }
+            {
+                Features tf = featuresAt(O.temporalFeatures, syi * O.W + sxi);
+                bool isTapBg = tf.transmittance == 1.f && tf.noReflectiveSurface;
+                if (isBgRes && !isTapBg) {
+                    if (!O.mvec.empty()) O.mvec[pixelId] = {(float)(rsx - x) / O.W, (float)(rsy - y) / O.H};
+                    return;
+                }
+            }
+            sxi = (int)sx; syi = (int)sy;
+            rsx = sxi; rsy = syi;
+            if (inRange(sxi, syi, O.W, O.H)) {
+                numUsed++;
+                taps[1] = O.temporalRes[syi * O.W + sxi];
+            }
+            if (numUsed > 1) useFallback = false;
+        }
+        if (useFallback) {
+            numUsed++;
+            rsx = x; rsy = y;
+            taps[1] = O.temporalRes[pixelId];
+        }
+        if (!O.mvec.empty()) O.mvec[pixelId] = {(float)(rsx - x) / O.W, (float)(rsy - y) / O.H};
+        float curM = taps[0].M;
+        float maxPrevM = O.P.temporalReuseMThreshold * curM;
+        if (numUsed == 2) {
+            temporalOriginalDepth = taps[1].depth;
+            if (taps[1].depth != kRayTMax) {
+                f3 pd = normalize(rayDirNN(pc, rsx, rsy, O.W, O.H));
+                f3 wp = pc.pos + taps[1].depth * pd;
+                taps[1].depth = length(wp - ray.origin);
+            }
+        }
+        float centerPrevDepth = taps[0].depth;
+        if (centerPrevDepth != kRayTMax) {
+            f3 wp = ray.at(centerPrevDepth);
+            centerPrevDepth = length(wp - pc.pos);
+        }
+        bool hasSel = output.runningSum > 0.f;
+        int start = talbot ? 0 : 1;
+        if (start == 1) selectedId = 0;
+        for (int i = start; i < numUsed; i++) {
+            float talbotW = 1.f;
+            float nPy = 0.f;
+            if (taps[i].p_y > 0.f) {
+                nPy = taps[i].p_y;
+                if (std::isnan(taps[i].runningSum) || std::isinf(taps[i].runningSum)) taps[i].runningSum = 0.f;
+                if (i > 0 && taps[i].runningSum != 0.f) {
+                    // resampleNeighbor (VR/ReSTIRHelper.slang:582-598)
+                    float ph = evaluatePHat(O, ray, sg, O.temporalExtra.data(), O.spatialOpt, taps[i]);
+                    float w = ph / taps[i].p_y;
+                    if (std::isinf(w) || std::isnan(w)) w = 0.f;
+                    taps[i].runningSum *= w;
+                    taps[i].p_y = ph;
+                }
+            } else {
+                taps[i].p_y = 0.f;
+                taps[i].runningSum = 0.f;
+            }
+            if (talbot && taps[i].runningSum > 0.f) {
+                float psum = 0, pqi = 0, k = 0;
+                for (int j = 0; j < numUsed; j++) {
+                    int tx = j == 0 ? x : rsx, ty = j == 0 ? y : rsy;
+                    float cM = std::min(maxPrevM, taps[j].M);
+                    k += cM;
+                    if (j == 0) {
+                        pqi = taps[i].p_y;
+                        psum += taps[i].p_y * cM;
+                    } else if (i == j) {
+                        pqi = nPy;
+                        psum += nPy * cM;
+                    } else {
+                        Ray nr;
+                        if (j == 0) nr = {O.S.cam.pos, rayDir(O.S.cam, tx, ty, O.W, O.H), 0, 0};
+                        else nr = {pc.pos, normalize(rayDirNN(pc, tx, ty, O.W, O.H)), 0, 0};
+                        float usedDepth = j == 0 ? taps[i].depth : (i == 0 ? centerPrevDepth : temporalOriginalDepth);
+                        nr.tMax = usedDepth;
+                        float backup = taps[i].depth;
+                        taps[i].depth = usedDepth;
+                        float py = evaluatePHat(O, nr, sg, i == 0 ? curExtra.data() : O.temporalExtra.data(),
+                                                O.spatialOpt, taps[i]);
+                        taps[i].depth = backup;
+                        if (std::isinf(py) || std::isnan(py)) py = 0.f;
+                        psum += py * cM;
+                    }
+                }
+                if (psum > 0) talbotW = pqi * k / psum;
+            }
+            taps[i].runningSum *= talbotW;
+            bool curSel = resampleStepMaxM(taps[i], maxPrevM, output, sg);
+            hasSel |= curSel;
+            if (curSel) selectedId = i;
+        }
+        if (b > 1) {
+            if (hasSel && selectedId > 0) {
+                int mi = decodeMaxIndirect(output.sampledPixel);
+                for (int k = 0; k < mi; k++) curExtra[centerExtra + k] = O.temporalExtra[output.extraBounceStartId + k];
+            }
+            output.extraBounceStartId = centerExtra;
+        }
+        cur[pixelId] = output;
+    });
+}
+
+// R2 / Hammersley neighbour offsets (VR/SpatialReuse.cs.slang:64-81). The R2
+// constants are single-precision literals widened to double.
+static void neighborOffset(int sampleId, int sampleCount, float radius, int frameSeed, int upperBound, int sampler,
+                           int& ox, int& oy) {
+    f2 u;
+    if (sampler == 0) {
+        uint32_t i = (uint32_t)sampleId;
+        i = (i & 0x55555555u) << 1 | (i & 0xAAAAAAAAu) >> 1;
+        i = (i & 0x33333333u) << 2 | (i & 0xCCCCCCCCu) >> 2;
+        i = (i & 0x0F0F0F0Fu) << 4 | (i & 0xF0F0F0F0u) >> 4;
+        i = (i & 0x00FF00FFu) << 8 | (i & 0xFF00FF00u) >> 8;
+        i = (i << 16) | (i >> 16);
+        u = {(float)sampleId / (float)sampleCount, (float)i * 2.3283064365386963e-10f};
+    } else {
+        double mult = (double)frameSeed * upperBound + sampleId;
+        if (sampleId == 0) u = {0, 0};
+        else {
+            double a = (double)0.754877669f * mult, bb = (double)0.569840296f * mult;
+            u = {(float)(a - std::floor(a)), (float)(bb - std::floor(bb))};
+        }
+    }
+    float r = std::sqrt(u.x);
+    float phi = 2.f * kPi * u.y;
+    ox = (int)(radius * (r * std::cos(phi)));
+    oy = (int)(radius * (r * std::sin(phi)));
+}
+
+// SpatialReuse (VR/SpatialReuse.cs.slang:94-265).
+static void stageSpatial(Oracle& O, int roundId) {
+    const int b = maxB(O);
+    int inIdx = O.totalRoundId % 2, outIdx = (O.totalRoundId + 1) % 2;
+    const std::vector<Reservoir>& in = O.res[inIdx];
+    std::vector<Reservoir>& out = O.res[outIdx];
+    const std::vector<f3>& inExtra = O.extra[inIdx];
+    std::vector<f3>& outExtra = O.extra[outIdx];
+    int roundOffset = (O.P.enableTemporalReuse ? 1 : 0) + 1;
+    int numRounds = O.P.spatialReuseRounds + roundOffset + 1;
+    int rid = roundId + roundOffset;
+    int r2Seed = ((O.P.spatialReuseRounds + 1) * O.frameCount + roundId) % 16;
+    bool talbot = O.P.spatialMISMethod == 1;
+    int K = O.P.spatialSampleCount;
+    std::vector<int> offx(K), offy(K);
+    for (int s = 0; s < K; s++)
+        neighborOffset(s, K, O.P.sampleRadius, r2Seed, O.P.spatialSampleCount, (int)O.P.randomSamplerType, offx[s],
+                       offy[s]);
+    forRows(O, [&](int x, int y) {
+        SampleGen sg = makeSampleGen(x, y, (uint32_t)(numRounds * O.frameCount + rid));
+        int pixelId = y * O.W + x;
+        Reservoir output = in[pixelId];
+        int centerExtra = output.extraBounceStartId;
+        f3 d = rayDir(O.S.cam, x, y, O.W, O.H);
+        Ray ray{O.S.cam.pos, d, 0, kRayTMax};
+        if (talbot) output = newReservoir();
+        bool hasSel = output.runningSum > 0.f;
+        Features cf = O.features[pixelId];
+        bool selfBg = !(cf.transmittance != 1.f);
+        if (selfBg) {
+            out[pixelId] = talbot ? in[pixelId] : output;
+            if (b > 1) {
+                int mi = decodeMaxIndirect(output.sampledPixel);
+                for (int k = 0; k < mi; k++) outExtra[output.extraBounceStartId + k] = inExtra[output.extraBounceStartId + k];
+            }
+            return;
+        }
+        int start = talbot ? 0 : 1;
+        for (int s = start; s < K; s++) {
+            int tx = x + offx[s], ty = y + offy[s];
+            if (!inRange(tx, ty, O.W, O.H)) continue;
+            Reservoir tap = in[ty * O.W + tx];
+            float misW = 1.f;
+            if (s > 0) {
+                // resampleNeighborSpatialReuse (VR/ReSTIRHelper.slang:563-580)
+                if (tap.runningSum != 0.f) {
+                    float ph = evaluatePHat(O, ray, sg, inExtra.data(), O.spatialOpt, tap);
+                    float w = ph / tap.p_y;
+                    if (std::isinf(w) || std::isnan(w)) w = 0.f;
+                    tap.runningSum *= w;
+                    tap.p_y = ph;
+                }
+            }
+            if (talbot && tap.runningSum > 0.f) {
+                float psum = 0, pqi = 0, k = 0;
+                for (int j = 0; j < K; j++) {
+                    int t2x = x + offx[j], t2y = y + offy[j];
+                    if (!inRange(t2x, t2y, O.W, O.H)) continue;
+                    const Reservoir& tap2 = in[t2y * O.W + t2x];
+                    k += tap2.M;
+                    if (j == 0) {
+                        pqi = tap.p_y;
+                        psum += tap.p_y * tap2.M;
+                    } else if (s == j) {
+                        pqi = tap2.p_y;
+                        psum += tap2.p_y * tap2.M;
+                    } else {
+                        Ray nr{ray.origin, rayDir(O.S.cam, t2x, t2y, O.W, O.H), 0, tap.depth};
+                        float py = evaluatePHat(O, nr, sg, inExtra.data(), O.spatialOpt, tap);
+                        if (std::isinf(py) || std::isnan(py)) py = 0.f;
+                        psum += py * tap2.M;
+                    }
+                }
+                if (psum > 0) misW = pqi * k / psum;
+            }
+            tap.runningSum *= misW;
+            if (resampleStep(tap, output, sg)) hasSel = true;
+        }
+        if (b > 1 && hasSel) {
+            int mi = decodeMaxIndirect(output.sampledPixel);
+            for (int k = 0; k < mi; k++) outExtra[centerExtra + k] = inExtra[output.extraBounceStartId + k];
+        }
+        output.extraBounceStartId = b > 1 ? centerExtra : output.extraBounceStartId;
+        out[pixelId] = output;
+    });
+}
+
+// CopyReservoirs (VR/CopyReservoirs.cs.slang:84-95).
+static void stageCopy(Oracle& O) {
+    const int b = maxB(O);
+    int idx = O.totalRoundId % 2;
+    forRows(O, [&](int x, int y) {
+        int pid = y * O.W + x;
+        Reservoir r = O.res[idx][pid];
+        O.temporalRes[pid] = r;
+        if (b > 1) {
+            int mi = decodeMaxIndirect(r.sampledPixel);
+            for (int k = 0; k < mi; k++) O.temporalExtra[r.extraBounceStartId + k] = O.extra[idx][r.extraBounceStartId + k];
+        }
+    });
+}
+
+// FinalShading (VR/FinalShading.cs.slang:71-141).
+static void stageFinal(Oracle& O) {
+    int idx = O.totalRoundId % 2;
+    bool noReuse = !O.P.enableSpatialReuse && !O.P.enableTemporalReuse;
+    forRows(O, [&](int x, int y) {
+        int pid = y * O.W + x;
+        SampleGen sg = makeSampleGen(x, y, (uint32_t)(O.numTotalRounds * O.frameCount + O.numTotalRounds - 1));
+        f3 outC = mk3(0.f);
+        if (O.P.useReference) {
+            f4 c = O.color[idx][pid];
+            outC = {c.x, c.y, c.z};
+        } else if (O.P.visualizeTotalTransmittance) {
+            outC = mk3(std::pow(O.features[pid].transmittance, 2.2f));
+        } else {
+            Reservoir r = O.res[idx][pid];
+            if (r.runningSum > 0.f) {
+                f3 d = rayDir(O.S.cam, x, y, O.W, O.H);
+                Ray ray{O.S.cam.pos, d, 0, r.depth};
+                f3 c = evaluateF(O, r, O.extra[idx].data(), ray, sg, O.finalOpt, noReuse, true);
+                float Wt = r.p_y == 0.0f ? 1.f : r.runningSum / (r.p_y * r.M);
+                outC = outC + c * Wt;
+            }
+        }
+        f4 col{outC.x, outC.y, outC.z, 1.f};
+        if (std::isnan(col.x) || std::isnan(col.y) || std::isnan(col.z) || std::isinf(col.x) || std::isinf(col.y) ||
+            std::isinf(col.z))
+            col = {0, 0, 0, 0};
+        O.output[pid] = col;
+    });
+}
+
+// Frame set-up: option structs and round bookkeeping (VR/VolumetricReSTIR.cpp:339-496).
+static void prepareFrame(Oracle& O, int W, int H) {
+    const vrestir_params& p = O.P;
+    bool sizeChanged = W != O.W || H != O.H;
+    if (O.optionsChanged) {
+        O.frameCount = 0;
+        O.temporalAccumulated = false;
+        O.optionsChanged = false;
+    }
+    if (sizeChanged || O.res[0].empty()) {
+        O.W = W; O.H = H;
+        size_t n = (size_t)W * H;
+        O.features.assign(n, {0, 0.f});
+        O.temporalFeatures.assign(n, {0, 0.f});
+        for (int i = 0; i < 2; i++) {
+            O.res[i].assign(n, Reservoir{0, 0, 0, 0, {0, 0}, 0, 0, 0});
+            O.color[i].assign(n, {0, 0, 0, 0});
+        }
+        O.temporalRes.assign(n, Reservoir{0, 0, 0, 0, {0, 0}, 0, 0, 0});
+        size_t ne = n * (size_t)std::max(0, p.maxBounces - 1);
+        for (int i = 0; i < 2; i++) O.extra[i].assign(ne, {0, 0, 0});
+        O.temporalExtra.assign(ne, {0, 0, 0});
+        O.output.assign(n, {0, 0, 0, 0});
+        O.mvec.assign(n, {0, 0});
+        O.temporalAccumulated = false;
+    }
+    O.rowBegin = std::max(0, std::min(O.rowBegin, H));
+    O.rowEnd = O.rowEnd <= 0 ? H : std::min(O.rowEnd, H);
+    O.numTotalRounds = (p.enableSpatialReuse ? p.spatialReuseRounds : 0) + (p.enableTemporalReuse ? 1 : 0) + 1 + 1;
+    O.initialOpt = {VRESTIR_ANALYTIC_TRACKING, p.initialLightingTrackingMethod, p.initialLightSamples,
+                    p.initialLightingMipLevel, 1,
+                    p.initialVisibilityUseLinearSampler ? p.initialBaseMipLevel : p.initialBaseMipLevel + 8,
+                    p.initialVisibilityUseLinearSampler != 0, p.initialLightingUseLinearSampler != 0,
+                    p.initialVisibilityTStepScale, p.initialLightingTStepScale, p.useEnvironmentLights != 0,
+                    p.useAnalyticLights != 0, p.useEmissiveLights != 0, p.vertexReuseStartBounce};
+    O.spatialOpt = {p.spatialVisibilityTrackingMethod, p.spatialLightingTrackingMethod, 1, p.spatialLightingMipLevel, 1,
+                    p.spatialVisibilityMipLevel, p.spatialVisibilityUseLinearSampler != 0,
+                    p.spatialLightingUseLinearSampler != 0, p.spatialVisibilityTStepScale, p.spatialLightingTStepScale,
+                    p.useEnvironmentLights != 0, p.useAnalyticLights != 0, p.useEmissiveLights != 0,
+                    p.vertexReuseStartBounce};
+    bool fa = p.finalLightTrackingMethod == 1 || p.finalLightTrackingMethod == 2;
+    bool fv = p.finalVisibilityTrackingMethod == 1 || p.finalVisibilityTrackingMethod == 2;
+    O.finalOpt = {p.finalVisibilityTrackingMethod, p.finalLightTrackingMethod, fa ? 1 : p.finalLightSamples, 0,
+                  fv ? 1 : p.finalVisibilitySamples, 0, true, true, p.finalTStepScale, p.finalTStepScale,
+                  p.useEnvironmentLights != 0, p.useAnalyticLights != 0, p.useEmissiveLights != 0,
+                  p.vertexReuseStartBounce};
+    O.totalRoundId = 0;
+}
+
+static void endFrame(Oracle& O) {
+    O.temporalAccumulated = true;
+    O.S.prevCam = O.S.cam;
+    O.frameCount++;
+}
+
+template <class F>
+static void timed(Oracle& O, int stage, F&& fn) {
+    auto t0 = std::chrono::steady_clock::now();
+    fn();
+    O.stageSeconds[stage] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+static void runStage(Oracle& O, int stage, int round) {
+    const vrestir_params& p = O.P;
+    switch (stage) {
+        case VRESTIR_STAGE_GENERATE_FEATURES: timed(O, 0, [&] { stageFeatures(O); }); break;
+        case VRESTIR_STAGE_TRACE_RAYS: timed(O, 1, [&] { stageTrace(O); }); break;
+        case VRESTIR_STAGE_TEMPORAL_REUSE:
+            timed(O, 2, [&] {
+                stageTemporal(O, O.totalRoundId);
+                if (!p.enableSpatialReuse) {
+                    O.temporalRes = O.res[O.totalRoundId];
+                    O.temporalExtra = O.extra[O.totalRoundId];
+                }
+                O.temporalFeatures = O.features;
+            });
+            break;
+        case VRESTIR_STAGE_SPATIAL_REUSE:
+            timed(O, 3, [&] { stageSpatial(O, round); });
+            O.totalRoundId++;
+            break;
+        case VRESTIR_STAGE_COPY_RESERVOIRS: timed(O, 4, [&] { stageCopy(O); }); break;
+        case VRESTIR_STAGE_FINAL_SHADING: timed(O, 5, [&] { stageFinal(O); }); break;
+    }
+}
+
+// VolumetricReSTIR::execute (VR/VolumetricReSTIR.cpp:303-772), dispatch order.
+static void executeFrame(Oracle& O, int W, int H) {
+    const vrestir_params& p = O.P;
+    prepareFrame(O, W, H);
+    runStage(O, VRESTIR_STAGE_GENERATE_FEATURES, 0);
+    runStage(O, VRESTIR_STAGE_TRACE_RAYS, 0);
+    if (!p.useReference && p.enableTemporalReuse) runStage(O, VRESTIR_STAGE_TEMPORAL_REUSE, 0);
+    if (!p.useReference && p.enableSpatialReuse)
+        for (int r = 0; r < p.spatialReuseRounds; r++) runStage(O, VRESTIR_STAGE_SPATIAL_REUSE, r);
+    if (!p.useReference && p.enableTemporalReuse) runStage(O, VRESTIR_STAGE_COPY_RESERVOIRS, 0);
+    runStage(O, VRESTIR_STAGE_FINAL_SHADING, 0);
+    endFrame(O);
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+// ---------------------------------------------------------------------------
+// C interface for the tests (ctypes) and bench.py's CPU baseline.
+extern "C" {
+
+void* orc_create(void) {
+    Oracle* O = new Oracle();
+    vrestir_default_params_oracle(&O->P);
+    return O;
+}
+void orc_destroy(void* h) { delete (Oracle*)h; }
+
+// Defaults of VolumetricReSTIRParams (VR/VolumetricReSTIR.h:132-206).
+void vrestir_default_params_oracle(vrestir_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->maxBounces = 1;
+    p->enableTemporalReuse = 1;
+    p->enableSpatialReuse = 1;
+    p->vertexReuse = 0;
+    p->vertexReuseStartBounce = 1;
+    p->useReference = 0;
+    p->useEnvironmentLights = 1;
+    p->useAnalyticLights = 0;
+    p->useEmissiveLights = 0;
+    p->baselineSamplePerPixel = 1;
+    p->visualizeTotalTransmittance = 0;
+    p->useSurfaceScene = 0;
+    p->usePrevVolumeForReproj = 1;
+    p->initialBaseMipLevel = 1;
+    p->initialM = 4;
+    p->initialLightSamples = 1;
+    p->initialLightingMipLevel = 2;
+    p->initialVisibilityUseLinearSampler = 0;
+    p->initialLightingUseLinearSampler = 1;
+    p->initialLightingTrackingMethod = VRESTIR_RAY_MARCHING;
+    p->initialVisibilityTStepScale = 1.f;
+    p->initialLightingTStepScale = 2.f;
+    p->initialUseRussianRoulette = 1;
+    p->initialUseCoarserGridForIndirectBounce = 1;
+    p->temporalReuseMThreshold = 4.f;
+    p->temporalReprojectionMode = VRESTIR_REPROJECTION_LINEAR;
+    p->temporalMISMethod = VRESTIR_MIS_TALBOT;
+    p->temporalReprojectionMipLevel = 1;
+    p->spatialReuseRounds = 1;
+    p->spatialVisibilityMipLevel = 1;
+    p->spatialLightingMipLevel = 1;
+    p->spatialVisibilityUseLinearSampler = 1;
+    p->spatialLightingUseLinearSampler = 1;
+    p->spatialVisibilityTStepScale = 1.f;
+    p->spatialLightingTStepScale = 1.f;
+    p->spatialVisibilityTrackingMethod = VRESTIR_RAY_MARCHING;
+    p->spatialLightingTrackingMethod = VRESTIR_RAY_MARCHING;
+    p->randomSamplerType = VRESTIR_SAMPLER_R2;
+    p->sampleRadius = 10.f;
+    p->spatialSampleCount = 4;
+    p->enableVisibilitySimilarityRejection = 0;
+    p->spatialMISMethod = VRESTIR_MIS_TALBOT;
+    p->finalLightSamples = 1;
+    p->finalVisibilitySamples = 1;
+    p->finalVisibilityTrackingMethod = VRESTIR_ANALYTIC_TRACKING;
+    p->finalLightTrackingMethod = VRESTIR_ANALYTIC_TRACKING;
+    p->finalRandomSamplerType = VRESTIR_SAMPLER_R2;
+    p->finalTStepScale = 0.2f;
+}
+
+int orc_set_params(void* h, const vrestir_params* p) {
+    Oracle* O = (Oracle*)h;
+    if (p->vertexReuse || p->useSurfaceScene || p->useEmissiveLights) return VRESTIR_EUNSUPPORTED;
+    if (p->maxBounces < 1 || p->maxBounces > 8) return VRESTIR_EINVAL;
+    bool bChanged = p->maxBounces != O->P.maxBounces;
+    O->P = *p;
+    O->optionsChanged = true;
+    if (bChanged) { O->W = 0; O->H = 0; }
+    return VRESTIR_OK;
+}
+
+int orc_set_volume(void* h, const vrestir_volume_desc* d) {
+    Oracle* O = (Oracle*)h;
+    if (d->numMips < 1 || d->numMips > 8) return VRESTIR_EINVAL;
+    int align = 8 << (d->numMips - 1);
+    if (d->nx % align || d->ny % align || d->nz % align) return VRESTIR_EINVAL;
+    Medium& m = O->S.med;
+    for (auto& g : m.grid) g = GridSlot();
+    DenseGrid base;
+    base.nx = d->nx; base.ny = d->ny; base.nz = d->nz;
+    base.v.assign(d->density, d->density + (size_t)d->nx * d->ny * d->nz);
+    for (float& v : base.v) v = v > 0.f ? v : 0.f;
+    bool fp16 = d->brickFormat == VRESTIR_BRICK_FP16;
+    DenseGrid cons = dilateConservative(base);
+    DenseGrid curN = base, curC = cons;
+    for (int k = 0; k < d->numMips; k++) {
+        if (k > 0) { curN = downsample2(curN); curC = downsample2(curC); }
+        buildSlot(m.grid[k], curN, d->indexToWorld, 1 << k, fp16);
+        buildSlot(m.grid[8 + k], curC, d->indexToWorld, 1 << k, fp16);
+    }
+    m.numMips = d->numMips;
+    m.sigma_s = {d->sigmaS[0], d->sigmaS[1], d->sigmaS[2]};
+    m.sigma_a = {d->sigmaA[0], d->sigmaA[1], d->sigmaA[2]};
+    m.sigma_t = d->sigmaS[0] + d->sigmaA[0];
+    m.g = d->g;
+    m.densityScale = d->densityScale;
+    const float* X = m.grid[0].xform;
+    auto rl = [&](int r) { return std::sqrt(X[r * 4] * X[r * 4] + X[r * 4 + 1] * X[r * 4 + 1] + X[r * 4 + 2] * X[r * 4 + 2]); };
+    m.tStep = (rl(0) + rl(1) + rl(2)) / 3.f;
+    auto rd = [&](int r) { return X[r * 4] * X[r * 4] + X[r * 4 + 1] * X[r * 4 + 1] + X[r * 4 + 2] * X[r * 4 + 2]; };
+    m.superVoxelDiag = 8.f * std::sqrt(rd(0) + rd(1) + rd(2));
+    m.worldScaling = 1.f;
+    O->volumeSet = true;
+    O->optionsChanged = true;
+    return VRESTIR_OK;
+}
+
+int orc_set_lights(void* h, const vrestir_point_light* L, int n, const float env[3]) {
+    Oracle* O = (Oracle*)h;
+    O->S.lights.clear();
+    for (int i = 0; i < n; i++)
+        O->S.lights.push_back({{L[i].posW[0], L[i].posW[1], L[i].posW[2]},
+                               {L[i].intensity[0], L[i].intensity[1], L[i].intensity[2]}});
+    O->S.env = {env[0], env[1], env[2]};
+    return VRESTIR_OK;
+}
+
+int orc_set_camera(void* h, const vrestir_camera* c, int W, int H) {
+    Oracle* O = (Oracle*)h;
+    buildCamera(O->S.cam, c->posW, c->target, c->up, c->fovY, (float)W / (float)H, c->nearZ, c->farZ);
+    if (!O->camSet) O->S.prevCam = O->S.cam;
+    O->camSet = true;
+    return VRESTIR_OK;
+}
+
+int orc_execute(void* h, int W, int H, float* outRGBA, int rowBegin, int rowEnd, int nthreads) {
+    Oracle* O = (Oracle*)h;
+    if (!O->volumeSet || !O->camSet) return VRESTIR_ESTATE;
+    O->rowBegin = rowBegin; O->rowEnd = rowEnd; O->nthreads = nthreads;
+    executeFrame(*O, W, H);
+    if (outRGBA) std::memcpy(outRGBA, O->output.data(), sizeof(f4) * O->output.size());
+    return VRESTIR_OK;
+}
+
+int orc_prepare_frame(void* h, int W, int H, int nthreads) {
+    Oracle* O = (Oracle*)h;
+    if (!O->volumeSet || !O->camSet) return VRESTIR_ESTATE;
+    O->rowBegin = 0; O->rowEnd = H; O->nthreads = nthreads;
+    prepareFrame(*O, W, H);
+    return VRESTIR_OK;
+}
+int orc_run_stage(void* h, int stage, int round) {
+    runStage(*(Oracle*)h, stage, round);
+    return VRESTIR_OK;
+}
+int orc_end_frame(void* h) { endFrame(*(Oracle*)h); return VRESTIR_OK; }
+
+static void* bufPtr(Oracle* O, int id, size_t& bytes) {
+    switch (id) {
+        case VRESTIR_BUF_FEATURES: bytes = O->features.size() * 8; return O->features.data();
+        case VRESTIR_BUF_TEMPORAL_FEATURES: bytes = O->temporalFeatures.size() * 8; return O->temporalFeatures.data();
+        case VRESTIR_BUF_RESERVOIRS0: bytes = O->res[0].size() * 36; return O->res[0].data();
+        case VRESTIR_BUF_RESERVOIRS1: bytes = O->res[1].size() * 36; return O->res[1].data();
+        case VRESTIR_BUF_TEMPORAL_RESERVOIRS: bytes = O->temporalRes.size() * 36; return O->temporalRes.data();
+        case VRESTIR_BUF_EXTRA0: bytes = O->extra[0].size() * 12; return O->extra[0].data();
+        case VRESTIR_BUF_EXTRA1: bytes = O->extra[1].size() * 12; return O->extra[1].data();
+        case VRESTIR_BUF_TEMPORAL_EXTRA: bytes = O->temporalExtra.size() * 12; return O->temporalExtra.data();
+        case VRESTIR_BUF_COLOR: bytes = O->output.size() * 16; return O->output.data();
+    }
+    bytes = 0;
+    return nullptr;
+}
+int orc_read_buffer(void* h, int id, void* dst, size_t bytes) {
+    size_t n; void* p = bufPtr((Oracle*)h, id, n);
+    if (!p || bytes != n) return VRESTIR_EINVAL;
+    std::memcpy(dst, p, n);
+    return VRESTIR_OK;
+}
+int orc_write_buffer(void* h, int id, const void* src, size_t bytes) {
+    size_t n; void* p = bufPtr((Oracle*)h, id, n);
+    if (!p || bytes != n) return VRESTIR_EINVAL;
+    std::memcpy(p, src, n);
+    return VRESTIR_OK;
+}
+int orc_frame_count(void* h) { return ((Oracle*)h)->frameCount; }
+int orc_total_round_id(void* h) { return ((Oracle*)h)->totalRoundId; }
+void orc_stage_seconds(void* h, double out[6]) { std::memcpy(out, ((Oracle*)h)->stageSeconds, sizeof(double) * 6); }
+
+// Batched tracking primitive: same contract as vrestir_track_batch, on host memory.
+int orc_track_batch(void* h, int mode, int mip, int linear, float tStepScale, int numSamples, int n, const float* rays,
+                    const uint32_t* seeds, float* out) {
+    Oracle* O = (Oracle*)h;
+    const Medium& m = O->S.med;
+    for (int i = 0; i < n; i++) {
+        const float* r = rays + 8 * i;
+        Ray ray{{r[0], r[1], r[2]}, {r[3], r[4], r[5]}, 0.f, r[6]};
+        SampleGen sg = makeSampleGen(seeds[3 * i], seeds[3 * i + 1], seeds[3 * i + 2]);
+        float* o = out + 8 * i;
+        for (int k = 0; k < 8; k++) o[k] = 0.f;
+        switch (mode) {
+            case VRESTIR_TRACK_TR_ANALYTIC: o[0] = trAnalytic(m, ray, mip, sg, linear != 0); break;
+            case VRESTIR_TRACK_TR_RAY_MARCHING: o[0] = trRayMarch(m, ray, mip, linear != 0, tStepScale, sg); break;
+            case VRESTIR_TRACK_TR_RATIO: o[0] = trResidualRatio(m, ray, mip, sg, false, true); break;
+            case VRESTIR_TRACK_TR_RESIDUAL_RATIO: o[0] = trResidualRatio(m, ray, mip, sg, false, false); break;
+            case VRESTIR_TRACK_TR_ANALOG_RESIDUAL_RATIO: o[0] = trResidualRatio(m, ray, mip, sg, true, false); break;
+            case VRESTIR_TRACK_SAMPLE_ANALYTIC: {
+                float hh[4], pd[4], tr[4];
+                int ns = std::max(1, std::min(4, numSamples));
+                sampleAnalytic(m, ray, sg, linear != 0, hh, mip, pd, tr, ns);
+                // packed: hit[0..ns-1] then tr / pdf are reported for the first two samples
+                o[0] = hh[0]; o[1] = pd[0]; o[2] = tr[0];
+                o[3] = ns > 1 ? hh[1] : 0.f; o[4] = ns > 1 ? pd[1] : 0.f; o[5] = ns > 1 ? tr[1] : 0.f;
+                o[6] = ns > 2 ? hh[2] : 0.f; o[7] = ns > 3 ? hh[3] : 0.f;
+                break;
+            }
+            case VRESTIR_TRACK_SAMPLE_DECOMPOSITION: {
+                MediumInteraction mi{};
+                sampleDecomposition(m, ray, sg, mi, mip);
+                o[0] = mi.valid ? 1.f : 0.f;
+                o[1] = mi.p.x; o[2] = mi.p.y; o[3] = mi.p.z;
+                break;
+            }
+            case VRESTIR_TRACK_SAMPLE_CELL_BY_DENSITY: {
+                float bound = 0.f;
+                f2 s = cellByDensity(m, ray, sg, mip, bound);
+                o[0] = s.x; o[1] = s.y; o[2] = bound;
+                o[3] = rejectionSampleDepth(m, ray, sg, mip);
+                break;
+            }
+            case VRESTIR_TRACK_FEATURE_MARCH: o[0] = featureMarch(m, ray, mip, linear != 0, tStepScale, sg); break;
+            case VRESTIR_TRACK_DENSITY_AT_POINT: o[0] = densityWorld(m, ray.origin, mip); break;
+            default: return VRESTIR_EINVAL;
+        }
+    }
+    return VRESTIR_OK;
+}
+
+// RNG stream of the pass's sample generator for one pixel / sample number.
+void orc_rng_stream(uint32_t px, uint32_t py, uint32_t sampleNumber, int n, uint32_t* out) {
+    SampleGen g = makeSampleGen(px, py, sampleNumber);
+    for (int i = 0; i < n; i++) out[i] = xoshiroNext(g);
+}
+uint32_t orc_morton(uint32_t x, uint32_t y) { return interleave32(x, y); }
+void orc_splitmix64(uint64_t seed, int n, uint64_t* out) {
+    for (int i = 0; i < n; i++) out[i] = splitmix64Next(seed);
+}
+void orc_xoshiro128ss(const uint32_t seed[4], int n, uint32_t* out) {
+    SampleGen g;
+    for (int i = 0; i < 4; i++) g.s[i] = seed[i];
+    for (int i = 0; i < n; i++) out[i] = xoshiroNext(g);
+}
+float orc_round_to_half(float v) { return roundToHalf(v); }
+
+// Neighbour offsets of one spatial round.
+void orc_neighbor_offsets(int sampleCount, float radius, int frameSeed, int sampler, int* out) {
+    for (int s = 0; s < sampleCount; s++) neighborOffset(s, sampleCount, radius, frameSeed, sampleCount, sampler, out[2 * s], out[2 * s + 1]);
+}
+
+// Volume introspection for the grid tests.
+int orc_grid_info(void* h, int slot, int* dims, float* maxValue, int* numBricks) {
+    Oracle* O = (Oracle*)h;
+    const GridSlot& g = O->S.med.grid[slot];
+    if (!g.valid) return VRESTIR_EINVAL;
+    dims[0] = g.nx; dims[1] = g.ny; dims[2] = g.nz;
+    *maxValue = g.maxValue;
+    *numBricks = (int)g.level0.size();
+    return VRESTIR_OK;
+}
+// Brick bounds of the brick containing mip-local voxel (x,y,z), or -1 if empty.
+int orc_brick_bounds(void* h, int slot, int x, int y, int z, float* out4) {
+    Oracle* O = (Oracle*)h;
+    const GridSlot& g = O->S.med.grid[slot];
+    const LeafNode* leaf = leafAtPoint(g, {x + 0.5f, y + 0.5f, z + 0.5f});
+    if (!leaf) return -1;
+    out4[0] = leaf->bounds.x; out4[1] = leaf->bounds.y; out4[2] = leaf->bounds.z; out4[3] = leaf->bounds.w;
+    return 0;
+}
+// Stored voxel value (point lookup through the tree).
+float orc_voxel(void* h, int slot, int x, int y, int z) {
+    Oracle* O = (Oracle*)h;
+    return valueAtPoint(O->S.med.grid[slot], {x + 0.5f, y + 0.5f, z + 0.5f}, false);
+}
+
+}  // extern "C"

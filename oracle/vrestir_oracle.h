@@ -1,0 +1,42 @@
+/* vrestir_oracle.h -- TEST INFRASTRUCTURE ONLY: C interface of the CPU oracle.
+ *
+ * Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * only, always as the checker. The product library never links it. */
+#ifndef VRESTIR_ORACLE_H
+#define VRESTIR_ORACLE_H
+#include "../include/vrestir.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+void vrestir_default_params_oracle(vrestir_params* p);
+void* orc_create(void);
+void orc_destroy(void* h);
+int orc_set_params(void* h, const vrestir_params* p);
+int orc_set_volume(void* h, const vrestir_volume_desc* d);
+int orc_set_lights(void* h, const vrestir_point_light* lights, int n, const float env[3]);
+int orc_set_camera(void* h, const vrestir_camera* c, int width, int height);
+int orc_execute(void* h, int width, int height, float* outRGBA, int rowBegin, int rowEnd, int nthreads);
+int orc_prepare_frame(void* h, int width, int height, int nthreads);
+int orc_run_stage(void* h, int stage, int round);
+int orc_end_frame(void* h);
+int orc_read_buffer(void* h, int id, void* dst, size_t bytes);
+int orc_write_buffer(void* h, int id, const void* src, size_t bytes);
+int orc_frame_count(void* h);
+int orc_total_round_id(void* h);
+void orc_stage_seconds(void* h, double out[6]);
+int orc_track_batch(void* h, int mode, int mip, int linear, float tStepScale, int numSamples, int n,
+                    const float* rays, const uint32_t* seeds, float* out);
+void orc_rng_stream(uint32_t px, uint32_t py, uint32_t sampleNumber, int n, uint32_t* out);
+uint32_t orc_morton(uint32_t x, uint32_t y);
+void orc_splitmix64(uint64_t seed, int n, uint64_t* out);
+void orc_xoshiro128ss(const uint32_t seed[4], int n, uint32_t* out);
+float orc_round_to_half(float v);
+void orc_neighbor_offsets(int sampleCount, float radius, int frameSeed, int sampler, int* out);
+int orc_grid_info(void* h, int slot, int* dims, float* maxValue, int* numBricks);
+int orc_brick_bounds(void* h, int slot, int x, int y, int z, float* out4);
+float orc_voxel(void* h, int slot, int x, int y, int z);
+#ifdef __cplusplus
+}
+#endif
+#endif

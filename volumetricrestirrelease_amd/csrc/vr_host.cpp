@@ -1,0 +1,755 @@
+// vr_host.cpp -- the C ABI of the pass (include/vrestir.h): handle lifecycle,
+// scene upload, option surface, and the per-frame dispatch sequence of
+// VolumetricReSTIR::execute (VR/VolumetricReSTIR.cpp:303-906) on a HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/vrestir.h"
+#include "vr_launch.h"
+#include "vr_volume.h"
+
+namespace vr {
+struct ResStore { float4 a, b; };
+}
+
+using namespace vr;
+
+static thread_local std::string g_createError;
+
+struct vrestir_handle {
+    int device = 0;
+    std::string err;
+    vrestir_params P;
+    bool optionsChanged = true;
+    bool offsetsDirty = true;
+    // scene
+    Scene sceneHost;
+    Scene* sceneDev = nullptr;
+    std::vector<void*> volumeAllocs;
+    HostVolume meta;  // slot metadata only (pools are released after upload)
+    bool volumeSet = false, camSet = false;
+    vrestir_camera camDesc;
+    CameraFrame cam{}, prevCam{};
+    float view[16], proj[16], prevView[16], prevProj[16];
+    bool havePrev = false;
+    // frame state
+    int W = 0, H = 0, allocB = 0;
+    int frameCount = 0;
+    bool temporalAccumulated = false;
+    int totalRoundId = 0;
+    Frame frame{};
+    bool frameOpen = false;
+    // buffers (VR/VolumetricReSTIR.cpp:419-450)
+    ResStore* res[2] = {nullptr, nullptr};
+    ResStore* resT = nullptr;
+    float* extra[2] = {nullptr, nullptr};
+    float* extraT = nullptr;
+    float2* features = nullptr;
+    float2* featuresT = nullptr;
+    float4* color[2] = {nullptr, nullptr};
+    float4* outInternal = nullptr;
+    int* offsetsDev = nullptr;
+};
+
+static int fail(vrestir_handle* h, int code, const std::string& msg) {
+    if (h) h->err = msg;
+    else g_createError = msg;
+    return code;
+}
+#define HIPCHK(h, call)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail((h), VRESTIR_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+static void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+static void freeFrameBuffers(vrestir_handle* h) {
+    for (int i = 0; i < 2; i++) { dfree(h->res[i]); dfree(h->extra[i]); dfree(h->color[i]); }
+    dfree(h->resT); dfree(h->extraT); dfree(h->features); dfree(h->featuresT); dfree(h->outInternal);
+    h->W = h->H = 0;
+}
+static void freeVolume(vrestir_handle* h) {
+    for (void* p : h->volumeAllocs) (void)hipFree(p);
+    h->volumeAllocs.clear();
+}
+
+// ------------------------------------------------------------------ params
+extern "C" int32_t vrestir_version(void) { return VRESTIR_ABI_VERSION; }
+
+// Defaults of VolumetricReSTIRParams (VR/VolumetricReSTIR.h:132-206).
+extern "C" void vrestir_default_params(vrestir_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->maxBounces = 1;
+    p->enableTemporalReuse = 1;
+    p->enableSpatialReuse = 1;
+    p->vertexReuseStartBounce = 1;
+    p->useEnvironmentLights = 1;
+    p->baselineSamplePerPixel = 1;
+    p->usePrevVolumeForReproj = 1;
+    p->initialBaseMipLevel = 1;
+    p->initialM = 4;
+    p->initialLightSamples = 1;
+    p->initialLightingMipLevel = 2;
+    p->initialLightingUseLinearSampler = 1;
+    p->initialLightingTrackingMethod = VRESTIR_RAY_MARCHING;
+    p->initialVisibilityTStepScale = 1.f;
+    p->initialLightingTStepScale = 2.f;
+    p->initialUseRussianRoulette = 1;
+    p->initialUseCoarserGridForIndirectBounce = 1;
+    p->temporalReuseMThreshold = 4.f;
+    p->temporalReprojectionMode = VRESTIR_REPROJECTION_LINEAR;
+    p->temporalMISMethod = VRESTIR_MIS_TALBOT;
+    p->temporalReprojectionMipLevel = 1;
+    p->spatialReuseRounds = 1;
+    p->spatialVisibilityMipLevel = 1;
+    p->spatialLightingMipLevel = 1;
+    p->spatialVisibilityUseLinearSampler = 1;
+    p->spatialLightingUseLinearSampler = 1;
+    p->spatialVisibilityTStepScale = 1.f;
+    p->spatialLightingTStepScale = 1.f;
+    p->spatialVisibilityTrackingMethod = VRESTIR_RAY_MARCHING;
+    p->spatialLightingTrackingMethod = VRESTIR_RAY_MARCHING;
+    p->randomSamplerType = VRESTIR_SAMPLER_R2;
+    p->sampleRadius = 10.f;
+    p->spatialSampleCount = 4;
+    p->spatialMISMethod = VRESTIR_MIS_TALBOT;
+    p->finalLightSamples = 1;
+    p->finalVisibilitySamples = 1;
+    p->finalVisibilityTrackingMethod = VRESTIR_ANALYTIC_TRACKING;
+    p->finalLightTrackingMethod = VRESTIR_ANALYTIC_TRACKING;
+    p->finalRandomSamplerType = VRESTIR_SAMPLER_R2;
+    p->finalTStepScale = 0.2f;
+}
+
+static int checkParams(vrestir_handle* h, const vrestir_params* p) {
+    if (!p) return fail(h, VRESTIR_EINVAL, "params is NULL");
+    if (p->useSurfaceScene) return fail(h, VRESTIR_EUNSUPPORTED, "surface scenes are not part of this pass build");
+    if (p->vertexReuse) return fail(h, VRESTIR_EUNSUPPORTED, "vertex reuse is not implemented");
+    if (p->useEmissiveLights) return fail(h, VRESTIR_EUNSUPPORTED, "emissive-triangle lights need a surface scene");
+    if (p->maxBounces < 1 || p->maxBounces > 4) return fail(h, VRESTIR_EINVAL, "maxBounces must be in [1, 4]");
+    if (p->initialM < 1) return fail(h, VRESTIR_EINVAL, "initialM must be >= 1");
+    if (p->spatialSampleCount < 1 || p->spatialSampleCount > kMaxSpatialSamples)
+        return fail(h, VRESTIR_EINVAL, "spatialSampleCount must be in [1, 32]");
+    if (p->spatialReuseRounds < 0) return fail(h, VRESTIR_EINVAL, "spatialReuseRounds must be >= 0");
+    if (p->baselineSamplePerPixel < 1) return fail(h, VRESTIR_EINVAL, "baselineSamplePerPixel must be >= 1");
+    auto method = [](uint32_t m) { return m <= 4u; };
+    if (!method(p->initialLightingTrackingMethod) || !method(p->spatialVisibilityTrackingMethod) ||
+        !method(p->spatialLightingTrackingMethod) || !method(p->finalVisibilityTrackingMethod) ||
+        !method(p->finalLightTrackingMethod))
+        return fail(h, VRESTIR_EINVAL, "tracking method out of range");
+    if (p->initialBaseMipLevel < 0 || p->initialBaseMipLevel > 7 || p->initialLightingMipLevel < 0 ||
+        p->initialLightingMipLevel > 7 || p->spatialVisibilityMipLevel < 0 || p->spatialVisibilityMipLevel > 7 ||
+        p->spatialLightingMipLevel < 0 || p->spatialLightingMipLevel > 7 || p->temporalReprojectionMipLevel < 0 ||
+        p->temporalReprojectionMipLevel > 7)
+        return fail(h, VRESTIR_EINVAL, "mip level out of range");
+    return VRESTIR_OK;
+}
+
+extern "C" int32_t vrestir_create(const vrestir_params* params, vrestir_handle** out) {
+    if (!out) return fail(nullptr, VRESTIR_EINVAL, "out is NULL");
+    *out = nullptr;
+    vrestir_params p;
+    if (params) p = *params;
+    else vrestir_default_params(&p);
+    int rc = checkParams(nullptr, &p);
+    if (rc) return rc;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(nullptr, VRESTIR_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    vrestir_handle* h = new vrestir_handle();
+    h->device = dev;
+    h->P = p;
+    std::memset(&h->sceneHost, 0, sizeof(Scene));
+    if (hipMalloc((void**)&h->sceneDev, sizeof(Scene)) != hipSuccess ||
+        hipMalloc((void**)&h->offsetsDev, sizeof(int) * 16 * kMaxSpatialSamples * 2) != hipSuccess) {
+        dfree(h->sceneDev);
+        delete h;
+        return fail(nullptr, VRESTIR_ENOMEM, "device allocation failed");
+    }
+    *out = h;
+    return VRESTIR_OK;
+}
+
+extern "C" int32_t vrestir_destroy(vrestir_handle* h) {
+    if (!h) return VRESTIR_OK;
+    (void)hipSetDevice(h->device);
+    (void)hipDeviceSynchronize();
+    freeFrameBuffers(h);
+    freeVolume(h);
+    dfree(h->sceneDev);
+    dfree(h->offsetsDev);
+    delete h;
+    return VRESTIR_OK;
+}
+
+extern "C" const char* vrestir_last_error(const vrestir_handle* h) {
+    return h ? h->err.c_str() : g_createError.c_str();
+}
+
+extern "C" int32_t vrestir_set_params(vrestir_handle* h, const vrestir_params* p) {
+    if (!h) return VRESTIR_EINVAL;
+    int rc = checkParams(h, p);
+    if (rc) return rc;
+    h->P = *p;
+    h->optionsChanged = true;
+    h->offsetsDirty = true;
+    return VRESTIR_OK;
+}
+extern "C" int32_t vrestir_get_params(const vrestir_handle* h, vrestir_params* out) {
+    if (!h || !out) return VRESTIR_EINVAL;
+    *out = h->P;
+    return VRESTIR_OK;
+}
+extern "C" int32_t vrestir_reset_history(vrestir_handle* h) {
+    if (!h) return VRESTIR_EINVAL;
+    h->optionsChanged = true;
+    return VRESTIR_OK;
+}
+extern "C" int32_t vrestir_frame_count(const vrestir_handle* h, int32_t* out) {
+    if (!h || !out) return VRESTIR_EINVAL;
+    *out = h->frameCount;
+    return VRESTIR_OK;
+}
+
+// ------------------------------------------------------------------ scene
+static int uploadScene(vrestir_handle* h) {
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipDeviceSynchronize());
+    HIPCHK(h, hipMemcpy(h->sceneDev, &h->sceneHost, sizeof(Scene), hipMemcpyHostToDevice));
+    return VRESTIR_OK;
+}
+
+template <class T>
+static int upload(vrestir_handle* h, const std::vector<T>& v, const void** dst) {
+    void* p = nullptr;
+    size_t bytes = std::max<size_t>(v.size() * sizeof(T), 16);
+    HIPCHK(h, hipMalloc(&p, bytes));
+    h->volumeAllocs.push_back(p);
+    if (!v.empty()) HIPCHK(h, hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    *dst = p;
+    return VRESTIR_OK;
+}
+
+extern "C" int32_t vrestir_set_volume(vrestir_handle* h, const vrestir_volume_desc* d) {
+    if (!h || !d) return VRESTIR_EINVAL;
+    HostVolume hv;
+    int rc = buildVolume(*d, hv, h->err);
+    if (rc) return rc;
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipDeviceSynchronize());
+    freeVolume(h);
+    Scene& S = h->sceneHost;
+    for (int s = 0; s < kMaxSlots; s++) {
+        Grid& G = S.grid[s];
+        std::memset(&G, 0, sizeof(Grid));
+        HostGrid& hg = hv.slots[s];
+        if (!hg.valid) continue;
+        const void* p;
+        if ((rc = upload(h, hg.brickTable, &p))) return rc;
+        G.brickTable = (const uint32_t*)p;
+        if ((rc = upload(h, hg.nodeTable, &p))) return rc;
+        G.nodeTable = (const uint32_t*)p;
+        if ((rc = upload(h, hg.bounds, &p))) return rc;
+        G.bounds = (const float*)p;
+        if (hg.fp16) { if ((rc = upload(h, hg.pool16, &p))) return rc; }
+        else { if ((rc = upload(h, hg.pool32, &p))) return rc; }
+        G.pool = p;
+        G.nbx = hg.nbx; G.nby = hg.nby; G.nbz = hg.nbz;
+        G.nnx = hg.nnx; G.nny = hg.nny; G.nnz = hg.nnz;
+        G.bmax[0] = (float)hg.nx; G.bmax[1] = (float)hg.ny; G.bmax[2] = (float)hg.nz;
+        std::memcpy(G.w2i, hg.w2i, sizeof(G.w2i));
+        G.maxValue = hg.maxValue;
+        G.fp16 = hg.fp16 ? 1 : 0;
+        G.valid = 1;
+        // keep metadata, drop the host copies of the pools
+        HostGrid& m = h->meta.slots[s];
+        m = HostGrid();
+        m.valid = true; m.nx = hg.nx; m.ny = hg.ny; m.nz = hg.nz; m.numBricks = hg.numBricks;
+        m.maxValue = hg.maxValue; m.fp16 = hg.fp16;
+        m.nbx = hg.nbx; m.nby = hg.nby; m.nbz = hg.nbz;
+    }
+    h->meta.numMips = hv.numMips;
+    S.numMips = hv.numMips;
+    S.sigma_t = d->sigmaS[0] + d->sigmaA[0];
+    for (int i = 0; i < 3; i++) { S.sigma_s[i] = d->sigmaS[i]; S.sigma_a[i] = d->sigmaA[i]; }
+    S.g = d->g;
+    S.densityScale = d->densityScale;
+    S.tStep = hv.tStep;
+    S.worldScaling = 1.f;
+    S.superVoxelDiag = hv.superVoxelDiag;
+    h->volumeSet = true;
+    h->optionsChanged = true;
+    return uploadScene(h);
+}
+
+extern "C" int32_t vrestir_load_vbx(vrestir_handle* h, const char* pathPrefix, int32_t numMips, int32_t brickFormat,
+                                    const float sigmaA[3], const float sigmaS[3], float g, float densityScale) {
+    if (!h || !pathPrefix) return VRESTIR_EINVAL;
+    std::string path = std::string(pathPrefix) + "_mip0.vbx";
+    int nx = 0, ny = 0, nz = 0;
+    vrestir_volume_desc d;
+    std::memset(&d, 0, sizeof(d));
+    int rc = vrestir_read_vbx(path.c_str(), nullptr, &nx, &ny, &nz, d.indexToWorld);
+    if (rc) return fail(h, rc, "cannot read " + path);
+    std::vector<float> dens((size_t)nx * ny * nz);
+    rc = vrestir_read_vbx(path.c_str(), dens.data(), &nx, &ny, &nz, d.indexToWorld);
+    if (rc) return fail(h, rc, "cannot read " + path);
+    d.nx = nx; d.ny = ny; d.nz = nz;
+    d.density = dens.data();
+    d.numMips = numMips;
+    d.brickFormat = brickFormat;
+    for (int i = 0; i < 3; i++) { d.sigmaA[i] = sigmaA[i]; d.sigmaS[i] = sigmaS[i]; }
+    d.g = g;
+    d.densityScale = densityScale;
+    return vrestir_set_volume(h, &d);
+}
+
+extern "C" int32_t vrestir_set_lights(vrestir_handle* h, const vrestir_point_light* lights, int32_t count,
+                                      const float env[3]) {
+    if (!h) return VRESTIR_EINVAL;
+    if (count < 0 || count > kMaxLights || (count > 0 && !lights))
+        return fail(h, VRESTIR_EINVAL, "light count must be in [0, 256]");
+    Scene& S = h->sceneHost;
+    S.numLights = count;
+    for (int i = 0; i < count; i++)
+        for (int k = 0; k < 3; k++) {
+            S.lights[6 * i + k] = lights[i].posW[k];
+            S.lights[6 * i + 3 + k] = lights[i].intensity[k];
+        }
+    for (int k = 0; k < 3; k++) S.env[k] = env ? env[k] : 0.f;
+    return uploadScene(h);
+}
+
+extern "C" int32_t vrestir_set_camera(vrestir_handle* h, const vrestir_camera* c) {
+    if (!h || !c) return VRESTIR_EINVAL;
+    if (!(c->fovY > 0.f && c->fovY < 3.14159f)) return fail(h, VRESTIR_EINVAL, "fovY must be in (0, pi)");
+    h->camDesc = *c;
+    h->camSet = true;
+    return VRESTIR_OK;
+}
+
+// UVW frame (SC/Camera/Camera.cpp:183-189) and lookAt / perspective matrices
+// in double precision, stored as float (row-vector convention).
+static void buildCamera(const vrestir_camera& c, float aspect, CameraFrame& cf, float* view, float* proj) {
+    double P[3] = {c.posW[0], c.posW[1], c.posW[2]};
+    double w[3] = {(double)c.target[0] - P[0], (double)c.target[1] - P[1], (double)c.target[2] - P[2]};
+    double up[3] = {c.up[0], c.up[1], c.up[2]};
+    auto nrm = [](double* v) { double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); v[0] /= l; v[1] /= l; v[2] /= l; };
+    auto crs = [](const double* a, const double* b, double* o) {
+        o[0] = a[1] * b[2] - a[2] * b[1]; o[1] = a[2] * b[0] - a[0] * b[2]; o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    nrm(w);
+    double u[3], v[3];
+    crs(w, up, u); nrm(u);
+    crs(u, w, v); nrm(v);
+    double th = std::tan((double)c.fovY * 0.5);
+    double ul = th * aspect, vl = th;
+    for (int i = 0; i < 3; i++) {
+        cf.pos[i] = c.posW[i];
+        cf.W[i] = (float)w[i];
+        cf.U[i] = (float)(u[i] * ul);
+        cf.V[i] = (float)(v[i] * vl);
+    }
+    double vw[16] = {u[0], v[0], -w[0], 0, u[1], v[1], -w[1], 0, u[2], v[2], -w[2], 0,
+                     -(u[0] * P[0] + u[1] * P[1] + u[2] * P[2]), -(v[0] * P[0] + v[1] * P[1] + v[2] * P[2]),
+                     (w[0] * P[0] + w[1] * P[1] + w[2] * P[2]), 1};
+    double pr[16] = {0};
+    pr[0] = 1.0 / (aspect * th);
+    pr[5] = 1.0 / th;
+    pr[10] = c.farZ / (c.nearZ - c.farZ);
+    pr[11] = -1.0;
+    pr[14] = -((double)c.farZ * c.nearZ) / ((double)c.farZ - c.nearZ);
+    for (int i = 0; i < 16; i++) { view[i] = (float)vw[i]; proj[i] = (float)pr[i]; }
+}
+
+// Neighbour offsets (VR/SpatialReuse.cs.slang:64-81) for the 16 R2 frame seeds:
+// u from the R2 sequence (single-precision constants, double arithmetic) or
+// Hammersley, then int(radius * sample_disk(u)).
+static void neighbourOffsets(const vrestir_params& p, std::vector<int>& out) {
+    out.assign(16 * kMaxSpatialSamples * 2, 0);
+    int K = p.spatialSampleCount;
+    for (int seed = 0; seed < 16; seed++)
+        for (int s = 0; s < K; s++) {
+            float ux, uy;
+            if (p.randomSamplerType == VRESTIR_SAMPLER_HAMMERSLEY) {
+                uint32_t i = (uint32_t)s;
+                i = (i & 0x55555555u) << 1 | (i & 0xAAAAAAAAu) >> 1;
+                i = (i & 0x33333333u) << 2 | (i & 0xCCCCCCCCu) >> 2;
+                i = (i & 0x0F0F0F0Fu) << 4 | (i & 0xF0F0F0F0u) >> 4;
+                i = (i & 0x00FF00FFu) << 8 | (i & 0xFF00FF00u) >> 8;
+                i = (i << 16) | (i >> 16);
+                ux = (float)s / (float)K;
+                uy = (float)i * 2.3283064365386963e-10f;
+            } else if (s == 0) {
+                ux = 0.f; uy = 0.f;
+            } else {
+                double m = (double)seed * p.spatialSampleCount + s;
+                double a = (double)0.754877669f * m, b = (double)0.569840296f * m;
+                ux = (float)(a - std::floor(a));
+                uy = (float)(b - std::floor(b));
+            }
+            float r = std::sqrt(ux);
+            float phi = 2.f * 3.14159265358979323846f * uy;
+            out[(seed * kMaxSpatialSamples + s) * 2 + 0] = (int)(p.sampleRadius * (r * std::cos(phi)));
+            out[(seed * kMaxSpatialSamples + s) * 2 + 1] = (int)(p.sampleRadius * (r * std::sin(phi)));
+        }
+}
+
+static void fillOptions(Options& o, uint32_t vm, uint32_t lm, int ls, int lmip, int vs, int vmip, bool vlin,
+                        bool llin, float vts, float lts, const vrestir_params& p) {
+    o.visibilityTrackingMethod = vm; o.lightingTrackingMethod = lm;
+    o.lightSamples = ls; o.lightingMipLevel = lmip; o.visibilitySamples = vs; o.visibilityMipLevel = vmip;
+    o.visibilityUseLinearSampler = vlin; o.lightingUseLinearSampler = llin;
+    o.visibilityTStepScale = vts; o.lightingTStepScale = lts;
+    o.useEnv = p.useEnvironmentLights; o.useAnalytic = p.useAnalyticLights; o.useEmissive = p.useEmissiveLights;
+    o.vertexReuseStartBounce = p.vertexReuseStartBounce;
+}
+
+static int allocFrameBuffers(vrestir_handle* h, int W, int H, int B) {
+    freeFrameBuffers(h);
+    size_t n = (size_t)W * H, ne = n * (size_t)std::max(B - 1, 1);
+    auto al = [&](void** p, size_t bytes) -> int {
+        HIPCHK(h, hipMalloc(p, bytes));
+        HIPCHK(h, hipMemset(*p, 0, bytes));
+        return VRESTIR_OK;
+    };
+    int rc;
+    for (int i = 0; i < 2; i++) {
+        if ((rc = al((void**)&h->res[i], n * sizeof(ResStore)))) return rc;
+        if ((rc = al((void**)&h->extra[i], ne * 12))) return rc;
+        if ((rc = al((void**)&h->color[i], n * 16))) return rc;
+    }
+    if ((rc = al((void**)&h->resT, n * sizeof(ResStore)))) return rc;
+    if ((rc = al((void**)&h->extraT, ne * 12))) return rc;
+    if ((rc = al((void**)&h->features, n * 8))) return rc;
+    if ((rc = al((void**)&h->featuresT, n * 8))) return rc;
+    if ((rc = al((void**)&h->outInternal, n * 16))) return rc;
+    h->W = W; h->H = H; h->allocB = B;
+    return VRESTIR_OK;
+}
+
+// Frame set-up: reallocation, history reset on option change, round
+// bookkeeping and the three SamplingOptions (VR/VolumetricReSTIR.cpp:339-496).
+extern "C" int32_t vrestir_prepare_frame(vrestir_handle* h, int32_t W, int32_t H) {
+    if (!h) return VRESTIR_EINVAL;
+    if (!h->volumeSet) return fail(h, VRESTIR_ESTATE, "no volume bound (vrestir_set_volume)");
+    if (!h->camSet) return fail(h, VRESTIR_ESTATE, "no camera bound (vrestir_set_camera)");
+    if (W <= 0 || H <= 0 || W > 16384 || H > 16384) return fail(h, VRESTIR_EINVAL, "resolution out of range");
+    const vrestir_params& p = h->P;
+    HIPCHK(h, hipSetDevice(h->device));
+    if (h->optionsChanged) {
+        h->frameCount = 0;
+        h->temporalAccumulated = false;
+        h->optionsChanged = false;
+    }
+    if (W != h->W || H != h->H || p.maxBounces != h->allocB) {
+        int rc = allocFrameBuffers(h, W, H, p.maxBounces);
+        if (rc) return rc;
+        h->temporalAccumulated = false;
+    }
+    if (h->offsetsDirty) {
+        std::vector<int> offs;
+        neighbourOffsets(p, offs);
+        HIPCHK(h, hipMemcpy(h->offsetsDev, offs.data(), offs.size() * sizeof(int), hipMemcpyHostToDevice));
+        h->offsetsDirty = false;
+    }
+    float aspect = (float)W / (float)H;
+    buildCamera(h->camDesc, aspect, h->cam, h->view, h->proj);
+    if (!h->havePrev) {
+        h->prevCam = h->cam;
+        std::memcpy(h->prevView, h->view, sizeof(h->view));
+        std::memcpy(h->prevProj, h->proj, sizeof(h->proj));
+        h->havePrev = true;
+    }
+    Frame& F = h->frame;
+    std::memset(&F, 0, sizeof(F));
+    F.width = W; F.height = H;
+    F.frameCount = h->frameCount;
+    F.numTotalRounds = (p.enableSpatialReuse ? p.spatialReuseRounds : 0) + (p.enableTemporalReuse ? 1 : 0) + 2;
+    F.maxBounces = p.maxBounces;
+    F.initialM = p.initialM;
+    F.useRussianRoulette = p.initialUseRussianRoulette;
+    F.noReuse = !p.enableSpatialReuse && !p.enableTemporalReuse;
+    F.coarserIndirect = p.initialUseCoarserGridForIndirectBounce;
+    F.useReference = p.useReference;
+    F.baselineSpp = p.baselineSamplePerPixel;
+    F.visualizeTransmittance = p.visualizeTotalTransmittance;
+    F.isFirstFrame = h->temporalAccumulated ? 0 : 1;
+    F.temporalMIS = (int)p.temporalMISMethod;
+    F.reprojectionMode = (int)p.temporalReprojectionMode;
+    F.reprojectionMip = p.temporalReprojectionMipLevel;
+    F.temporalThreshold = p.temporalReuseMThreshold;
+    F.spatialMIS = (int)p.spatialMISMethod;
+    F.spatialSampleCount = p.spatialSampleCount;
+    F.enableTemporal = p.enableTemporalReuse;
+    F.enableSpatial = p.enableSpatialReuse;
+    F.spatialRounds = p.spatialReuseRounds;
+    F.cam = h->cam;
+    F.prevCam = h->prevCam;
+    std::memcpy(F.prevView, h->prevView, sizeof(F.prevView));
+    std::memcpy(F.prevProj, h->prevProj, sizeof(F.prevProj));
+    fillOptions(F.initialOpt, VRESTIR_ANALYTIC_TRACKING, p.initialLightingTrackingMethod, p.initialLightSamples,
+                p.initialLightingMipLevel, 1,
+                p.initialVisibilityUseLinearSampler ? p.initialBaseMipLevel : p.initialBaseMipLevel + 8,
+                p.initialVisibilityUseLinearSampler, p.initialLightingUseLinearSampler, p.initialVisibilityTStepScale,
+                p.initialLightingTStepScale, p);
+    fillOptions(F.spatialOpt, p.spatialVisibilityTrackingMethod, p.spatialLightingTrackingMethod, 1,
+                p.spatialLightingMipLevel, 1, p.spatialVisibilityMipLevel, p.spatialVisibilityUseLinearSampler,
+                p.spatialLightingUseLinearSampler, p.spatialVisibilityTStepScale, p.spatialLightingTStepScale, p);
+    bool fa = p.finalLightTrackingMethod == 1 || p.finalLightTrackingMethod == 2;
+    bool fv = p.finalVisibilityTrackingMethod == 1 || p.finalVisibilityTrackingMethod == 2;
+    fillOptions(F.finalOpt, p.finalVisibilityTrackingMethod, p.finalLightTrackingMethod, fa ? 1 : p.finalLightSamples,
+                0, fv ? 1 : p.finalVisibilitySamples, 0, true, true, p.finalTStepScale, p.finalTStepScale, p);
+    // every mip level the options reference must exist
+    int maxMip = std::max({p.initialBaseMipLevel, p.initialLightingMipLevel, p.spatialVisibilityMipLevel,
+                           p.spatialLightingMipLevel, p.temporalReprojectionMipLevel});
+    if (maxMip >= h->meta.numMips)
+        return fail(h, VRESTIR_EINVAL, "an option references a mip level the bound volume does not have");
+    if (p.useAnalyticLights && h->sceneHost.numLights == 0)
+        return fail(h, VRESTIR_EINVAL, "useAnalyticLights is set but no point light is bound");
+    h->totalRoundId = 0;
+    h->frameOpen = true;
+    return VRESTIR_OK;
+}
+
+extern "C" int32_t vrestir_run_stage(vrestir_handle* h, void* stream, int32_t stage, int32_t round, float* outRGBA) {
+    if (!h) return VRESTIR_EINVAL;
+    if (!h->frameOpen) return fail(h, VRESTIR_ESTATE, "vrestir_prepare_frame must precede stages");
+    const vrestir_params& p = h->P;
+    hipStream_t st = (hipStream_t)stream;
+    LaunchArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.stream = st;
+    a.stage = stage;
+    a.round = round;
+    a.maxBounces = p.maxBounces;
+    a.width = h->W;
+    a.height = h->H;
+    a.scene = h->sceneDev;
+    a.frame = h->frame;
+    a.offsets = h->offsetsDev;
+    a.features = h->features;
+    a.featuresT = h->featuresT;
+    a.resT = h->resT;
+    a.extraT = h->extraT;
+    size_t n = (size_t)h->W * h->H, ne = n * (size_t)std::max(p.maxBounces - 1, 1);
+    int t = h->totalRoundId;
+    switch (stage) {
+        case VRESTIR_STAGE_GENERATE_FEATURES: break;
+        case VRESTIR_STAGE_TRACE_RAYS:
+            a.res0 = h->res[0]; a.extra0 = h->extra[0]; a.color0 = h->color[0];
+            break;
+        case VRESTIR_STAGE_TEMPORAL_REUSE:
+            a.res0 = h->res[t % 2]; a.extra0 = h->extra[t % 2];
+            a.mvec = (float2*)outRGBA;  // motion vectors ride in the output slot for this stage
+            a.frame.outputMotionVec = outRGBA != nullptr;
+            break;
+        case VRESTIR_STAGE_SPATIAL_REUSE:
+            a.res0 = h->res[t % 2]; a.res1 = h->res[(t + 1) % 2];
+            a.extra0 = h->extra[t % 2]; a.extra1 = h->extra[(t + 1) % 2];
+            break;
+        case VRESTIR_STAGE_COPY_RESERVOIRS:
+            a.res0 = h->res[t % 2]; a.extra0 = h->extra[t % 2];
+            break;
+        case VRESTIR_STAGE_FINAL_SHADING:
+            a.res0 = h->res[t % 2]; a.extra0 = h->extra[t % 2]; a.color0 = h->color[t % 2];
+            a.out = outRGBA ? (float4*)outRGBA : h->outInternal;
+            break;
+        default: return fail(h, VRESTIR_EINVAL, "unknown stage");
+    }
+    HIPCHK(h, launchStage(a));
+    if (stage == VRESTIR_STAGE_TEMPORAL_REUSE) {
+        if (!p.enableSpatialReuse) {
+            HIPCHK(h, hipMemcpyAsync(h->resT, h->res[t % 2], n * sizeof(ResStore), hipMemcpyDeviceToDevice, st));
+            if (p.maxBounces > 1)
+                HIPCHK(h, hipMemcpyAsync(h->extraT, h->extra[t % 2], ne * 12, hipMemcpyDeviceToDevice, st));
+        }
+        HIPCHK(h, hipMemcpyAsync(h->featuresT, h->features, n * 8, hipMemcpyDeviceToDevice, st));
+    }
+    if (stage == VRESTIR_STAGE_SPATIAL_REUSE) h->totalRoundId++;
+    return VRESTIR_OK;
+}
+
+extern "C" int32_t vrestir_end_frame(vrestir_handle* h) {
+    if (!h) return VRESTIR_EINVAL;
+    h->temporalAccumulated = true;
+    h->prevCam = h->cam;
+    std::memcpy(h->prevView, h->view, sizeof(h->view));
+    std::memcpy(h->prevProj, h->proj, sizeof(h->proj));
+    h->frameCount++;
+    h->frameOpen = false;
+    return VRESTIR_OK;
+}
+
+extern "C" int32_t vrestir_execute(vrestir_handle* h, void* stream, int32_t W, int32_t H, float* outRGBA,
+                                   float* outMotionVec) {
+    if (!h) return VRESTIR_EINVAL;
+    if (!outRGBA) return fail(h, VRESTIR_EINVAL, "outRGBA is NULL");
+    int rc = vrestir_prepare_frame(h, W, H);
+    if (rc) return rc;
+    const vrestir_params& p = h->P;
+    if ((rc = vrestir_run_stage(h, stream, VRESTIR_STAGE_GENERATE_FEATURES, 0, nullptr))) return rc;
+    if ((rc = vrestir_run_stage(h, stream, VRESTIR_STAGE_TRACE_RAYS, 0, nullptr))) return rc;
+    if (!p.useReference && p.enableTemporalReuse)
+        if ((rc = vrestir_run_stage(h, stream, VRESTIR_STAGE_TEMPORAL_REUSE, 0, outMotionVec))) return rc;
+    if (!p.useReference && p.enableSpatialReuse)
+        for (int r = 0; r < p.spatialReuseRounds; r++)
+            if ((rc = vrestir_run_stage(h, stream, VRESTIR_STAGE_SPATIAL_REUSE, r, nullptr))) return rc;
+    if (!p.useReference && p.enableTemporalReuse)
+        if ((rc = vrestir_run_stage(h, stream, VRESTIR_STAGE_COPY_RESERVOIRS, 0, nullptr))) return rc;
+    if ((rc = vrestir_run_stage(h, stream, VRESTIR_STAGE_FINAL_SHADING, 0, outRGBA))) return rc;
+    return vrestir_end_frame(h);
+}
+
+// ------------------------------------------------------------------ buffers (exchange format)
+static int bufInfo(vrestir_handle* h, int id, void** dev, size_t& elems, int& kind) {
+    size_t n = (size_t)h->W * h->H;
+    int B = h->allocB;
+    size_t ne = n * (size_t)(B > 1 ? B - 1 : 0);
+    kind = 0;
+    switch (id) {
+        case VRESTIR_BUF_FEATURES: *dev = h->features; elems = n * 8; return 0;
+        case VRESTIR_BUF_TEMPORAL_FEATURES: *dev = h->featuresT; elems = n * 8; return 0;
+        case VRESTIR_BUF_RESERVOIRS0: *dev = h->res[0]; elems = n; kind = 1; return 0;
+        case VRESTIR_BUF_RESERVOIRS1: *dev = h->res[1]; elems = n; kind = 1; return 0;
+        case VRESTIR_BUF_TEMPORAL_RESERVOIRS: *dev = h->resT; elems = n; kind = 1; return 0;
+        case VRESTIR_BUF_EXTRA0: *dev = h->extra[0]; elems = ne * 12; return 0;
+        case VRESTIR_BUF_EXTRA1: *dev = h->extra[1]; elems = ne * 12; return 0;
+        case VRESTIR_BUF_TEMPORAL_EXTRA: *dev = h->extraT; elems = ne * 12; return 0;
+        case VRESTIR_BUF_COLOR: *dev = h->outInternal; elems = n * 16; return 0;
+    }
+    return -1;
+}
+
+extern "C" int32_t vrestir_read_buffer(vrestir_handle* h, int32_t id, void* dst, size_t bytes) {
+    if (!h || !dst) return VRESTIR_EINVAL;
+    void* dev;
+    size_t elems;
+    int kind;
+    if (bufInfo(h, id, &dev, elems, kind) || !dev) return fail(h, VRESTIR_EINVAL, "unknown or unallocated buffer");
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipDeviceSynchronize());
+    if (kind == 0) {
+        if (bytes != elems) return fail(h, VRESTIR_EINVAL, "byte count does not match the buffer");
+        HIPCHK(h, hipMemcpy(dst, dev, bytes, hipMemcpyDeviceToHost));
+        return VRESTIR_OK;
+    }
+    if (bytes != elems * 36) return fail(h, VRESTIR_EINVAL, "byte count does not match the buffer");
+    std::vector<ResStore> tmp(elems);
+    HIPCHK(h, hipMemcpy(tmp.data(), dev, elems * sizeof(ResStore), hipMemcpyDeviceToHost));
+    int B = h->allocB;
+    char* o = (char*)dst;
+    for (size_t i = 0; i < elems; i++) {
+        float f[8] = {tmp[i].a.x, tmp[i].a.y, tmp[i].a.z, tmp[i].a.w, tmp[i].b.x, tmp[i].b.y, tmp[i].b.z, tmp[i].b.w};
+        std::memcpy(o + 36 * i, f, 32);
+        int32_t ex = B > 1 ? (int32_t)(i * (size_t)(B - 1)) : 0;
+        std::memcpy(o + 36 * i + 32, &ex, 4);
+    }
+    return VRESTIR_OK;
+}
+
+extern "C" int32_t vrestir_write_buffer(vrestir_handle* h, int32_t id, const void* src, size_t bytes) {
+    if (!h || !src) return VRESTIR_EINVAL;
+    void* dev;
+    size_t elems;
+    int kind;
+    if (bufInfo(h, id, &dev, elems, kind) || !dev) return fail(h, VRESTIR_EINVAL, "unknown or unallocated buffer");
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipDeviceSynchronize());
+    if (kind == 0) {
+        if (bytes != elems) return fail(h, VRESTIR_EINVAL, "byte count does not match the buffer");
+        HIPCHK(h, hipMemcpy(dev, src, bytes, hipMemcpyHostToDevice));
+        return VRESTIR_OK;
+    }
+    if (bytes != elems * 36) return fail(h, VRESTIR_EINVAL, "byte count does not match the buffer");
+    std::vector<ResStore> tmp(elems);
+    const char* s = (const char*)src;
+    for (size_t i = 0; i < elems; i++) {
+        float f[8];
+        std::memcpy(f, s + 36 * i, 32);
+        tmp[i].a = make_float4(f[0], f[1], f[2], f[3]);
+        tmp[i].b = make_float4(f[4], f[5], f[6], f[7]);
+    }
+    HIPCHK(h, hipMemcpy(dev, tmp.data(), elems * sizeof(ResStore), hipMemcpyHostToDevice));
+    return VRESTIR_OK;
+}
+
+extern "C" int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t mode, int32_t mip,
+                                       int32_t useLinearSampler, float tStepScale, int32_t numSamples, int32_t n,
+                                       const float* rays, const uint32_t* seeds, float* out) {
+    if (!h) return VRESTIR_EINVAL;
+    if (!h->volumeSet) return fail(h, VRESTIR_ESTATE, "no volume bound");
+    if (mode < 0 || mode > VRESTIR_TRACK_DENSITY_AT_POINT) return fail(h, VRESTIR_EINVAL, "unknown tracking mode");
+    if (mip < 0 || mip >= kMaxSlots || !h->meta.slots[mip].valid) return fail(h, VRESTIR_EINVAL, "grid slot not bound");
+    if (n < 0 || (n > 0 && (!rays || !seeds || !out))) return fail(h, VRESTIR_EINVAL, "bad ray batch");
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, launchTrack((hipStream_t)stream, h->sceneDev, mode, mip, useLinearSampler, tStepScale, numSamples, n,
+                          rays, seeds, out));
+    return VRESTIR_OK;
+}
+
+// ------------------------------------------------------------------ host helpers
+extern "C" int32_t vrestir_synth_fbm(uint64_t seed, int32_t nx, int32_t ny, int32_t nz, int32_t octaves,
+                                     float baseFrequency, float lacunarity, float gain, int32_t sphereMask,
+                                     float threshold, float amplitude, float* out) {
+    return synthFbm(seed, nx, ny, nz, octaves, baseFrequency, lacunarity, gain, sphereMask, threshold, amplitude, out);
+}
+
+extern "C" int32_t vrestir_count_bricks(const float* d, int32_t nx, int32_t ny, int32_t nz, int64_t* out) {
+    if (!d || !out || nx % 8 || ny % 8 || nz % 8) return VRESTIR_EINVAL;
+    int64_t c = 0;
+    for (int bz = 0; bz < nz / 8; bz++)
+        for (int by = 0; by < ny / 8; by++)
+            for (int bx = 0; bx < nx / 8; bx++) {
+                bool any = false;
+                for (int z = 0; z < 8 && !any; z++)
+                    for (int y = 0; y < 8 && !any; y++)
+                        for (int x = 0; x < 8 && !any; x++)
+                            if (d[(size_t)(bx * 8 + x) + (size_t)nx * ((size_t)(by * 8 + y) + (size_t)ny * (bz * 8 + z))] > 0.f)
+                                any = true;
+                c += any;
+            }
+    *out = c;
+    return VRESTIR_OK;
+}
+
+// Orbit about `center` (camera animation mode 2, VR/VolumetricReSTIR.cpp:879-904):
+// rotate the start position about the vertical axis through the point of the
+// initial view ray closest to the centre, looking at that point.
+extern "C" int32_t vrestir_orbit_camera(const vrestir_camera* base, const float center[3], int32_t frame,
+                                        float degreesPerFrame, vrestir_camera* out) {
+    if (!base || !center || !out) return VRESTIR_EINVAL;
+    double P[3] = {base->posW[0], base->posW[1], base->posW[2]}, T[3] = {base->target[0], base->target[1], base->target[2]};
+    double f[3] = {T[0] - P[0], T[1] - P[1], T[2] - P[2]};
+    double fl = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    for (double& x : f) x /= fl;
+    double cp[3] = {center[0] - P[0], center[1] - P[1], center[2] - P[2]};
+    double proj = cp[0] * f[0] + cp[1] * f[1] + cp[2] * f[2];
+    double rc[3] = {P[0] + proj * f[0], P[1] + proj * f[1], P[2] + proj * f[2]};
+    double X[3] = {P[0] - rc[0], 0.0, P[2] - rc[2]};
+    double xl = std::sqrt(X[0] * X[0] + X[2] * X[2]);
+    *out = *base;
+    if (xl <= 0.0) return VRESTIR_OK;
+    X[0] /= xl; X[2] /= xl;
+    double Y[3] = {X[2], 0.0, -X[0]};  // cross((0,1,0), X)
+    double ang = (double)frame * degreesPerFrame * 3.14159265358979323846 / 180.0;
+    double arc[3] = {rc[0], P[1], rc[2]};
+    for (int i = 0; i < 3; i++) {
+        out->posW[i] = (float)(arc[i] + xl * (X[i] * std::cos(ang) + Y[i] * std::sin(ang)));
+        out->target[i] = (float)rc[i];
+    }
+    return VRESTIR_OK;
+}

@@ -1,0 +1,535 @@
+// vr_kernels.hip -- the six per-frame kernels of the VolumetricReSTIR pass, the
+// reference path tracer and the batched tracking primitive, for gfx950.
+//
+// One lane per pixel, 256-lane workgroups covering 16x16 pixel tiles (4 wave64
+// of 16x4 pixels each, so neighbouring lanes trace neighbouring rays).
+// Reservoirs are 32 B per pixel (two float4 loads, coalesced across the wave),
+// extra-bounce records 12 B per bounce, features 8 B, colour 16 B.
+#include "vr_launch.h"
+#include "vr_pass.hpp"
+
+namespace vr {
+
+struct PixelIdx { int x, y, id; bool ok; };
+VR_DEV PixelIdx pixelOf(const Frame& F) {
+    PixelIdx p;
+    p.x = blockIdx.x * 16 + (threadIdx.x & 15);
+    p.y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    p.ok = p.x < F.width && p.y < F.height;
+    p.id = p.y * F.width + p.x;
+    return p;
+}
+VR_DEV bool inRange(int x, int y, int W, int H) { return x >= 0 && x < W && y >= 0 && y < H; }
+
+// GenerateFeatures (VR/GenerateFeatures.cs.slang:57-102).
+__global__ __launch_bounds__(256) void kFeatures(const Scene* __restrict__ sc, const Frame fr,
+                                                 float2* __restrict__ features) {
+    const Scene& S = *sc;
+    const Frame& F = fr;
+    PixelIdx px = pixelOf(F);
+    if (!px.ok || F.useReference) return;
+    Rng rng{0, 0, 0, 0};
+    Ray r{ld3(F.cam.pos), rayDir(F.cam, px.x, px.y, F.width, F.height), 0.f, kTMax};
+    float accT = featureMarch(S, r, 0, true, F.initialOpt.visibilityTStepScale, rng);
+    features[px.id] = make_float2(__int_as_float(1), accT);
+}
+
+// IntegrateByVolumePathTracing (VR/VolumePathTracingFunctions.slang:3-132), NEE on.
+VR_DEV v3 pathTrace(const Scene& S, const Frame& Fr, Ray ray, Rng& rng, int lightSamples, int maxBounces) {
+    MediumInteraction mi{V3(0.f), V3(0.f), S.g, false};
+    v3 beta = V3(1.f), L = V3(0.f);
+    for (int bounce = 0; bounce < maxBounces; bounce++) {
+        mi.valid = false;
+        sampleDecomposition(S, ray, rng, mi, 0);
+        if (mi.valid) {
+            v3 albedo = ld3(S.sigma_s) / S.sigma_t;
+            beta = beta * albedo;
+            v3 Ld = directLightingPT(S, rng, mi, lightSamples, Fr.initialOpt, 0, 3u);
+            L = L + beta * Ld;
+            v3 wi = V3(0.f);
+            if (maxBounces > 1) {
+                float u0 = u01(rng);
+                float u1 = u01(rng);
+                samplePhase(mi.g, -ray.d, wi, u0, u1);
+            }
+            if (bounce < maxBounces - 1) {
+                ray = {mi.p, wi, 0.f, kTMax};
+                if (Fr.useRussianRoulette && bounce >= 2) {
+                    if (u01(rng) < albedo.x) beta = beta / albedo.x;
+                    else bounce = maxBounces;
+                }
+            }
+        } else {
+            if (bounce == 0) L = L + beta * ld3(S.env);
+            bounce = maxBounces;
+        }
+    }
+    return L;
+}
+
+// TraceRays (VR/TraceRays.cs.slang:64-201).
+template <int B>
+__global__ __launch_bounds__(256) void kTrace(const Scene* __restrict__ sc, const Frame fr,
+                                              ResStore* __restrict__ outRes, float* __restrict__ outExtra,
+                                              float4* __restrict__ outColor) {
+    const Scene& S = *sc;
+    const Frame& F = fr;
+    PixelIdx px = pixelOf(F);
+    if (!px.ok) return;
+    Rng rng = rngCreate(px.x, px.y, (uint32_t)(F.numTotalRounds * F.frameCount));
+    Ray ray{ld3(F.cam.pos), rayDir(F.cam, px.x, px.y, F.width, F.height), 0.f, kTMax};
+    if (F.useReference) {
+        v3 avg = V3(0.f);
+        for (int s = 0; s < F.baselineSpp; s++)
+            avg = avg + pathTrace(S, F, ray, rng, max(1, F.initialOpt.lightSamples), B);
+        avg = avg / (float)F.baselineSpp;
+        outColor[px.id] = make_float4(avg.x, avg.y, avg.z, 1.f);
+        return;
+    }
+    constexpr int NE = B > 1 ? B - 1 : 1;
+    v3 finalEx[NE];
+#pragma unroll
+    for (int i = 0; i < NE; i++) finalEx[i] = V3(0.f);
+    Res fin = newRes();
+    const int M = F.initialM;
+    const int rounds = (M + 3) / 4;
+    for (int r = 0; r < rounds; r++) {
+        float h[4] = {0, 0, 0, 0}, pd[4] = {0, 0, 0, 0}, tr[4] = {0, 0, 0, 0};
+        int rs = r == rounds - 1 ? (M - 4 * (rounds - 1)) : 4;
+        if (!F.noReuse)
+            sampleAnalytic(S, ray, rng, F.initialOpt.visibilityUseLinearSampler != 0, F.initialOpt.visibilityMipLevel,
+                           rs, h, pd, tr);
+        for (int s = 0; s < rs; s++) {
+            v3 ex[NE];
+#pragma unroll
+            for (int i = 0; i < NE; i++) ex[i] = V3(0.f);
+            float hs = s == 0 ? h[0] : (s == 1 ? h[1] : (s == 2 ? h[2] : h[3]));
+            float ps = s == 0 ? pd[0] : (s == 1 ? pd[1] : (s == 2 ? pd[2] : pd[3]));
+            float ts = s == 0 ? tr[0] : (s == 1 ? tr[1] : (s == 2 ? tr[2] : tr[3]));
+            Res o = computeInitialSample<B>(S, F, ray, hs, ps, ts, rng, F.initialOpt, ex);
+            bool sel = resample(o, fin, rng);
+            if (sel && B > 1) {
+                int mi = decodeMaxIndirect(fin.sampledPixel);
+#pragma unroll
+                for (int k = 0; k < NE; k++)
+                    if (k < mi) finalEx[k] = ex[k];
+            }
+        }
+    }
+    // Write the selected extra-bounce records first: the p-hat re-evaluation
+    // below reads them back from the same slots (VR/TraceRays.cs.slang:169-198).
+    const int exStart = px.id * (B - 1);
+    if (B > 1) {
+        int mi = decodeMaxIndirect(fin.sampledPixel);
+#pragma unroll
+        for (int k = 0; k < NE; k++)
+            if (k < mi) {
+                float* e = outExtra + 3 * (size_t)(exStart + k);
+                e[0] = finalEx[k].x; e[1] = finalEx[k].y; e[2] = finalEx[k].z;
+            }
+    }
+    fin.exStart = exStart;
+    float ph = lum(evaluateF<B>(S, fin, outExtra, ray, rng, F.spatialOpt, false, false));
+    if (fin.runningSum > 0.f) {
+        fin.runningSum *= fin.p_y == 0.f ? 0.f : ph / fin.p_y;
+        fin.p_y = ph;
+    }
+    storeRes(outRes, px.id, fin);
+}
+
+VR_DEV float2 featuresAt(const float2* f, int id, int n) {
+    if (id < 0 || id >= n) return make_float2(0.f, 0.f);  // out-of-range loads read 0
+    return f[id];
+}
+VR_DEV bool isBackground(float2 f) { return f.y == 1.f && __float_as_int(f.x) != 0; }
+
+// TemporalReuse (VR/TemporalReuse.cs.slang:80-377).
+template <int B>
+__global__ __launch_bounds__(256) void kTemporal(const Scene* __restrict__ sc, const Frame fr,
+                                                 ResStore* __restrict__ cur, float* __restrict__ curExtra,
+                                                 const ResStore* __restrict__ temporal,
+                                                 const float* __restrict__ temporalExtra,
+                                                 const float2* __restrict__ features,
+                                                 const float2* __restrict__ temporalFeatures,
+                                                 float2* __restrict__ mvec) {
+    const Scene& S = *sc;
+    const Frame& F = fr;
+    PixelIdx px = pixelOf(F);
+    if (!px.ok || F.isFirstFrame) return;
+    const int W = F.width, H = F.height, N = W * H;
+    Rng rng = rngCreate(px.x, px.y, (uint32_t)(F.numTotalRounds * F.frameCount + 1));
+    int selectedId = -1;
+    int numUsed = 1;
+    Res taps[2];
+    taps[0] = loadRes(cur, px.id, B);
+    taps[1] = newRes();
+    Ray ray{ld3(F.cam.pos), rayDir(F.cam, px.x, px.y, W, H), 0.f, kTMax};
+    const bool talbot = F.temporalMIS == 1;
+    Res output = talbot ? newRes() : taps[0];
+    const int centerEx = taps[0].exStart;
+    float temporalOriginalDepth = 0.f;
+    int rsx = 0, rsy = 0;
+    bool isBgRes = isBackground(features[px.id]);
+    bool useFallback = true;
+    if (F.reprojectionMode != 1) {
+        float reprojDepth = taps[0].depth;
+        if (reprojDepth == kTMax && F.reprojectionMode != 2 && !isBgRes)
+            reprojDepth = rejectionSampleDepth(S, ray, rng, 8 + F.reprojectionMip);
+        v3 p = ray.o + ray.d * reprojDepth;
+        const float* V = F.prevView;
+        const float* P = F.prevProj;
+        float vx = p.x * V[0] + p.y * V[4] + p.z * V[8] + V[12];
+        float vy = p.x * V[1] + p.y * V[5] + p.z * V[9] + V[13];
+        float vz = p.x * V[2] + p.y * V[6] + p.z * V[10] + V[14];
+        float vw = p.x * V[3] + p.y * V[7] + p.z * V[11] + V[15];
+        float cx = vx * P[0] + vy * P[4] + vz * P[8] + vw * P[12];
+        float cy = vx * P[1] + vy * P[5] + vz * P[9] + vw * P[13];
+        float cw = vx * P[3] + vy * P[7] + vz * P[11] + vw * P[15];
+        float sx = cx / cw, sy = cy / cw;
+        int sxi, syi;
+        if (reprojDepth == kTMax) {
+            sx = (float)px.x + 0.5f; sy = (float)px.y + 0.5f;
+            sxi = px.x; syi = px.y;
+        } else {
+            sx = 0.5f * sx + 0.5f;
+            sy = -0.5f * sy + 0.5f;
+            sx *= (float)W; sy *= (float)H;
+            sxi = (int)sx; syi = (int)sy;
+        }
+        {
+            float2 tf = featuresAt(temporalFeatures, syi * W + sxi, N);
+            if (isBgRes && !isBackground(tf)) {
+                if (F.outputMotionVec) mvec[px.id] = make_float2((float)(rsx - px.x) / W, (float)(rsy - px.y) / H);
+                return;
+            }
+        }
+        sxi = (int)sx; syi = (int)sy;
+        rsx = sxi; rsy = syi;
+        if (inRange(sxi, syi, W, H)) {
+            numUsed++;
+            taps[1] = loadRes(temporal, syi * W + sxi, B);
+        }
+        if (numUsed > 1) useFallback = false;
+    }
+    if (useFallback) {
+        numUsed++;
+        rsx = px.x; rsy = px.y;
+        taps[1] = loadRes(temporal, px.id, B);
+    }
+    if (F.outputMotionVec) mvec[px.id] = make_float2((float)(rsx - px.x) / W, (float)(rsy - px.y) / H);
+    float maxPrevM = F.temporalThreshold * taps[0].M;
+    if (numUsed == 2) {
+        temporalOriginalDepth = taps[1].depth;
+        if (taps[1].depth != kTMax) {
+            v3 pd = norm3(rayDirNN(F.prevCam, rsx, rsy, W, H));
+            v3 wp = ld3(F.prevCam.pos) + taps[1].depth * pd;
+            taps[1].depth = len3(wp - ray.o);
+        }
+    }
+    float centerPrevDepth = taps[0].depth;
+    if (centerPrevDepth != kTMax) centerPrevDepth = len3(at(ray, centerPrevDepth) - ld3(F.prevCam.pos));
+    bool hasSel = output.runningSum > 0.f;
+    int start = talbot ? 0 : 1;
+    if (start == 1) selectedId = 0;
+    for (int i = start; i < numUsed; i++) {
+        Res& ti = i == 0 ? taps[0] : taps[1];
+        float talbotW = 1.f;
+        float nPy = 0.f;
+        if (ti.p_y > 0.f) {
+            nPy = ti.p_y;
+            if (isnan(ti.runningSum) || isinf(ti.runningSum)) ti.runningSum = 0.f;
+            if (i > 0 && ti.runningSum != 0.f) {
+                float ph = lum(evaluateF<B>(S, ti, temporalExtra, ray, rng, F.spatialOpt, false, false));
+                float w = ph / ti.p_y;
+                if (isinf(w) || isnan(w)) w = 0.f;
+                ti.runningSum *= w;
+                ti.p_y = ph;
+            }
+        } else {
+            ti.p_y = 0.f;
+            ti.runningSum = 0.f;
+        }
+        if (talbot && ti.runningSum > 0.f) {
+            float psum = 0.f, pqi = 0.f, k = 0.f;
+            for (int j = 0; j < numUsed; j++) {
+                const Res& tj = j == 0 ? taps[0] : taps[1];
+                float cM = fminf(maxPrevM, tj.M);
+                k += cM;
+                if (j == 0) {
+                    pqi = ti.p_y;
+                    psum += ti.p_y * cM;
+                } else if (i == j) {
+                    pqi = nPy;
+                    psum += nPy * cM;
+                } else {
+                    // j == 1, i == 0: the current sample seen from the previous camera
+                    Ray nr{ld3(F.prevCam.pos), norm3(rayDirNN(F.prevCam, rsx, rsy, W, H)), 0.f, centerPrevDepth};
+                    Res tmp = ti;
+                    tmp.depth = centerPrevDepth;
+                    float py = lum(evaluateF<B>(S, tmp, curExtra, nr, rng, F.spatialOpt, false, false));
+                    if (isinf(py) || isnan(py)) py = 0.f;
+                    psum += py * cM;
+                }
+            }
+            if (psum > 0) talbotW = pqi * k / psum;
+        }
+        ti.runningSum *= talbotW;
+        bool curSel = resampleMaxM(ti, maxPrevM, output, rng);
+        hasSel |= curSel;
+        if (curSel) selectedId = i;
+    }
+    if (B > 1) {
+        if (hasSel && selectedId > 0) {
+            int mi = decodeMaxIndirect(output.sampledPixel);
+            for (int k = 0; k < B - 1; k++) {
+                if (k >= mi) break;
+                const float* s = temporalExtra + 3 * (size_t)(output.exStart + k);
+                float* d = curExtra + 3 * (size_t)(centerEx + k);
+                d[0] = s[0]; d[1] = s[1]; d[2] = s[2];
+            }
+        }
+        output.exStart = centerEx;
+    }
+    storeRes(cur, px.id, output);
+}
+
+// SpatialReuse (VR/SpatialReuse.cs.slang:94-265) for one round.
+template <int B>
+__global__ __launch_bounds__(256) void kSpatial(const Scene* __restrict__ sc, const Frame fr,
+                                                int roundId, const int* __restrict__ offsets,
+                                                const ResStore* __restrict__ in,
+                                                ResStore* __restrict__ out, const float* __restrict__ inExtra,
+                                                float* __restrict__ outExtra, const float2* __restrict__ features) {
+    const Scene& S = *sc;
+    const Frame& F = fr;
+    PixelIdx px = pixelOf(F);
+    if (!px.ok) return;
+    const int W = F.width, H = F.height;
+    const int roundOffset = (F.enableTemporal ? 1 : 0) + 1;
+    const int numRounds = F.spatialRounds + roundOffset + 1;
+    const int rid = roundId + roundOffset;
+    const int r2Seed = ((F.spatialRounds + 1) * F.frameCount + roundId) % 16;
+    const int K = F.spatialSampleCount;
+    const bool talbot = F.spatialMIS == 1;
+    Rng rng = rngCreate(px.x, px.y, (uint32_t)(numRounds * F.frameCount + rid));
+    Res inCenter = loadRes(in, px.id, B);
+    Res output = inCenter;
+    const int centerEx = output.exStart;
+    Ray ray{ld3(F.cam.pos), rayDir(F.cam, px.x, px.y, W, H), 0.f, kTMax};
+    if (talbot) output = newRes();
+    bool hasSel = output.runningSum > 0.f;
+    float2 cf = features[px.id];
+    if (!(cf.y != 1.f)) {  // background pixel: pass through
+        storeRes(out, px.id, talbot ? inCenter : output);
+        if (B > 1) {
+            int mi = decodeMaxIndirect(output.sampledPixel);
+            for (int k = 0; k < B - 1; k++) {
+                if (k >= mi) break;
+                const float* s = inExtra + 3 * (size_t)(output.exStart + k);
+                float* d = outExtra + 3 * (size_t)(output.exStart + k);
+                d[0] = s[0]; d[1] = s[1]; d[2] = s[2];
+            }
+        }
+        return;
+    }
+    const int* offs = offsets + r2Seed * kMaxSpatialSamples * 2;
+    for (int s = talbot ? 0 : 1; s < K; s++) {
+        int tx = px.x + offs[2 * s], ty = px.y + offs[2 * s + 1];
+        if (!inRange(tx, ty, W, H)) continue;
+        Res tap = loadRes(in, ty * W + tx, B);
+        float misW = 1.f;
+        if (s > 0 && tap.runningSum != 0.f) {
+            float ph = lum(evaluateF<B>(S, tap, inExtra, ray, rng, F.spatialOpt, false, false));
+            float w = ph / tap.p_y;
+            if (isinf(w) || isnan(w)) w = 0.f;
+            tap.runningSum *= w;
+            tap.p_y = ph;
+        }
+        if (talbot && tap.runningSum > 0.f) {
+            float psum = 0.f, pqi = 0.f, k = 0.f;
+            for (int j = 0; j < K; j++) {
+                int t2x = px.x + offs[2 * j], t2y = px.y + offs[2 * j + 1];
+                if (!inRange(t2x, t2y, W, H)) continue;
+                int t2 = t2y * W + t2x;
+                float m2 = in[t2].a.y;
+                k += m2;
+                if (j == 0) {
+                    pqi = tap.p_y;
+                    psum += tap.p_y * m2;
+                } else if (s == j) {
+                    float py2 = in[t2].a.w;
+                    pqi = py2;
+                    psum += py2 * m2;
+                } else {
+                    Ray nr{ray.o, rayDir(F.cam, t2x, t2y, W, H), 0.f, tap.depth};
+                    float py = lum(evaluateF<B>(S, tap, inExtra, nr, rng, F.spatialOpt, false, false));
+                    if (isinf(py) || isnan(py)) py = 0.f;
+                    psum += py * m2;
+                }
+            }
+            if (psum > 0) misW = pqi * k / psum;
+        }
+        tap.runningSum *= misW;
+        if (resample(tap, output, rng)) hasSel = true;
+    }
+    if (B > 1 && hasSel) {
+        int mi = decodeMaxIndirect(output.sampledPixel);
+        for (int k = 0; k < B - 1; k++) {
+            if (k >= mi) break;
+            const float* s2 = inExtra + 3 * (size_t)(output.exStart + k);
+            float* d = outExtra + 3 * (size_t)(centerEx + k);
+            d[0] = s2[0]; d[1] = s2[1]; d[2] = s2[2];
+        }
+    }
+    if (B > 1) output.exStart = centerEx;
+    storeRes(out, px.id, output);
+}
+
+// CopyReservoirs (VR/CopyReservoirs.cs.slang:84-95).
+template <int B>
+__global__ __launch_bounds__(256) void kCopy(const Frame fr, const ResStore* __restrict__ cur,
+                                             const float* __restrict__ curExtra, ResStore* __restrict__ temporal,
+                                             float* __restrict__ temporalExtra) {
+    const Frame& F = fr;
+    PixelIdx px = pixelOf(F);
+    if (!px.ok) return;
+    ResStore r = cur[px.id];
+    temporal[px.id] = r;
+    if (B > 1) {
+        int mi = decodeMaxIndirect(__float_as_int(r.b.w));
+        int st = px.id * (B - 1);
+        for (int k = 0; k < B - 1; k++) {
+            if (k >= mi) break;
+            const float* s = curExtra + 3 * (size_t)(st + k);
+            float* d = temporalExtra + 3 * (size_t)(st + k);
+            d[0] = s[0]; d[1] = s[1]; d[2] = s[2];
+        }
+    }
+}
+
+// FinalShading (VR/FinalShading.cs.slang:71-141).
+template <int B>
+__global__ __launch_bounds__(256) void kFinal(const Scene* __restrict__ sc, const Frame fr,
+                                              const ResStore* __restrict__ cur, const float* __restrict__ curExtra,
+                                              const float2* __restrict__ features, const float4* __restrict__ colorIn,
+                                              float4* __restrict__ out) {
+    const Scene& S = *sc;
+    const Frame& F = fr;
+    PixelIdx px = pixelOf(F);
+    if (!px.ok) return;
+    Rng rng = rngCreate(px.x, px.y, (uint32_t)(F.numTotalRounds * F.frameCount + F.numTotalRounds - 1));
+    v3 c = V3(0.f);
+    if (F.useReference) {
+        float4 v = colorIn[px.id];
+        c = V3(v.x, v.y, v.z);
+    } else if (F.visualizeTransmittance) {
+        c = V3(powf(features[px.id].y, 2.2f));
+    } else {
+        Res r = loadRes(cur, px.id, B);
+        if (r.runningSum > 0.f) {
+            Ray ray{ld3(F.cam.pos), rayDir(F.cam, px.x, px.y, F.width, F.height), 0.f, r.depth};
+            v3 f = evaluateF<B>(S, r, curExtra, ray, rng, F.finalOpt, F.noReuse != 0, true);
+            float Wt = r.p_y == 0.0f ? 1.f : r.runningSum / (r.p_y * r.M);
+            c = c + f * Wt;
+        }
+    }
+    float4 col = make_float4(c.x, c.y, c.z, 1.f);
+    if (isnan(c.x) || isnan(c.y) || isnan(c.z) || isinf(c.x) || isinf(c.y) || isinf(c.z))
+        col = make_float4(0.f, 0.f, 0.f, 0.f);
+    out[px.id] = col;
+}
+
+// Batched tracking primitive: one lane per ray.
+__global__ __launch_bounds__(256) void kTrack(const Scene* __restrict__ sc, int mode, int mip, int linear,
+                                              float tStepScale, int numSamples, int n, const float* __restrict__ rays,
+                                              const uint32_t* __restrict__ seeds, float* __restrict__ out) {
+    const Scene& S = *sc;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* r = rays + 8 * (size_t)i;
+    Ray ray{V3(r[0], r[1], r[2]), V3(r[3], r[4], r[5]), 0.f, r[6]};
+    Rng rng = rngCreate(seeds[3 * i], seeds[3 * i + 1], seeds[3 * i + 2]);
+    float o[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    switch (mode) {
+        case 0: o[0] = trAnalytic(S, ray, mip, rng, linear != 0); break;
+        case 1: o[0] = trMarch(S, ray, mip, linear != 0, tStepScale, rng); break;
+        case 2: o[0] = trResidual(S, ray, mip, rng, false, true); break;
+        case 3: o[0] = trResidual(S, ray, mip, rng, false, false); break;
+        case 4: o[0] = trResidual(S, ray, mip, rng, true, false); break;
+        case 5: {
+            float h[4], pd[4], tr[4];
+            int ns = max(1, min(4, numSamples));
+            sampleAnalytic(S, ray, rng, linear != 0, mip, ns, h, pd, tr);
+            o[0] = h[0]; o[1] = pd[0]; o[2] = tr[0];
+            o[3] = ns > 1 ? h[1] : 0.f; o[4] = ns > 1 ? pd[1] : 0.f; o[5] = ns > 1 ? tr[1] : 0.f;
+            o[6] = ns > 2 ? h[2] : 0.f; o[7] = ns > 3 ? h[3] : 0.f;
+            break;
+        }
+        case 6: {
+            MediumInteraction mi{V3(0.f), V3(0.f), S.g, false};
+            sampleDecomposition(S, ray, rng, mi, mip);
+            o[0] = mi.valid ? 1.f : 0.f; o[1] = mi.p.x; o[2] = mi.p.y; o[3] = mi.p.z;
+            break;
+        }
+        case 7: {
+            float sx, sy, bound;
+            cellByDensity(S, ray, rng, mip, sx, sy, bound);
+            o[0] = sx; o[1] = sy; o[2] = bound;
+            o[3] = rejectionSampleDepth(S, ray, rng, mip);
+            break;
+        }
+        case 8: o[0] = featureMarch(S, ray, mip, linear != 0, tStepScale, rng); break;
+        case 9: o[0] = densityWorld(S, mip, ray.o); break;
+        default: break;
+    }
+    float* d = out + 8 * (size_t)i;
+#pragma unroll
+    for (int k = 0; k < 8; k++) d[k] = o[k];
+}
+
+// ------------------------------------------------------------------ launchers
+static dim3 tiles(int W, int H) { return dim3((W + 15) / 16, (H + 15) / 16, 1); }
+
+template <int B>
+static hipError_t launchStageB(const LaunchArgs& a) {
+    dim3 g = tiles(a.width, a.height), blk(256, 1, 1);
+    switch (a.stage) {
+        case 0: hipLaunchKernelGGL(kFeatures, g, blk, 0, a.stream, a.scene, a.frame, a.features); break;
+        case 1: hipLaunchKernelGGL(kTrace<B>, g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0, a.color0); break;
+        case 2:
+            hipLaunchKernelGGL(kTemporal<B>, g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0, a.resT, a.extraT,
+                               a.features, a.featuresT, a.mvec);
+            break;
+        case 3:
+            hipLaunchKernelGGL(kSpatial<B>, g, blk, 0, a.stream, a.scene, a.frame, a.round, a.offsets, a.res0, a.res1, a.extra0,
+                               a.extra1, a.features);
+            break;
+        case 4: hipLaunchKernelGGL(kCopy<B>, g, blk, 0, a.stream, a.frame, a.res0, a.extra0, a.resT, a.extraT); break;
+        case 5:
+            hipLaunchKernelGGL(kFinal<B>, g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0, a.features, a.color0,
+                               a.out);
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchStage(const LaunchArgs& a) {
+    switch (a.maxBounces) {
+        case 1: return launchStageB<1>(a);
+        case 2: return launchStageB<2>(a);
+        case 3: return launchStageB<3>(a);
+        case 4: return launchStageB<4>(a);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launchTrack(hipStream_t stream, const Scene* scene, int mode, int mip, int linear, float tStepScale,
+                       int numSamples, int n, const float* rays, const uint32_t* seeds, float* out) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kTrack, dim3((n + 255) / 256), dim3(256), 0, stream, scene, mode, mip, linear, tStepScale,
+                       numSamples, n, rays, seeds, out);
+    return hipGetLastError();
+}
+
+}  // namespace vr

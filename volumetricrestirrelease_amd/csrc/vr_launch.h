@@ -1,0 +1,36 @@
+// vr_launch.h -- host-side launch interface of the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "vr_types.h"
+
+namespace vr {
+
+struct ResStore;
+
+// Buffers of one stage launch. res0/extra0 are the stage's "current" (or input)
+// reservoirs, res1/extra1 the spatial-reuse output, resT/extraT the temporal history.
+struct LaunchArgs {
+    hipStream_t stream;
+    int stage, round, maxBounces, width, height;
+    const Scene* scene;
+    Frame frame;
+    const int* offsets;
+    ResStore* res0;
+    ResStore* res1;
+    ResStore* resT;
+    float* extra0;
+    float* extra1;
+    float* extraT;
+    float2* features;
+    float2* featuresT;
+    float4* color0;
+    float4* out;
+    float2* mvec;
+};
+
+hipError_t launchStage(const LaunchArgs& a);
+hipError_t launchTrack(hipStream_t stream, const Scene* scene, int mode, int mip, int linear, float tStepScale,
+                       int numSamples, int n, const float* rays, const uint32_t* seeds, float* out);
+
+}  // namespace vr

@@ -1,0 +1,100 @@
+// vr_types.h -- plain-old-data descriptors shared by the host library and the
+// HIP kernels. Everything the kernels read about the scene and the frame is in
+// these structs, uploaded once per change into device memory and passed to the
+// kernels as `const T* __restrict__` (uniform loads go through the scalar cache).
+#pragma once
+#include <stdint.h>
+
+namespace vr {
+
+static const uint32_t kEmpty = 0xFFFFFFFFu;
+static const int kMaxSlots = 16;    // mips 0..7, conservative 8..15 (VR/HostDeviceSharedConstants.slang:19-20)
+static const int kBrick = 8;        // voxels per brick side
+static const int kApron = 10;       // stored brick side (one-voxel apron on each face)
+static const int kBrickVals = 1000; // kApron^3 values per stored brick
+static const int kNodeBricks = 16;  // bricks per level-1 node side (128 voxels)
+static const int kMaxLights = 256;
+
+// One grid slot in HBM: a flat brick table over the grid's brick extent, an
+// occupancy table over 128^3-voxel blocks (the hierarchical-DDA level above
+// bricks), per-brick density bounds and the brick pool.
+struct Grid {
+    const uint32_t* brickTable;   // nbx*nby*nbz brick ids (x fastest), kEmpty = no brick
+    const uint32_t* nodeTable;    // nnx*nny*nnz, non-zero = block holds a brick
+    const float* bounds;          // 4 floats per brick: min, max, avg, 0 (raw values)
+    const void* pool;             // 1000 values per brick, fp32 or fp16, x fastest
+    int32_t nbx, nby, nbz;
+    int32_t nnx, nny, nnz;
+    float bmax[3];                // [0, bmax) index-space bounds (nx, ny, nz)
+    float w2i[16];                // world -> index, row vector
+    float maxValue;
+    int32_t fp16;
+    int32_t valid;
+    int32_t pad_;
+};
+
+struct Scene {
+    Grid grid[kMaxSlots];
+    float sigma_t;
+    float sigma_s[3];
+    float sigma_a[3];
+    float g;
+    float densityScale;
+    float tStep;
+    float worldScaling;
+    float superVoxelDiag;
+    int32_t numMips;
+    int32_t numLights;
+    float env[3];
+    float pad_;
+    float lights[kMaxLights * 6];  // pos.xyz, intensity.xyz
+};
+
+// SamplingOptions (VR/HostDeviceSharedDefinitions.h:100-138).
+struct Options {
+    uint32_t visibilityTrackingMethod, lightingTrackingMethod;
+    int32_t lightSamples, lightingMipLevel, visibilitySamples, visibilityMipLevel;
+    int32_t visibilityUseLinearSampler, lightingUseLinearSampler;
+    float visibilityTStepScale, lightingTStepScale;
+    int32_t useEnv, useAnalytic, useEmissive;
+    int32_t vertexReuseStartBounce;
+};
+
+struct CameraFrame {
+    float pos[3], U[3], V[3], W[3];
+};
+
+// Per-frame constants (VR/VolumetricReSTIR.cpp:453-496 and the kernels' cbuffers),
+// passed by value as a kernel argument.
+struct Frame {
+    int32_t width, height;
+    int32_t frameCount;
+    int32_t numTotalRounds;
+    int32_t maxBounces;
+    int32_t initialM;
+    int32_t useRussianRoulette;
+    int32_t noReuse;
+    int32_t coarserIndirect;
+    int32_t useReference;
+    int32_t baselineSpp;
+    int32_t visualizeTransmittance;
+    int32_t isFirstFrame;
+    int32_t temporalMIS;
+    int32_t reprojectionMode;
+    int32_t reprojectionMip;
+    float temporalThreshold;
+    int32_t spatialMIS;
+    int32_t spatialSampleCount;
+    int32_t enableTemporal;
+    int32_t enableSpatial;
+    int32_t spatialRounds;
+    int32_t outputMotionVec;
+    int32_t pad_;
+    CameraFrame cam, prevCam;
+    float prevView[16], prevProj[16];
+    Options initialOpt, spatialOpt, finalOpt;
+};
+// Neighbour offsets of spatial reuse: [16 frame seeds][kMaxSpatialSamples][2] ints.
+static const int kMaxSpatialSamples = 32;
+
+}  // namespace vr

@@ -1,0 +1,50 @@
+// vr_volume.h -- host-side volume preparation (see vr_volume.cpp).
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/vrestir.h"
+#include "vr_types.h"
+
+namespace vr {
+
+struct DenseF {
+    int nx = 0, ny = 0, nz = 0;
+    std::vector<float> v;
+    size_t idx(int x, int y, int z) const { return (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * (size_t)z); }
+    float at(int x, int y, int z) const {
+        if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) return 0.f;
+        return v[idx(x, y, z)];
+    }
+};
+
+struct HostGrid {
+    bool valid = false;
+    int nx = 0, ny = 0, nz = 0;
+    int nbx = 0, nby = 0, nbz = 0, nnx = 0, nny = 0, nnz = 0;
+    bool fp16 = false;
+    float maxValue = 0.f;
+    float w2i[16];
+    uint32_t numBricks = 0;
+    std::vector<uint32_t> brickTable, nodeTable;
+    std::vector<float> bounds;
+    std::vector<float> pool32;
+    std::vector<uint16_t> pool16;
+};
+
+struct HostVolume {
+    HostGrid slots[kMaxSlots];
+    int numMips = 0;
+    float tStep = 1.f;
+    float superVoxelDiag = 1.f;
+};
+
+uint16_t floatToHalfBits(float f);
+float halfBitsToFloat(uint16_t h);
+int buildVolume(const vrestir_volume_desc& d, HostVolume& out, std::string& err);
+int synthFbm(uint64_t seed, int nx, int ny, int nz, int octaves, float baseFreq, float lacunarity, float gain,
+             int sphereMask, float threshold, float amplitude, float* out);
+
+}  // namespace vr

@@ -1,0 +1,168 @@
+"""Python mirror of the reference render pass (VR/VolumetricReSTIR.{h,cpp}) over the C ABI.
+
+Names follow the reference: ``VolumetricReSTIRParams`` fields keep their
+``m``-prefixed names (VR/VolumetricReSTIR.cpp:1356-1426), ``execute`` renders
+one frame, ``getScriptingDictionary`` / ``updateDict`` serialise the options.
+Device memory and streams come from PyTorch; all work runs in libvrestir.so.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import abi
+
+
+class VolumetricReSTIRError(RuntimeError):
+    pass
+
+
+class VolumetricReSTIRParams:
+    """Option surface with the reference's field names (``mMaxBounces`` ...)."""
+
+    def __init__(self, **kwargs):
+        object.__setattr__(self, "_p", abi.default_params())
+        for k, v in kwargs.items():
+            setattr(self, k, v)
+
+    @staticmethod
+    def _cname(name: str) -> str:
+        if not name.startswith("m") or len(name) < 2:
+            raise AttributeError(name)
+        c = name[1].lower() + name[2:]
+        if c not in dict(abi.PARAM_FIELDS):
+            raise AttributeError(f"VolumetricReSTIRParams has no field {name}")
+        return c
+
+    def __getattr__(self, name):
+        return getattr(object.__getattribute__(self, "_p"), self._cname(name))
+
+    def __setattr__(self, name, value):
+        c = self._cname(name)
+        setattr(self._p, c, value)
+
+    def to_dict(self) -> dict:
+        return {"m" + k[0].upper() + k[1:]: getattr(self._p, k) for k, _ in abi.PARAM_FIELDS}
+
+    @property
+    def struct(self) -> abi.Params:
+        return self._p
+
+
+class VolumetricReSTIR:
+    """One instance of the pass, bound to the current HIP device."""
+
+    def __init__(self, params: VolumetricReSTIRParams | abi.Params | None = None):
+        self.lib = abi.load_library()
+        p = params.struct if isinstance(params, VolumetricReSTIRParams) else params
+        h = ctypes.c_void_p()
+        rc = self.lib.vrestir_create(ctypes.byref(p) if p is not None else None, ctypes.byref(h))
+        if rc != 0:
+            raise VolumetricReSTIRError(f"vrestir_create: {self.lib.vrestir_last_error(None).decode()} ({rc})")
+        self.h = h
+
+    @classmethod
+    def create(cls, params=None):
+        return cls(params)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.vrestir_last_error(self.h).decode()
+            raise VolumetricReSTIRError(f"{what}: {msg} (status {rc})")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.vrestir_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- scene (setScene, VR/VolumetricReSTIR.cpp:1218-1256)
+    def setScene(self, spec, frame: int = 0):
+        d = spec.volume_desc()
+        self._check(self.lib.vrestir_set_volume(self.h, ctypes.byref(d)), "set_volume")
+        arr, n, env = spec.light_array()
+        self._check(self.lib.vrestir_set_lights(self.h, arr, n, env), "set_lights")
+        self.setCamera(spec, frame)
+        self.setParams(spec.make_params())
+
+    def setCamera(self, spec, frame: int = 0):
+        c = spec.camera_struct(frame)
+        self._check(self.lib.vrestir_set_camera(self.h, ctypes.byref(c)), "set_camera")
+
+    def setParams(self, p):
+        st = p.struct if isinstance(p, VolumetricReSTIRParams) else p
+        self._check(self.lib.vrestir_set_params(self.h, ctypes.byref(st)), "set_params")
+
+    def getParams(self) -> abi.Params:
+        p = abi.Params()
+        self._check(self.lib.vrestir_get_params(self.h, ctypes.byref(p)), "get_params")
+        return p
+
+    # ---- serialisation (VR/VolumetricReSTIR.cpp:142-147, 1280-1347)
+    def getScriptingDictionary(self) -> dict:
+        p = self.getParams()
+        return {"mParams": {"m" + k[0].upper() + k[1:]: getattr(p, k) for k, _ in abi.PARAM_FIELDS}}
+
+    def updateDict(self, d: dict):
+        p = self.getParams()
+        for k, v in d.get("mParams", d).items():
+            setattr(p, VolumetricReSTIRParams._cname(k), v)
+        self.setParams(p)
+
+    # ---- per frame (execute, VR/VolumetricReSTIR.cpp:303-906)
+    def execute(self, out_rgba, out_mvec=None, stream=None):
+        """Render one frame into ``out_rgba`` (a CUDA/HIP float32 tensor [H, W, 4])."""
+        H, W = int(out_rgba.shape[0]), int(out_rgba.shape[1])
+        if out_rgba.dtype.itemsize != 4 or out_rgba.shape[-1] != 4 or not out_rgba.is_contiguous():
+            raise VolumetricReSTIRError("out_rgba must be a contiguous float32 [H, W, 4] device tensor")
+        s = ctypes.c_void_p(stream.cuda_stream if stream is not None else 0)
+        mv = ctypes.c_void_p(out_mvec.data_ptr()) if out_mvec is not None else None
+        self._check(self.lib.vrestir_execute(self.h, s, W, H, ctypes.c_void_p(out_rgba.data_ptr()), mv), "execute")
+
+    def frameCount(self) -> int:
+        v = ctypes.c_int32()
+        self._check(self.lib.vrestir_frame_count(self.h, ctypes.byref(v)), "frame_count")
+        return v.value
+
+    # ---- stage-level access for parity tests
+    def prepare_frame(self, W, H):
+        self._check(self.lib.vrestir_prepare_frame(self.h, W, H), "prepare_frame")
+
+    def run_stage(self, stage, rnd=0, out=None, stream=None):
+        s = ctypes.c_void_p(stream.cuda_stream if stream is not None else 0)
+        o = ctypes.c_void_p(out.data_ptr()) if out is not None else None
+        self._check(self.lib.vrestir_run_stage(self.h, s, stage, rnd, o), f"run_stage({stage})")
+
+    def end_frame(self):
+        self._check(self.lib.vrestir_end_frame(self.h), "end_frame")
+
+    def read(self, buf, nbytes) -> np.ndarray:
+        out = np.zeros(nbytes, dtype=np.uint8)
+        self._check(self.lib.vrestir_read_buffer(self.h, buf, ctypes.c_void_p(out.ctypes.data), nbytes), "read")
+        return out
+
+    def write(self, buf, data: np.ndarray):
+        data = np.ascontiguousarray(data)
+        self._check(self.lib.vrestir_write_buffer(self.h, buf, ctypes.c_void_p(data.ctypes.data), data.nbytes),
+                    "write")
+
+    def track(self, mode, mip, linear, tstep, nsamples, rays, seeds, out, stream=None):
+        """Batched tracking primitive over device tensors rays [n,8] f32, seeds [n,3] u32/i32, out [n,8] f32."""
+        n = int(rays.shape[0])
+        s = ctypes.c_void_p(stream.cuda_stream if stream is not None else 0)
+        self._check(self.lib.vrestir_track_batch(self.h, s, mode, mip, linear, tstep, nsamples, n,
+                                                 ctypes.c_void_p(rays.data_ptr()), ctypes.c_void_p(seeds.data_ptr()),
+                                                 ctypes.c_void_p(out.data_ptr())), "track_batch")
+
+
+# Reservoir / features exchange layouts (include/vrestir.h)
+RESERVOIR_DTYPE = np.dtype([("runningSum", "<f4"), ("M", "<f4"), ("depth", "<f4"), ("p_y", "<f4"),
+                            ("lightUV", "<f4", 2), ("lightID", "<i4"), ("sampledPixel", "<i4"),
+                            ("extraBounceStartId", "<i4")])
+FEATURES_DTYPE = np.dtype([("noReflectiveSurface", "<i4"), ("transmittance", "<f4")])
