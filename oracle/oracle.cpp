@@ -1038,6 +1038,29 @@ int orc_frame_count(void* h) { return ((Oracle*)h)->frameCount; }
 int orc_total_round_id(void* h) { return ((Oracle*)h)->totalRoundId; }
 void orc_stage_seconds(void* h, double out[6]) { std::memcpy(out, ((Oracle*)h)->stageSeconds, sizeof(double) * 6); }
 
+// Per-pixel traversal work of the final-shading stage on the current buffers
+// (diagnostics: out[2*i] = DDA steps, out[2*i+1] = cell steps).
+int orc_final_costs(void* h, long long* out) {
+    Oracle* O = (Oracle*)h;
+    int idx = O->totalRoundId % 2;
+    bool noReuse = !O->P.enableSpatialReuse && !O->P.enableTemporalReuse;
+    for (int y = 0; y < O->H; y++)
+        for (int x = 0; x < O->W; x++) {
+            int pid = y * O->W + x;
+            g_ddaSteps = 0; g_cellSteps = 0;
+            SampleGen sg = makeSampleGen(x, y, (uint32_t)(O->numTotalRounds * O->frameCount + O->numTotalRounds - 1));
+            Reservoir r = O->res[idx][pid];
+            if (r.runningSum > 0.f) {
+                f3 d = rayDir(O->S.cam, x, y, O->W, O->H);
+                Ray ray{O->S.cam.pos, d, 0, r.depth};
+                (void)evaluateF(*O, r, O->extra[idx].data(), ray, sg, O->finalOpt, noReuse, true);
+            }
+            out[2 * pid] = g_ddaSteps;
+            out[2 * pid + 1] = g_cellSteps;
+        }
+    return VRESTIR_OK;
+}
+
 // Batched tracking primitive: same contract as vrestir_track_batch, on host memory.
 int orc_track_batch(void* h, int mode, int mip, int linear, float tStepScale, int numSamples, int n, const float* rays,
                     const uint32_t* seeds, float* out) {

@@ -114,12 +114,20 @@ struct HDDA {
         mask.z = (int)((tSide.z < tSide.x) & (tSide.z <= tSide.y));
         t.y = mask.x ? tSide.x : (mask.y ? tSide.y : tSide.z);
     }
+    // The reference adds float(mask) * tDel; for an axis the ray never crosses
+    // (direction component exactly 0, tDel = inf) that is 0 * inf = NaN, which
+    // stalls the walk until the 4096-step cap. Adding tDel only on the stepped
+    // axis is the same arithmetic everywhere else and keeps such rays walking.
     void step() {
         t.x = t.y;
-        tSide = tSide + f3of(mask) * tDel;
+        tSide = {mask.x ? tSide.x + tDel.x : tSide.x, mask.y ? tSide.y + tDel.y : tSide.y,
+                 mask.z ? tSide.z + tDel.z : tSide.z};
         p = {p.x + mask.x * pStep.x, p.y + mask.y * pStep.y, p.z + mask.z * pStep.z};
     }
 };
+
+// Work counters (traversal steps, per-brick cell steps) for cost diagnostics.
+inline thread_local long long g_ddaSteps = 0, g_cellSteps = 0;
 
 // Tree constants for the 3-level layout: res, log2 res, child extent (voxels).
 static const int kRes[3] = {8, 16, 32};
@@ -156,6 +164,7 @@ void trackGVDB(const Medium& m, const Ray& rWorld, int slot, SampleGen& sg, A& a
         while (it++ < 3 && outside(lev)) { dda.next(); dda.step(); dda.t.x += eps; }
     }
     for (int iter = 0; iter < 4096 && lev > 0 && lev <= topLev && !outside(lev); iter++) {
+        g_ddaSteps++;
         dda.next();
         int b = (((dda.p.z << kDim[lev]) + dda.p.y) << kDim[lev]) + dda.p.x;
         uint32_t child = childOf(g, lev, nodeList[lev], b);
@@ -242,6 +251,7 @@ struct TrAnalytic {
         HDDA dl = dda;
         dl.prepareLeaf(vminLeaf);
         for (int iter = 0; iter < 128 && leafInside(dl); iter++) {
+            g_cellSteps++;
             dl.next();
             float maxDeltaT = dl.t.y - t;
             if (linear) {

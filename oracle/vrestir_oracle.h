@@ -36,6 +36,7 @@ void orc_neighbor_offsets(int sampleCount, float radius, int frameSeed, int samp
 int orc_grid_info(void* h, int slot, int* dims, float* maxValue, int* numBricks);
 int orc_brick_bounds(void* h, int slot, int x, int y, int z, float* out4);
 float orc_voxel(void* h, int slot, int x, int y, int z);
+int orc_final_costs(void* h, long long* out);
 #ifdef __cplusplus
 }
 #endif

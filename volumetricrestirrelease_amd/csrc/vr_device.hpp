@@ -221,9 +221,12 @@ struct Dda {
         mask.z = (int)((tSide.z < tSide.x) & (tSide.z <= tSide.y));
         ty = mask.x ? tSide.x : (mask.y ? tSide.y : tSide.z);
     }
+    // Adds tDel on the stepped axis only: float(mask) * tDel would be 0 * inf =
+    // NaN on an axis with a zero direction component and stall the walk.
     VR_DEV void step() {
         tx = ty;
-        tSide = tSide + tof(mask) * tDel;
+        tSide = {mask.x ? tSide.x + tDel.x : tSide.x, mask.y ? tSide.y + tDel.y : tSide.y,
+                 mask.z ? tSide.z + tDel.z : tSide.z};
         p = {p.x + mask.x * pStep.x, p.y + mask.y * pStep.y, p.z + mask.z * pStep.z};
     }
 };
@@ -642,8 +645,11 @@ VR_DEV float trResidual(const Scene& S, const Ray& r, int slot, Rng& rng, bool a
 VR_DEV void sampleAnalytic(const Scene& S, const Ray& r, Rng& rng, bool linear, int slot, int ns, float hit[4],
                            float pdf[4], float tr[4]) {
     SampleAnalytic a; a.S = &S; a.G = &S.grid[slot]; a.linear = linear; a.numSamples = ns;
-    if (linear)
-        for (int i = 0; i < ns; i++) a.tr[i] = -logf(1 - u01(rng));
+    if (linear) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (i < ns) a.tr[i] = -logf(1 - u01(rng));
+    }
     traverse(S, r, slot, rng, a, linear);
 #pragma unroll
     for (int i = 0; i < 4; i++) { hit[i] = a.hit[i]; pdf[i] = a.pdf[i]; tr[i] = a.tr[i]; }
@@ -664,16 +670,31 @@ VR_DEV float featureMarch(const Scene& S, const Ray& r, int slot, bool linear, f
     traverse(S, r, slot, rng, a, false);
     return a.accuT;
 }
-// computeVisibility (VR/VolumeUtils.slang:553-571).
-VR_DEV float visibility(const Scene& S, const Ray& r, Rng& rng, int samples, int slot, bool linear, uint32_t method,
-                        float tStepScale) {
+// computeVisibility (VR/VolumeUtils.slang:553-571). M fixes the tracking method
+// at compile time (0..4) so a kernel carries only the tracker it uses; M = -1
+// dispatches on `method` at run time.
+template <int M>
+VR_DEV float visibilityT(const Scene& S, const Ray& r, Rng& rng, int samples, int slot, bool linear, uint32_t method,
+                         float tStepScale) {
     float vis = 0.f;
     for (int i = 0; i < samples; i++) {
-        if (method == 0u || method == 3u || method == 4u) vis += trResidual(S, r, slot, rng, method == 4u, method == 0u);
-        else if (method == 2u) vis += trMarch(S, r, slot, linear, tStepScale, rng);
-        else if (method == 1u) vis += trAnalytic(S, r, slot, rng, linear);
+        if constexpr (M < 0) {
+            if (method == 0u || method == 3u || method == 4u) vis += trResidual(S, r, slot, rng, method == 4u, method == 0u);
+            else if (method == 2u) vis += trMarch(S, r, slot, linear, tStepScale, rng);
+            else if (method == 1u) vis += trAnalytic(S, r, slot, rng, linear);
+        } else if constexpr (M == 0 || M == 3 || M == 4) {
+            vis += trResidual(S, r, slot, rng, M == 4, M == 0);
+        } else if constexpr (M == 2) {
+            vis += trMarch(S, r, slot, linear, tStepScale, rng);
+        } else {
+            vis += trAnalytic(S, r, slot, rng, linear);
+        }
     }
     return vis / (float)samples;
+}
+VR_DEV float visibility(const Scene& S, const Ray& r, Rng& rng, int samples, int slot, bool linear, uint32_t method,
+                        float tStepScale) {
+    return visibilityT<-1>(S, r, rng, samples, slot, linear, method, tStepScale);
 }
 // RejectionSampleRandomPointByDensity (VR/VolumeUtils.slang:573-582).
 VR_DEV float rejectionSampleDepth(const Scene& S, const Ray& r, Rng& rng, int slot) {

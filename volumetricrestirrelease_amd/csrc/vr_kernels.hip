@@ -67,25 +67,34 @@ VR_DEV v3 pathTrace(const Scene& S, const Frame& Fr, Ray ray, Rng& rng, int ligh
     return L;
 }
 
-// TraceRays (VR/TraceRays.cs.slang:64-201).
+// TraceRays in reference mode (VR/TraceRays.cs.slang:85-103): the volume path tracer.
 template <int B>
-__global__ __launch_bounds__(256) void kTrace(const Scene* __restrict__ sc, const Frame fr,
-                                              ResStore* __restrict__ outRes, float* __restrict__ outExtra,
-                                              float4* __restrict__ outColor) {
+__global__ __launch_bounds__(256) void kReference(const Scene* __restrict__ sc, const Frame fr,
+                                                  float4* __restrict__ outColor) {
     const Scene& S = *sc;
     const Frame& F = fr;
     PixelIdx px = pixelOf(F);
     if (!px.ok) return;
     Rng rng = rngCreate(px.x, px.y, (uint32_t)(F.numTotalRounds * F.frameCount));
     Ray ray{ld3(F.cam.pos), rayDir(F.cam, px.x, px.y, F.width, F.height), 0.f, kTMax};
-    if (F.useReference) {
-        v3 avg = V3(0.f);
-        for (int s = 0; s < F.baselineSpp; s++)
-            avg = avg + pathTrace(S, F, ray, rng, max(1, F.initialOpt.lightSamples), B);
-        avg = avg / (float)F.baselineSpp;
-        outColor[px.id] = make_float4(avg.x, avg.y, avg.z, 1.f);
-        return;
-    }
+    v3 avg = V3(0.f);
+    for (int s = 0; s < F.baselineSpp; s++)
+        avg = avg + pathTrace(S, F, ray, rng, max(1, F.initialOpt.lightSamples), B);
+    avg = avg / (float)F.baselineSpp;
+    outColor[px.id] = make_float4(avg.x, avg.y, avg.z, 1.f);
+}
+
+// TraceRays (VR/TraceRays.cs.slang:64-201). IM: initial lighting tracking
+// method; VM / LM: the spatial options' methods used by the final p-hat.
+template <int B, int IM, int VM, int LM>
+__global__ __launch_bounds__(256) void kTrace(const Scene* __restrict__ sc, const Frame fr,
+                                              ResStore* __restrict__ outRes, float* __restrict__ outExtra) {
+    const Scene& S = *sc;
+    const Frame& F = fr;
+    PixelIdx px = pixelOf(F);
+    if (!px.ok) return;
+    Rng rng = rngCreate(px.x, px.y, (uint32_t)(F.numTotalRounds * F.frameCount));
+    Ray ray{ld3(F.cam.pos), rayDir(F.cam, px.x, px.y, F.width, F.height), 0.f, kTMax};
     constexpr int NE = B > 1 ? B - 1 : 1;
     v3 finalEx[NE];
 #pragma unroll
@@ -106,7 +115,7 @@ __global__ __launch_bounds__(256) void kTrace(const Scene* __restrict__ sc, cons
             float hs = s == 0 ? h[0] : (s == 1 ? h[1] : (s == 2 ? h[2] : h[3]));
             float ps = s == 0 ? pd[0] : (s == 1 ? pd[1] : (s == 2 ? pd[2] : pd[3]));
             float ts = s == 0 ? tr[0] : (s == 1 ? tr[1] : (s == 2 ? tr[2] : tr[3]));
-            Res o = computeInitialSample<B>(S, F, ray, hs, ps, ts, rng, F.initialOpt, ex);
+            Res o = computeInitialSample<B, IM>(S, F, ray, hs, ps, ts, rng, F.initialOpt, ex);
             bool sel = resample(o, fin, rng);
             if (sel && B > 1) {
                 int mi = decodeMaxIndirect(fin.sampledPixel);
@@ -129,7 +138,7 @@ __global__ __launch_bounds__(256) void kTrace(const Scene* __restrict__ sc, cons
             }
     }
     fin.exStart = exStart;
-    float ph = lum(evaluateF<B>(S, fin, outExtra, ray, rng, F.spatialOpt, false, false));
+    float ph = lum(evaluateF<B, VM, LM>(S, fin, outExtra, ray, rng, F.spatialOpt, false, false));
     if (fin.runningSum > 0.f) {
         fin.runningSum *= fin.p_y == 0.f ? 0.f : ph / fin.p_y;
         fin.p_y = ph;
@@ -144,7 +153,7 @@ VR_DEV float2 featuresAt(const float2* f, int id, int n) {
 VR_DEV bool isBackground(float2 f) { return f.y == 1.f && __float_as_int(f.x) != 0; }
 
 // TemporalReuse (VR/TemporalReuse.cs.slang:80-377).
-template <int B>
+template <int B, int VM, int LM>
 __global__ __launch_bounds__(256) void kTemporal(const Scene* __restrict__ sc, const Frame fr,
                                                  ResStore* __restrict__ cur, float* __restrict__ curExtra,
                                                  const ResStore* __restrict__ temporal,
@@ -239,7 +248,7 @@ __global__ __launch_bounds__(256) void kTemporal(const Scene* __restrict__ sc, c
             nPy = ti.p_y;
             if (isnan(ti.runningSum) || isinf(ti.runningSum)) ti.runningSum = 0.f;
             if (i > 0 && ti.runningSum != 0.f) {
-                float ph = lum(evaluateF<B>(S, ti, temporalExtra, ray, rng, F.spatialOpt, false, false));
+                float ph = lum(evaluateF<B, VM, LM>(S, ti, temporalExtra, ray, rng, F.spatialOpt, false, false));
                 float w = ph / ti.p_y;
                 if (isinf(w) || isnan(w)) w = 0.f;
                 ti.runningSum *= w;
@@ -266,7 +275,7 @@ __global__ __launch_bounds__(256) void kTemporal(const Scene* __restrict__ sc, c
                     Ray nr{ld3(F.prevCam.pos), norm3(rayDirNN(F.prevCam, rsx, rsy, W, H)), 0.f, centerPrevDepth};
                     Res tmp = ti;
                     tmp.depth = centerPrevDepth;
-                    float py = lum(evaluateF<B>(S, tmp, curExtra, nr, rng, F.spatialOpt, false, false));
+                    float py = lum(evaluateF<B, VM, LM>(S, tmp, curExtra, nr, rng, F.spatialOpt, false, false));
                     if (isinf(py) || isnan(py)) py = 0.f;
                     psum += py * cM;
                 }
@@ -294,7 +303,7 @@ __global__ __launch_bounds__(256) void kTemporal(const Scene* __restrict__ sc, c
 }
 
 // SpatialReuse (VR/SpatialReuse.cs.slang:94-265) for one round.
-template <int B>
+template <int B, int VM, int LM>
 __global__ __launch_bounds__(256) void kSpatial(const Scene* __restrict__ sc, const Frame fr,
                                                 int roundId, const int* __restrict__ offsets,
                                                 const ResStore* __restrict__ in,
@@ -339,7 +348,7 @@ __global__ __launch_bounds__(256) void kSpatial(const Scene* __restrict__ sc, co
         Res tap = loadRes(in, ty * W + tx, B);
         float misW = 1.f;
         if (s > 0 && tap.runningSum != 0.f) {
-            float ph = lum(evaluateF<B>(S, tap, inExtra, ray, rng, F.spatialOpt, false, false));
+            float ph = lum(evaluateF<B, VM, LM>(S, tap, inExtra, ray, rng, F.spatialOpt, false, false));
             float w = ph / tap.p_y;
             if (isinf(w) || isnan(w)) w = 0.f;
             tap.runningSum *= w;
@@ -362,7 +371,7 @@ __global__ __launch_bounds__(256) void kSpatial(const Scene* __restrict__ sc, co
                     psum += py2 * m2;
                 } else {
                     Ray nr{ray.o, rayDir(F.cam, t2x, t2y, W, H), 0.f, tap.depth};
-                    float py = lum(evaluateF<B>(S, tap, inExtra, nr, rng, F.spatialOpt, false, false));
+                    float py = lum(evaluateF<B, VM, LM>(S, tap, inExtra, nr, rng, F.spatialOpt, false, false));
                     if (isinf(py) || isnan(py)) py = 0.f;
                     psum += py * m2;
                 }
@@ -408,7 +417,7 @@ __global__ __launch_bounds__(256) void kCopy(const Frame fr, const ResStore* __r
 }
 
 // FinalShading (VR/FinalShading.cs.slang:71-141).
-template <int B>
+template <int B, int VM, int LM>
 __global__ __launch_bounds__(256) void kFinal(const Scene* __restrict__ sc, const Frame fr,
                                               const ResStore* __restrict__ cur, const float* __restrict__ curExtra,
                                               const float2* __restrict__ features, const float4* __restrict__ colorIn,
@@ -428,7 +437,7 @@ __global__ __launch_bounds__(256) void kFinal(const Scene* __restrict__ sc, cons
         Res r = loadRes(cur, px.id, B);
         if (r.runningSum > 0.f) {
             Ray ray{ld3(F.cam.pos), rayDir(F.cam, px.x, px.y, F.width, F.height), 0.f, r.depth};
-            v3 f = evaluateF<B>(S, r, curExtra, ray, rng, F.finalOpt, F.noReuse != 0, true);
+            v3 f = evaluateF<B, VM, LM>(S, r, curExtra, ray, rng, F.finalOpt, F.noReuse != 0, true);
             float Wt = r.p_y == 0.0f ? 1.f : r.runningSum / (r.p_y * r.M);
             c = c + f * Wt;
         }
@@ -490,24 +499,52 @@ __global__ __launch_bounds__(256) void kTrack(const Scene* __restrict__ sc, int 
 // ------------------------------------------------------------------ launchers
 static dim3 tiles(int W, int H) { return dim3((W + 15) / 16, (H + 15) / 16, 1); }
 
+// Each reuse kernel is compiled for the tracking methods of the reference's
+// default options (ray marching for the spatial/temporal options and initial
+// lighting, analytic tracking for final shading) and once with run-time
+// dispatch for any other option combination.
 template <int B>
 static hipError_t launchStageB(const LaunchArgs& a) {
     dim3 g = tiles(a.width, a.height), blk(256, 1, 1);
+    const Options& so = a.frame.spatialOpt;
+    const Options& fo = a.frame.finalOpt;
+    const bool spMarch = so.visibilityTrackingMethod == 2u && so.lightingTrackingMethod == 2u;
+    const bool inMarch = a.frame.initialOpt.lightingTrackingMethod == 2u;
+    const bool fiAnalytic = fo.visibilityTrackingMethod == 1u && fo.lightingTrackingMethod == 1u;
     switch (a.stage) {
         case 0: hipLaunchKernelGGL(kFeatures, g, blk, 0, a.stream, a.scene, a.frame, a.features); break;
-        case 1: hipLaunchKernelGGL(kTrace<B>, g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0, a.color0); break;
+        case 1:
+            if (a.frame.useReference)
+                hipLaunchKernelGGL(kReference<B>, g, blk, 0, a.stream, a.scene, a.frame, a.color0);
+            else if (spMarch && inMarch)
+                hipLaunchKernelGGL((kTrace<B, 2, 2, 2>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0);
+            else
+                hipLaunchKernelGGL((kTrace<B, -1, -1, -1>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0);
+            break;
         case 2:
-            hipLaunchKernelGGL(kTemporal<B>, g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0, a.resT, a.extraT,
-                               a.features, a.featuresT, a.mvec);
+            if (spMarch)
+                hipLaunchKernelGGL((kTemporal<B, 2, 2>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0, a.resT,
+                                   a.extraT, a.features, a.featuresT, a.mvec);
+            else
+                hipLaunchKernelGGL((kTemporal<B, -1, -1>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0,
+                                   a.resT, a.extraT, a.features, a.featuresT, a.mvec);
             break;
         case 3:
-            hipLaunchKernelGGL(kSpatial<B>, g, blk, 0, a.stream, a.scene, a.frame, a.round, a.offsets, a.res0, a.res1, a.extra0,
-                               a.extra1, a.features);
+            if (spMarch)
+                hipLaunchKernelGGL((kSpatial<B, 2, 2>), g, blk, 0, a.stream, a.scene, a.frame, a.round, a.offsets, a.res0,
+                                   a.res1, a.extra0, a.extra1, a.features);
+            else
+                hipLaunchKernelGGL((kSpatial<B, -1, -1>), g, blk, 0, a.stream, a.scene, a.frame, a.round, a.offsets,
+                                   a.res0, a.res1, a.extra0, a.extra1, a.features);
             break;
         case 4: hipLaunchKernelGGL(kCopy<B>, g, blk, 0, a.stream, a.frame, a.res0, a.extra0, a.resT, a.extraT); break;
         case 5:
-            hipLaunchKernelGGL(kFinal<B>, g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0, a.features, a.color0,
-                               a.out);
+            if (fiAnalytic)
+                hipLaunchKernelGGL((kFinal<B, 1, 1>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0, a.features,
+                                   a.color0, a.out);
+            else
+                hipLaunchKernelGGL((kFinal<B, -1, -1>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0,
+                                   a.features, a.color0, a.out);
             break;
         default: return hipErrorInvalidValue;
     }

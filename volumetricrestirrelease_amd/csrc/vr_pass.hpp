@@ -158,6 +158,7 @@ VR_DEV bool sampleLights(const Scene& S, v3 origin, bool useEnv, bool useAnalyti
 }
 
 // SampleDirectLighting (VR/VolumeUtils.slang:454-492).
+template <int LM>
 VR_DEV v3 sampleDirect(const Scene& S, Rng& rng, float& pdf, const MediumInteraction& mi, const Options& o,
                        bool enableShadow, int& lightIndex, float& uvx, float& uvy) {
     pdf = 0.f;
@@ -168,8 +169,8 @@ VR_DEV v3 sampleDirect(const Scene& S, Rng& rng, float& pdf, const MediumInterac
     if (!valid) { pdf = 0.f; return V3(0.f); }
     Ray sh{mi.p, ls.rayDir, 0.f, ls.rayDistance};
     if (enableShadow) {
-        float vis = visibility(S, sh, rng, o.lightSamples, o.lightingMipLevel, o.lightingUseLinearSampler != 0,
-                               o.lightingTrackingMethod, o.lightingTStepScale);
+        float vis = visibilityT<LM>(S, sh, rng, o.lightSamples, o.lightingMipLevel, o.lightingUseLinearSampler != 0,
+                                    o.lightingTrackingMethod, o.lightingTStepScale);
         ls.Li = ls.Li * vis;
     }
     float p = phaseHG(dot3(mi.wo, ls.dir), mi.g);
@@ -186,7 +187,7 @@ VR_DEV v3 directLightingPT(const Scene& S, Rng& rng, const MediumInteraction& mi
         bool valid = sampleLights(S, mi.p, o.useEnv, o.useAnalytic, o.useEmissive, rng, ls, idx, ux, uy);
         if (!valid) continue;
         Ray sh{mi.p, ls.rayDir, 0.f, ls.rayDistance};
-        float vis = visibility(S, sh, rng, 1, mip, true, method, 1.f);
+        float vis = visibilityT<3>(S, sh, rng, 1, mip, true, method, 1.f);
         ls.Li = ls.Li * vis;
         float p = phaseHG(dot3(mi.wo, ls.dir), mi.g);
         Ld = Ld + V3(p) * ls.Li / 1.f;
@@ -197,7 +198,8 @@ VR_DEV v3 directLightingPT(const Scene& S, Rng& rng, const MediumInteraction& mi
 // ------------------------------------------------------------------ target function
 // evaluate_F_ (VR/ReSTIRHelper.slang:91-423) + evaluate_L_in_volume (:443-496).
 // `extra` points at the extra-bounce records (3 floats each) the tap indexes.
-template <int B>
+// VM / LM: visibility and lighting tracking methods (-1 = chosen at run time).
+template <int B, int VM, int LM>
 VR_DEV v3 evaluateF(const Scene& S, Res tap, const float* extra, Ray ray, Rng& rng, const Options& opt, bool noReuse,
                     bool isFinal) {
     bool isBg = tap.depth == kTMax;
@@ -212,8 +214,9 @@ VR_DEV v3 evaluateF(const Scene& S, Res tap, const float* extra, Ray ray, Rng& r
         float density = (isBg || noReuse) ? 1.f : densityWorld(S, 0, pW);
         if (density == 0.f) return V3(0.f);
         if (!noReuse)
-            vis = visibility(S, ray, rng, opt.visibilitySamples, opt.visibilityMipLevel,
-                             opt.visibilityUseLinearSampler != 0, opt.visibilityTrackingMethod, opt.visibilityTStepScale);
+            vis = visibilityT<VM>(S, ray, rng, opt.visibilitySamples, opt.visibilityMipLevel,
+                                  opt.visibilityUseLinearSampler != 0, opt.visibilityTrackingMethod,
+                                  opt.visibilityTStepScale);
         v3 ss = isBg ? V3(1.f) : (isSelfEmission ? ld3(S.sigma_a) : ld3(S.sigma_s));
         if (noReuse && !isBg) ss = ss / S.sigma_t;
         F = F * (V3(vis * density) * ss);
@@ -264,7 +267,7 @@ VR_DEV v3 evaluateF(const Scene& S, Res tap, const float* extra, Ray ray, Rng& r
                     if (allZero(F)) return V3(0.f);
                     float sv = 1.f;
                     if (!noReuse)
-                        sv = visibility(S, sr, rng, opt.visibilitySamples, opt.visibilityMipLevel,
+                        sv = visibilityT<VM>(S, sr, rng, opt.visibilitySamples, opt.visibilityMipLevel,
                                         opt.visibilityUseLinearSampler != 0, opt.visibilityTrackingMethod,
                                         opt.visibilityTStepScale);
                     F = F * sv;
@@ -294,7 +297,7 @@ VR_DEV v3 evaluateF(const Scene& S, Res tap, const float* extra, Ray ray, Rng& r
                 } else {
                     return V3(0.f);
                 }
-                float Tr = visibility(S, sh, rng, opt.lightSamples, isFinal ? 0 : opt.lightingMipLevel,
+                float Tr = visibilityT<LM>(S, sh, rng, opt.lightSamples, isFinal ? 0 : opt.lightingMipLevel,
                                       opt.lightingUseLinearSampler != 0, opt.lightingTrackingMethod,
                                       opt.lightingTStepScale);
                 F = F * (Ld * Tr);
@@ -305,8 +308,8 @@ VR_DEV v3 evaluateF(const Scene& S, Res tap, const float* extra, Ray ray, Rng& r
 }
 
 // ComputeInitialSample (VR/ComputeInitialSample.slang:4-395). ex[] receives the
-// candidate's extra-bounce records.
-template <int B>
+// candidate's extra-bounce records; IM is the initial lighting tracking method.
+template <int B, int IM>
 VR_DEV Res computeInitialSample(const Scene& S, const Frame& Fr, Ray ray, float preHit, float prePdf, float preTr,
                                 Rng& rng, const Options& opt, v3 ex[B > 1 ? B - 1 : 1]) {
     float pathPdf = 1.f, pathPHat = 1.f;
@@ -348,7 +351,12 @@ VR_DEV Res computeInitialSample(const Scene& S, const Frame& Fr, Ray ray, float 
         } else {
             out.depth = primaryDepth;
             out.sampledPixel = encodeMaxIndirect(out.sampledPixel, bounce);
-            if (B > 1) ex[bounce - 1] = encodeWiDist(ray.d, !mi.valid ? kTMax : curHit);
+            if (B > 1) {
+                v3 e = encodeWiDist(ray.d, !mi.valid ? kTMax : curHit);
+#pragma unroll
+                for (int k = 0; k < (B > 1 ? B - 1 : 1); k++)
+                    if (k == bounce - 1) ex[k] = e;
+            }
             out.p_y = pathPdf;
         }
         if (mi.valid) actualDensity = noReuse ? 1.f : densityWorld(S, 0, mi.p);
@@ -367,7 +375,7 @@ VR_DEV Res computeInitialSample(const Scene& S, const Frame& Fr, Ray ray, float 
                 v3 Le = V3(0.f);
                 v3 oma = V3(1.f) - albedo;
                 bool shadow = opt.lightSamples == 0 ? false : true;
-                v3 Ld = sampleDirect(S, rng, outLightPdf, mi, opt, shadow, out.lightID, out.uvx, out.uvy);
+                v3 Ld = sampleDirect<IM>(S, rng, outLightPdf, mi, opt, shadow, out.lightID, out.uvx, out.uvy);
                 v3 wi = V3(0.f);
                 if (B > 1) {
                     float u0 = u01(rng);
