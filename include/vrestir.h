@@ -217,7 +217,8 @@ enum {
     VRESTIR_TRACK_SAMPLE_DECOMPOSITION = 6,/* delta tracking against per-brick bounds */
     VRESTIR_TRACK_SAMPLE_CELL_BY_DENSITY = 7,
     VRESTIR_TRACK_FEATURE_MARCH = 8,
-    VRESTIR_TRACK_DENSITY_AT_POINT = 9
+    VRESTIR_TRACK_DENSITY_AT_POINT = 9,
+    VRESTIR_TRACK_RNG_STREAM = 10          /* first 8 uint32 of the sample generator (bit-cast to float) */
 };
 int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t mode, int32_t mipLevel,
                             int32_t useLinearSampler, float tStepScale, int32_t numSamples, int32_t n,

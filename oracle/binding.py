@@ -55,6 +55,9 @@ def lib():
         "orc_grid_info": (c_int, [c_void_p, c_int, POINTER(c_int), POINTER(c_float), POINTER(c_int)]),
         "orc_brick_bounds": (c_int, [c_void_p, c_int, c_int, c_int, c_int, POINTER(c_float)]),
         "orc_voxel": (c_float, [c_void_p, c_int, c_int, c_int, c_int]),
+        "orc_final_costs": (c_int, [c_void_p, c_void_p]),
+        "orc_sample_phase": (c_float, [c_float, c_void_p, c_float, c_float, c_void_p]),
+        "orc_phase_hg": (c_float, [c_float, c_float]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

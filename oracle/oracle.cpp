@@ -1104,6 +1104,9 @@ int orc_track_batch(void* h, int mode, int mip, int linear, float tStepScale, in
             }
             case VRESTIR_TRACK_FEATURE_MARCH: o[0] = featureMarch(m, ray, mip, linear != 0, tStepScale, sg); break;
             case VRESTIR_TRACK_DENSITY_AT_POINT: o[0] = densityWorld(m, ray.origin, mip); break;
+            case VRESTIR_TRACK_RNG_STREAM:
+                for (int k = 0; k < 8; k++) { uint32_t u = xoshiroNext(sg); std::memcpy(&o[k], &u, 4); }
+                break;
             default: return VRESTIR_EINVAL;
         }
     }
@@ -1125,6 +1128,15 @@ void orc_xoshiro128ss(const uint32_t seed[4], int n, uint32_t* out) {
     for (int i = 0; i < n; i++) out[i] = xoshiroNext(g);
 }
 float orc_round_to_half(float v) { return roundToHalf(v); }
+
+// Henyey-Greenstein sampling (MediumInteraction::Sample_p): wi and the returned pdf.
+float orc_sample_phase(float g, const float wo[3], float u0, float u1, float wi[3]) {
+    f3 w{0, 0, 0};
+    float pdf = samplePhase(g, {wo[0], wo[1], wo[2]}, w, {u0, u1});
+    wi[0] = w.x; wi[1] = w.y; wi[2] = w.z;
+    return pdf;
+}
+float orc_phase_hg(float cosTheta, float g) { return phaseHG(cosTheta, g); }
 
 // Neighbour offsets of one spatial round.
 void orc_neighbor_offsets(int sampleCount, float radius, int frameSeed, int sampler, int* out) {

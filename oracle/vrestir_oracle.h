@@ -37,6 +37,8 @@ int orc_grid_info(void* h, int slot, int* dims, float* maxValue, int* numBricks)
 int orc_brick_bounds(void* h, int slot, int x, int y, int z, float* out4);
 float orc_voxel(void* h, int slot, int x, int y, int z);
 int orc_final_costs(void* h, long long* out);
+float orc_sample_phase(float g, const float wo[3], float u0, float u1, float wi[3]);
+float orc_phase_hg(float cosTheta, float g);
 #ifdef __cplusplus
 }
 #endif

@@ -35,7 +35,7 @@ BUF_FEATURES, BUF_TEMPORAL_FEATURES, BUF_RESERVOIRS0, BUF_RESERVOIRS1, BUF_TEMPO
 
 TRACK_TR_ANALYTIC, TRACK_TR_RAY_MARCHING, TRACK_TR_RATIO, TRACK_TR_RESIDUAL_RATIO, \
     TRACK_TR_ANALOG_RESIDUAL_RATIO, TRACK_SAMPLE_ANALYTIC, TRACK_SAMPLE_DECOMPOSITION, \
-    TRACK_SAMPLE_CELL_BY_DENSITY, TRACK_FEATURE_MARCH, TRACK_DENSITY_AT_POINT = range(10)
+    TRACK_SAMPLE_CELL_BY_DENSITY, TRACK_FEATURE_MARCH, TRACK_DENSITY_AT_POINT, TRACK_RNG_STREAM = range(11)
 
 # (name, ctype) in declaration order of vrestir_params
 PARAM_FIELDS = [

@@ -692,7 +692,7 @@ extern "C" int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t 
                                        const float* rays, const uint32_t* seeds, float* out) {
     if (!h) return VRESTIR_EINVAL;
     if (!h->volumeSet) return fail(h, VRESTIR_ESTATE, "no volume bound");
-    if (mode < 0 || mode > VRESTIR_TRACK_DENSITY_AT_POINT) return fail(h, VRESTIR_EINVAL, "unknown tracking mode");
+    if (mode < 0 || mode > VRESTIR_TRACK_RNG_STREAM) return fail(h, VRESTIR_EINVAL, "unknown tracking mode");
     if (mip < 0 || mip >= kMaxSlots || !h->meta.slots[mip].valid) return fail(h, VRESTIR_EINVAL, "grid slot not bound");
     if (n < 0 || (n > 0 && (!rays || !seeds || !out))) return fail(h, VRESTIR_EINVAL, "bad ray batch");
     HIPCHK(h, hipSetDevice(h->device));

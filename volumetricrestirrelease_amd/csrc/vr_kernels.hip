@@ -489,6 +489,10 @@ __global__ __launch_bounds__(256) void kTrack(const Scene* __restrict__ sc, int 
         }
         case 8: o[0] = featureMarch(S, ray, mip, linear != 0, tStepScale, rng); break;
         case 9: o[0] = densityWorld(S, mip, ray.o); break;
+        case 10:
+#pragma unroll
+            for (int k = 0; k < 8; k++) o[k] = __uint_as_float(rngNext(rng));
+            break;
         default: break;
     }
     float* d = out + 8 * (size_t)i;
