@@ -106,6 +106,13 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         return _lib
     if not os.path.exists(path):
         raise LibraryMissing(f"{path} not built; run __graft_entry__.build() (make -C volumetricrestirrelease_amd/csrc)")
+    # libvrestir.so binds to whichever libamdhip64.so.7 the process has loaded. PyTorch
+    # ships its own copy; load torch first so the library and torch share one HIP runtime
+    # (loading the system runtime first leaves torch unable to initialise its devices).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     vp, i32, f32, u32 = c_void_p, c_int32, c_float, c_uint32
     sig = {
