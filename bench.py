@@ -62,6 +62,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=64)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--calibration-copy-mb", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB first (a known-size kernel that "
+                         "calibrates FETCH_SIZE / WRITE_SIZE counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -74,6 +77,13 @@ def main():
     from volumetricrestirrelease_amd.volumetric_restir import VolumetricReSTIR
 
     torch.cuda.set_device(0)
+    if args.calibration_copy_mb > 0:
+        n = args.calibration_copy_mb * (1 << 20) // 4
+        src = torch.ones(n, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     t0 = time.perf_counter()
     spec = scenes.config1() if args.config == "config1" else scenes.config0()
     if args.config == "config0":

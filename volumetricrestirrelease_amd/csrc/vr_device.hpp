@@ -85,24 +85,55 @@ VR_DEV uint32_t rngNext(Rng& g) {
 VR_DEV float u01(Rng& g) { return (float)(rngNext(g) >> 8) * 5.9604644775390625e-08f; }
 
 // ------------------------------------------------------------------ volume access
-VR_DEV float poolValue(const Grid& G, uint32_t brick, int idx) {
-    if (G.fp16) return (float)((const _Float16*)G.pool)[(size_t)brick * kBrickVals + idx];
-    return ((const float*)G.pool)[(size_t)brick * kBrickVals + idx];
-}
+// Pointers into the grid buffers are read from the Scene struct in device
+// memory, so the compiler only sees generic pointers; casting them to the
+// global address space turns flat loads (which also count against the LDS
+// counter and force joint waits) into global loads.
+template <class T> using gptr = const __attribute__((address_space(1))) T*;
+template <class T> VR_DEV gptr<T> gbl(const void* p) { return (gptr<T>)p; }
+
 // Brick id of brick cell (bx, by, bz), kEmpty outside the table.
 VR_DEV uint32_t brickAt(const Grid& G, int bx, int by, int bz) {
     if ((unsigned)bx >= (unsigned)G.nbx || (unsigned)by >= (unsigned)G.nby || (unsigned)bz >= (unsigned)G.nbz)
         return kEmpty;
-    return G.brickTable[bx + G.nbx * (by + G.nby * bz)];
+    return gbl<uint32_t>(G.brickTable)[bx + G.nbx * (by + G.nby * bz)];
 }
 VR_DEV bool nodeAt(const Grid& G, int nx, int ny, int nz) {
     if ((unsigned)nx >= (unsigned)G.nnx || (unsigned)ny >= (unsigned)G.nny || (unsigned)nz >= (unsigned)G.nnz)
         return false;
-    return G.nodeTable[nx + G.nnx * (ny + G.nny * nz)] != 0u;
+    return gbl<uint32_t>(G.nodeTable)[nx + G.nnx * (ny + G.nny * nz)] != 0u;
 }
-// Stored value at apron coordinate (i,j,k) in [0,10)^3 of a brick.
+// Stored value at apron index idx in [0,1000) of a brick. The format test is
+// uniform (one grid per call), so it costs a scalar branch.
+VR_DEV float apronAt(const Grid& G, uint32_t brick, int idx) {
+    const size_t o = (size_t)brick * kBrickVals + idx;
+    if (G.fp16) return (float)gbl<_Float16>(G.pool)[o];
+    return gbl<float>(G.pool)[o];
+}
 VR_DEV float apronVal(const Grid& G, uint32_t brick, int i, int j, int k) {
-    return poolValue(G, brick, i + kApron * (j + kApron * k));
+    return apronAt(G, brick, i + kApron * (j + kApron * k));
+}
+// The 2x2x2 footprint whose lowest corner is apron index `base`, in the
+// order v000 v100 v010 v110 v001 v101 v011 v111. One format branch for all
+// eight loads, so they issue back to back (adjacent x pairs merge into one
+// 32-bit load for fp16, 64-bit for fp32).
+VR_DEV void footprint(const Grid& G, uint32_t brick, int base, float v[8]) {
+    const size_t o = (size_t)brick * kBrickVals + base;
+    if (G.fp16) {
+        gptr<_Float16> q = gbl<_Float16>(G.pool) + o;
+        v[0] = (float)q[0]; v[1] = (float)q[1]; v[2] = (float)q[10]; v[3] = (float)q[11];
+        v[4] = (float)q[100]; v[5] = (float)q[101]; v[6] = (float)q[110]; v[7] = (float)q[111];
+    } else {
+        gptr<float> q = gbl<float>(G.pool) + o;
+        v[0] = q[0]; v[1] = q[1]; v[2] = q[10]; v[3] = q[11];
+        v[4] = q[100]; v[5] = q[101]; v[6] = q[110]; v[7] = q[111];
+    }
+}
+VR_DEV float trilerp(const float v[8], v3 f) {
+    float a0 = v[0] + (v[1] - v[0]) * f.x, a1 = v[2] + (v[3] - v[2]) * f.x;
+    float a2 = v[4] + (v[5] - v[4]) * f.x, a3 = v[6] + (v[7] - v[6]) * f.x;
+    float b0 = a0 + (a1 - a0) * f.y, b1 = a2 + (a3 - a2) * f.y;
+    return b0 + (b1 - b0) * f.z;
 }
 // Texture-style sampling of a brick at brick-local continuous coordinate q
 // (q = p - brickMin, p in index space): point = voxel floor(q); linear =
@@ -117,29 +148,19 @@ VR_DEV float sampleBrick(const Grid& G, uint32_t brick, v3 q, bool linear) {
     v3 i0 = floor3(x);
     v3 f = x - i0;
     int ix = (int)i0.x, iy = (int)i0.y, iz = (int)i0.z;
+    float v[8];
     if ((unsigned)ix >= 9u || (unsigned)iy >= 9u || (unsigned)iz >= 9u) {
         // only reachable for q outside [0,8]: sample with zero border
-        auto tx = [&](int a, int b, int c) -> float {
-            if ((unsigned)a >= 10u || (unsigned)b >= 10u || (unsigned)c >= 10u) return 0.f;
-            return apronVal(G, brick, a, b, c);
-        };
-        float v000 = tx(ix, iy, iz), v100 = tx(ix + 1, iy, iz), v010 = tx(ix, iy + 1, iz), v110 = tx(ix + 1, iy + 1, iz);
-        float v001 = tx(ix, iy, iz + 1), v101 = tx(ix + 1, iy, iz + 1), v011 = tx(ix, iy + 1, iz + 1),
-              v111 = tx(ix + 1, iy + 1, iz + 1);
-        float a0 = v000 + (v100 - v000) * f.x, a1 = v010 + (v110 - v010) * f.x;
-        float a2 = v001 + (v101 - v001) * f.x, a3 = v011 + (v111 - v011) * f.x;
-        float b0 = a0 + (a1 - a0) * f.y, b1 = a2 + (a3 - a2) * f.y;
-        return b0 + (b1 - b0) * f.z;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            int a = ix + (c & 1), b = iy + ((c >> 1) & 1), d = iz + (c >> 2);
+            v[c] = ((unsigned)a >= 10u || (unsigned)b >= 10u || (unsigned)d >= 10u) ? 0.f
+                                                                                   : apronVal(G, brick, a, b, d);
+        }
+        return trilerp(v, f);
     }
-    int base = ix + kApron * (iy + kApron * iz);
-    float v000 = poolValue(G, brick, base), v100 = poolValue(G, brick, base + 1);
-    float v010 = poolValue(G, brick, base + 10), v110 = poolValue(G, brick, base + 11);
-    float v001 = poolValue(G, brick, base + 100), v101 = poolValue(G, brick, base + 101);
-    float v011 = poolValue(G, brick, base + 110), v111 = poolValue(G, brick, base + 111);
-    float a0 = v000 + (v100 - v000) * f.x, a1 = v010 + (v110 - v010) * f.x;
-    float a2 = v001 + (v101 - v001) * f.x, a3 = v011 + (v111 - v011) * f.x;
-    float b0 = a0 + (a1 - a0) * f.y, b1 = a2 + (a3 - a2) * f.y;
-    return b0 + (b1 - b0) * f.z;
+    footprint(G, brick, ix + kApron * (iy + kApron * iz), v);
+    return trilerp(v, f);
 }
 
 struct Ray { v3 o, d; float tMin, tMax; };
@@ -289,7 +310,7 @@ VR_DEV void traverse(const Scene& S, const Ray& rWorld, int slot, Rng& rng, A& a
                 BrickCtx bc;
                 bc.brick = brick;
                 bc.vmin = V3((float)(cx * 8), (float)(cy * 8), (float)(cz * 8));
-                const float* bd = G.bounds + 4 * (size_t)brick;
+                gptr<float> bd = gbl<float>(G.bounds) + 4 * (size_t)brick;
                 bc.bounds[0] = bd[0] * S.densityScale; bc.bounds[1] = bd[1] * S.densityScale;
                 bc.bounds[2] = bd[2] * S.densityScale; bc.bounds[3] = bd[3] * S.densityScale;
                 if (ad.brickStep(dda, bc, slot, t, rng)) return;
@@ -318,16 +339,11 @@ VR_DEV void traverse(const Scene& S, const Ray& rWorld, int slot, Rng& rng, A& a
 struct Poly { float c3, c2, c1, c0; };
 VR_DEV Poly cellPoly(const Scene& S, const Grid& G, const Dda& dl, const BrickCtx& bc, v3 d) {
     int base = (dl.p.x + 1) + kApron * ((dl.p.y + 1) + kApron * (dl.p.z + 1));
-    float sc = S.densityScale * S.sigma_t;
-    float v000 = poolValue(G, bc.brick, base) * S.densityScale * S.sigma_t;
-    float v100 = poolValue(G, bc.brick, base + 1) * S.densityScale * S.sigma_t;
-    float v010 = poolValue(G, bc.brick, base + 10) * S.densityScale * S.sigma_t;
-    float v110 = poolValue(G, bc.brick, base + 11) * S.densityScale * S.sigma_t;
-    float v001 = poolValue(G, bc.brick, base + 100) * S.densityScale * S.sigma_t;
-    float v101 = poolValue(G, bc.brick, base + 101) * S.densityScale * S.sigma_t;
-    float v011 = poolValue(G, bc.brick, base + 110) * S.densityScale * S.sigma_t;
-    float v111 = poolValue(G, bc.brick, base + 111) * S.densityScale * S.sigma_t;
-    (void)sc;
+    float v[8];
+    footprint(G, bc.brick, base, v);
+    const float ds = S.densityScale, st = S.sigma_t;
+    const float v000 = v[0] * ds * st, v100 = v[1] * ds * st, v010 = v[2] * ds * st, v110 = v[3] * ds * st;
+    const float v001 = v[4] * ds * st, v101 = v[5] * ds * st, v011 = v[6] * ds * st, v111 = v[7] * ds * st;
     float mxyz = v111 - v011 - v101 - v110 + v100 + v010 + v001 - v000;
     float mxy = v000 - v100 - v010 + v110;
     float mxz = v000 - v100 - v001 + v101;
