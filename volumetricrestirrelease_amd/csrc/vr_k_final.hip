@@ -1,0 +1,21 @@
+// vr_k_final.hip -- launcher of the FinalShading stage.
+#include "vr_stages.hpp"
+
+namespace vr {
+
+hipError_t launchFinalShading(const LaunchArgs& a) {
+    const dim3 g = tiles(a.width, a.height), blk(256, 1, 1);
+    const Options& fo = a.frame.finalOpt;
+    const bool fiAnalytic = fo.visibilityTrackingMethod == 1u && fo.lightingTrackingMethod == 1u;
+    VR_DISPATCH_B(a, {
+        if (fiAnalytic)
+            hipLaunchKernelGGL((kFinal<B, 1, 1>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0, a.features,
+                               a.color0, a.out);
+        else
+            hipLaunchKernelGGL((kFinal<B, -1, -1>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0,
+                               a.features, a.color0, a.out);
+    })
+    return hipGetLastError();
+}
+
+}  // namespace vr

@@ -1,0 +1,97 @@
+// vr_k_misc.hip -- GenerateFeatures, the batched tracking primitive, and the
+// stage dispatcher (CopyReservoirs is instantiated here).
+#include "vr_stages.hpp"
+
+namespace vr {
+
+// GenerateFeatures (VR/GenerateFeatures.cs.slang:57-102).
+__global__ __launch_bounds__(256) void kFeatures(const Scene* __restrict__ sc, const Frame fr,
+                                                 float2* __restrict__ features) {
+    const Scene& S = *sc;
+    const Frame& F = fr;
+    PixelIdx px = pixelOf(F);
+    if (!px.ok || F.useReference) return;
+    Rng rng{0, 0, 0, 0};
+    Ray r{ld3(F.cam.pos), rayDir(F.cam, px.x, px.y, F.width, F.height), 0.f, kTMax};
+    float accT = featureMarch(S, r, 0, true, F.initialOpt.visibilityTStepScale, rng);
+    features[px.id] = make_float2(__int_as_float(1), accT);
+}
+
+
+// Batched tracking primitive: one lane per ray.
+__global__ __launch_bounds__(256) void kTrack(const Scene* __restrict__ sc, int mode, int mip, int linear,
+                                              float tStepScale, int numSamples, int n, const float* __restrict__ rays,
+                                              const uint32_t* __restrict__ seeds, float* __restrict__ out) {
+    const Scene& S = *sc;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* r = rays + 8 * (size_t)i;
+    Ray ray{V3(r[0], r[1], r[2]), V3(r[3], r[4], r[5]), 0.f, r[6]};
+    Rng rng = rngCreate(seeds[3 * i], seeds[3 * i + 1], seeds[3 * i + 2]);
+    float o[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    switch (mode) {
+        case 0: o[0] = trAnalytic(S, ray, mip, rng, linear != 0); break;
+        case 1: o[0] = trMarch(S, ray, mip, linear != 0, tStepScale, rng); break;
+        case 2: o[0] = trResidual(S, ray, mip, rng, false, true); break;
+        case 3: o[0] = trResidual(S, ray, mip, rng, false, false); break;
+        case 4: o[0] = trResidual(S, ray, mip, rng, true, false); break;
+        case 5: {
+            float h[4], pd[4], tr[4];
+            int ns = max(1, min(4, numSamples));
+            sampleAnalytic(S, ray, rng, linear != 0, mip, ns, h, pd, tr);
+            o[0] = h[0]; o[1] = pd[0]; o[2] = tr[0];
+            o[3] = ns > 1 ? h[1] : 0.f; o[4] = ns > 1 ? pd[1] : 0.f; o[5] = ns > 1 ? tr[1] : 0.f;
+            o[6] = ns > 2 ? h[2] : 0.f; o[7] = ns > 3 ? h[3] : 0.f;
+            break;
+        }
+        case 6: {
+            MediumInteraction mi{V3(0.f), V3(0.f), S.g, false};
+            sampleDecomposition(S, ray, rng, mi, mip);
+            o[0] = mi.valid ? 1.f : 0.f; o[1] = mi.p.x; o[2] = mi.p.y; o[3] = mi.p.z;
+            break;
+        }
+        case 7: {
+            float sx, sy, bound;
+            cellByDensity(S, ray, rng, mip, sx, sy, bound);
+            o[0] = sx; o[1] = sy; o[2] = bound;
+            o[3] = rejectionSampleDepth(S, ray, rng, mip);
+            break;
+        }
+        case 8: o[0] = featureMarch(S, ray, mip, linear != 0, tStepScale, rng); break;
+        case 9: o[0] = densityWorld(S, mip, ray.o); break;
+        case 10:
+#pragma unroll
+            for (int k = 0; k < 8; k++) o[k] = __uint_as_float(rngNext(rng));
+            break;
+        default: break;
+    }
+    float* d = out + 8 * (size_t)i;
+#pragma unroll
+    for (int k = 0; k < 8; k++) d[k] = o[k];
+}
+
+
+hipError_t launchStage(const LaunchArgs& a) {
+    const dim3 g = tiles(a.width, a.height), blk(256, 1, 1);
+    switch (a.stage) {
+        case 0: hipLaunchKernelGGL(kFeatures, g, blk, 0, a.stream, a.scene, a.frame, a.features); break;
+        case 1: return launchTraceRays(a);
+        case 2: return launchTemporalReuse(a);
+        case 3: return launchSpatialReuse(a);
+        case 4: VR_DISPATCH_B(a, hipLaunchKernelGGL(kCopy<B>, g, blk, 0, a.stream, a.frame, a.res0, a.extra0, a.resT,
+                                                     a.extraT)) break;
+        case 5: return launchFinalShading(a);
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchTrack(hipStream_t stream, const Scene* scene, int mode, int mip, int linear, float tStepScale,
+                       int numSamples, int n, const float* rays, const uint32_t* seeds, float* out) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kTrack, dim3((n + 255) / 256), dim3(256), 0, stream, scene, mode, mip, linear, tStepScale,
+                       numSamples, n, rays, seeds, out);
+    return hipGetLastError();
+}
+
+}  // namespace vr

@@ -1,0 +1,21 @@
+// vr_k_temporal.hip -- launcher of the TemporalReuse stage.
+#include "vr_stages.hpp"
+
+namespace vr {
+
+hipError_t launchTemporalReuse(const LaunchArgs& a) {
+    const dim3 g = tiles(a.width, a.height), blk(256, 1, 1);
+    const Options& so = a.frame.spatialOpt;
+    const bool spMarch = so.visibilityTrackingMethod == 2u && so.lightingTrackingMethod == 2u;
+    VR_DISPATCH_B(a, {
+        if (spMarch)
+            hipLaunchKernelGGL((kTemporal<B, 2, 2>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0, a.resT,
+                               a.extraT, a.features, a.featuresT, a.mvec);
+        else
+            hipLaunchKernelGGL((kTemporal<B, -1, -1>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0,
+                               a.resT, a.extraT, a.features, a.featuresT, a.mvec);
+    })
+    return hipGetLastError();
+}
+
+}  // namespace vr

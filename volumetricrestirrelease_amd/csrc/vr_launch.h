@@ -30,6 +30,11 @@ struct LaunchArgs {
 };
 
 hipError_t launchStage(const LaunchArgs& a);
+// Per-stage launchers, one translation unit each (vr_k_*.hip).
+hipError_t launchTraceRays(const LaunchArgs& a);
+hipError_t launchTemporalReuse(const LaunchArgs& a);
+hipError_t launchSpatialReuse(const LaunchArgs& a);
+hipError_t launchFinalShading(const LaunchArgs& a);
 hipError_t launchTrack(hipStream_t stream, const Scene* scene, int mode, int mip, int linear, float tStepScale,
                        int numSamples, int n, const float* rays, const uint32_t* seeds, float* out);
 

@@ -1,10 +1,13 @@
-// vr_kernels.hip -- the six per-frame kernels of the VolumetricReSTIR pass, the
-// reference path tracer and the batched tracking primitive, for gfx950.
+// vr_stages.hpp -- kernel templates of the VolumetricReSTIR pass for gfx950:
+// the per-frame stages that trace through the volume, and the reference path
+// tracer. Each stage's launcher lives in its own translation unit
+// (vr_k_*.hip), which instantiates only that stage's kernels.
 //
 // One lane per pixel, 256-lane workgroups covering 16x16 pixel tiles (4 wave64
 // of 16x4 pixels each, so neighbouring lanes trace neighbouring rays).
 // Reservoirs are 32 B per pixel (two float4 loads, coalesced across the wave),
 // extra-bounce records 12 B per bounce, features 8 B, colour 16 B.
+#pragma once
 #include "vr_launch.h"
 #include "vr_pass.hpp"
 
@@ -21,18 +24,22 @@ VR_DEV PixelIdx pixelOf(const Frame& F) {
 }
 VR_DEV bool inRange(int x, int y, int W, int H) { return x >= 0 && x < W && y >= 0 && y < H; }
 
-// GenerateFeatures (VR/GenerateFeatures.cs.slang:57-102).
-__global__ __launch_bounds__(256) void kFeatures(const Scene* __restrict__ sc, const Frame fr,
-                                                 float2* __restrict__ features) {
-    const Scene& S = *sc;
-    const Frame& F = fr;
-    PixelIdx px = pixelOf(F);
-    if (!px.ok || F.useReference) return;
-    Rng rng{0, 0, 0, 0};
-    Ray r{ld3(F.cam.pos), rayDir(F.cam, px.x, px.y, F.width, F.height), 0.f, kTMax};
-    float accT = featureMarch(S, r, 0, true, F.initialOpt.visibilityTStepScale, rng);
-    features[px.id] = make_float2(__int_as_float(1), accT);
-}
+// The tracing kernels are latency bound (dependent brick-table and voxel
+// loads); four resident waves per SIMD (<= 128 VGPRs) hide more of that
+// latency than three with fewer spills.
+#define VR_TRACING_KERNEL __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+
+static inline dim3 tiles(int W, int H) { return dim3((W + 15) / 16, (H + 15) / 16, 1); }
+
+// Expands `body` with a compile-time B for the four supported path lengths.
+#define VR_DISPATCH_B(a, body)                                   \
+    switch ((a).maxBounces) {                                    \
+        case 1: { constexpr int B = 1; body; } break;            \
+        case 2: { constexpr int B = 2; body; } break;            \
+        case 3: { constexpr int B = 3; body; } break;            \
+        case 4: { constexpr int B = 4; body; } break;            \
+        default: return hipErrorInvalidValue;                    \
+    }
 
 // IntegrateByVolumePathTracing (VR/VolumePathTracingFunctions.slang:3-132), NEE on.
 VR_DEV v3 pathTrace(const Scene& S, const Frame& Fr, Ray ray, Rng& rng, int lightSamples, int maxBounces) {
@@ -87,7 +94,7 @@ __global__ __launch_bounds__(256) void kReference(const Scene* __restrict__ sc, 
 // TraceRays (VR/TraceRays.cs.slang:64-201). IM: initial lighting tracking
 // method; VM / LM: the spatial options' methods used by the final p-hat.
 template <int B, int IM, int VM, int LM>
-__global__ __launch_bounds__(256) void kTrace(const Scene* __restrict__ sc, const Frame fr,
+VR_TRACING_KERNEL void kTrace(const Scene* __restrict__ sc, const Frame fr,
                                               ResStore* __restrict__ outRes, float* __restrict__ outExtra) {
     const Scene& S = *sc;
     const Frame& F = fr;
@@ -154,7 +161,7 @@ VR_DEV bool isBackground(float2 f) { return f.y == 1.f && __float_as_int(f.x) !=
 
 // TemporalReuse (VR/TemporalReuse.cs.slang:80-377).
 template <int B, int VM, int LM>
-__global__ __launch_bounds__(256) void kTemporal(const Scene* __restrict__ sc, const Frame fr,
+VR_TRACING_KERNEL void kTemporal(const Scene* __restrict__ sc, const Frame fr,
                                                  ResStore* __restrict__ cur, float* __restrict__ curExtra,
                                                  const ResStore* __restrict__ temporal,
                                                  const float* __restrict__ temporalExtra,
@@ -304,7 +311,7 @@ __global__ __launch_bounds__(256) void kTemporal(const Scene* __restrict__ sc, c
 
 // SpatialReuse (VR/SpatialReuse.cs.slang:94-265) for one round.
 template <int B, int VM, int LM>
-__global__ __launch_bounds__(256) void kSpatial(const Scene* __restrict__ sc, const Frame fr,
+VR_TRACING_KERNEL void kSpatial(const Scene* __restrict__ sc, const Frame fr,
                                                 int roundId, const int* __restrict__ offsets,
                                                 const ResStore* __restrict__ in,
                                                 ResStore* __restrict__ out, const float* __restrict__ inExtra,
@@ -448,129 +455,5 @@ __global__ __launch_bounds__(256) void kFinal(const Scene* __restrict__ sc, cons
     out[px.id] = col;
 }
 
-// Batched tracking primitive: one lane per ray.
-__global__ __launch_bounds__(256) void kTrack(const Scene* __restrict__ sc, int mode, int mip, int linear,
-                                              float tStepScale, int numSamples, int n, const float* __restrict__ rays,
-                                              const uint32_t* __restrict__ seeds, float* __restrict__ out) {
-    const Scene& S = *sc;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float* r = rays + 8 * (size_t)i;
-    Ray ray{V3(r[0], r[1], r[2]), V3(r[3], r[4], r[5]), 0.f, r[6]};
-    Rng rng = rngCreate(seeds[3 * i], seeds[3 * i + 1], seeds[3 * i + 2]);
-    float o[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    switch (mode) {
-        case 0: o[0] = trAnalytic(S, ray, mip, rng, linear != 0); break;
-        case 1: o[0] = trMarch(S, ray, mip, linear != 0, tStepScale, rng); break;
-        case 2: o[0] = trResidual(S, ray, mip, rng, false, true); break;
-        case 3: o[0] = trResidual(S, ray, mip, rng, false, false); break;
-        case 4: o[0] = trResidual(S, ray, mip, rng, true, false); break;
-        case 5: {
-            float h[4], pd[4], tr[4];
-            int ns = max(1, min(4, numSamples));
-            sampleAnalytic(S, ray, rng, linear != 0, mip, ns, h, pd, tr);
-            o[0] = h[0]; o[1] = pd[0]; o[2] = tr[0];
-            o[3] = ns > 1 ? h[1] : 0.f; o[4] = ns > 1 ? pd[1] : 0.f; o[5] = ns > 1 ? tr[1] : 0.f;
-            o[6] = ns > 2 ? h[2] : 0.f; o[7] = ns > 3 ? h[3] : 0.f;
-            break;
-        }
-        case 6: {
-            MediumInteraction mi{V3(0.f), V3(0.f), S.g, false};
-            sampleDecomposition(S, ray, rng, mi, mip);
-            o[0] = mi.valid ? 1.f : 0.f; o[1] = mi.p.x; o[2] = mi.p.y; o[3] = mi.p.z;
-            break;
-        }
-        case 7: {
-            float sx, sy, bound;
-            cellByDensity(S, ray, rng, mip, sx, sy, bound);
-            o[0] = sx; o[1] = sy; o[2] = bound;
-            o[3] = rejectionSampleDepth(S, ray, rng, mip);
-            break;
-        }
-        case 8: o[0] = featureMarch(S, ray, mip, linear != 0, tStepScale, rng); break;
-        case 9: o[0] = densityWorld(S, mip, ray.o); break;
-        case 10:
-#pragma unroll
-            for (int k = 0; k < 8; k++) o[k] = __uint_as_float(rngNext(rng));
-            break;
-        default: break;
-    }
-    float* d = out + 8 * (size_t)i;
-#pragma unroll
-    for (int k = 0; k < 8; k++) d[k] = o[k];
-}
-
-// ------------------------------------------------------------------ launchers
-static dim3 tiles(int W, int H) { return dim3((W + 15) / 16, (H + 15) / 16, 1); }
-
-// Each reuse kernel is compiled for the tracking methods of the reference's
-// default options (ray marching for the spatial/temporal options and initial
-// lighting, analytic tracking for final shading) and once with run-time
-// dispatch for any other option combination.
-template <int B>
-static hipError_t launchStageB(const LaunchArgs& a) {
-    dim3 g = tiles(a.width, a.height), blk(256, 1, 1);
-    const Options& so = a.frame.spatialOpt;
-    const Options& fo = a.frame.finalOpt;
-    const bool spMarch = so.visibilityTrackingMethod == 2u && so.lightingTrackingMethod == 2u;
-    const bool inMarch = a.frame.initialOpt.lightingTrackingMethod == 2u;
-    const bool fiAnalytic = fo.visibilityTrackingMethod == 1u && fo.lightingTrackingMethod == 1u;
-    switch (a.stage) {
-        case 0: hipLaunchKernelGGL(kFeatures, g, blk, 0, a.stream, a.scene, a.frame, a.features); break;
-        case 1:
-            if (a.frame.useReference)
-                hipLaunchKernelGGL(kReference<B>, g, blk, 0, a.stream, a.scene, a.frame, a.color0);
-            else if (spMarch && inMarch)
-                hipLaunchKernelGGL((kTrace<B, 2, 2, 2>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0);
-            else
-                hipLaunchKernelGGL((kTrace<B, -1, -1, -1>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0);
-            break;
-        case 2:
-            if (spMarch)
-                hipLaunchKernelGGL((kTemporal<B, 2, 2>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0, a.resT,
-                                   a.extraT, a.features, a.featuresT, a.mvec);
-            else
-                hipLaunchKernelGGL((kTemporal<B, -1, -1>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0,
-                                   a.resT, a.extraT, a.features, a.featuresT, a.mvec);
-            break;
-        case 3:
-            if (spMarch)
-                hipLaunchKernelGGL((kSpatial<B, 2, 2>), g, blk, 0, a.stream, a.scene, a.frame, a.round, a.offsets, a.res0,
-                                   a.res1, a.extra0, a.extra1, a.features);
-            else
-                hipLaunchKernelGGL((kSpatial<B, -1, -1>), g, blk, 0, a.stream, a.scene, a.frame, a.round, a.offsets,
-                                   a.res0, a.res1, a.extra0, a.extra1, a.features);
-            break;
-        case 4: hipLaunchKernelGGL(kCopy<B>, g, blk, 0, a.stream, a.frame, a.res0, a.extra0, a.resT, a.extraT); break;
-        case 5:
-            if (fiAnalytic)
-                hipLaunchKernelGGL((kFinal<B, 1, 1>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0, a.features,
-                                   a.color0, a.out);
-            else
-                hipLaunchKernelGGL((kFinal<B, -1, -1>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0,
-                                   a.features, a.color0, a.out);
-            break;
-        default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-hipError_t launchStage(const LaunchArgs& a) {
-    switch (a.maxBounces) {
-        case 1: return launchStageB<1>(a);
-        case 2: return launchStageB<2>(a);
-        case 3: return launchStageB<3>(a);
-        case 4: return launchStageB<4>(a);
-        default: return hipErrorInvalidValue;
-    }
-}
-
-hipError_t launchTrack(hipStream_t stream, const Scene* scene, int mode, int mip, int linear, float tStepScale,
-                       int numSamples, int n, const float* rays, const uint32_t* seeds, float* out) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(kTrack, dim3((n + 255) / 256), dim3(256), 0, stream, scene, mode, mip, linear, tStepScale,
-                       numSamples, n, rays, seeds, out);
-    return hipGetLastError();
-}
 
 }  // namespace vr
