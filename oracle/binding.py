@@ -58,6 +58,18 @@ def lib():
         "orc_final_costs": (c_int, [c_void_p, c_void_p]),
         "orc_sample_phase": (c_float, [c_float, c_void_p, c_float, c_float, c_void_p]),
         "orc_phase_hg": (c_float, [c_float, c_float]),
+        "orc_oct_to_dir": (None, [c_float, c_float, c_void_p]),
+        "orc_light_uv_encode": (None, [c_void_p, c_void_p, POINTER(c_int)]),
+        "orc_light_uv_decode": (None, [c_void_p, c_int, c_void_p]),
+        "orc_wi_dist_encode": (None, [c_void_p, c_void_p]),
+        "orc_wi_dist_decode": (None, [c_void_p, c_void_p]),
+        "orc_pack_sampled_pixel": (c_int, [c_int, c_int, c_int]),
+        "orc_unpack_sampled_pixel": (None, [c_int, POINTER(c_int), POINTER(c_int)]),
+        "orc_wrs": (c_int, [c_int, c_void_p, c_void_p, c_float, c_uint32, c_uint32, c_uint32, POINTER(c_float),
+                            POINTER(c_float)]),
+        "orc_camera_ray": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p]),
+        "orc_reproject": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p]),
+        "orc_trace_bricks": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -3,9 +3,15 @@
 // A scalar FP32 restatement of the reference's shaders (FP64 only where the
 // reference uses it: the R2 neighbour sequence). It is the checker for the
 // HIP product and the CPU baseline of bench.py; nothing in the product links,
-// loads or calls it. Parity status of each part is listed in the oracle
-// section of the design notes; functions below cite the reference file:line
-// they follow.
+// loads or calls it. Functions below cite the reference file:line they follow.
+//
+// Pinned by closed forms / golden vectors in tests/: the RNG (the reference's
+// own xoshiro / splitmix64 sources), indexing and mips, the trackers, the
+// neighbour offsets, codecs, WRS, camera, reprojection and the HDDA walk.
+// Parity unpinned: D3D12 trilinear / BC4 and Slang transcendental ulps (not
+// reproducible without the reference's GPU path); the composition of the
+// estimator (evaluateF's product, Talbot weights, RNG draw order across a
+// stage), checked only against the HIP path.
 #include "vrestir_oracle.h"
 
 #include <chrono>
@@ -445,6 +451,18 @@ static Features featuresAt(const std::vector<Features>& f, int id) {
     return f[id];
 }
 
+// World point -> continuous pixel coordinates through camera c's view and
+// projection (VR/TemporalReuse.cs.slang:168-185).
+static f2 projectToPixel(const Camera& c, f3 p, int W, int H) {
+    f4 vp = mulRow({p.x, p.y, p.z, 1.f}, c.view);
+    f4 cp = mulRow(vp, c.proj);
+    float sx = cp.x / cp.w, sy = cp.y / cp.w;
+    sx = 0.5f * sx + 0.5f;
+    sy = -0.5f * sy + 0.5f;
+    sx *= (float)W; sy *= (float)H;
+    return {sx, sy};
+}
+
 // TemporalReuse (VR/TemporalReuse.cs.slang:80-377).
 static void stageTemporal(Oracle& O, int curIdx) {
     if (!O.temporalAccumulated) return;  // gIsFirstFrame
@@ -476,17 +494,14 @@ static void stageTemporal(Oracle& O, int curIdx) {
             if (reprojDepth == kRayTMax && O.P.temporalReprojectionMode != 2 && !isBgRes)
                 reprojDepth = rejectionSampleDepth(O.S.med, ray, sg, 8 + O.P.temporalReprojectionMipLevel);
             f3 p = ray.origin + ray.dir * reprojDepth;
-            f4 vp = mulRow({p.x, p.y, p.z, 1.f}, pc.view);
-            f4 cp = mulRow(vp, pc.proj);
-            float sx = cp.x / cp.w, sy = cp.y / cp.w;
+            float sx, sy;
             int sxi, syi;
             if (reprojDepth == kRayTMax) {
                 sx = (float)x + 0.5f; sy = (float)y + 0.5f;
                 sxi = x; syi = y;
             } else {
-                sx = 0.5f * sx + 0.5f;
-                sy = -0.5f * sy + 0.5f;
-                sx *= (float)O.W; sy *= (float)O.H;
+                f2 sp = projectToPixel(pc, p, O.W, O.H);
+                sx = sp.x; sy = sp.y;
                 sxi = (int)sx; syi = (int)sy;
             }
             {
@@ -1141,6 +1156,99 @@ float orc_phase_hg(float cosTheta, float g) { return phaseHG(cosTheta, g); }
 // Neighbour offsets of one spatial round.
 void orc_neighbor_offsets(int sampleCount, float radius, int frameSeed, int sampler, int* out) {
     for (int s = 0; s < sampleCount; s++) neighborOffset(s, sampleCount, radius, frameSeed, sampleCount, sampler, out[2 * s], out[2 * s + 1]);
+}
+
+// ---- helpers exposed so the tests can check the oracle against closed forms
+void orc_oct_to_dir(float u, float v, float out[3]) {
+    f3 d = octToDirEqualArea({u, v});
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+void orc_light_uv_encode(const float dir[3], float uv[2], int* id) {
+    f2 e = encodeLightUV({dir[0], dir[1], dir[2]}, *id);
+    uv[0] = e.x; uv[1] = e.y;
+}
+void orc_light_uv_decode(const float uv[2], int id, float out[3]) {
+    f3 d = decodeLightUV({uv[0], uv[1]}, id);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+void orc_wi_dist_encode(const float wiDist[4], float out[3]) {
+    f3 e = encodeWiDist({wiDist[0], wiDist[1], wiDist[2], wiDist[3]});
+    out[0] = e.x; out[1] = e.y; out[2] = e.z;
+}
+void orc_wi_dist_decode(const float in[3], float out[4]) {
+    f4 d = decodeWiDist({in[0], in[1], in[2]});
+    out[0] = d.x; out[1] = d.y; out[2] = d.z; out[3] = d.w;
+}
+int orc_pack_sampled_pixel(int pixel, int tag, int maxIndirect) {
+    return encodePathTag(encodeMaxIndirect(pixel, maxIndirect), tag);
+}
+void orc_unpack_sampled_pixel(int s, int* tag, int* maxIndirect) {
+    *tag = decodePathTag(s);
+    *maxIndirect = decodeMaxIndirect(s);
+}
+// Streams n candidates (weights w, counts M; candidate i carries depth = i)
+// through one reservoir with the sample generator of (px, py, sn); cap < 0
+// uses the plain step, otherwise the M-capped one. Returns the selected index
+// (-1 if none) and the reservoir's running sum and M.
+int orc_wrs(int n, const float* w, const float* M, float cap, uint32_t px, uint32_t py, uint32_t sn, float* runningSum,
+            float* outM) {
+    SampleGen sg = makeSampleGen(px, py, sn);
+    Reservoir st = newReservoir();
+    st.depth = -1.f;
+    for (int i = 0; i < n; i++) {
+        Reservoir r = newReservoir();
+        r.runningSum = w[i];
+        r.M = M[i];
+        r.depth = (float)i;
+        if (cap < 0.f) resampleStep(r, st, sg);
+        else resampleStepMaxM(r, cap, st, sg);
+    }
+    *runningSum = st.runningSum;
+    *outM = st.M;
+    return (int)st.depth;
+}
+// Normalised primary-ray direction of pixel (px, py) for the current camera.
+int orc_camera_ray(void* h, int px, int py, int W, int H, float out[3]) {
+    Oracle* O = (Oracle*)h;
+    if (!O->camSet) return VRESTIR_ESTATE;
+    f3 d = rayDir(O->S.cam, px, py, W, H);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+    return VRESTIR_OK;
+}
+// Continuous pixel coordinates of world point p through the current camera's
+// view and projection, the mapping TemporalReuse applies with the previous camera.
+int orc_reproject(void* h, const float p[3], int W, int H, float out[2]) {
+    Oracle* O = (Oracle*)h;
+    if (!O->camSet) return VRESTIR_ESTATE;
+    f2 s = projectToPixel(O->S.cam, {p[0], p[1], p[2]}, W, H);
+    out[0] = s.x; out[1] = s.y;
+    return VRESTIR_OK;
+}
+// Occupied bricks the hierarchical DDA hands to a tracker along a world-space
+// ray (origin, dir, tMax in ray[0..6]) on grid `slot`: per brick, the entry t,
+// exit t and the brick's index-space minimum corner. Returns the count.
+struct BrickLog {
+    float* out; int maxOut; int n = 0;
+    void setRayInfo(float, float, const Ray&) {}
+    void start() {}
+    bool mainStep(const HDDA& dda, f3 vminLeaf, f3, f4, int, float& t, SampleGen&) {
+        if (n < maxOut) {
+            float* o = out + 5 * n;
+            o[0] = t; o[1] = dda.t.y; o[2] = vminLeaf.x; o[3] = vminLeaf.y; o[4] = vminLeaf.z;
+        }
+        n++;
+        return false;
+    }
+    void end() {}
+};
+int orc_trace_bricks(void* h, int slot, const float ray[8], int vertexCenter, int maxOut, float* out) {
+    Oracle* O = (Oracle*)h;
+    if (!O->volumeSet || slot < 0 || slot >= 16 || !O->S.med.grid[slot].valid) return -1;
+    Ray r{{ray[0], ray[1], ray[2]}, {ray[3], ray[4], ray[5]}, 0.f, ray[6]};
+    SampleGen sg = makeSampleGen(0, 0, 0);
+    BrickLog log{out, maxOut};
+    trackGVDB(O->S.med, r, slot, sg, log, vertexCenter != 0);
+    return log.n;
 }
 
 // Volume introspection for the grid tests.

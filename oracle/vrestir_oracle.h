@@ -39,6 +39,18 @@ float orc_voxel(void* h, int slot, int x, int y, int z);
 int orc_final_costs(void* h, long long* out);
 float orc_sample_phase(float g, const float wo[3], float u0, float u1, float wi[3]);
 float orc_phase_hg(float cosTheta, float g);
+void orc_oct_to_dir(float u, float v, float out[3]);
+void orc_light_uv_encode(const float dir[3], float uv[2], int* id);
+void orc_light_uv_decode(const float uv[2], int id, float out[3]);
+void orc_wi_dist_encode(const float wiDist[4], float out[3]);
+void orc_wi_dist_decode(const float in[3], float out[4]);
+int orc_pack_sampled_pixel(int pixel, int tag, int maxIndirect);
+void orc_unpack_sampled_pixel(int s, int* tag, int* maxIndirect);
+int orc_wrs(int n, const float* w, const float* M, float cap, uint32_t px, uint32_t py, uint32_t sn, float* runningSum,
+            float* outM);
+int orc_camera_ray(void* h, int px, int py, int W, int H, float out[3]);
+int orc_reproject(void* h, const float p[3], int W, int H, float out[2]);
+int orc_trace_bricks(void* h, int slot, const float ray[8], int vertexCenter, int maxOut, float* out);
 #ifdef __cplusplus
 }
 #endif

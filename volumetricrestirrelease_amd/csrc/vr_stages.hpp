@@ -14,10 +14,19 @@
 namespace vr {
 
 struct PixelIdx { int x, y, id; bool ok; };
+// Workgroups go to the 8 XCDs round-robin by linear workgroup id, and each XCD
+// has its own L2. The tile of workgroup `lin` is chosen so that XCD x works
+// through one contiguous run of tiles (a band of image rows): its rays stay in
+// one part of the volume and of the reservoir buffers, which its L2 then holds.
+static constexpr int kXcds = 8;
 VR_DEV PixelIdx pixelOf(const Frame& F) {
+    const int gx = gridDim.x, n = gridDim.x * gridDim.y;
+    const int lin = blockIdx.y * gx + blockIdx.x;
+    const int xcd = lin % kXcds, k = lin / kXcds, q = n / kXcds, r = n % kXcds;
+    const int tile = xcd * q + min(xcd, r) + k;
     PixelIdx p;
-    p.x = blockIdx.x * 16 + (threadIdx.x & 15);
-    p.y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    p.x = (tile % gx) * 16 + (threadIdx.x & 15);
+    p.y = (tile / gx) * 16 + (threadIdx.x >> 4);
     p.ok = p.x < F.width && p.y < F.height;
     p.id = p.y * F.width + p.x;
     return p;
