@@ -224,6 +224,19 @@ int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t mode, int32
                             int32_t useLinearSampler, float tStepScale, int32_t numSamples, int32_t n,
                             const float* raysDev, const uint32_t* seedsDev, float* outDev);
 
+/* ---- ground truth and error (AccumulatePass double mode, AccumulatePass.cpp:42,93;
+ *      ImageCompare "rmse", Source/Tools/ImageCompare/ImageCompare.cpp:172-180) ---- */
+/* Adds one RGBA32F frame (device memory, width*height pixels) to the handle's FP64
+ * running sum; reset != 0 (or a new size) starts a new sum. */
+int32_t vrestir_accumulate(vrestir_handle* h, void* stream, const float* frameRGBA, int32_t width, int32_t height,
+                           int32_t reset);
+/* Writes the mean of the accumulated frames (RGBA32F, device memory); frames may be NULL. */
+int32_t vrestir_accumulated(vrestir_handle* h, void* stream, float* outRGBA, int64_t* frames);
+/* relMSE = mean over pixels of mean over RGB of (ref - img)^2 / (ref^2 + 1e-3), in FP64;
+ * both images RGBA32F in device memory; the result is written to *out (host) after the stream drains. */
+int32_t vrestir_relmse(vrestir_handle* h, void* stream, const float* imageRGBA, const float* referenceRGBA,
+                       int32_t width, int32_t height, double* out);
+
 /* ---- host-side helpers (no GPU needed) ---- */
 /* Seeded fBm density (4-octave Perlin by default) for the synthetic workloads. */
 int32_t vrestir_synth_fbm(uint64_t seed, int32_t nx, int32_t ny, int32_t nz, int32_t octaves,

@@ -136,6 +136,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "vrestir_read_buffer": (i32, [vp, i32, vp, c_size_t]),
         "vrestir_write_buffer": (i32, [vp, i32, vp, c_size_t]),
         "vrestir_track_batch": (i32, [vp, vp, i32, i32, i32, f32, i32, i32, vp, vp, vp]),
+        "vrestir_accumulate": (i32, [vp, vp, vp, i32, i32, i32]),
+        "vrestir_accumulated": (i32, [vp, vp, vp, POINTER(c_int64)]),
+        "vrestir_relmse": (i32, [vp, vp, vp, vp, i32, i32, POINTER(ctypes.c_double)]),
         "vrestir_synth_fbm": (i32, [c_uint64, i32, i32, i32, i32, f32, f32, f32, i32, f32, f32, POINTER(f32)]),
         "vrestir_orbit_camera": (i32, [POINTER(Camera), POINTER(f32), i32, f32, POINTER(Camera)]),
         "vrestir_count_bricks": (i32, [POINTER(f32), i32, i32, i32, POINTER(c_int64)]),
@@ -155,7 +158,8 @@ EXPORTED_SYMBOLS = [
     "vrestir_set_volume", "vrestir_load_vbx", "vrestir_set_lights", "vrestir_set_camera", "vrestir_set_params",
     "vrestir_get_params", "vrestir_reset_history", "vrestir_execute", "vrestir_frame_count",
     "vrestir_prepare_frame", "vrestir_run_stage", "vrestir_end_frame", "vrestir_read_buffer",
-    "vrestir_write_buffer", "vrestir_track_batch", "vrestir_synth_fbm", "vrestir_orbit_camera",
+    "vrestir_write_buffer", "vrestir_track_batch", "vrestir_accumulate", "vrestir_accumulated", "vrestir_relmse",
+    "vrestir_synth_fbm", "vrestir_orbit_camera",
     "vrestir_count_bricks", "vrestir_write_vbx", "vrestir_read_vbx",
 ]
 

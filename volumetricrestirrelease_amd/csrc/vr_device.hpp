@@ -98,6 +98,51 @@ VR_DEV uint32_t brickAt(const Grid& G, int bx, int by, int bz) {
         return kEmpty;
     return gbl<uint32_t>(G.brickTable)[bx + G.nbx * (by + G.nby * bz)];
 }
+
+// ------------------------------------------------------------------ occupancy tables in LDS
+// A traversal's empty-brick steps are each a dependent brick-table load. The
+// kernels that trace copy the {word, prefix} occupancy tables of up to two grid
+// slots into LDS once per workgroup (occCacheLoad, first statement of every
+// kernel that traverses), and brickLookup answers from LDS for those slots:
+// occupancy is a bit test and the id of an occupied brick its rank
+// (prefix + popcount of the lower bits), the id the brick table holds.
+extern __shared__ uint2 sOccTab[];        // dynamic LDS: the tables of sOccSlot[0], then sOccSlot[1]
+static __shared__ int sOccSlot[2], sOccOff[2];
+
+// slotA / slotB: grid slots to cache (-1: none). The launch must provide
+// (occWords(slotA) + occWords(slotB)) * 8 bytes of dynamic LDS.
+VR_DEV void occCacheLoad(const Scene& S, int slotA, int slotB) {
+    if (slotB == slotA) slotB = -1;
+    const int nA = slotA >= 0 ? S.grid[slotA].occWords : 0;
+    const int nB = slotB >= 0 ? S.grid[slotB].occWords : 0;
+    if (threadIdx.x == 0) {
+        sOccSlot[0] = slotA; sOccSlot[1] = slotB;
+        sOccOff[0] = 0; sOccOff[1] = nA;
+    }
+    if (nA > 0) {
+        gptr<uint32_t> a = gbl<uint32_t>(S.grid[slotA].occ);
+        for (int i = threadIdx.x; i < nA; i += blockDim.x) sOccTab[i] = make_uint2(a[2 * i], a[2 * i + 1]);
+    }
+    if (nB > 0) {
+        gptr<uint32_t> b = gbl<uint32_t>(S.grid[slotB].occ);
+        for (int i = threadIdx.x; i < nB; i += blockDim.x) sOccTab[nA + i] = make_uint2(b[2 * i], b[2 * i + 1]);
+    }
+    __syncthreads();
+}
+
+// brickAt for grid `slot`, from LDS when the kernel cached that slot's table.
+VR_DEV uint32_t brickLookup(const Grid& G, int slot, int bx, int by, int bz) {
+    if ((unsigned)bx >= (unsigned)G.nbx || (unsigned)by >= (unsigned)G.nby || (unsigned)bz >= (unsigned)G.nbz)
+        return kEmpty;
+    const uint32_t idx = (uint32_t)(bx + G.nbx * (by + G.nby * bz));
+    const int k = slot == sOccSlot[0] ? 0 : (slot == sOccSlot[1] ? 1 : -1);
+    if (k < 0) return gbl<uint32_t>(G.brickTable)[idx];
+    const uint2 e = sOccTab[sOccOff[k] + (idx >> 5)];
+    const uint32_t bit = 1u << (idx & 31u);
+    if (!(e.x & bit)) return kEmpty;
+    const uint32_t id = e.y + (uint32_t)__popc(e.x & (bit - 1u));
+    return id < G.numBricks ? id : kEmpty;
+}
 VR_DEV bool nodeAt(const Grid& G, int nx, int ny, int nz) {
     if ((unsigned)nx >= (unsigned)G.nnx || (unsigned)ny >= (unsigned)G.nny || (unsigned)nz >= (unsigned)G.nnz)
         return false;
@@ -183,7 +228,7 @@ VR_DEV float densityIndex(const Scene& S, int slot, v3 p) {
     const Grid& G = S.grid[slot];
     if (p.x < 0.f || p.y < 0.f || p.z < 0.f || p.x >= G.bmax[0] || p.y >= G.bmax[1] || p.z >= G.bmax[2]) return 0.f;
     int bx = (int)p.x / 8, by = (int)p.y / 8, bz = (int)p.z / 8;
-    uint32_t b = brickAt(G, bx, by, bz);
+    uint32_t b = brickLookup(G, slot, bx, by, bz);
     if (b == kEmpty) return 0.f;
     v3 q = p - V3((float)(bx * 8), (float)(by * 8), (float)(bz * 8));
     return sampleBrick(G, b, q, true) * S.densityScale;
@@ -301,7 +346,7 @@ VR_DEV void traverse(const Scene& S, const Ray& rWorld, int slot, Rng& rng, A& a
         }
         uint32_t brick = kEmpty;
         if (occupied && lev == 1) {
-            brick = brickAt(G, cx, cy, cz);
+            brick = brickLookup(G, slot, cx, cy, cz);
             occupied = brick != kEmpty;
         }
         if (occupied) {

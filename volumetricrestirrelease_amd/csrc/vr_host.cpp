@@ -55,6 +55,13 @@ struct vrestir_handle {
     float4* color[2] = {nullptr, nullptr};
     float4* outInternal = nullptr;
     int* offsetsDev = nullptr;
+    // ground-truth accumulation and relMSE scratch
+    double4* accum = nullptr;
+    int accumPixels = 0;
+    int64_t accumFrames = 0;
+    double* relPartial = nullptr;
+    int relPartialN = 0;
+    double* relOut = nullptr;
 };
 
 static int fail(vrestir_handle* h, int code, const std::string& msg) {
@@ -191,6 +198,9 @@ extern "C" int32_t vrestir_destroy(vrestir_handle* h) {
     freeVolume(h);
     dfree(h->sceneDev);
     dfree(h->offsetsDev);
+    dfree(h->accum);
+    dfree(h->relPartial);
+    dfree(h->relOut);
     delete h;
     return VRESTIR_OK;
 }
@@ -262,6 +272,10 @@ extern "C" int32_t vrestir_set_volume(vrestir_handle* h, const vrestir_volume_de
         G.brickTable = (const uint32_t*)p;
         if ((rc = upload(h, hg.nodeTable, &p))) return rc;
         G.nodeTable = (const uint32_t*)p;
+        if ((rc = upload(h, hg.occ, &p))) return rc;
+        G.occ = (const uint32_t*)p;
+        G.occWords = (int32_t)(hg.occ.size() / 2);
+        G.numBricks = hg.numBricks;
         if ((rc = upload(h, hg.bounds, &p))) return rc;
         G.bounds = (const float*)p;
         if (hg.fp16) { if ((rc = upload(h, hg.pool16, &p))) return rc; }
@@ -525,6 +539,21 @@ extern "C" int32_t vrestir_prepare_frame(vrestir_handle* h, int32_t W, int32_t H
     return VRESTIR_OK;
 }
 
+// Grid slots whose occupancy tables a launch keeps in LDS (see occCacheLoad):
+// at most kOccLdsBytes per workgroup, so four 256-lane workgroups still fit a CU.
+static const int kOccLdsBytes = 24 * 1024;
+static int occPlan(const vrestir_handle* h, int a, int b, int slots[2]) {
+    auto words = [&](int s) {
+        return (s >= 0 && s < kMaxSlots && h->sceneHost.grid[s].valid) ? h->sceneHost.grid[s].occWords : -1;
+    };
+    slots[0] = slots[1] = -1;
+    int bytes = 0;
+    const int wa = words(a), wb = b != a ? words(b) : -1;
+    if (wa >= 0 && wa * 8 <= kOccLdsBytes) { slots[0] = a; bytes += wa * 8; }
+    if (wb >= 0 && bytes + wb * 8 <= kOccLdsBytes) { slots[1] = b; bytes += wb * 8; }
+    return bytes;
+}
+
 extern "C" int32_t vrestir_run_stage(vrestir_handle* h, void* stream, int32_t stage, int32_t round, float* outRGBA) {
     if (!h) return VRESTIR_EINVAL;
     if (!h->frameOpen) return fail(h, VRESTIR_ESTATE, "vrestir_prepare_frame must precede stages");
@@ -570,6 +599,23 @@ extern "C" int32_t vrestir_run_stage(vrestir_handle* h, void* stream, int32_t st
             break;
         default: return fail(h, VRESTIR_EINVAL, "unknown stage");
     }
+    // the grids each stage traverses most: its options' visibility and lighting mips
+    const Frame& F = h->frame;
+    int oa = -1, ob = -1;
+    switch (stage) {
+        case VRESTIR_STAGE_GENERATE_FEATURES: oa = 0; break;
+        case VRESTIR_STAGE_TRACE_RAYS:
+            if (p.useReference) oa = 0;
+            else { oa = F.initialOpt.visibilityMipLevel; ob = F.initialOpt.lightingMipLevel; }
+            break;
+        case VRESTIR_STAGE_TEMPORAL_REUSE:
+        case VRESTIR_STAGE_SPATIAL_REUSE:
+            oa = F.spatialOpt.visibilityMipLevel; ob = F.spatialOpt.lightingMipLevel;
+            break;
+        case VRESTIR_STAGE_FINAL_SHADING: oa = F.finalOpt.visibilityMipLevel; ob = 0; break;
+        default: break;
+    }
+    a.occBytes = occPlan(h, oa, ob, a.frame.occSlot);
     HIPCHK(h, launchStage(a));
     if (stage == VRESTIR_STAGE_TEMPORAL_REUSE) {
         if (!p.enableSpatialReuse) {
@@ -687,6 +733,58 @@ extern "C" int32_t vrestir_write_buffer(vrestir_handle* h, int32_t id, const voi
     return VRESTIR_OK;
 }
 
+// ------------------------------------------------------------------ ground truth and error
+// FP64 accumulation of frames (AccumulatePass's double mode,
+// Source/RenderPasses/AccumulatePass/AccumulatePass.cpp:42,93).
+extern "C" int32_t vrestir_accumulate(vrestir_handle* h, void* stream, const float* frameRGBA, int32_t width,
+                                      int32_t height, int32_t reset) {
+    if (!h || !frameRGBA || width <= 0 || height <= 0) return VRESTIR_EINVAL;
+    HIPCHK(h, hipSetDevice(h->device));
+    const int n = width * height;
+    if (n != h->accumPixels) {
+        dfree(h->accum);
+        HIPCHK(h, hipMalloc((void**)&h->accum, (size_t)n * sizeof(double4)));
+        h->accumPixels = n;
+        reset = 1;
+    }
+    if (reset) h->accumFrames = 0;
+    HIPCHK(h, launchAccumulate((hipStream_t)stream, (const float4*)frameRGBA, h->accum, n, reset != 0));
+    h->accumFrames++;
+    return VRESTIR_OK;
+}
+
+// Mean of the accumulated frames, rounded once to float.
+extern "C" int32_t vrestir_accumulated(vrestir_handle* h, void* stream, float* outRGBA, int64_t* frames) {
+    if (!h || !outRGBA) return VRESTIR_EINVAL;
+    if (h->accumFrames == 0) return fail(h, VRESTIR_ESTATE, "nothing accumulated");
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, launchResolve((hipStream_t)stream, h->accum, 1.0 / (double)h->accumFrames, (float4*)outRGBA,
+                            h->accumPixels));
+    if (frames) *frames = h->accumFrames;
+    return VRESTIR_OK;
+}
+
+// relMSE of an RGBA image against a reference (ImageCompare's "rmse",
+// Source/Tools/ImageCompare/ImageCompare.cpp:172-180). Waits for the stream.
+extern "C" int32_t vrestir_relmse(vrestir_handle* h, void* stream, const float* imageRGBA, const float* referenceRGBA,
+                                  int32_t width, int32_t height, double* out) {
+    if (!h || !imageRGBA || !referenceRGBA || !out || width <= 0 || height <= 0) return VRESTIR_EINVAL;
+    HIPCHK(h, hipSetDevice(h->device));
+    const int n = width * height;
+    const int m = relMsePartialCount(n);
+    if (m > h->relPartialN) {
+        dfree(h->relPartial);
+        HIPCHK(h, hipMalloc((void**)&h->relPartial, (size_t)m * sizeof(double)));
+        h->relPartialN = m;
+    }
+    if (!h->relOut) HIPCHK(h, hipMalloc((void**)&h->relOut, sizeof(double)));
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(h, launchRelMse(st, (const float4*)imageRGBA, (const float4*)referenceRGBA, n, h->relPartial, h->relOut));
+    HIPCHK(h, hipMemcpyAsync(out, h->relOut, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipStreamSynchronize(st));
+    return VRESTIR_OK;
+}
+
 extern "C" int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t mode, int32_t mip,
                                        int32_t useLinearSampler, float tStepScale, int32_t numSamples, int32_t n,
                                        const float* rays, const uint32_t* seeds, float* out) {
@@ -696,8 +794,10 @@ extern "C" int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t 
     if (mip < 0 || mip >= kMaxSlots || !h->meta.slots[mip].valid) return fail(h, VRESTIR_EINVAL, "grid slot not bound");
     if (n < 0 || (n > 0 && (!rays || !seeds || !out))) return fail(h, VRESTIR_EINVAL, "bad ray batch");
     HIPCHK(h, hipSetDevice(h->device));
+    int occSlots[2];
+    const int occBytes = occPlan(h, mip, -1, occSlots);
     HIPCHK(h, launchTrack((hipStream_t)stream, h->sceneDev, mode, mip, useLinearSampler, tStepScale, numSamples, n,
-                          rays, seeds, out));
+                          rays, seeds, out, occSlots[0], occBytes));
     return VRESTIR_OK;
 }
 

@@ -9,10 +9,10 @@ hipError_t launchFinalShading(const LaunchArgs& a) {
     const bool fiAnalytic = fo.visibilityTrackingMethod == 1u && fo.lightingTrackingMethod == 1u;
     VR_DISPATCH_B(a, {
         if (fiAnalytic)
-            hipLaunchKernelGGL((kFinal<B, 1, 1>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0, a.features,
+            hipLaunchKernelGGL((kFinal<B, 1, 1>), g, blk, a.occBytes, a.stream, a.scene, a.frame, a.res0, a.extra0, a.features,
                                a.color0, a.out);
         else
-            hipLaunchKernelGGL((kFinal<B, -1, -1>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0,
+            hipLaunchKernelGGL((kFinal<B, -1, -1>), g, blk, a.occBytes, a.stream, a.scene, a.frame, a.res0, a.extra0,
                                a.features, a.color0, a.out);
     })
     return hipGetLastError();

@@ -9,6 +9,7 @@ __global__ __launch_bounds__(256) void kFeatures(const Scene* __restrict__ sc, c
                                                  float2* __restrict__ features) {
     const Scene& S = *sc;
     const Frame& F = fr;
+    occCacheLoad(S, F.occSlot[0], F.occSlot[1]);
     PixelIdx px = pixelOf(F);
     if (!px.ok || F.useReference) return;
     Rng rng{0, 0, 0, 0};
@@ -21,8 +22,10 @@ __global__ __launch_bounds__(256) void kFeatures(const Scene* __restrict__ sc, c
 // Batched tracking primitive: one lane per ray.
 __global__ __launch_bounds__(256) void kTrack(const Scene* __restrict__ sc, int mode, int mip, int linear,
                                               float tStepScale, int numSamples, int n, const float* __restrict__ rays,
-                                              const uint32_t* __restrict__ seeds, float* __restrict__ out) {
+                                              const uint32_t* __restrict__ seeds, float* __restrict__ out,
+                                              int occSlot) {
     const Scene& S = *sc;
+    occCacheLoad(S, occSlot, -1);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float* r = rays + 8 * (size_t)i;
@@ -74,7 +77,7 @@ __global__ __launch_bounds__(256) void kTrack(const Scene* __restrict__ sc, int 
 hipError_t launchStage(const LaunchArgs& a) {
     const dim3 g = tiles(a.width, a.height), blk(256, 1, 1);
     switch (a.stage) {
-        case 0: hipLaunchKernelGGL(kFeatures, g, blk, 0, a.stream, a.scene, a.frame, a.features); break;
+        case 0: hipLaunchKernelGGL(kFeatures, g, blk, a.occBytes, a.stream, a.scene, a.frame, a.features); break;
         case 1: return launchTraceRays(a);
         case 2: return launchTemporalReuse(a);
         case 3: return launchSpatialReuse(a);
@@ -87,10 +90,11 @@ hipError_t launchStage(const LaunchArgs& a) {
 }
 
 hipError_t launchTrack(hipStream_t stream, const Scene* scene, int mode, int mip, int linear, float tStepScale,
-                       int numSamples, int n, const float* rays, const uint32_t* seeds, float* out) {
+                       int numSamples, int n, const float* rays, const uint32_t* seeds, float* out, int occSlot,
+                       int occBytes) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(kTrack, dim3((n + 255) / 256), dim3(256), 0, stream, scene, mode, mip, linear, tStepScale,
-                       numSamples, n, rays, seeds, out);
+    hipLaunchKernelGGL(kTrack, dim3((n + 255) / 256), dim3(256), occBytes, stream, scene, mode, mip, linear,
+                       tStepScale, numSamples, n, rays, seeds, out, occSlot);
     return hipGetLastError();
 }
 

@@ -9,10 +9,10 @@ hipError_t launchSpatialReuse(const LaunchArgs& a) {
     const bool spMarch = so.visibilityTrackingMethod == 2u && so.lightingTrackingMethod == 2u;
     VR_DISPATCH_B(a, {
         if (spMarch)
-            hipLaunchKernelGGL((kSpatial<B, 2, 2>), g, blk, 0, a.stream, a.scene, a.frame, a.round, a.offsets, a.res0,
+            hipLaunchKernelGGL((kSpatial<B, 2, 2>), g, blk, a.occBytes, a.stream, a.scene, a.frame, a.round, a.offsets, a.res0,
                                a.res1, a.extra0, a.extra1, a.features);
         else
-            hipLaunchKernelGGL((kSpatial<B, -1, -1>), g, blk, 0, a.stream, a.scene, a.frame, a.round, a.offsets,
+            hipLaunchKernelGGL((kSpatial<B, -1, -1>), g, blk, a.occBytes, a.stream, a.scene, a.frame, a.round, a.offsets,
                                a.res0, a.res1, a.extra0, a.extra1, a.features);
     })
     return hipGetLastError();

@@ -9,10 +9,10 @@ hipError_t launchTemporalReuse(const LaunchArgs& a) {
     const bool spMarch = so.visibilityTrackingMethod == 2u && so.lightingTrackingMethod == 2u;
     VR_DISPATCH_B(a, {
         if (spMarch)
-            hipLaunchKernelGGL((kTemporal<B, 2, 2>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0, a.resT,
+            hipLaunchKernelGGL((kTemporal<B, 2, 2>), g, blk, a.occBytes, a.stream, a.scene, a.frame, a.res0, a.extra0, a.resT,
                                a.extraT, a.features, a.featuresT, a.mvec);
         else
-            hipLaunchKernelGGL((kTemporal<B, -1, -1>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0,
+            hipLaunchKernelGGL((kTemporal<B, -1, -1>), g, blk, a.occBytes, a.stream, a.scene, a.frame, a.res0, a.extra0,
                                a.resT, a.extraT, a.features, a.featuresT, a.mvec);
     })
     return hipGetLastError();

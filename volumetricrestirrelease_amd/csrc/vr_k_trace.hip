@@ -11,11 +11,11 @@ hipError_t launchTraceRays(const LaunchArgs& a) {
     const bool inMarch = a.frame.initialOpt.lightingTrackingMethod == 2u;
     VR_DISPATCH_B(a, {
         if (a.frame.useReference)
-            hipLaunchKernelGGL(kReference<B>, g, blk, 0, a.stream, a.scene, a.frame, a.color0);
+            hipLaunchKernelGGL(kReference<B>, g, blk, a.occBytes, a.stream, a.scene, a.frame, a.color0);
         else if (spMarch && inMarch)
-            hipLaunchKernelGGL((kTrace<B, 2, 2, 2>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0);
+            hipLaunchKernelGGL((kTrace<B, 2, 2, 2>), g, blk, a.occBytes, a.stream, a.scene, a.frame, a.res0, a.extra0);
         else
-            hipLaunchKernelGGL((kTrace<B, -1, -1, -1>), g, blk, 0, a.stream, a.scene, a.frame, a.res0, a.extra0);
+            hipLaunchKernelGGL((kTrace<B, -1, -1, -1>), g, blk, a.occBytes, a.stream, a.scene, a.frame, a.res0, a.extra0);
     })
     return hipGetLastError();
 }

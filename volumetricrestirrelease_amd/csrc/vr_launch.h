@@ -13,6 +13,7 @@ struct ResStore;
 struct LaunchArgs {
     hipStream_t stream;
     int stage, round, maxBounces, width, height;
+    int occBytes;  // dynamic LDS of the occupancy tables named in frame.occSlot
     const Scene* scene;
     Frame frame;
     const int* offsets;
@@ -35,7 +36,13 @@ hipError_t launchTraceRays(const LaunchArgs& a);
 hipError_t launchTemporalReuse(const LaunchArgs& a);
 hipError_t launchSpatialReuse(const LaunchArgs& a);
 hipError_t launchFinalShading(const LaunchArgs& a);
+// Ground-truth accumulation and relMSE (vr_k_accum.hip).
+hipError_t launchAccumulate(hipStream_t st, const float4* frame, double4* sum, int n, bool reset);
+hipError_t launchResolve(hipStream_t st, const double4* sum, double inv, float4* out, int n);
+int relMsePartialCount(int n);
+hipError_t launchRelMse(hipStream_t st, const float4* img, const float4* ref, int n, double* partial, double* out);
 hipError_t launchTrack(hipStream_t stream, const Scene* scene, int mode, int mip, int linear, float tStepScale,
-                       int numSamples, int n, const float* rays, const uint32_t* seeds, float* out);
+                       int numSamples, int n, const float* rays, const uint32_t* seeds, float* out, int occSlot,
+                       int occBytes);
 
 }  // namespace vr

@@ -14,19 +14,10 @@
 namespace vr {
 
 struct PixelIdx { int x, y, id; bool ok; };
-// Workgroups go to the 8 XCDs round-robin by linear workgroup id, and each XCD
-// has its own L2. The tile of workgroup `lin` is chosen so that XCD x works
-// through one contiguous run of tiles (a band of image rows): its rays stay in
-// one part of the volume and of the reservoir buffers, which its L2 then holds.
-static constexpr int kXcds = 8;
 VR_DEV PixelIdx pixelOf(const Frame& F) {
-    const int gx = gridDim.x, n = gridDim.x * gridDim.y;
-    const int lin = blockIdx.y * gx + blockIdx.x;
-    const int xcd = lin % kXcds, k = lin / kXcds, q = n / kXcds, r = n % kXcds;
-    const int tile = xcd * q + min(xcd, r) + k;
     PixelIdx p;
-    p.x = (tile % gx) * 16 + (threadIdx.x & 15);
-    p.y = (tile / gx) * 16 + (threadIdx.x >> 4);
+    p.x = blockIdx.x * 16 + (threadIdx.x & 15);
+    p.y = blockIdx.y * 16 + (threadIdx.x >> 4);
     p.ok = p.x < F.width && p.y < F.height;
     p.id = p.y * F.width + p.x;
     return p;
@@ -89,6 +80,7 @@ __global__ __launch_bounds__(256) void kReference(const Scene* __restrict__ sc, 
                                                   float4* __restrict__ outColor) {
     const Scene& S = *sc;
     const Frame& F = fr;
+    occCacheLoad(S, F.occSlot[0], F.occSlot[1]);
     PixelIdx px = pixelOf(F);
     if (!px.ok) return;
     Rng rng = rngCreate(px.x, px.y, (uint32_t)(F.numTotalRounds * F.frameCount));
@@ -107,6 +99,7 @@ VR_TRACING_KERNEL void kTrace(const Scene* __restrict__ sc, const Frame fr,
                                               ResStore* __restrict__ outRes, float* __restrict__ outExtra) {
     const Scene& S = *sc;
     const Frame& F = fr;
+    occCacheLoad(S, F.occSlot[0], F.occSlot[1]);
     PixelIdx px = pixelOf(F);
     if (!px.ok) return;
     Rng rng = rngCreate(px.x, px.y, (uint32_t)(F.numTotalRounds * F.frameCount));
@@ -179,6 +172,7 @@ VR_TRACING_KERNEL void kTemporal(const Scene* __restrict__ sc, const Frame fr,
                                                  float2* __restrict__ mvec) {
     const Scene& S = *sc;
     const Frame& F = fr;
+    occCacheLoad(S, F.occSlot[0], F.occSlot[1]);
     PixelIdx px = pixelOf(F);
     if (!px.ok || F.isFirstFrame) return;
     const int W = F.width, H = F.height, N = W * H;
@@ -327,6 +321,7 @@ VR_TRACING_KERNEL void kSpatial(const Scene* __restrict__ sc, const Frame fr,
                                                 float* __restrict__ outExtra, const float2* __restrict__ features) {
     const Scene& S = *sc;
     const Frame& F = fr;
+    occCacheLoad(S, F.occSlot[0], F.occSlot[1]);
     PixelIdx px = pixelOf(F);
     if (!px.ok) return;
     const int W = F.width, H = F.height;
@@ -440,6 +435,7 @@ __global__ __launch_bounds__(256) void kFinal(const Scene* __restrict__ sc, cons
                                               float4* __restrict__ out) {
     const Scene& S = *sc;
     const Frame& F = fr;
+    occCacheLoad(S, F.occSlot[0], F.occSlot[1]);
     PixelIdx px = pixelOf(F);
     if (!px.ok) return;
     Rng rng = rngCreate(px.x, px.y, (uint32_t)(F.numTotalRounds * F.frameCount + F.numTotalRounds - 1));

@@ -30,7 +30,12 @@ struct Grid {
     float maxValue;
     int32_t fp16;
     int32_t valid;
-    int32_t pad_;
+    int32_t occWords;             // entries of occ
+    uint32_t numBricks;
+    // Brick occupancy as 32-bit words over the brick-table index, each with the
+    // number of occupied bricks in all earlier words: brick ids are assigned in
+    // table order, so id = prefix + popcount(word bits below the brick's bit).
+    const uint32_t* occ;          // occWords pairs {word, prefix}
 };
 
 struct Scene {
@@ -90,6 +95,7 @@ struct Frame {
     int32_t spatialRounds;
     int32_t outputMotionVec;
     int32_t pad_;
+    int32_t occSlot[2];           // grid slots whose occupancy tables the kernel keeps in LDS (-1: none)
     CameraFrame cam, prevCam;
     float prevView[16], prevProj[16];
     Options initialOpt, spatialOpt, finalOpt;

@@ -167,6 +167,19 @@ static void buildSlot(HostGrid& g, const DenseF& src, bool fp16) {
                 g.nodeTable[(bx / 16) + (size_t)g.nnx * ((by / 16) + (size_t)g.nny * (bz / 16))] = 1u;
             }
     g.numBricks = nb;
+    const size_t words = (nbTot + 31) / 32;
+    g.occ.assign(2 * words, 0u);
+    uint32_t before = 0;
+    for (size_t w = 0; w < words; w++) {
+        uint32_t bits = 0;
+        for (int b = 0; b < 32; b++) {
+            size_t c = w * 32 + b;
+            if (c < nbTot && occ[c]) bits |= 1u << b;
+        }
+        g.occ[2 * w] = bits;
+        g.occ[2 * w + 1] = before;
+        before += (uint32_t)__builtin_popcount(bits);
+    }
     g.bounds.assign((size_t)nb * 4, 0.f);
     if (fp16) g.pool16.assign((size_t)nb * kBrickVals, 0);
     else g.pool32.assign((size_t)nb * kBrickVals, 0.f);

@@ -29,6 +29,7 @@ struct HostGrid {
     float w2i[16];
     uint32_t numBricks = 0;
     std::vector<uint32_t> brickTable, nodeTable;
+    std::vector<uint32_t> occ;    // {word, prefix} pairs over the brick-table index
     std::vector<float> bounds;
     std::vector<float> pool32;
     std::vector<uint16_t> pool16;

@@ -50,6 +50,10 @@ class VolumetricReSTIRParams:
         return self._p
 
 
+def _stream_ptr(stream):
+    return ctypes.c_void_p(stream.cuda_stream if stream is not None else 0)
+
+
 class VolumetricReSTIR:
     """One instance of the pass, bound to the current HIP device."""
 
@@ -151,6 +155,26 @@ class VolumetricReSTIR:
         data = np.ascontiguousarray(data)
         self._check(self.lib.vrestir_write_buffer(self.h, buf, ctypes.c_void_p(data.ctypes.data), data.nbytes),
                     "write")
+
+    # ---- ground truth (AccumulatePass double mode) and relMSE (ImageCompare "rmse")
+    def accumulate(self, frame_rgba, reset=False, stream=None):
+        H, W = int(frame_rgba.shape[0]), int(frame_rgba.shape[1])
+        st = _stream_ptr(stream)
+        self._check(self.lib.vrestir_accumulate(self.h, st, frame_rgba.data_ptr(), W, H, 1 if reset else 0),
+                    "accumulate")
+
+    def accumulated(self, out_rgba, stream=None) -> int:
+        n = ctypes.c_int64()
+        self._check(self.lib.vrestir_accumulated(self.h, _stream_ptr(stream), out_rgba.data_ptr(), ctypes.byref(n)),
+                    "accumulated")
+        return n.value
+
+    def relmse(self, image_rgba, reference_rgba, stream=None) -> float:
+        H, W = int(image_rgba.shape[0]), int(image_rgba.shape[1])
+        out = ctypes.c_double()
+        self._check(self.lib.vrestir_relmse(self.h, _stream_ptr(stream), image_rgba.data_ptr(),
+                                            reference_rgba.data_ptr(), W, H, ctypes.byref(out)), "relmse")
+        return out.value
 
     def track(self, mode, mip, linear, tstep, nsamples, rays, seeds, out, stream=None):
         """Batched tracking primitive over device tensors rays [n,8] f32, seeds [n,3] u32/i32, out [n,8] f32."""
