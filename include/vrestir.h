@@ -224,6 +224,15 @@ int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t mode, int32
                             int32_t useLinearSampler, float tStepScale, int32_t numSamples, int32_t n,
                             const float* raysDev, const uint32_t* seedsDev, float* outDev);
 
+/* ---- implementation choices (no counterpart in the reference) ----
+ * VRESTIR_TUNE_SPLIT_REUSE (default 0): 1 runs spatial reuse as three launches
+ * -- record its transmittance rays, trace them from per-wave balanced queues,
+ * replay the reuse with their values -- when its ray kinds use ray marching;
+ * 0 runs it as one kernel. Results are the same either way; on the 1080p
+ * cloud workload both take the same time. */
+enum { VRESTIR_TUNE_SPLIT_REUSE = 1 };
+int32_t vrestir_set_tuning(vrestir_handle* h, int32_t key, int32_t value);
+
 /* ---- ground truth and error (AccumulatePass double mode, AccumulatePass.cpp:42,93;
  *      ImageCompare "rmse", Source/Tools/ImageCompare/ImageCompare.cpp:172-180) ---- */
 /* Adds one RGBA32F frame (device memory, width*height pixels) to the handle's FP64

@@ -146,3 +146,30 @@ def test_whole_frames_uninjected():
         say(f"   frame {f}: frac within 1e-3 {frac:.4f}, mean radiance oracle {co[:, :3].mean():.5f} gpu {cg[:, :3].mean():.5f}")
         assert frac >= 0.97
         assert abs(co[:, :3].mean() - cg[:, :3].mean()) <= 2e-3 * co[:, :3].mean()
+
+
+def test_split_spatial_reuse_matches_single_kernel():
+    say("case: spatial reuse as record / balanced trace / replay vs one kernel, fp16 b=3, 2 rounds, 3 frames")
+    import torch
+    from volumetricrestirrelease_amd.volumetric_restir import VolumetricReSTIR
+    spec = small_scene(fp16=True, maxBounces=3, initialM=8, temporalReuseMThreshold=2.5, spatialReuseRounds=2,
+                       spatialSampleCount=6)
+    spec.orbit_degrees_per_frame = 2.0
+    outs = []
+    for split in (0, 1):
+        g = VolumetricReSTIR()
+        g.setScene(spec)
+        assert abi.load_library().vrestir_set_tuning(g.h, abi.TUNE_SPLIT_REUSE, split) == 0
+        out = torch.zeros((spec.height, spec.width, 4), device="cuda")
+        frames = []
+        for f in range(3):
+            g.setCamera(spec, f)
+            g.execute(out)
+            torch.cuda.synchronize()
+            frames.append(out.cpu().numpy().copy())
+        outs.append(frames)
+    for f in range(3):
+        a, b = outs[0][f][..., :3], outs[1][f][..., :3]
+        frac = float((rel(a.reshape(-1, 3), b.reshape(-1, 3)).max(axis=1) <= 1e-5).mean())
+        say(f"   frame {f}: pixels equal within 1e-5: {frac:.4f}")
+        assert frac >= 0.999

@@ -28,6 +28,8 @@ SAMPLER_HAMMERSLEY, SAMPLER_R2 = 0, 1
 REPROJECTION_LINEAR, REPROJECTION_NONE, REPROJECTION_NO_BACKGROUND = 0, 1, 2
 BRICK_FP32, BRICK_FP16 = 0, 1
 
+TUNE_SPLIT_REUSE = 1
+
 STAGE_GENERATE_FEATURES, STAGE_TRACE_RAYS, STAGE_TEMPORAL_REUSE, STAGE_SPATIAL_REUSE, \
     STAGE_COPY_RESERVOIRS, STAGE_FINAL_SHADING = range(6)
 BUF_FEATURES, BUF_TEMPORAL_FEATURES, BUF_RESERVOIRS0, BUF_RESERVOIRS1, BUF_TEMPORAL_RESERVOIRS, \
@@ -136,6 +138,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "vrestir_read_buffer": (i32, [vp, i32, vp, c_size_t]),
         "vrestir_write_buffer": (i32, [vp, i32, vp, c_size_t]),
         "vrestir_track_batch": (i32, [vp, vp, i32, i32, i32, f32, i32, i32, vp, vp, vp]),
+        "vrestir_set_tuning": (i32, [vp, i32, i32]),
         "vrestir_accumulate": (i32, [vp, vp, vp, i32, i32, i32]),
         "vrestir_accumulated": (i32, [vp, vp, vp, POINTER(c_int64)]),
         "vrestir_relmse": (i32, [vp, vp, vp, vp, i32, i32, POINTER(ctypes.c_double)]),
@@ -158,7 +161,7 @@ EXPORTED_SYMBOLS = [
     "vrestir_set_volume", "vrestir_load_vbx", "vrestir_set_lights", "vrestir_set_camera", "vrestir_set_params",
     "vrestir_get_params", "vrestir_reset_history", "vrestir_execute", "vrestir_frame_count",
     "vrestir_prepare_frame", "vrestir_run_stage", "vrestir_end_frame", "vrestir_read_buffer",
-    "vrestir_write_buffer", "vrestir_track_batch", "vrestir_accumulate", "vrestir_accumulated", "vrestir_relmse",
+    "vrestir_write_buffer", "vrestir_track_batch", "vrestir_set_tuning", "vrestir_accumulate", "vrestir_accumulated", "vrestir_relmse",
     "vrestir_synth_fbm", "vrestir_orbit_camera",
     "vrestir_count_bricks", "vrestir_write_vbx", "vrestir_read_vbx",
 ]

@@ -62,6 +62,10 @@ struct vrestir_handle {
     double* relPartial = nullptr;
     int relPartialN = 0;
     double* relOut = nullptr;
+    // record / trace / replay buffers of the reuse stages
+    bool splitReuse = false;
+    SplitBufs split{};
+    size_t splitPixels = 0;
 };
 
 static int fail(vrestir_handle* h, int code, const std::string& msg) {
@@ -82,7 +86,13 @@ static void dfree(T*& p) {
     p = nullptr;
 }
 
+static void freeSplit(vrestir_handle* h) {
+    dfree(h->split.recs); dfree(h->split.trs); dfree(h->split.counts); dfree(h->split.plan);
+    h->split.cap = 0;
+    h->splitPixels = 0;
+}
 static void freeFrameBuffers(vrestir_handle* h) {
+    freeSplit(h);
     for (int i = 0; i < 2; i++) { dfree(h->res[i]); dfree(h->extra[i]); dfree(h->color[i]); }
     dfree(h->resT); dfree(h->extraT); dfree(h->features); dfree(h->featuresT); dfree(h->outInternal);
     h->W = h->H = 0;
@@ -201,6 +211,7 @@ extern "C" int32_t vrestir_destroy(vrestir_handle* h) {
     dfree(h->accum);
     dfree(h->relPartial);
     dfree(h->relOut);
+    freeSplit(h);
     delete h;
     return VRESTIR_OK;
 }
@@ -554,6 +565,44 @@ static int occPlan(const vrestir_handle* h, int a, int b, int slots[2]) {
     return bytes;
 }
 
+// The spatial stage runs as record / trace / replay (balanced tracing of its
+// transmittance rays) when both of its ray kinds use ray marching and the ray
+// records of a frame fit kSplitBudget; otherwise it runs in one kernel.
+static const size_t kSplitBudget = (size_t)48 << 30;
+static int splitPlan(vrestir_handle* h, LaunchArgs& a) {
+    a.splitTK = 0;
+    const vrestir_params& p = h->P;
+    const Options& so = h->frame.spatialOpt;
+    if (!h->splitReuse || so.visibilityTrackingMethod != 2u || so.lightingTrackingMethod != 2u) return VRESTIR_OK;
+    const int K = p.spatialSampleCount;
+    const int evals = p.spatialMISMethod == VRESTIR_MIS_TALBOT ? K * (K - 1) : K - 1;
+    const int cap = std::max(1, evals * (p.maxBounces + 1));
+    const size_t n = (size_t)h->W * h->H;
+    const size_t bytes = n * (size_t)cap * (sizeof(TraceRec) + sizeof(float)) + n * 8;
+    if (bytes > kSplitBudget) return VRESTIR_OK;
+    if (h->split.cap != cap || h->splitPixels != n) {
+        freeSplit(h);
+        HIPCHK(h, hipMalloc((void**)&h->split.recs, n * (size_t)cap * sizeof(TraceRec)));
+        HIPCHK(h, hipMalloc((void**)&h->split.trs, n * (size_t)cap * sizeof(float)));
+        HIPCHK(h, hipMalloc((void**)&h->split.counts, n * sizeof(int32_t)));
+        HIPCHK(h, hipMalloc((void**)&h->split.plan, n * sizeof(uint32_t)));
+        h->split.cap = cap;
+        h->splitPixels = n;
+    }
+    a.split = h->split;
+    a.splitTK = 2;
+    return VRESTIR_OK;
+}
+
+// Implementation choices with no counterpart in the reference.
+extern "C" int32_t vrestir_set_tuning(vrestir_handle* h, int32_t key, int32_t value) {
+    if (!h) return VRESTIR_EINVAL;
+    switch (key) {
+        case VRESTIR_TUNE_SPLIT_REUSE: h->splitReuse = value != 0; return VRESTIR_OK;
+        default: return fail(h, VRESTIR_EINVAL, "unknown tuning key");
+    }
+}
+
 extern "C" int32_t vrestir_run_stage(vrestir_handle* h, void* stream, int32_t stage, int32_t round, float* outRGBA) {
     if (!h) return VRESTIR_EINVAL;
     if (!h->frameOpen) return fail(h, VRESTIR_ESTATE, "vrestir_prepare_frame must precede stages");
@@ -616,6 +665,10 @@ extern "C" int32_t vrestir_run_stage(vrestir_handle* h, void* stream, int32_t st
         default: break;
     }
     a.occBytes = occPlan(h, oa, ob, a.frame.occSlot);
+    if (stage == VRESTIR_STAGE_SPATIAL_REUSE) {
+        int rc = splitPlan(h, a);
+        if (rc) return rc;
+    }
     HIPCHK(h, launchStage(a));
     if (stage == VRESTIR_STAGE_TEMPORAL_REUSE) {
         if (!p.enableSpatialReuse) {

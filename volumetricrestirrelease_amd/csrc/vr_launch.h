@@ -14,6 +14,8 @@ struct LaunchArgs {
     hipStream_t stream;
     int stage, round, maxBounces, width, height;
     int occBytes;  // dynamic LDS of the occupancy tables named in frame.occSlot
+    int splitTK;   // > 0: run the reuse stage as record / trace / replay with tracker kind splitTK
+    SplitBufs split;
     const Scene* scene;
     Frame frame;
     const int* offsets;
@@ -36,6 +38,8 @@ hipError_t launchTraceRays(const LaunchArgs& a);
 hipError_t launchTemporalReuse(const LaunchArgs& a);
 hipError_t launchSpatialReuse(const LaunchArgs& a);
 hipError_t launchFinalShading(const LaunchArgs& a);
+// Balanced trace of the rays recorded in a.split (vr_k_queue.hip); tk: 2 ray marching, 1 analytic.
+hipError_t launchTraceQueue(const LaunchArgs& a, const Options& opt, bool isFinal, int tk);
 // Ground-truth accumulation and relMSE (vr_k_accum.hip).
 hipError_t launchAccumulate(hipStream_t st, const float4* frame, double4* sum, int n, bool reset);
 hipError_t launchResolve(hipStream_t st, const double4* sum, double inv, float4* out, int n);

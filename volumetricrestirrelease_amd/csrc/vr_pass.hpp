@@ -196,15 +196,72 @@ VR_DEV v3 directLightingPT(const Scene& S, Rng& rng, const MediumInteraction& mi
 }
 
 // ------------------------------------------------------------------ target function
+// How evaluateF obtains the visibility of a transmittance ray. The direct
+// policy traces it on the spot (visibilityT). The record and replay policies
+// split one reuse kernel into three launches: the record pass runs the pixel
+// program with every visibility taken as 1 and writes each ray it would trace;
+// a separate kernel traces all recorded rays from a balanced queue; the replay
+// pass runs the program again and reads each ray's transmittance back in the
+// same order. Only deterministic trackers (ray marching, analytic) qualify:
+// they draw no random numbers, so the split changes no result.
+//
+// To make record and replay enumerate the same rays, their early exits look
+// only at the transmittance-free product Fn (sample, phase, density and light
+// factors), never at a visibility: a zero visibility makes F zero and every
+// later factor is finite, so continuing changes the result by nothing.
+template <int VM, int LM>
+struct DirectTrace {
+    static constexpr bool kSplit = false;
+    // kind 0: visibility options; 1: lighting options (grid 0 in final shading)
+    VR_DEV float vis(const Scene& S, const Ray& r, Rng& rng, const Options& o, int kind, bool isFinal) {
+        if (kind == 0)
+            return visibilityT<VM>(S, r, rng, o.visibilitySamples, o.visibilityMipLevel,
+                                   o.visibilityUseLinearSampler != 0, o.visibilityTrackingMethod,
+                                   o.visibilityTStepScale);
+        return visibilityT<LM>(S, r, rng, o.lightSamples, isFinal ? 0 : o.lightingMipLevel,
+                               o.lightingUseLinearSampler != 0, o.lightingTrackingMethod, o.lightingTStepScale);
+    }
+};
+// visibilityT's average of `samples` identical transmittances (0/0 for none).
+VR_DEV float sameSamples(float tr, int samples) {
+    float v = 0.f;
+    for (int i = 0; i < samples; i++) v += tr;
+    return v / (float)samples;
+}
+struct RecordTrace {
+    static constexpr bool kSplit = true;
+    TraceRec* out;  // this pixel's record slots
+    int n;          // records written
+    VR_DEV float vis(const Scene&, const Ray& r, Rng&, const Options& o, int kind, bool) {
+        const int samples = kind == 0 ? o.visibilitySamples : o.lightSamples;
+        if (samples <= 0) return sameSamples(1.f, samples);
+        TraceRec rec;
+        rec.ox = r.o.x; rec.oy = r.o.y; rec.oz = r.o.z; rec.tMax = r.tMax;
+        rec.dx = r.d.x; rec.dy = r.d.y; rec.dz = r.d.z; rec.kind = kind;
+        out[n++] = rec;
+        return sameSamples(1.f, samples);
+    }
+};
+struct ReplayTrace {
+    static constexpr bool kSplit = true;
+    const float* tr;  // this pixel's transmittances, in record order
+    int n;
+    VR_DEV float vis(const Scene&, const Ray&, Rng&, const Options& o, int kind, bool) {
+        const int samples = kind == 0 ? o.visibilitySamples : o.lightSamples;
+        if (samples <= 0) return sameSamples(1.f, samples);
+        return sameSamples(tr[n++], samples);
+    }
+};
+
 // evaluate_F_ (VR/ReSTIRHelper.slang:91-423) + evaluate_L_in_volume (:443-496).
 // `extra` points at the extra-bounce records (3 floats each) the tap indexes.
-// VM / LM: visibility and lighting tracking methods (-1 = chosen at run time).
-template <int B, int VM, int LM>
+template <int B, class TP>
 VR_DEV v3 evaluateF(const Scene& S, Res tap, const float* extra, Ray ray, Rng& rng, const Options& opt, bool noReuse,
-                    bool isFinal) {
+                    bool isFinal, TP& tp) {
     bool isBg = tap.depth == kTMax;
     ray.tMax = tap.depth;
-    v3 F = V3(1.f);
+    v3 F = V3(1.f), Fn = V3(1.f);
+    auto D = [&]() -> const v3& { return TP::kSplit ? Fn : F; };  // what the early exits look at
     int maxInd = B > 1 ? decodeMaxIndirect(tap.sampledPixel) : 0;
     bool isSelfEmission = B > 1 ? (maxInd == 0 && tap.lightID == -3) : tap.lightID == -3;
     float vis = 1.f;
@@ -213,15 +270,13 @@ VR_DEV v3 evaluateF(const Scene& S, Res tap, const float* extra, Ray ray, Rng& r
     {
         float density = (isBg || noReuse) ? 1.f : densityWorld(S, 0, pW);
         if (density == 0.f) return V3(0.f);
-        if (!noReuse)
-            vis = visibilityT<VM>(S, ray, rng, opt.visibilitySamples, opt.visibilityMipLevel,
-                                  opt.visibilityUseLinearSampler != 0, opt.visibilityTrackingMethod,
-                                  opt.visibilityTStepScale);
+        if (!noReuse) vis = tp.vis(S, ray, rng, opt, 0, isFinal);
         v3 ss = isBg ? V3(1.f) : (isSelfEmission ? ld3(S.sigma_a) : ld3(S.sigma_s));
         if (noReuse && !isBg) ss = ss / S.sigma_t;
         F = F * (V3(vis * density) * ss);
+        if (TP::kSplit) Fn = Fn * (V3(density) * ss);
     }
-    if (anyPos(F)) {
+    if (anyPos(D())) {
         if (isBg) {
             F = F * ld3(S.env);
         } else if (isSelfEmission) {
@@ -250,8 +305,10 @@ VR_DEV v3 evaluateF(const Scene& S, Res tap, const float* extra, Ray ray, Rng& r
                     } else {
                         sr = {pW, wdir, 0.f, wdist};
                     }
-                    F = F * phaseHG(dot3(mi.wo, sr.d), mi.g);
-                    if (allZero(F)) return V3(0.f);
+                    const float ph = phaseHG(dot3(mi.wo, sr.d), mi.g);
+                    F = F * ph;
+                    if (TP::kSplit) Fn = Fn * ph;
+                    if (allZero(D())) return V3(0.f);
                     pW = wdist == -1.f ? wdir : at(sr, sr.tMax);
                     v3 ss;
                     float sd;
@@ -263,22 +320,23 @@ VR_DEV v3 evaluateF(const Scene& S, Res tap, const float* extra, Ray ray, Rng& r
                         sd = 1.f;
                     }
                     F = F * (V3(sd) * ss);
-                    if (curEmissive) F = F * (1.f / (dist * dist));
-                    if (allZero(F)) return V3(0.f);
+                    if (TP::kSplit) Fn = Fn * (V3(sd) * ss);
+                    if (curEmissive) {
+                        F = F * (1.f / (dist * dist));
+                        if (TP::kSplit) Fn = Fn * (1.f / (dist * dist));
+                    }
+                    if (allZero(D())) return V3(0.f);
                     float sv = 1.f;
-                    if (!noReuse)
-                        sv = visibilityT<VM>(S, sr, rng, opt.visibilitySamples, opt.visibilityMipLevel,
-                                        opt.visibilityUseLinearSampler != 0, opt.visibilityTrackingMethod,
-                                        opt.visibilityTStepScale);
+                    if (!noReuse) sv = tp.vis(S, sr, rng, opt, 0, isFinal);
                     F = F * sv;
                     mi.wo = -sr.d;
                     mi.p = pW;
-                    if (allZero(F)) return V3(0.f);
+                    if (allZero(D())) return V3(0.f);
                 }
                 if (scatterEmission) F = F * V3(0.f);
                 else mi = {at(sr, sr.tMax), -sr.d, S.g, true};
             }
-            if (!scatterEmission && anyPos(F)) {
+            if (!scatterEmission && anyPos(D())) {
                 Ray sh;
                 v3 Ld;
                 if (tap.lightID < 0) {
@@ -297,14 +355,19 @@ VR_DEV v3 evaluateF(const Scene& S, Res tap, const float* extra, Ray ray, Rng& r
                 } else {
                     return V3(0.f);
                 }
-                float Tr = visibilityT<LM>(S, sh, rng, opt.lightSamples, isFinal ? 0 : opt.lightingMipLevel,
-                                      opt.lightingUseLinearSampler != 0, opt.lightingTrackingMethod,
-                                      opt.lightingTStepScale);
+                float Tr = tp.vis(S, sh, rng, opt, 1, isFinal);
                 F = F * (Ld * Tr);
             }
         }
     }
     return F;
+}
+// The direct form: traces every transmittance ray where evaluateF needs it.
+template <int B, int VM, int LM>
+VR_DEV v3 evaluateF(const Scene& S, const Res& tap, const float* extra, const Ray& ray, Rng& rng, const Options& opt,
+                    bool noReuse, bool isFinal) {
+    DirectTrace<VM, LM> tp;
+    return evaluateF<B>(S, tap, extra, ray, rng, opt, noReuse, isFinal, tp);
 }
 
 // ComputeInitialSample (VR/ComputeInitialSample.slang:4-395). ex[] receives the

@@ -8,6 +8,7 @@
 // Reservoirs are 32 B per pixel (two float4 loads, coalesced across the wave),
 // extra-bounce records 12 B per bounce, features 8 B, colour 16 B.
 #pragma once
+#include <type_traits>
 #include "vr_launch.h"
 #include "vr_pass.hpp"
 
@@ -312,13 +313,22 @@ VR_TRACING_KERNEL void kTemporal(const Scene* __restrict__ sc, const Frame fr,
     storeRes(cur, px.id, output);
 }
 
-// SpatialReuse (VR/SpatialReuse.cs.slang:94-265) for one round.
-template <int B, int VM, int LM>
-VR_TRACING_KERNEL void kSpatial(const Scene* __restrict__ sc, const Frame fr,
-                                                int roundId, const int* __restrict__ offsets,
-                                                const ResStore* __restrict__ in,
-                                                ResStore* __restrict__ out, const float* __restrict__ inExtra,
-                                                float* __restrict__ outExtra, const float2* __restrict__ features) {
+enum { kDirect = 0, kRecord = 1, kReplay = 2 };
+template <int MODE, int VM, int LM>
+using TracePolicy = std::conditional_t<MODE == kDirect, DirectTrace<VM, LM>,
+                                       std::conditional_t<MODE == kRecord, RecordTrace, ReplayTrace>>;
+
+// SpatialReuse (VR/SpatialReuse.cs.slang:94-265) for one round. MODE: the
+// direct form, or the record / replay passes around a balanced trace of the
+// recorded rays (evaluateF's trace policies). The replay pass performs the
+// Talbot evaluations the record pass recorded; it uses their weights only
+// where the direct form would have made them (the record pass decides with
+// every visibility 1, a superset).
+template <int B, int VM, int LM, int MODE>
+VR_TRACING_KERNEL void kSpatial(const Scene* __restrict__ sc, const Frame fr, int roundId,
+                                const int* __restrict__ offsets, const ResStore* __restrict__ in,
+                                ResStore* __restrict__ out, const float* __restrict__ inExtra,
+                                float* __restrict__ outExtra, const float2* __restrict__ features, SplitBufs sb) {
     const Scene& S = *sc;
     const Frame& F = fr;
     occCacheLoad(S, F.occSlot[0], F.occSlot[1]);
@@ -331,6 +341,11 @@ VR_TRACING_KERNEL void kSpatial(const Scene* __restrict__ sc, const Frame fr,
     const int r2Seed = ((F.spatialRounds + 1) * F.frameCount + roundId) % 16;
     const int K = F.spatialSampleCount;
     const bool talbot = F.spatialMIS == 1;
+    TracePolicy<MODE, VM, LM> tp;
+    if constexpr (MODE == kRecord) { tp.out = sb.recs + (size_t)px.id * sb.cap; tp.n = 0; }
+    if constexpr (MODE == kReplay) { tp.tr = sb.trs + (size_t)px.id * sb.cap; tp.n = 0; }
+    uint32_t plan = 0;
+    if constexpr (MODE == kReplay) plan = sb.plan[px.id];
     Rng rng = rngCreate(px.x, px.y, (uint32_t)(numRounds * F.frameCount + rid));
     Res inCenter = loadRes(in, px.id, B);
     Res output = inCenter;
@@ -340,6 +355,11 @@ VR_TRACING_KERNEL void kSpatial(const Scene* __restrict__ sc, const Frame fr,
     bool hasSel = output.runningSum > 0.f;
     float2 cf = features[px.id];
     if (!(cf.y != 1.f)) {  // background pixel: pass through
+        if constexpr (MODE == kRecord) {
+            sb.counts[px.id] = 0;
+            sb.plan[px.id] = 0u;
+            return;
+        }
         storeRes(out, px.id, talbot ? inCenter : output);
         if (B > 1) {
             int mi = decodeMaxIndirect(output.sampledPixel);
@@ -359,13 +379,17 @@ VR_TRACING_KERNEL void kSpatial(const Scene* __restrict__ sc, const Frame fr,
         Res tap = loadRes(in, ty * W + tx, B);
         float misW = 1.f;
         if (s > 0 && tap.runningSum != 0.f) {
-            float ph = lum(evaluateF<B, VM, LM>(S, tap, inExtra, ray, rng, F.spatialOpt, false, false));
+            float ph = lum(evaluateF<B>(S, tap, inExtra, ray, rng, F.spatialOpt, false, false, tp));
             float w = ph / tap.p_y;
             if (isinf(w) || isnan(w)) w = 0.f;
             tap.runningSum *= w;
             tap.p_y = ph;
         }
-        if (talbot && tap.runningSum > 0.f) {
+        const bool talUse = talbot && tap.runningSum > 0.f;
+        bool talRun = talUse;
+        if constexpr (MODE == kRecord) { if (talUse) plan |= 1u << s; }
+        if constexpr (MODE == kReplay) talRun = (plan >> s) & 1u;
+        if (talRun) {
             float psum = 0.f, pqi = 0.f, k = 0.f;
             for (int j = 0; j < K; j++) {
                 int t2x = px.x + offs[2 * j], t2y = px.y + offs[2 * j + 1];
@@ -382,15 +406,22 @@ VR_TRACING_KERNEL void kSpatial(const Scene* __restrict__ sc, const Frame fr,
                     psum += py2 * m2;
                 } else {
                     Ray nr{ray.o, rayDir(F.cam, t2x, t2y, W, H), 0.f, tap.depth};
-                    float py = lum(evaluateF<B, VM, LM>(S, tap, inExtra, nr, rng, F.spatialOpt, false, false));
+                    float py = lum(evaluateF<B>(S, tap, inExtra, nr, rng, F.spatialOpt, false, false, tp));
                     if (isinf(py) || isnan(py)) py = 0.f;
                     psum += py * m2;
                 }
             }
-            if (psum > 0) misW = pqi * k / psum;
+            if (talUse && psum > 0) misW = pqi * k / psum;
         }
-        tap.runningSum *= misW;
-        if (resample(tap, output, rng)) hasSel = true;
+        if constexpr (MODE != kRecord) {
+            tap.runningSum *= misW;
+            if (resample(tap, output, rng)) hasSel = true;
+        }
+    }
+    if constexpr (MODE == kRecord) {
+        sb.counts[px.id] = tp.n;
+        sb.plan[px.id] = plan;
+        return;
     }
     if (B > 1 && hasSel) {
         int mi = decodeMaxIndirect(output.sampledPixel);

@@ -100,6 +100,23 @@ struct Frame {
     float prevView[16], prevProj[16];
     Options initialOpt, spatialOpt, finalOpt;
 };
+// A transmittance ray recorded by the record pass of a reuse kernel
+// (world-space origin and direction, tMax, and which options trace it:
+// 0 = the visibility options, 1 = the lighting options).
+struct alignas(16) TraceRec {
+    float ox, oy, oz, tMax;
+    float dx, dy, dz;
+    int32_t kind;
+};
+// Buffers of the record / trace / replay form of a reuse kernel.
+struct SplitBufs {
+    TraceRec* recs;   // cap records per pixel, in evaluation order
+    float* trs;       // their transmittances
+    int32_t* counts;  // records per pixel
+    uint32_t* plan;   // per pixel: bit s = candidate s's Talbot evaluations were recorded
+    int32_t cap;
+    int32_t pad_;
+};
 // Neighbour offsets of spatial reuse: [16 frame seeds][kMaxSpatialSamples][2] ints.
 static const int kMaxSpatialSamples = 32;
 

@@ -1,0 +1,212 @@
+// vr_walker.hpp -- resumable transmittance traversal for the balanced trace
+// of recorded rays (vr_k_queue.hip).
+//
+// A walker holds one ray's hierarchical-DDA and tracker state; advance() does
+// one unit of work (one DDA cell, or one ray-march sample / analytic cell
+// inside an occupied brick). The arithmetic is traverse<TrMarch> /
+// traverse<TrAnalytic> of vr_device.hpp step for step, so a walked ray's
+// transmittance is the direct form's.
+#pragma once
+#include "vr_device.hpp"
+
+namespace vr {
+
+// ------------------------------------------------------------------ resumable traversal
+// traverse<TrMarch> (TK = 2) and traverse<TrAnalytic> (TK = 1) of vr_device.hpp
+// with the loop state held in the struct: VolumeTrackingGVDB
+// (VR/VolumeUtils.slang:171-282) over the adapters of
+// VR/VolumeTrackingAdapterGVDB.slang:20-208. The state is kept lean (the DDA's
+// axis mask and exit distance are recomputed rather than stored) so many
+// waves fit a SIMD.
+template <int TK>
+struct Walker {
+    Dda dda;        // pos / dir: the index-space ray (vertex-shifted for linear analytic tracking)
+    float tNear, tFar, tMax1, tMax2, tStep, Tr, t;
+    int lev, iter, it, node;  // node: current 128^3 block, x | y << 8 | z << 16
+    int slot;
+    uint32_t brick;
+    bool linear, active, inBk;  // inBk: inside an occupied brick
+    v3 q;                       // ray marching: brick-local sample point
+    v3 vmin, lDel, lSide;       // analytic: brick minimum and the cell walk inside the brick
+    i3v lp;
+    float ltx;
+
+    VR_DEV bool outside(int res) const {
+        return dda.p.x < 0 || dda.p.y < 0 || dda.p.z < 0 || dda.p.x > res || dda.p.y > res || dda.p.z > res;
+    }
+    VR_DEV void levelUp() {
+        while (lev <= 2 && dda.tx > (lev == 1 ? tMax1 : tMax2)) {
+            lev++;
+            if (lev <= 2) dda.prepare(V3(0.f), 128.f);
+        }
+    }
+    VR_DEV Dda leaf() const {
+        Dda d;
+        d.pos = dda.pos; d.dir = dda.dir;
+        d.tDel = lDel; d.tSide = lSide; d.p = lp; d.tx = ltx;
+        return d;
+    }
+    VR_DEV void setLeaf(const Dda& d) { lDel = d.tDel; lSide = d.tSide; lp = d.p; ltx = d.tx; }
+
+    // Starts a traversal of world-space ray rWorld through grid G (slot `slot`).
+    // When the ray misses the grid the walk is complete on return (active ==
+    // false, Tr = 1, what end() gives for an adapter that never started).
+    VR_DEV void begin(const Scene& S, const Grid& G, int slot_, const Ray& rWorld, bool linear_, float tStepScale) {
+        linear = linear_;
+        slot = slot_;
+        const bool vc = TK == 1 && linear;
+        const float eps = 0.01f;
+        Ray ray{xformP(G.w2i, rWorld.o), xformV(G.w2i, rWorld.d), rWorld.tMin, rWorld.tMax};
+        if (vc) ray.o = ray.o - 0.5f;
+        if constexpr (TK == 2) {
+            int eff = slot_ >= 19 ? slot_ - 19 : slot_;
+            eff = eff >= 8 ? eff - 8 : eff;
+            tStep = S.tStep * S.worldScaling * tStepScale * (float)(eff + 1);
+        }
+        Tr = 0.f;
+        inBk = false;
+        if (!intersectBounds(G, ray, tNear, tFar, vc)) {
+            Tr = 1.f;
+            active = false;
+            return;
+        }
+        lev = 2;
+        tMax2 = tFar;
+        tMax1 = 0.f;
+        node = 0;
+        dda.setFromRay(ray.o, ray.d, tNear + eps);
+        dda.prepare(V3(0.f), 128.f);
+        if (vc) {
+            int k = 0;
+            while (k++ < 3 && outside(32)) { dda.next(); dda.step(); dda.tx += eps; }
+        }
+        iter = 0;
+        active = true;
+    }
+    VR_DEV void finish() {
+        Tr = expf(Tr);
+        active = false;
+    }
+    // After the brick: the step the DDA would have taken right after entering it
+    // (its axis mask is recomputed from the unchanged side distances).
+    VR_DEV void leaveBrick() {
+        const float eps = 0.01f;
+        inBk = false;
+        dda.next();
+        dda.step();
+        dda.tx += eps;
+        levelUp();
+        iter++;
+    }
+    // One hierarchical-DDA cell; when it enters an occupied brick, also the
+    // brick's first tracker step. G must be the grid begin() was given.
+    VR_DEV void advance(const Scene& S, const Grid& G) {
+        const float eps = 0.01f;
+        if (!inBk) {
+            if (!(iter < 4096 && lev > 0 && lev <= 2 && !outside(lev == 2 ? 32 : 16))) {
+                finish();
+                return;
+            }
+            dda.next();
+            bool occupied;
+            int cx, cy, cz;
+            if (lev == 2) {
+                int b = (((dda.p.z << 5) + dda.p.y) << 5) + dda.p.x;
+                occupied = b < 32768 && nodeAt(G, b & 31, (b >> 5) & 31, b >> 10);
+                cx = b & 31; cy = (b >> 5) & 31; cz = b >> 10;
+            } else {
+                int b = (((dda.p.z << 4) + dda.p.y) << 4) + dda.p.x;
+                cx = (node & 255) * 16 + (b & 15);
+                cy = ((node >> 8) & 255) * 16 + ((b >> 4) & 15);
+                cz = (node >> 16) * 16 + (b >> 8);
+                occupied = b < 4096;
+            }
+            uint32_t bk = kEmpty;
+            if (occupied && lev == 1) {
+                bk = brickLookup(G, slot, cx, cy, cz);
+                occupied = bk != kEmpty;
+            }
+            if (occupied && lev == 1) {
+                brick = bk;
+                const v3 bmin = V3((float)(cx * 8), (float)(cy * 8), (float)(cz * 8));
+                t = dda.tx - eps;
+                if constexpr (TK == 2) {
+                    t = tNear + (floorf((t - tNear) / tStep) + 0.5f) * tStep;
+                    if (t < dda.tx) t += tStep;
+                    q = at(Ray{dda.pos, dda.dir, 0.f, 0.f}, t) - bmin;
+                } else {
+                    vmin = bmin;
+                    Dda dl = dda;
+                    dl.prepareLeaf(vmin);
+                    setLeaf(dl);
+                }
+                it = 0;
+                inBk = true;
+            } else {
+                if (occupied) {
+                    lev = 1;
+                    node = cx | (cy << 8) | (cz << 16);
+                    tMax1 = dda.ty;
+                    dda.prepare(V3((float)(cx * 128), (float)(cy * 128), (float)(cz * 128)), 8.f);
+                } else {
+                    dda.step();
+                    dda.tx += eps;
+                }
+                levelUp();
+                iter++;
+                return;
+            }
+        }
+        BrickCtx bc;
+        bc.brick = brick;
+        if constexpr (TK == 2) {
+            // TrMarch::brickStep, one sample
+            if (!(it < 128 && inBrick(q))) { leaveBrick(); return; }
+            if (t >= tFar) { finish(); return; }
+            float st = sampleBrick(G, brick, q, linear) * S.densityScale * S.sigma_t;
+            Tr += -st * 1.f * tStep;
+            const v3 dq = 1.f * tStep * dda.dir;
+            q = q + dq;
+            t += 1.f * tStep;
+            it++;
+        } else {
+            // TrAnalytic::brickStep, one cell
+            Dda dl = leaf();
+            if (!(it < 128 && leafInside(dl))) { leaveBrick(); return; }
+            bc.vmin = vmin;
+            dl.next();
+            float maxDeltaT = dl.ty - t;
+            if (linear) {
+                Poly c = cellPoly(S, G, dl, bc, dda.dir);
+                float td = fminf(tFar - t, maxDeltaT);
+                float t2 = td * td, t3 = t2 * td, t4 = t2 * t2;
+                Tr += -(c.c3 * t4 / 4 + c.c2 * t3 / 3 + c.c1 * t2 / 2 + c.c0 * td);
+            } else {
+                float st = voxelDensity(S, G, bc, dl.p) * S.sigma_t;
+                Tr += -fminf(tFar - t, maxDeltaT) * st;
+            }
+            if (t + maxDeltaT >= tFar) { finish(); return; }
+            t += maxDeltaT;
+            dl.step();
+            setLeaf(dl);
+            it++;
+        }
+    }
+};
+
+// Starts / advances walker w on the grid of its ray kind. When both kinds use
+// the same grid (uniform flag `oneSlot`), the grid descriptor is read once per
+// wave through scalar loads instead of per lane.
+template <int TK>
+VR_DEV void walkBegin(Walker<TK>& w, const Scene& S, bool oneSlot, int uniformSlot, int laneSlot, const Ray& r,
+                      bool linear, float tss) {
+    if (oneSlot) w.begin(S, S.grid[uniformSlot], uniformSlot, r, linear, tss);
+    else w.begin(S, S.grid[laneSlot], laneSlot, r, linear, tss);
+}
+template <int TK>
+VR_DEV void walkAdvance(Walker<TK>& w, const Scene& S, bool oneSlot, int uniformSlot) {
+    if (oneSlot) w.advance(S, S.grid[uniformSlot]);
+    else w.advance(S, S.grid[w.slot]);
+}
+
+}  // namespace vr
