@@ -122,9 +122,26 @@ typedef struct vrestir_volume_desc {
     float sigmaS[3];             /* sigma_a + sigma_s must be grey (SC/Scene.cpp:3247) */
     float g;                     /* Henyey-Greenstein anisotropy */
     float densityScale;          /* densityScaleFactorByScaling */
-    const float* temperature;    /* optional emission grid (same dims) or NULL */
+    const float* temperature;    /* optional temperature grid (same dims and transform) or NULL: the volume
+                                    emits LeScale * blackbody(min(6400, (T - temperatureCutOff) *
+                                    temperatureScale)) (VR/VolumeBase.slang:211-232; needs
+                                    vrestir_set_blackbody_lut) */
     float LeScale, temperatureCutOff, temperatureScale;
 } vrestir_volume_desc;
+
+/* A lat-long environment map (Scene::setEnvMap / setEnvMapIntensity / setEnvMapRotation
+ * of the Mogwai scripts; LT/EnvMap.cpp:57-122, LT/EnvMapSampler.cpp:61-110). u runs
+ * with atan2(x, -z), v from +y (v = 0) to -y (world_to_latlong_map,
+ * UT/Math/MathHelpers.slang:92-99); bilinear filter, wrap in u, clamp in v. */
+typedef struct vrestir_env_map_desc {
+    int32_t width, height;       /* texels */
+    const float* rgb;            /* host memory, width*height RGB triples, row 0 at v = 0; finite, >= 0 */
+    float intensity;
+    float tint[3];
+    float rotationDegrees[3];    /* XYZ Euler angles: local -> world = Rz * Ry * Rx */
+    int32_t importanceDim;       /* importance map side, power of two in [2, 2048]; 0 = 512 (the reference's) */
+    int32_t importanceSpp;       /* samples per importance texel: 1, 4, 16, 64, 256, 1024; 0 = 64 */
+} vrestir_env_map_desc;
 
 /* Point light (LT/LightHelpers.slang:223-244): Li = intensity / dist^2. */
 typedef struct vrestir_point_light {
@@ -157,6 +174,22 @@ int32_t vrestir_load_vbx(vrestir_handle* h, const char* pathPrefix, int32_t numM
 int32_t vrestir_set_lights(vrestir_handle* h, const vrestir_point_light* lights, int32_t count,
                            const float envRadiance[3]);
 int32_t vrestir_set_camera(vrestir_handle* h, const vrestir_camera* cam);
+/* Environment map in place of the constant envRadiance of vrestir_set_lights; NULL
+ * returns to the constant. The importance map is built on the device. */
+int32_t vrestir_set_env_map(vrestir_handle* h, const vrestir_env_map_desc* desc);
+/* EnvMap::setRotation: the orientation from the next frame on; temporal reuse
+ * evaluates the previous frame's samples under the previous frame's orientation
+ * (EnvMap prevData, SC/Scene.cpp:761-775). Also the per-frame env animation of
+ * VR/VolumetricReSTIR.cpp:788-799. */
+int32_t vrestir_set_env_rotation(vrestir_handle* h, const float degreesXYZ[3]);
+int32_t vrestir_set_env_intensity(vrestir_handle* h, float intensity, const float tint[3]); /* setIntensity/setTint */
+/* Blackbody table for volume emission (SC/Scene.cpp:2877-2893 loads
+ * LUT/BlackBodyRadiationRGB_50K-6400K.txt, 128 RGB rows); rows in [1, 256]. */
+int32_t vrestir_set_blackbody_lut(vrestir_handle* h, const float* rgb, int32_t rows);
+/* Temperature grid <pathPrefix>_temperature.vbx (own transform) for the bound
+ * volume (addGVDBVolume hasEmission=True, SC/Scene.cpp:2789-2811). */
+int32_t vrestir_load_vbx_emission(vrestir_handle* h, const char* pathPrefix, float LeScale, float temperatureCutOff,
+                                  float temperatureScale);
 
 /* ---- options: updateDict / getScriptingDictionary (VR/VolumetricReSTIR.cpp:1280-1347).
  *      Any change resets the temporal history and the frame counter, like mOptionsChanged. ---- */
@@ -218,7 +251,10 @@ enum {
     VRESTIR_TRACK_SAMPLE_CELL_BY_DENSITY = 7,
     VRESTIR_TRACK_FEATURE_MARCH = 8,
     VRESTIR_TRACK_DENSITY_AT_POINT = 9,
-    VRESTIR_TRACK_RNG_STREAM = 10          /* first 8 uint32 of the sample generator (bit-cast to float) */
+    VRESTIR_TRACK_RNG_STREAM = 10,         /* first 8 uint32 of the sample generator (bit-cast to float) */
+    VRESTIR_TRACK_ENV_EVAL = 11,           /* environment radiance towards dir: out rgb */
+    VRESTIR_TRACK_ENV_SAMPLE = 12,         /* importance sample at u = origin.xy: out dir, pdf, Le, lightID */
+    VRESTIR_TRACK_EMISSION = 13            /* emission at the point origin: out rgb, temperature */
 };
 int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t mode, int32_t mipLevel,
                             int32_t useLinearSampler, float tStepScale, int32_t numSamples, int32_t n,

@@ -70,6 +70,21 @@ def lib():
         "orc_camera_ray": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p]),
         "orc_reproject": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p]),
         "orc_trace_bricks": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p]),
+        "orc_set_temperature": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_float, c_float,
+                                        c_float]),
+        "orc_set_blackbody_lut": (c_int, [c_void_p, c_void_p, c_int]),
+        "orc_set_env_map": (c_int, [c_void_p, c_int, c_int, c_void_p, c_float, c_void_p, c_void_p, c_int, c_int]),
+        "orc_set_env_rotation": (c_int, [c_void_p, c_void_p]),
+        "orc_set_env_intensity": (c_int, [c_void_p, c_float, c_void_p]),
+        "orc_env_rotation": (None, [c_void_p, c_int, c_void_p, c_void_p]),
+        "orc_env_eval": (None, [c_void_p, c_void_p, c_void_p]),
+        "orc_env_sample": (None, [c_void_p, c_float, c_float, c_void_p]),
+        "orc_env_pdf": (c_float, [c_void_p, c_void_p]),
+        "orc_env_importance": (c_int, [c_void_p, c_int, c_void_p]),
+        "orc_latlong": (None, [c_void_p, c_void_p]),
+        "orc_dir_to_oct": (None, [c_void_p, c_void_p]),
+        "orc_emission": (None, [c_void_p, c_void_p, c_void_p]),
+        "orc_blackbody": (None, [c_void_p, c_float, c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -104,14 +119,88 @@ class Oracle:
     def set_scene(self, spec, frame=0):
         d = spec.volume_desc()
         assert self.L.orc_set_volume(self.h, ctypes.byref(d)) == 0
+        if spec.temperature is not None:
+            lut, rows = spec.blackbody_array()
+            assert self.L.orc_set_blackbody_lut(self.h, lut.ctypes.data, rows) == 0
         arr, n, env = spec.light_array()
         assert self.L.orc_set_lights(self.h, arr, n, env) == 0
+        if spec.env_map is not None:
+            self.set_env_map(spec.env_map, spec.env_intensity, spec.env_tint, spec.env_rotation_at(0),
+                             spec.env_importance_dim, spec.env_importance_spp)
+        else:
+            self.set_env_map(None)
         self.set_camera(spec, frame)
         self.set_params(spec.make_params())
 
     def set_camera(self, spec, frame=0):
         c = spec.camera_struct(frame)
         assert self.L.orc_set_camera(self.h, ctypes.byref(c), spec.width, spec.height) == 0
+        if spec.env_map is not None:
+            self.set_env_rotation(spec.env_rotation_at(frame))
+
+    # ---- environment map and emission
+    def set_env_map(self, rgb, intensity=1.0, tint=(1.0, 1.0, 1.0), rotation=(0.0, 0.0, 0.0), dim=0, spp=0):
+        if rgb is None:
+            assert self.L.orc_set_env_map(self.h, 0, 0, None, 1.0, None, None, 0, 0) == 0
+            return
+        arr = np.ascontiguousarray(rgb, dtype=np.float32)
+        t = (c_float * 3)(*tint)
+        r = (c_float * 3)(*rotation)
+        assert self.L.orc_set_env_map(self.h, arr.shape[1], arr.shape[0], arr.ctypes.data, intensity, t, r, dim,
+                                      spp) == 0
+
+    def set_env_rotation(self, deg):
+        assert self.L.orc_set_env_rotation(self.h, (c_float * 3)(*deg)) == 0
+
+    def env_rotation(self, prev=False):
+        w = np.zeros(9, dtype=np.float32)
+        l = np.zeros(9, dtype=np.float32)
+        self.L.orc_env_rotation(self.h, 1 if prev else 0, w.ctypes.data, l.ctypes.data)
+        return w.reshape(3, 3), l.reshape(3, 3)
+
+    def env_eval(self, d):
+        d = np.ascontiguousarray(d, dtype=np.float32)
+        out = np.zeros(3, dtype=np.float32)
+        self.L.orc_env_eval(self.h, d.ctypes.data, out.ctypes.data)
+        return out
+
+    def env_sample(self, u, v):
+        out = np.zeros(8, dtype=np.float32)
+        self.L.orc_env_sample(self.h, u, v, out.ctypes.data)
+        return out
+
+    def env_pdf(self, d):
+        d = np.ascontiguousarray(d, dtype=np.float32)
+        return float(self.L.orc_env_pdf(self.h, d.ctypes.data))
+
+    def env_importance(self, mip):
+        n = self.L.orc_env_importance(self.h, mip, None)
+        assert n > 0
+        out = np.zeros((n, n), dtype=np.float32)
+        self.L.orc_env_importance(self.h, mip, out.ctypes.data)
+        return out
+
+    def set_temperature(self, t, index_to_world, fp16, le_scale, cutoff, scale):
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        m = np.ascontiguousarray(index_to_world, dtype=np.float32)
+        nz, ny, nx = t.shape
+        assert self.L.orc_set_temperature(self.h, t.ctypes.data, nx, ny, nz, m.ctypes.data, 1 if fp16 else 0,
+                                          le_scale, cutoff, scale) == 0
+
+    def set_blackbody(self, lut):
+        lut = np.ascontiguousarray(lut, dtype=np.float32)
+        assert self.L.orc_set_blackbody_lut(self.h, lut.ctypes.data, lut.shape[0]) == 0
+
+    def emission(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        out = np.zeros(4, dtype=np.float32)
+        self.L.orc_emission(self.h, p.ctypes.data, out.ctypes.data)
+        return out
+
+    def blackbody(self, temperature):
+        out = np.zeros(3, dtype=np.float32)
+        self.L.orc_blackbody(self.h, temperature, out.ctypes.data)
+        return out
 
     def execute(self, W, H, row_begin=0, row_end=0, nthreads=1):
         out = np.zeros((H, W, 4), dtype=np.float32)
@@ -161,6 +250,20 @@ class Oracle:
         rc = self.L.orc_grid_info(self.h, slot, dims, ctypes.byref(mx), ctypes.byref(nb))
         assert rc == 0
         return tuple(dims), mx.value, nb.value
+
+
+def latlong(d):
+    d = np.ascontiguousarray(d, dtype=np.float32)
+    out = np.zeros(2, dtype=np.float32)
+    lib().orc_latlong(d.ctypes.data, out.ctypes.data)
+    return out
+
+
+def dir_to_oct(d):
+    d = np.ascontiguousarray(d, dtype=np.float32)
+    out = np.zeros(2, dtype=np.float32)
+    lib().orc_dir_to_oct(d.ctypes.data, out.ctypes.data)
+    return out
 
 
 def rng_stream(px, py, sample_number, n):

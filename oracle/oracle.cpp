@@ -59,6 +59,9 @@ struct Oracle {
     SamplingOptions initialOpt, spatialOpt, finalOpt;
     int totalRoundId = 0;
     double stageSeconds[6] = {0, 0, 0, 0, 0, 0};
+    // environment orientation: set now, used by the last frame, and this frame's [current, previous]
+    EnvXf envCur{}, envLast{}, frameEnv[2]{};
+    bool haveEnvLast = false;
 };
 
 static int maxB(const Oracle& O) { return O.P.maxBounces; }
@@ -83,7 +86,7 @@ static void forRows(Oracle& O, F&& fn) {
 
 // ---------------------------------------------------------------------------
 // evaluate_F_ (VR/ReSTIRHelper.slang:91-423), no surface scene, no vertex reuse.
-static f3 evaluateF(const Oracle& O, Reservoir tap, const f3* extraBuf, Ray ray, SampleGen& sg,
+static f3 evaluateF(const Oracle& O, const EnvXf& X, Reservoir tap, const f3* extraBuf, Ray ray, SampleGen& sg,
                     const SamplingOptions& opt, bool noReuse, bool isFinalShading) {
     const Medium& m = O.S.med;
     int mipOff = 0;  // previous-frame grids are not bound (static volumes)
@@ -111,9 +114,9 @@ static f3 evaluateF(const Oracle& O, Reservoir tap, const f3* extraBuf, Ray ray,
     int bounceId = 0;
     if (anyPositive(F)) {
         if (isBg) {
-            F = F * O.S.env;
+            F = F * envEval(O.S, X, ray.dir);
         } else if (isSelfEmission) {
-            F = F * mk3(0.f);  // emission grids are not bound in this oracle build
+            F = F * emissionWorld(m, pW);
         } else {
             bool isScatterSelfEmission = false;
             if (maxB(O) > 1 && maxInd > 0) {
@@ -164,7 +167,7 @@ static f3 evaluateF(const Oracle& O, Reservoir tap, const f3* extraBuf, Ray ray,
                     mi.p = pW;
                     if (allZero(F)) return mk3(0.f);
                 }
-                if (isScatterSelfEmission) F = F * mk3(0.f);
+                if (isScatterSelfEmission) F = F * emissionWorld(m, pW);
                 else mi = {scatterRay.at(scatterRay.tMax), -scatterRay.dir, m.g, true};
             }
             if (!isScatterSelfEmission && anyPositive(F)) {
@@ -173,9 +176,9 @@ static f3 evaluateF(const Oracle& O, Reservoir tap, const f3* extraBuf, Ray ray,
                 Ray sh{};
                 f3 Ld = mk3(0.f);
                 if (tap.lightID < 0) {
-                    f3 wi = decodeLightUV(tap.lightUV, tap.lightID);
+                    f3 wi = decodeLightUV(O.S, X, tap.lightUV, tap.lightID);
                     sh = {mi.p, wi, 0, kRayTMax};
-                    Ld = O.S.env * phaseHG(dot(mi.wo, wi), mi.g);
+                    Ld = envEval(O.S, X, wi) * phaseHG(dot(mi.wo, wi), mi.g);
                 } else if (tap.lightID < (int)O.S.lights.size()) {
                     SceneLightSample ls = analyticLightSample(O.S, tap.lightID, mi.p);
                     sh = {mi.p, ls.rayDir, 0, ls.rayDistance};
@@ -193,9 +196,9 @@ static f3 evaluateF(const Oracle& O, Reservoir tap, const f3* extraBuf, Ray ray,
     return F;
 }
 
-static float evaluatePHat(const Oracle& O, const Ray& ray, SampleGen& sg, const f3* extraBuf,
+static float evaluatePHat(const Oracle& O, const EnvXf& X, const Ray& ray, SampleGen& sg, const f3* extraBuf,
                           const SamplingOptions& opt, const Reservoir& tap) {
-    f3 F = evaluateF(O, tap, extraBuf, ray, sg, opt, false, false);
+    f3 F = evaluateF(O, X, tap, extraBuf, ray, sg, opt, false, false);
     return luminance(F);
 }
 
@@ -266,9 +269,10 @@ static Reservoir computeInitialSample(const Oracle& O, Ray primaryRay, float pre
                 out.lightUV = {0, 0};
                 float outLightPdf = 0.f, outVis = 1.f;
                 f3 Ld = mk3(0.f), Le = mk3(0.f);
+                if (m.hasEmission && actualDensity > 0.f) Le = emissionWorld(m, mi.p);
                 f3 oma = mk3(1.f) - albedo;
                 bool shouldComputeLightVisibility = opt.lightSamples == 0 ? false : true;
-                Ld = sampleDirectLighting(O.S, sg, outLightPdf, mi, opt.useEnv, opt.useAnalytic, opt.useEmissive,
+                Ld = sampleDirectLighting(O.S, O.frameEnv[0], sg, outLightPdf, mi, opt.useEnv, opt.useAnalytic, opt.useEmissive,
                                           shouldComputeLightVisibility, opt.lightingMipLevel,
                                           opt.lightingTrackingMethod, opt.lightingUseLinearSampler, opt.lightSamples,
                                           opt.lightingTStepScale, out.lightID, out.lightUV, outVis);
@@ -320,7 +324,7 @@ static Reservoir computeInitialSample(const Oracle& O, Ray primaryRay, float pre
                     }
                 }
             } else {
-                f3 Le = O.S.env;
+                f3 Le = envEval(O.S, O.frameEnv[0], ray.dir);
                 pathPHat *= Tr;
                 float p_y = pathPHat * luminance(Le);
                 out.runningSum = out.p_y == 0.f ? 0.f : p_y / out.p_y;
@@ -346,8 +350,12 @@ static f3 pathTrace(const Oracle& O, Ray ray, SampleGen& sg, int lightSamples, b
         sampleDecomposition(m, ray, sg, mi, mip);
         if (mi.valid) {
             f3 albedo = m.sigma_s / m.sigma_t;
+            {
+                f3 Le = emissionWorld(m, mi.p);
+                L = L + Le * (mk3(1.f) - albedo) * beta;
+            }
             beta = beta * albedo;
-            f3 Ld = directLighting(O.S, sg, mi, lightSamples, useEnv, useAnalytic, useEmissive, mip, 3);
+            f3 Ld = directLighting(O.S, O.frameEnv[0], sg, mi, lightSamples, useEnv, useAnalytic, useEmissive, mip, 3);
             L = L + beta * Ld;
             f3 wo = -ray.dir, wi = mk3(0.f);
             if (maxBounces > 1) {
@@ -364,7 +372,7 @@ static f3 pathTrace(const Oracle& O, Ray ray, SampleGen& sg, int lightSamples, b
                 }
             }
         } else {
-            f3 Le = O.S.env;
+            f3 Le = envEval(O.S, O.frameEnv[0], ray.dir);
             if (bounce == 0) L = L + beta * Le;
             bounce = maxBounces;
         }
@@ -432,7 +440,7 @@ static void stageTrace(Oracle& O) {
         }
         // p-hat re-evaluation with the spatial options (VR/TraceRays.cs.slang:169-183)
         fin.extraBounceStartId = 0;
-        float ph = evaluatePHat(O, ray, sg, finalExtra, O.spatialOpt, fin);
+        float ph = evaluatePHat(O, O.frameEnv[0], ray, sg, finalExtra, O.spatialOpt, fin);
         if (fin.runningSum > 0.f) {
             fin.runningSum *= fin.p_y == 0.f ? 0.f : ph / fin.p_y;
             fin.p_y = ph;
@@ -552,7 +560,7 @@ static void stageTemporal(Oracle& O, int curIdx) {
                 if (std::isnan(taps[i].runningSum) || std::isinf(taps[i].runningSum)) taps[i].runningSum = 0.f;
                 if (i > 0 && taps[i].runningSum != 0.f) {
                     // resampleNeighbor (VR/ReSTIRHelper.slang:582-598)
-                    float ph = evaluatePHat(O, ray, sg, O.temporalExtra.data(), O.spatialOpt, taps[i]);
+                    float ph = evaluatePHat(O, O.frameEnv[0], ray, sg, O.temporalExtra.data(), O.spatialOpt, taps[i]);
                     float w = ph / taps[i].p_y;
                     if (std::isinf(w) || std::isnan(w)) w = 0.f;
                     taps[i].runningSum *= w;
@@ -582,8 +590,10 @@ static void stageTemporal(Oracle& O, int curIdx) {
                         nr.tMax = usedDepth;
                         float backup = taps[i].depth;
                         taps[i].depth = usedDepth;
-                        float py = evaluatePHat(O, nr, sg, i == 0 ? curExtra.data() : O.temporalExtra.data(),
-                                                O.spatialOpt, taps[i]);
+                        // isLastFrame = j > 0 (VR/TemporalReuse.cs.slang:335-338)
+                        float py = evaluatePHat(O, O.frameEnv[j > 0 ? 1 : 0], nr, sg,
+                                                i == 0 ? curExtra.data() : O.temporalExtra.data(), O.spatialOpt,
+                                                taps[i]);
                         taps[i].depth = backup;
                         if (std::isinf(py) || std::isnan(py)) py = 0.f;
                         psum += py * cM;
@@ -680,7 +690,7 @@ static void stageSpatial(Oracle& O, int roundId) {
             if (s > 0) {
                 // resampleNeighborSpatialReuse (VR/ReSTIRHelper.slang:563-580)
                 if (tap.runningSum != 0.f) {
-                    float ph = evaluatePHat(O, ray, sg, inExtra.data(), O.spatialOpt, tap);
+                    float ph = evaluatePHat(O, O.frameEnv[0], ray, sg, inExtra.data(), O.spatialOpt, tap);
                     float w = ph / tap.p_y;
                     if (std::isinf(w) || std::isnan(w)) w = 0.f;
                     tap.runningSum *= w;
@@ -702,7 +712,7 @@ static void stageSpatial(Oracle& O, int roundId) {
                         psum += tap2.p_y * tap2.M;
                     } else {
                         Ray nr{ray.origin, rayDir(O.S.cam, t2x, t2y, O.W, O.H), 0, tap.depth};
-                        float py = evaluatePHat(O, nr, sg, inExtra.data(), O.spatialOpt, tap);
+                        float py = evaluatePHat(O, O.frameEnv[0], nr, sg, inExtra.data(), O.spatialOpt, tap);
                         if (std::isinf(py) || std::isnan(py)) py = 0.f;
                         psum += py * tap2.M;
                     }
@@ -754,7 +764,7 @@ static void stageFinal(Oracle& O) {
             if (r.runningSum > 0.f) {
                 f3 d = rayDir(O.S.cam, x, y, O.W, O.H);
                 Ray ray{O.S.cam.pos, d, 0, r.depth};
-                f3 c = evaluateF(O, r, O.extra[idx].data(), ray, sg, O.finalOpt, noReuse, true);
+                f3 c = evaluateF(O, O.frameEnv[0], r, O.extra[idx].data(), ray, sg, O.finalOpt, noReuse, true);
                 float Wt = r.p_y == 0.0f ? 1.f : r.runningSum / (r.p_y * r.M);
                 outC = outC + c * Wt;
             }
@@ -814,11 +824,16 @@ static void prepareFrame(Oracle& O, int W, int H) {
                   p.useEnvironmentLights != 0, p.useAnalyticLights != 0, p.useEmissiveLights != 0,
                   p.vertexReuseStartBounce};
     O.totalRoundId = 0;
+    // EnvMap data / prevData of this frame (SC/Scene.cpp:761-775)
+    O.frameEnv[0] = O.envCur;
+    O.frameEnv[1] = O.haveEnvLast ? O.envLast : O.envCur;
 }
 
 static void endFrame(Oracle& O) {
     O.temporalAccumulated = true;
     O.S.prevCam = O.S.cam;
+    O.envLast = O.envCur;
+    O.haveEnvLast = true;
     O.frameCount++;
 }
 
@@ -878,6 +893,9 @@ extern "C" {
 void* orc_create(void) {
     Oracle* O = new Oracle();
     vrestir_default_params_oracle(&O->P);
+    const float zero[3] = {0.f, 0.f, 0.f};
+    O->envCur = O->envLast = envRotation(zero);
+    O->frameEnv[0] = O->frameEnv[1] = O->envCur;
     return O;
 }
 void orc_destroy(void* h) { delete (Oracle*)h; }
@@ -977,9 +995,125 @@ int orc_set_volume(void* h, const vrestir_volume_desc* d) {
     auto rd = [&](int r) { return X[r * 4] * X[r * 4] + X[r * 4 + 1] * X[r * 4 + 1] + X[r * 4 + 2] * X[r * 4 + 2]; };
     m.superVoxelDiag = 8.f * std::sqrt(rd(0) + rd(1) + rd(2));
     m.worldScaling = 1.f;
+    m.hasEmission = false;
+    if (d->temperature) {
+        int rc = orc_set_temperature(h, d->temperature, d->nx, d->ny, d->nz, d->indexToWorld, fp16 ? 1 : 0,
+                                     d->LeScale, d->temperatureCutOff, d->temperatureScale);
+        if (rc) return rc;
+    }
     O->volumeSet = true;
     O->optionsChanged = true;
     return VRESTIR_OK;
+}
+
+// Temperature grid (slot 16) of an emissive volume; values <= 0 are stored as 0 and
+// the dimensions padded with zeros to whole bricks.
+int orc_set_temperature(void* h, const float* t, int nx, int ny, int nz, const float* indexToWorld, int fp16,
+                        float LeScale, float cutoff, float scale) {
+    Oracle* O = (Oracle*)h;
+    if (!t || nx <= 0 || ny <= 0 || nz <= 0) return VRESTIR_EINVAL;
+    Medium& m = O->S.med;
+    DenseGrid g;
+    g.nx = (nx + 7) / 8 * 8; g.ny = (ny + 7) / 8 * 8; g.nz = (nz + 7) / 8 * 8;
+    g.v.assign((size_t)g.nx * g.ny * g.nz, 0.f);
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                float v = t[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)];
+                g.v[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] = v > 0.f ? v : 0.f;
+            }
+    buildSlot(m.grid[kTemperatureSlot], g, indexToWorld, 1, fp16 != 0);
+    m.hasEmission = true;
+    m.LeScale = LeScale;
+    m.tempCutoff = cutoff;
+    m.tempScale = scale;
+    return VRESTIR_OK;
+}
+
+int orc_set_blackbody_lut(void* h, const float* rgb, int rows) {
+    Oracle* O = (Oracle*)h;
+    if (rows < 1 || rows > 256 || !rgb) return VRESTIR_EINVAL;
+    O->S.med.lut.clear();
+    for (int i = 0; i < rows; i++) O->S.med.lut.push_back({rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]});
+    return VRESTIR_OK;
+}
+
+int orc_set_env_map(void* h, int w, int hgt, const float* rgb, float intensity, const float tint[3],
+                    const float rotationDegrees[3], int impDim, int impSpp) {
+    Oracle* O = (Oracle*)h;
+    EnvMapState& E = O->S.envMap;
+    if (!rgb) { E = EnvMapState(); return VRESTIR_OK; }
+    if (w < 1 || hgt < 1) return VRESTIR_EINVAL;
+    E = EnvMapState();
+    E.textured = true;
+    E.w = w; E.h = hgt;
+    E.rgb.assign(rgb, rgb + (size_t)w * hgt * 3);
+    E.scale = {intensity * tint[0], intensity * tint[1], intensity * tint[2]};
+    buildImportance(E, impDim ? impDim : 512, impSpp ? impSpp : 64);
+    O->envCur = O->envLast = envRotation(rotationDegrees);
+    return VRESTIR_OK;
+}
+int orc_set_env_rotation(void* h, const float deg[3]) {
+    ((Oracle*)h)->envCur = envRotation(deg);
+    return VRESTIR_OK;
+}
+int orc_set_env_intensity(void* h, float intensity, const float tint[3]) {
+    ((Oracle*)h)->S.envMap.scale = {intensity * tint[0], intensity * tint[1], intensity * tint[2]};
+    return VRESTIR_OK;
+}
+// Test hooks on the current orientation (or the previous frame's, prev != 0).
+void orc_env_rotation(void* h, int prev, float toWorld[9], float toLocal[9]) {
+    Oracle* O = (Oracle*)h;
+    const EnvXf& X = prev ? O->envLast : O->envCur;
+    std::memcpy(toWorld, X.toWorld, sizeof(X.toWorld));
+    std::memcpy(toLocal, X.toLocal, sizeof(X.toLocal));
+}
+void orc_env_eval(void* h, const float dir[3], float out[3]) {
+    Oracle* O = (Oracle*)h;
+    f3 L = envEval(O->S, O->envCur, {dir[0], dir[1], dir[2]});
+    out[0] = L.x; out[1] = L.y; out[2] = L.z;
+}
+// out: dir(3), pdf, Le(3), lightID
+void orc_env_sample(void* h, float u, float v, float out[8]) {
+    Oracle* O = (Oracle*)h;
+    float pdf;
+    f3 d = envSample(O->S, O->envCur, {u, v}, pdf);
+    f3 L = envEval(O->S, O->envCur, d);
+    int id;
+    (void)encodeLightUV(O->S, O->envCur, d, id);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z; out[3] = pdf;
+    out[4] = L.x; out[5] = L.y; out[6] = L.z; out[7] = (float)id;
+}
+float orc_env_pdf(void* h, const float dir[3]) {
+    Oracle* O = (Oracle*)h;
+    return envEvalPdf(O->S, O->envCur, {dir[0], dir[1], dir[2]});
+}
+// Importance level `mip` into out ((dim >> mip)^2 floats); returns its side or -1.
+int orc_env_importance(void* h, int mip, float* out) {
+    const EnvMapState& E = ((Oracle*)h)->S.envMap;
+    if (!E.textured || mip < 0 || mip > E.impBaseMip) return -1;
+    if (out) std::memcpy(out, E.imp[mip].data(), E.imp[mip].size() * sizeof(float));
+    return E.impDim >> mip;
+}
+void orc_latlong(const float dir[3], float uv[2]) {
+    f2 r = worldToLatlong({dir[0], dir[1], dir[2]});
+    uv[0] = r.x; uv[1] = r.y;
+}
+void orc_dir_to_oct(const float dir[3], float uv[2]) {
+    f2 r = dirToOctEqualArea({dir[0], dir[1], dir[2]});
+    uv[0] = r.x; uv[1] = r.y;
+}
+// out: emission rgb, temperature
+void orc_emission(void* h, const float p[3], float out[4]) {
+    const Medium& m = ((Oracle*)h)->S.med;
+    f3 pW{p[0], p[1], p[2]};
+    f3 E = emissionWorld(m, pW);
+    out[0] = E.x; out[1] = E.y; out[2] = E.z;
+    out[3] = m.hasEmission ? temperatureWorld(m, pW) : 0.f;
+}
+void orc_blackbody(void* h, float temperature, float out[3]) {
+    f3 c = blackbody(((Oracle*)h)->S.med, temperature);
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
 }
 
 int orc_set_lights(void* h, const vrestir_point_light* L, int n, const float env[3]) {
@@ -1068,7 +1202,7 @@ int orc_final_costs(void* h, long long* out) {
             if (r.runningSum > 0.f) {
                 f3 d = rayDir(O->S.cam, x, y, O->W, O->H);
                 Ray ray{O->S.cam.pos, d, 0, r.depth};
-                (void)evaluateF(*O, r, O->extra[idx].data(), ray, sg, O->finalOpt, noReuse, true);
+                (void)evaluateF(*O, O->frameEnv[0], r, O->extra[idx].data(), ray, sg, O->finalOpt, noReuse, true);
             }
             out[2 * pid] = g_ddaSteps;
             out[2 * pid + 1] = g_cellSteps;
@@ -1122,6 +1256,9 @@ int orc_track_batch(void* h, int mode, int mip, int linear, float tStepScale, in
             case VRESTIR_TRACK_RNG_STREAM:
                 for (int k = 0; k < 8; k++) { uint32_t u = xoshiroNext(sg); std::memcpy(&o[k], &u, 4); }
                 break;
+            case VRESTIR_TRACK_ENV_EVAL: orc_env_eval(h, r + 3, o); break;
+            case VRESTIR_TRACK_ENV_SAMPLE: orc_env_sample(h, r[0], r[1], o); break;
+            case VRESTIR_TRACK_EMISSION: orc_emission(h, r, o); break;
             default: return VRESTIR_EINVAL;
         }
     }

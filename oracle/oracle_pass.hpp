@@ -42,10 +42,26 @@ struct Camera {
     float view[16], proj[16];  // row-vector matrices: clip = [p,1] * view * proj
 };
 
+// Orientation of the environment map for one frame (EnvMapData transform /
+// invTransform): world = toWorld * local, 3x3 row-major.
+struct EnvXf { float toWorld[9], toLocal[9]; };
+
+// A textured lat-long environment map and its importance mip chain
+// (LT/EnvMap.cpp, LT/EnvMapSampler.cpp:61-110).
+struct EnvMapState {
+    bool textured = false;
+    int w = 0, h = 0;
+    std::vector<float> rgb;                // w * h RGB, row 0 at v = 0
+    f3 scale{1, 1, 1};                     // intensity * tint
+    int impDim = 0, impBaseMip = 0;
+    std::vector<std::vector<float>> imp;   // level k: (impDim >> k)^2 texels, row-major
+};
+
 struct SceneState {
     Medium med;
     std::vector<PointLight> lights;
-    f3 env{0, 0, 0};
+    f3 env{0, 0, 0};                       // constant environment (envMap.textured == false)
+    EnvMapState envMap;
     Camera cam, prevCam;
 };
 
@@ -101,7 +117,75 @@ static inline f3 octToDirEqualArea(f2 p) {
     float f = r * std::sqrt(2.f - r * r);
     return {f * sgn(p.x) * std::cos(phi), f * sgn(p.y) * std::sin(phi), sgn(d) * (1.f - r * r)};
 }
-// EnvMapSampler::encodeLightUV / decodeLightUV (LT/EnvMapSampler.slang:75-90); env rotation = identity.
+// EnvMap::setRotation (LT/EnvMap.cpp:57-72): toWorld = Rz * Ry * Rx with the
+// right-handed axis rotations of glm::eulerAngleX/Y/Z, in double, rounded once;
+// toLocal is its transpose (the inverse of a rotation).
+static inline EnvXf envRotation(const float deg[3]) {
+    const double k = 3.14159265358979323846 / 180.0;
+    double a = deg[0] * k, b = deg[1] * k, c = deg[2] * k;
+    // Rz(c) * Ry(b) * Rx(a), multiplied out
+    double ca = std::cos(a), sa = std::sin(a), cb = std::cos(b), sb = std::sin(b), cc = std::cos(c), sc = std::sin(c);
+    double R[9] = {cc * cb, cc * sb * sa - sc * ca, cc * sb * ca + sc * sa,
+                   sc * cb, sc * sb * sa + cc * ca, sc * sb * ca - cc * sa,
+                   -sb,     cb * sa,                cb * ca};
+    EnvXf X;
+    for (int r = 0; r < 3; r++)
+        for (int q = 0; q < 3; q++) {
+            X.toWorld[r * 3 + q] = (float)R[r * 3 + q];
+            X.toLocal[r * 3 + q] = (float)R[q * 3 + r];
+        }
+    return X;
+}
+static inline f3 mat3v(const float* M, f3 d) {
+    return {M[0] * d.x + M[1] * d.y + M[2] * d.z, M[3] * d.x + M[4] * d.y + M[5] * d.z,
+            M[6] * d.x + M[7] * d.y + M[8] * d.z};
+}
+// world_to_latlong_map (UT/Math/MathHelpers.slang:92-99).
+static inline f2 worldToLatlong(f3 dir) {
+    f3 p = normalize(dir);
+    float py = std::min(std::max(p.y, -1.f), 1.f);
+    return {std::atan2(p.x, -p.z) * (0.5f / kPi) + 0.5f, std::acos(py) * (1.f / kPi)};
+}
+// ndir_to_oct_equal_area_unorm (UT/Math/MathHelpers.slang:174-193).
+static inline f2 dirToOctEqualArea(f3 n) {
+    float r = std::sqrt(1.f - std::fabs(n.z));
+    float phi = std::atan2(std::fabs(n.y), std::fabs(n.x));
+    f2 p;
+    p.y = r * phi * (2.f / kPi);
+    p.x = r - p.y;
+    if (n.z < 0.f) p = {1.f - p.y, 1.f - p.x};
+    p.x *= sgn(n.x);
+    p.y *= sgn(n.y);
+    return {p.x * 0.5f + 0.5f, p.y * 0.5f + 0.5f};
+}
+// Texture2D SampleLevel(lod 0) with a linear filter, wrap in u and clamp in v
+// (LT/EnvMap.cpp:113-121): bilinear over texel centres.
+static inline f3 envTexel(const EnvMapState& E, int x, int y) {
+    x = ((x % E.w) + E.w) % E.w;
+    y = std::min(std::max(y, 0), E.h - 1);
+    const float* t = &E.rgb[3 * ((size_t)y * E.w + x)];
+    return {t[0], t[1], t[2]};
+}
+static inline f3 envBilinear(const EnvMapState& E, f2 uv) {
+    float x = uv.x * (float)E.w - 0.5f, y = uv.y * (float)E.h - 0.5f;
+    if (!(std::fabs(x) < 1e30f) || !(std::fabs(y) < 1e30f)) return mk3(0.f);
+    float x0 = std::floor(x), y0 = std::floor(y);
+    float fx = x - x0, fy = y - y0;
+    long xi = (long)std::fmod(x0, (float)E.w);
+    int yi = (int)std::min(std::max(y0, -1.f), (float)E.h);
+    f3 a = envTexel(E, (int)xi, yi), b = envTexel(E, (int)xi + 1, yi);
+    f3 c = envTexel(E, (int)xi, yi + 1), d = envTexel(E, (int)xi + 1, yi + 1);
+    f3 top = a + (b - a) * fx, bot = c + (d - c) * fx;
+    return top + (bot - top) * fy;
+}
+// EnvMap::eval (LT/EnvMap.slang:45-50) for a textured map, the constant otherwise.
+static inline f3 envEval(const SceneState& S, const EnvXf& X, f3 dWorld) {
+    if (!S.envMap.textured) return S.env;
+    return S.envMap.scale * envBilinear(S.envMap, worldToLatlong(mat3v(X.toLocal, dWorld)));
+}
+// EnvMapSampler::encodeLightUV / decodeLightUV (LT/EnvMapSampler.slang:75-90): the
+// local direction's xy, the sign of its z in the light id. A constant environment
+// has no orientation (local = world).
 static inline f2 encodeLightUV(f3 dir, int& id) { id = dir.z < 0 ? -2 : -1; return {dir.x, dir.y}; }
 static inline f3 decodeLightUV(f2 uv, int id) {
     f3 d{uv.x, uv.y, 0};
@@ -110,14 +194,120 @@ static inline f3 decodeLightUV(f2 uv, int id) {
     if (id == -2) d.z = -d.z;
     return d;
 }
+static inline f2 encodeLightUV(const SceneState& S, const EnvXf& X, f3 dWorld, int& id) {
+    return encodeLightUV(S.envMap.textured ? mat3v(X.toLocal, dWorld) : dWorld, id);
+}
+static inline f3 decodeLightUV(const SceneState& S, const EnvXf& X, f2 uv, int id) {
+    f3 d = decodeLightUV(uv, id);
+    return S.envMap.textured ? mat3v(X.toWorld, d) : d;
+}
+// EnvMapSampler::createImportanceMap + EnvMapSamplerSetup.cs.slang:48-70: the base
+// level averages the luminance of spp stratified samples per texel (taken at the
+// equal-area octahedral directions), each further level the 2x2 blocks below it.
+static inline void buildImportance(EnvMapState& E, int dim, int spp) {
+    int sx = std::max(1, (int)std::sqrt((double)spp));
+    int sy = spp / sx;
+    E.impDim = dim;
+    E.imp.clear();
+    std::vector<float> base((size_t)dim * dim);
+    for (int y = 0; y < dim; y++)
+        for (int x = 0; x < dim; x++) {
+            float L = 0.f;
+            for (int j = 0; j < sy; j++)
+                for (int i = 0; i < sx; i++) {
+                    f2 p{((float)(x * sx + i) + 0.5f) / (float)(dim * sx), ((float)(y * sy + j) + 0.5f) / (float)(dim * sy)};
+                    L += luminance(envBilinear(E, worldToLatlong(octToDirEqualArea(p))));
+                }
+            base[(size_t)y * dim + x] = L * (1.f / (float)(sx * sy));
+        }
+    E.imp.push_back(std::move(base));
+    for (int d = dim / 2; d >= 1; d /= 2) {
+        const std::vector<float>& s = E.imp.back();
+        std::vector<float> n((size_t)d * d);
+        for (int y = 0; y < d; y++)
+            for (int x = 0; x < d; x++) {
+                const size_t o = (size_t)(2 * y) * (2 * d) + 2 * x;
+                n[(size_t)y * d + x] = ((s[o] + s[o + 1]) + (s[o + 2 * d] + s[o + 2 * d + 1])) * 0.25f;
+            }
+        E.imp.push_back(std::move(n));
+    }
+    E.impBaseMip = (int)E.imp.size() - 1;
+}
+// EnvMapSampler::sample (LT/EnvMapSampler.slang:94-146). Returns the world direction
+// and its solid-angle pdf; a constant environment is sampled uniformly.
+static inline f3 envSample(const SceneState& S, const EnvXf& X, f2 p, float& pdf) {
+    if (!S.envMap.textured) {
+        pdf = 1.f * kInv4Pi;
+        return octToDirEqualArea(p);
+    }
+    const EnvMapState& E = S.envMap;
+    int px = 0, py = 0;
+    for (int mip = E.impBaseMip - 1; mip >= 0; mip--) {
+        px *= 2; py *= 2;
+        const std::vector<float>& L = E.imp[mip];
+        const int d = E.impDim >> mip;
+        float w[4] = {L[(size_t)py * d + px], L[(size_t)py * d + px + 1], L[(size_t)(py + 1) * d + px],
+                      L[(size_t)(py + 1) * d + px + 1]};
+        float q[2] = {w[0] + w[2], w[1] + w[3]};
+        int ox, oy;
+        float dd = q[0] / (q[0] + q[1]);
+        if (p.x < dd) { ox = 0; p.x = p.x / dd; }
+        else { ox = 1; p.x = (p.x - dd) / (1.f - dd); }
+        float e = w[ox] / q[ox];
+        if (p.y < e) { oy = 0; p.y = p.y / e; }
+        else { oy = 1; p.y = (p.y - e) / (1.f - e); }
+        px += ox; py += oy;
+    }
+    const float inv = 1.f / (float)E.impDim;
+    f2 uv{((float)px + p.x) * inv, ((float)py + p.y) * inv};
+    f3 dir = octToDirEqualArea(uv);
+    float avg = E.imp[E.impBaseMip][0];
+    pdf = E.imp[0][(size_t)py * E.impDim + px] / avg * kInv4Pi;
+    return mat3v(X.toWorld, dir);
+}
+// EnvMapSampler::evalPdf (LT/EnvMapSampler.slang:148-160): point lookup (clamp)
+// of the base level at the direction's octahedral coordinate.
+static inline float envEvalPdf(const SceneState& S, const EnvXf& X, f3 dWorld) {
+    if (!S.envMap.textured) return kInv4Pi;
+    const EnvMapState& E = S.envMap;
+    f2 uv = dirToOctEqualArea(mat3v(X.toLocal, dWorld));
+    int x = std::min(std::max((int)std::floor(uv.x * E.impDim), 0), E.impDim - 1);
+    int y = std::min(std::max((int)std::floor(uv.y * E.impDim), 0), E.impDim - 1);
+    return E.imp[0][(size_t)y * E.impDim + x] / E.imp[E.impBaseMip][0] * kInv4Pi;
+}
+
+// ---------------------------------------------------------------- volume emission
+// TemperatureWorldSpace + ConvertTempToColor + EmissionWorldSpace
+// (VR/VolumeBase.slang:196-232): trilinear temperature lookup in the temperature
+// grid, T' = min(6400, (T - cutoff) * scale), and the blackbody table as a 1-D
+// texture over u = (T' - 25) / 6400 with a linear filter and border colour 0.
+static inline float temperatureWorld(const Medium& m, f3 pW) {
+    return valueAtPoint(m.grid[kTemperatureSlot], worldToMediumP(m, pW, kTemperatureSlot), true);
+}
+static inline f3 blackbody(const Medium& m, float temp) {
+    temp = std::min(6400.f, (temp - m.tempCutoff) * m.tempScale);
+    float q = (temp - 25.f) / 6400.f;
+    const int N = (int)m.lut.size();
+    float x = q * (float)N - 0.5f;
+    if (!(x > -1.f && x < (float)N)) return mk3(0.f);
+    float x0 = std::floor(x);
+    float f = x - x0;
+    int i0 = (int)x0, i1 = i0 + 1;
+    f3 a = i0 >= 0 ? m.lut[i0] : mk3(0.f), b = i1 < N ? m.lut[i1] : mk3(0.f);
+    return m.LeScale * (a + (b - a) * f);
+}
+static inline f3 emissionWorld(const Medium& m, f3 pW) {
+    if (!m.hasEmission) return mk3(0.f);
+    return blackbody(m, temperatureWorld(m, pW));
+}
 
 struct SceneLightSample { f3 dir; float distance; f3 Li; float pdf, pdfArea; f3 rayDir; float rayDistance; };
 
-// sampleSceneLights (VR/VolumeUtils.slang:12-149). The environment is constant:
-// the hierarchical warp of LT/EnvMapSampler.slang:94-167 through a constant
-// importance map is the identity on [0,1)^2 and its pdf is 1/(4 pi).
-static inline bool sampleSceneLights(const SceneState& S, f3 origin, bool useEnv, bool useAnalytic, bool useEmissive,
-                                     SampleGen& sg, SceneLightSample& ls, int& lightIndex, f2& lightUV) {
+// sampleSceneLights (VR/VolumeUtils.slang:12-149): environment (constant or
+// textured) and point lights.
+static inline bool sampleSceneLights(const SceneState& S, const EnvXf& X, f3 origin, bool useEnv, bool useAnalytic,
+                                     bool useEmissive, SampleGen& sg, SceneLightSample& ls, int& lightIndex,
+                                     f2& lightUV) {
     if (!useEnv && !useAnalytic && !useEmissive) return false;
     float p[3] = {useEnv ? 1.f : 0.f, useAnalytic ? 1.f : 0.f, useEmissive ? 1.f : 0.f};
     float sum = p[0] + p[1] + p[2];
@@ -131,9 +321,9 @@ static inline bool sampleSceneLights(const SceneState& S, f3 origin, bool useEnv
             f2 rnd;
             rnd.x = next1D(sg);
             rnd.y = next1D(sg);
-            f3 dir = octToDirEqualArea(rnd);
-            float lpdf = 1.f * kInv4Pi;
-            f3 Le = S.env;
+            float lpdf;
+            f3 dir = envSample(S, X, rnd, lpdf);
+            f3 Le = envEval(S, X, dir);
             float pdf = sel * lpdf;
             ls.rayDir = ls.dir = dir;
             ls.rayDistance = ls.distance = kRayTMax;
@@ -141,7 +331,7 @@ static inline bool sampleSceneLights(const SceneState& S, f3 origin, bool useEnv
             ls.pdfArea = pdf;
             ls.Li = pdf > 0.f ? Le / pdf : mk3(0.f);
             lightIndex = -1;
-            lightUV = encodeLightUV(ls.rayDir, lightIndex);
+            lightUV = encodeLightUV(S, X, ls.rayDir, lightIndex);
             return !(std::isnan(dir.x) || std::isnan(dir.y) || std::isnan(dir.z));
         }
         u -= p[0];
@@ -189,13 +379,14 @@ static inline SceneLightSample analyticLightSample(const SceneState& S, int id, 
 }
 
 // SampleDirectLighting (VR/VolumeUtils.slang:454-492).
-static inline f3 sampleDirectLighting(const SceneState& S, SampleGen& sg, float& pdf, const MediumInteraction& mi,
+static inline f3 sampleDirectLighting(const SceneState& S, const EnvXf& X, SampleGen& sg, float& pdf,
+                                      const MediumInteraction& mi,
                                       bool useEnv, bool useAnalytic, bool useEmissive, bool enableShadow, int mip,
                                       uint32_t method, bool linear, int visSamples, float tStepScale, int& lightIndex,
                                       f2& lightUV, float& outVis) {
     pdf = 0.f;
     SceneLightSample ls{};
-    bool valid = sampleSceneLights(S, mi.p, useEnv, useAnalytic, useEmissive, sg, ls, lightIndex, lightUV);
+    bool valid = sampleSceneLights(S, X, mi.p, useEnv, useAnalytic, useEmissive, sg, ls, lightIndex, lightUV);
     pdf = ls.pdfArea;
     if (!valid) { pdf = 0.f; return mk3(0.f); }
     ls.pdf = 1.f;
@@ -210,13 +401,14 @@ static inline f3 sampleDirectLighting(const SceneState& S, SampleGen& sg, float&
 }
 
 // directLighting (VR/VolumeUtils.slang:416-450), used by the path tracer.
-static inline f3 directLighting(const SceneState& S, SampleGen& sg, const MediumInteraction& mi, int nSamples,
+static inline f3 directLighting(const SceneState& S, const EnvXf& X, SampleGen& sg, const MediumInteraction& mi,
+                                int nSamples,
                                 bool useEnv, bool useAnalytic, bool useEmissive, int mip, uint32_t method) {
     f3 Ld = mk3(0.f);
     for (int i = 0; i < nSamples; i++) {
         SceneLightSample ls{};
         int idx; f2 uv;
-        bool valid = sampleSceneLights(S, mi.p, useEnv, useAnalytic, useEmissive, sg, ls, idx, uv);
+        bool valid = sampleSceneLights(S, X, mi.p, useEnv, useAnalytic, useEmissive, sg, ls, idx, uv);
         if (!valid) continue;
         ls.pdf = 1.f;
         Ray sh{mi.p, ls.rayDir, 0, ls.rayDistance};

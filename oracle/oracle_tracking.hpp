@@ -14,8 +14,10 @@ namespace orc {
 struct Ray { f3 origin, dir; float tMin, tMax; f3 at(float t) const { return origin + dir * t; } };
 
 // Medium parameters (SC/SceneTypes.slang:85-110) plus the grid slots.
+static const int kTemperatureSlot = 16;  // VR/HostDeviceSharedConstants.slang:23
+
 struct Medium {
-    GridSlot grid[16];
+    GridSlot grid[17];                      // mips 0..7, conservative 8..15, temperature 16
     int numMips = 0;
     float sigma_t = 1.f;
     f3 sigma_s{1, 1, 1}, sigma_a{0, 0, 0};
@@ -25,6 +27,8 @@ struct Medium {
     float worldScaling = 1.f;
     float superVoxelDiag = 1.f; // SC/Scene.cpp:3077-3080
     bool hasEmission = false;
+    float LeScale = 1.f, tempCutoff = 0.f, tempScale = 0.f;
+    std::vector<f3> lut;                    // blackbody table rows
 };
 
 // ---------------------------------------------------------------- transforms

@@ -16,6 +16,22 @@ int orc_set_params(void* h, const vrestir_params* p);
 int orc_set_volume(void* h, const vrestir_volume_desc* d);
 int orc_set_lights(void* h, const vrestir_point_light* lights, int n, const float env[3]);
 int orc_set_camera(void* h, const vrestir_camera* c, int width, int height);
+int orc_set_temperature(void* h, const float* t, int nx, int ny, int nz, const float* indexToWorld, int fp16,
+                        float LeScale, float cutoff, float scale);
+int orc_set_blackbody_lut(void* h, const float* rgb, int rows);
+int orc_set_env_map(void* h, int w, int hgt, const float* rgb, float intensity, const float tint[3],
+                    const float rotationDegrees[3], int impDim, int impSpp);
+int orc_set_env_rotation(void* h, const float deg[3]);
+int orc_set_env_intensity(void* h, float intensity, const float tint[3]);
+void orc_env_rotation(void* h, int prev, float toWorld[9], float toLocal[9]);
+void orc_env_eval(void* h, const float dir[3], float out[3]);
+void orc_env_sample(void* h, float u, float v, float out[8]);
+float orc_env_pdf(void* h, const float dir[3]);
+int orc_env_importance(void* h, int mip, float* out);
+void orc_latlong(const float dir[3], float uv[2]);
+void orc_dir_to_oct(const float dir[3], float uv[2]);
+void orc_emission(void* h, const float p[3], float out[4]);
+void orc_blackbody(void* h, float temperature, float out[3]);
 int orc_execute(void* h, int width, int height, float* outRGBA, int rowBegin, int rowEnd, int nthreads);
 int orc_prepare_frame(void* h, int width, int height, int nthreads);
 int orc_run_stage(void* h, int stage, int round);

@@ -8,75 +8,11 @@ flow matches); whole uninjected frames within 1e-3 on >= 97% of pixels."""
 import numpy as np
 import pytest
 
-from gpu_common import Pair, compare_reservoirs, rel, say, small_scene
+from gpu_common import RADIANCE_RTOL, Pair, compare_reservoirs, rel, run_injected_frames, say, small_scene
 from volumetricrestirrelease_amd import abi
 from volumetricrestirrelease_amd.volumetric_restir import FEATURES_DTYPE
 
 pytestmark = pytest.mark.gpu
-
-STAGE_MATCH = 0.995
-RADIANCE_RTOL = 1e-4
-
-
-def run_injected_frames(pair, frames=3):
-    B = pair.B
-    report = []
-    for frame in range(frames):
-        say(f"  frame {frame}")
-        pair.begin(frame)
-        if frame > 0:
-            pair.sync_history()
-        pair.stage(abi.STAGE_GENERATE_FEATURES)
-        fo = pair.o.read(abi.BUF_FEATURES, pair.n * 8).view(FEATURES_DTYPE)
-        fg = pair.g.read(abi.BUF_FEATURES, pair.n * 8).view(FEATURES_DTYPE)
-        assert np.array_equal(fo["noReflectiveSurface"], fg["noReflectiveSurface"])
-        assert rel(fo["transmittance"], fg["transmittance"]).max() <= 1e-4
-        pair.inject(abi.BUF_FEATURES, pair.n * 8)
-
-        pair.stage(abi.STAGE_TRACE_RAYS)
-        m, _ = compare_reservoirs(pair.o.read(abi.BUF_RESERVOIRS0, pair.nres), pair.g.read(abi.BUF_RESERVOIRS0, pair.nres))
-        report.append(("trace", frame, m))
-        assert m >= STAGE_MATCH, ("trace", frame, m)
-        pair.inject(abi.BUF_RESERVOIRS0, pair.nres)
-        pair.inject(abi.BUF_EXTRA0, pair.nex)
-
-        if pair.spec.params.get("enableTemporalReuse", 1):
-            pair.stage(abi.STAGE_TEMPORAL_REUSE)
-            m, _ = compare_reservoirs(pair.o.read(abi.BUF_RESERVOIRS0, pair.nres),
-                                      pair.g.read(abi.BUF_RESERVOIRS0, pair.nres))
-            report.append(("temporal", frame, m))
-            assert m >= STAGE_MATCH, ("temporal", frame, m)
-            pair.inject(abi.BUF_RESERVOIRS0, pair.nres)
-            pair.inject(abi.BUF_EXTRA0, pair.nex)
-        t = 0
-        if pair.spec.params.get("enableSpatialReuse", 1):
-            for rnd in range(pair.spec.params.get("spatialReuseRounds", 1)):
-                src = abi.BUF_RESERVOIRS0 if t % 2 == 0 else abi.BUF_RESERVOIRS1
-                dst = abi.BUF_RESERVOIRS1 if t % 2 == 0 else abi.BUF_RESERVOIRS0
-                dex = abi.BUF_EXTRA1 if t % 2 == 0 else abi.BUF_EXTRA0
-                pair.inject(src, pair.nres)
-                pair.stage(abi.STAGE_SPATIAL_REUSE, rnd)
-                m, _ = compare_reservoirs(pair.o.read(dst, pair.nres), pair.g.read(dst, pair.nres))
-                report.append((f"spatial{rnd}", frame, m))
-                assert m >= STAGE_MATCH, ("spatial", rnd, frame, m)
-                pair.inject(dst, pair.nres)
-                pair.inject(dex, pair.nex)
-                t += 1
-        if pair.spec.params.get("enableTemporalReuse", 1):
-            pair.stage(abi.STAGE_COPY_RESERVOIRS)
-        cur = abi.BUF_RESERVOIRS0 if t % 2 == 0 else abi.BUF_RESERVOIRS1
-        _, match = compare_reservoirs(pair.o.read(cur, pair.nres), pair.g.read(cur, pair.nres))
-        pair.stage(abi.STAGE_FINAL_SHADING)
-        co, cg = pair.color()
-        r = rel(co[:, :3], cg[:, :3]).max(axis=1)
-        ok = r <= RADIANCE_RTOL
-        report.append(("final", frame, float(ok[match].mean())))
-        assert ok[match].mean() >= STAGE_MATCH, ("final", frame, ok[match].mean(), r.max())
-        pair.end()
-    for row in report:
-        say("   ", row)
-    return report
-
 
 def test_stage_parity_config0_small():
     say("case: config0 64^3 fp32, 64x48, b=2, M=4, 1 spatial round k=4")

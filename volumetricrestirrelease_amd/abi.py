@@ -37,7 +37,8 @@ BUF_FEATURES, BUF_TEMPORAL_FEATURES, BUF_RESERVOIRS0, BUF_RESERVOIRS1, BUF_TEMPO
 
 TRACK_TR_ANALYTIC, TRACK_TR_RAY_MARCHING, TRACK_TR_RATIO, TRACK_TR_RESIDUAL_RATIO, \
     TRACK_TR_ANALOG_RESIDUAL_RATIO, TRACK_SAMPLE_ANALYTIC, TRACK_SAMPLE_DECOMPOSITION, \
-    TRACK_SAMPLE_CELL_BY_DENSITY, TRACK_FEATURE_MARCH, TRACK_DENSITY_AT_POINT, TRACK_RNG_STREAM = range(11)
+    TRACK_SAMPLE_CELL_BY_DENSITY, TRACK_FEATURE_MARCH, TRACK_DENSITY_AT_POINT, TRACK_RNG_STREAM, \
+    TRACK_ENV_EVAL, TRACK_ENV_SAMPLE, TRACK_EMISSION = range(14)
 
 # (name, ctype) in declaration order of vrestir_params
 PARAM_FIELDS = [
@@ -79,6 +80,15 @@ class VolumeDesc(ctypes.Structure):
         ("g", c_float), ("densityScale", c_float),
         ("temperature", POINTER(c_float)),
         ("LeScale", c_float), ("temperatureCutOff", c_float), ("temperatureScale", c_float),
+    ]
+
+
+class EnvMapDesc(ctypes.Structure):
+    _fields_ = [
+        ("width", c_int32), ("height", c_int32),
+        ("rgb", POINTER(c_float)),
+        ("intensity", c_float), ("tint", c_float * 3), ("rotationDegrees", c_float * 3),
+        ("importanceDim", c_int32), ("importanceSpp", c_int32),
     ]
 
 
@@ -127,6 +137,11 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "vrestir_load_vbx": (i32, [vp, c_char_p, i32, i32, POINTER(f32), POINTER(f32), f32, f32]),
         "vrestir_set_lights": (i32, [vp, POINTER(PointLight), i32, POINTER(f32)]),
         "vrestir_set_camera": (i32, [vp, POINTER(Camera)]),
+        "vrestir_set_env_map": (i32, [vp, POINTER(EnvMapDesc)]),
+        "vrestir_set_env_rotation": (i32, [vp, POINTER(f32)]),
+        "vrestir_set_env_intensity": (i32, [vp, f32, POINTER(f32)]),
+        "vrestir_set_blackbody_lut": (i32, [vp, POINTER(f32), i32]),
+        "vrestir_load_vbx_emission": (i32, [vp, c_char_p, f32, f32, f32]),
         "vrestir_set_params": (i32, [vp, POINTER(Params)]),
         "vrestir_get_params": (i32, [vp, POINTER(Params)]),
         "vrestir_reset_history": (i32, [vp]),
@@ -158,7 +173,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
 
 EXPORTED_SYMBOLS = [
     "vrestir_version", "vrestir_default_params", "vrestir_create", "vrestir_destroy", "vrestir_last_error",
-    "vrestir_set_volume", "vrestir_load_vbx", "vrestir_set_lights", "vrestir_set_camera", "vrestir_set_params",
+    "vrestir_set_volume", "vrestir_load_vbx", "vrestir_set_lights", "vrestir_set_camera", "vrestir_set_env_map",
+    "vrestir_set_env_rotation", "vrestir_set_env_intensity", "vrestir_set_blackbody_lut", "vrestir_load_vbx_emission",
+    "vrestir_set_params",
     "vrestir_get_params", "vrestir_reset_history", "vrestir_execute", "vrestir_frame_count",
     "vrestir_prepare_frame", "vrestir_run_stage", "vrestir_end_frame", "vrestir_read_buffer",
     "vrestir_write_buffer", "vrestir_track_batch", "vrestir_set_tuning", "vrestir_accumulate", "vrestir_accumulated", "vrestir_relmse",

@@ -66,6 +66,13 @@ struct vrestir_handle {
     bool splitReuse = false;
     SplitBufs split{};
     size_t splitPixels = 0;
+    // textured environment map: texels, importance mip chain, orientation now and at the last frame
+    float* envTexDev = nullptr;
+    float* envImpDev = nullptr;
+    EnvXf envCur{}, envLast{};
+    bool haveEnvLast = false;
+    // temperature grid bound after the volume (vrestir_load_vbx_emission)
+    std::vector<void*> tempAllocs;
 };
 
 static int fail(vrestir_handle* h, int code, const std::string& msg) {
@@ -100,6 +107,36 @@ static void freeFrameBuffers(vrestir_handle* h) {
 static void freeVolume(vrestir_handle* h) {
     for (void* p : h->volumeAllocs) (void)hipFree(p);
     h->volumeAllocs.clear();
+    for (void* p : h->tempAllocs) (void)hipFree(p);
+    h->tempAllocs.clear();
+}
+static void freeEnv(vrestir_handle* h) {
+    dfree(h->envTexDev);
+    dfree(h->envImpDev);
+}
+// EnvMap::setRotation (LT/EnvMap.cpp:57-72): local -> world = Rz * Ry * Rx of
+// glm::eulerAngleX/Y/Z (right-handed rotations by the XYZ angles in degrees),
+// computed in double and rounded once; world -> local is its transpose.
+static void envRotation(const float deg[3], EnvXf& X) {
+    const double k = 3.14159265358979323846 / 180.0;
+    const double cx = std::cos(deg[0] * k), sx = std::sin(deg[0] * k);
+    const double cy = std::cos(deg[1] * k), sy = std::sin(deg[1] * k);
+    const double cz = std::cos(deg[2] * k), sz = std::sin(deg[2] * k);
+    const double Rx[9] = {1, 0, 0, 0, cx, -sx, 0, sx, cx};
+    const double Ry[9] = {cy, 0, sy, 0, 1, 0, -sy, 0, cy};
+    const double Rz[9] = {cz, -sz, 0, sz, cz, 0, 0, 0, 1};
+    double T[9], R[9];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            T[r * 3 + c] = Ry[r * 3] * Rx[c] + Ry[r * 3 + 1] * Rx[3 + c] + Ry[r * 3 + 2] * Rx[6 + c];
+        }
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) R[r * 3 + c] = Rz[r * 3] * T[c] + Rz[r * 3 + 1] * T[3 + c] + Rz[r * 3 + 2] * T[6 + c];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            X.toWorld[r * 3 + c] = (float)R[r * 3 + c];
+            X.toLocal[r * 3 + c] = (float)R[c * 3 + r];
+        }
 }
 
 // ------------------------------------------------------------------ params
@@ -190,6 +227,9 @@ extern "C" int32_t vrestir_create(const vrestir_params* params, vrestir_handle**
     h->device = dev;
     h->P = p;
     std::memset(&h->sceneHost, 0, sizeof(Scene));
+    const float zero[3] = {0.f, 0.f, 0.f};
+    envRotation(zero, h->envCur);
+    h->envLast = h->envCur;
     if (hipMalloc((void**)&h->sceneDev, sizeof(Scene)) != hipSuccess ||
         hipMalloc((void**)&h->offsetsDev, sizeof(int) * 16 * kMaxSpatialSamples * 2) != hipSuccess) {
         dfree(h->sceneDev);
@@ -206,6 +246,7 @@ extern "C" int32_t vrestir_destroy(vrestir_handle* h) {
     (void)hipDeviceSynchronize();
     freeFrameBuffers(h);
     freeVolume(h);
+    freeEnv(h);
     dfree(h->sceneDev);
     dfree(h->offsetsDev);
     dfree(h->accum);
@@ -253,14 +294,47 @@ static int uploadScene(vrestir_handle* h) {
     return VRESTIR_OK;
 }
 
-template <class T>
-static int upload(vrestir_handle* h, const std::vector<T>& v, const void** dst) {
-    void* p = nullptr;
-    size_t bytes = std::max<size_t>(v.size() * sizeof(T), 16);
-    HIPCHK(h, hipMalloc(&p, bytes));
-    h->volumeAllocs.push_back(p);
-    if (!v.empty()) HIPCHK(h, hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-    *dst = p;
+// Uploads one built slot into S.grid[s]; device allocations are recorded in allocs.
+static int uploadSlot(vrestir_handle* h, int s, HostGrid& hg, std::vector<void*>& allocs) {
+    Grid& G = h->sceneHost.grid[s];
+    std::memset(&G, 0, sizeof(Grid));
+    h->meta.slots[s] = HostGrid();
+    if (!hg.valid) return VRESTIR_OK;
+    auto up = [&](const void* src, size_t bytes, const void** dst) -> int {
+        void* p = nullptr;
+        HIPCHK(h, hipMalloc(&p, std::max<size_t>(bytes, 16)));
+        allocs.push_back(p);
+        if (bytes) HIPCHK(h, hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+        *dst = p;
+        return VRESTIR_OK;
+    };
+    int rc;
+    const void* p;
+    if ((rc = up(hg.brickTable.data(), hg.brickTable.size() * 4, &p))) return rc;
+    G.brickTable = (const uint32_t*)p;
+    if ((rc = up(hg.nodeTable.data(), hg.nodeTable.size() * 4, &p))) return rc;
+    G.nodeTable = (const uint32_t*)p;
+    if ((rc = up(hg.occ.data(), hg.occ.size() * 4, &p))) return rc;
+    G.occ = (const uint32_t*)p;
+    G.occWords = (int32_t)(hg.occ.size() / 2);
+    G.numBricks = hg.numBricks;
+    if ((rc = up(hg.bounds.data(), hg.bounds.size() * 4, &p))) return rc;
+    G.bounds = (const float*)p;
+    if (hg.fp16) { if ((rc = up(hg.pool16.data(), hg.pool16.size() * 2, &p))) return rc; }
+    else { if ((rc = up(hg.pool32.data(), hg.pool32.size() * 4, &p))) return rc; }
+    G.pool = p;
+    G.nbx = hg.nbx; G.nby = hg.nby; G.nbz = hg.nbz;
+    G.nnx = hg.nnx; G.nny = hg.nny; G.nnz = hg.nnz;
+    G.bmax[0] = (float)hg.nx; G.bmax[1] = (float)hg.ny; G.bmax[2] = (float)hg.nz;
+    std::memcpy(G.w2i, hg.w2i, sizeof(G.w2i));
+    G.maxValue = hg.maxValue;
+    G.fp16 = hg.fp16 ? 1 : 0;
+    G.valid = 1;
+    // keep metadata, drop the host copies of the pools
+    HostGrid& m = h->meta.slots[s];
+    m.valid = true; m.nx = hg.nx; m.ny = hg.ny; m.nz = hg.nz; m.numBricks = hg.numBricks;
+    m.maxValue = hg.maxValue; m.fp16 = hg.fp16;
+    m.nbx = hg.nbx; m.nby = hg.nby; m.nbz = hg.nbz;
     return VRESTIR_OK;
 }
 
@@ -273,39 +347,8 @@ extern "C" int32_t vrestir_set_volume(vrestir_handle* h, const vrestir_volume_de
     HIPCHK(h, hipDeviceSynchronize());
     freeVolume(h);
     Scene& S = h->sceneHost;
-    for (int s = 0; s < kMaxSlots; s++) {
-        Grid& G = S.grid[s];
-        std::memset(&G, 0, sizeof(Grid));
-        HostGrid& hg = hv.slots[s];
-        if (!hg.valid) continue;
-        const void* p;
-        if ((rc = upload(h, hg.brickTable, &p))) return rc;
-        G.brickTable = (const uint32_t*)p;
-        if ((rc = upload(h, hg.nodeTable, &p))) return rc;
-        G.nodeTable = (const uint32_t*)p;
-        if ((rc = upload(h, hg.occ, &p))) return rc;
-        G.occ = (const uint32_t*)p;
-        G.occWords = (int32_t)(hg.occ.size() / 2);
-        G.numBricks = hg.numBricks;
-        if ((rc = upload(h, hg.bounds, &p))) return rc;
-        G.bounds = (const float*)p;
-        if (hg.fp16) { if ((rc = upload(h, hg.pool16, &p))) return rc; }
-        else { if ((rc = upload(h, hg.pool32, &p))) return rc; }
-        G.pool = p;
-        G.nbx = hg.nbx; G.nby = hg.nby; G.nbz = hg.nbz;
-        G.nnx = hg.nnx; G.nny = hg.nny; G.nnz = hg.nnz;
-        G.bmax[0] = (float)hg.nx; G.bmax[1] = (float)hg.ny; G.bmax[2] = (float)hg.nz;
-        std::memcpy(G.w2i, hg.w2i, sizeof(G.w2i));
-        G.maxValue = hg.maxValue;
-        G.fp16 = hg.fp16 ? 1 : 0;
-        G.valid = 1;
-        // keep metadata, drop the host copies of the pools
-        HostGrid& m = h->meta.slots[s];
-        m = HostGrid();
-        m.valid = true; m.nx = hg.nx; m.ny = hg.ny; m.nz = hg.nz; m.numBricks = hg.numBricks;
-        m.maxValue = hg.maxValue; m.fp16 = hg.fp16;
-        m.nbx = hg.nbx; m.nby = hg.nby; m.nbz = hg.nbz;
-    }
+    for (int s = 0; s < kMaxSlots; s++)
+        if ((rc = uploadSlot(h, s, hv.slots[s], h->volumeAllocs))) return rc;
     h->meta.numMips = hv.numMips;
     S.numMips = hv.numMips;
     S.sigma_t = d->sigmaS[0] + d->sigmaA[0];
@@ -315,8 +358,144 @@ extern "C" int32_t vrestir_set_volume(vrestir_handle* h, const vrestir_volume_de
     S.tStep = hv.tStep;
     S.worldScaling = 1.f;
     S.superVoxelDiag = hv.superVoxelDiag;
+    // addGVDBVolume(hasEmission, LeScale, temperatureCutOff, temperatureScale), SC/Scene.cpp:2754-2790
+    S.hasEmission = hv.slots[kTemperatureSlot].valid ? 1 : 0;
+    S.LeScale = d->LeScale;
+    S.tempCutoff = d->temperatureCutOff;
+    S.tempScale = d->temperatureScale;
     h->volumeSet = true;
     h->optionsChanged = true;
+    return uploadScene(h);
+}
+
+// <prefix>_temperature.vbx next to <prefix>_mip0.vbx (SC/Scene.cpp:2789-2791, 2808-2811).
+extern "C" int32_t vrestir_load_vbx_emission(vrestir_handle* h, const char* pathPrefix, float LeScale,
+                                             float temperatureCutOff, float temperatureScale) {
+    if (!h || !pathPrefix) return VRESTIR_EINVAL;
+    if (!h->volumeSet) return fail(h, VRESTIR_ESTATE, "bind the density volume first");
+    std::string path = std::string(pathPrefix) + "_temperature.vbx";
+    int nx = 0, ny = 0, nz = 0;
+    float i2w[16];
+    int rc = vrestir_read_vbx(path.c_str(), nullptr, &nx, &ny, &nz, i2w);
+    if (rc) return fail(h, rc, "cannot read " + path);
+    std::vector<float> t((size_t)nx * ny * nz);
+    rc = vrestir_read_vbx(path.c_str(), t.data(), &nx, &ny, &nz, i2w);
+    if (rc) return fail(h, rc, "cannot read " + path);
+    HostGrid hg;
+    rc = buildTemperatureSlot(t.data(), nx, ny, nz, i2w, h->meta.slots[0].fp16, hg, h->err);
+    if (rc) return rc;
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipDeviceSynchronize());
+    for (void* p : h->tempAllocs) (void)hipFree(p);
+    h->tempAllocs.clear();
+    if ((rc = uploadSlot(h, kTemperatureSlot, hg, h->tempAllocs))) return rc;
+    Scene& S = h->sceneHost;
+    S.hasEmission = 1;
+    S.LeScale = LeScale;
+    S.tempCutoff = temperatureCutOff;
+    S.tempScale = temperatureScale;
+    return uploadScene(h);
+}
+
+// The blackbody table of SC/Scene.cpp:2877-2893 (LUT/BlackBodyRadiationRGB_50K-6400K.txt,
+// 128 rows of "r g b"), handed over by the caller.
+extern "C" int32_t vrestir_set_blackbody_lut(vrestir_handle* h, const float* rgb, int32_t rows) {
+    if (!h) return VRESTIR_EINVAL;
+    if (rows < 1 || rows > kMaxLutRows || !rgb) return fail(h, VRESTIR_EINVAL, "blackbody table needs 1..256 rows");
+    for (int i = 0; i < 3 * rows; i++)
+        if (!std::isfinite(rgb[i])) return fail(h, VRESTIR_EINVAL, "blackbody table holds a non-finite value");
+    Scene& S = h->sceneHost;
+    std::memset(S.lut, 0, sizeof(S.lut));
+    std::memcpy(S.lut, rgb, sizeof(float) * 3 * rows);
+    S.lutRows = rows;
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipDeviceSynchronize());
+    return uploadScene(h);
+}
+
+// ------------------------------------------------------------------ environment map
+// EnvMap::create + EnvMapSampler::create (LT/EnvMap.cpp:107-122, LT/EnvMapSampler.cpp:61-110):
+// texels to HBM as RGBA32F, the importance mip chain built on the device.
+extern "C" int32_t vrestir_set_env_map(vrestir_handle* h, const vrestir_env_map_desc* d) {
+    if (!h) return VRESTIR_EINVAL;
+    Scene& S = h->sceneHost;
+    if (!d) {
+        HIPCHK(h, hipSetDevice(h->device));
+        HIPCHK(h, hipDeviceSynchronize());
+        freeEnv(h);
+        S.envTextured = 0;
+        S.envTex = S.envImp = nullptr;
+        return uploadScene(h);
+    }
+    const int W = d->width, H = d->height;
+    if (W < 1 || H < 1 || (int64_t)W * H > (int64_t)1 << 26 || !d->rgb)
+        return fail(h, VRESTIR_EINVAL, "environment map needs 1..2^26 texels and rgb data");
+    const int dim = d->importanceDim ? d->importanceDim : 512;
+    const int spp = d->importanceSpp ? d->importanceSpp : 64;
+    if (dim < 2 || dim > 2048 || (dim & (dim - 1)))
+        return fail(h, VRESTIR_EINVAL, "importanceDim must be a power of two in [2, 2048]");
+    int sx = 1;
+    while ((sx * 2) * (sx * 2) <= spp) sx *= 2;
+    if (spp < 1 || spp > 1024 || (spp & (spp - 1)) || sx * sx != spp)
+        return fail(h, VRESTIR_EINVAL, "importanceSpp must be 1, 4, 16, 64, 256 or 1024");
+    std::vector<float> rgba((size_t)W * H * 4);
+    float lumMax = 0.f;
+    for (size_t i = 0; i < (size_t)W * H; i++) {
+        const float r = d->rgb[3 * i], g = d->rgb[3 * i + 1], b = d->rgb[3 * i + 2];
+        if (!(std::isfinite(r) && std::isfinite(g) && std::isfinite(b)) || r < 0.f || g < 0.f || b < 0.f)
+            return fail(h, VRESTIR_EINVAL, "environment texels must be finite and non-negative");
+        rgba[4 * i] = r; rgba[4 * i + 1] = g; rgba[4 * i + 2] = b; rgba[4 * i + 3] = 1.f;
+        lumMax = std::max(lumMax, r * 0.2126f + g * 0.7152f + b * 0.0722f);
+    }
+    if (!(lumMax > 0.f)) return fail(h, VRESTIR_EINVAL, "environment map is black (bind a zero constant environment instead)");
+    int mips = 0, off[kMaxImpMips];
+    size_t total = 0;
+    for (int dd = dim; dd >= 1; dd /= 2) { off[mips++] = (int)total; total += (size_t)dd * dd; }
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipDeviceSynchronize());
+    freeEnv(h);
+    HIPCHK(h, hipMalloc((void**)&h->envTexDev, rgba.size() * sizeof(float)));
+    HIPCHK(h, hipMalloc((void**)&h->envImpDev, total * sizeof(float)));
+    HIPCHK(h, hipMemcpy(h->envTexDev, rgba.data(), rgba.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(h, launchEnvImportance(nullptr, h->envTexDev, W, H, h->envImpDev, off, dim, spp));
+    HIPCHK(h, hipDeviceSynchronize());
+    float avg = 0.f;
+    HIPCHK(h, hipMemcpy(&avg, h->envImpDev + off[mips - 1], sizeof(float), hipMemcpyDeviceToHost));
+    if (!(avg > 0.f)) {
+        freeEnv(h);
+        S.envTextured = 0;
+        (void)uploadScene(h);
+        return fail(h, VRESTIR_EINVAL, "the importance map of the environment is zero everywhere");
+    }
+    S.envTex = h->envTexDev;
+    S.envImp = h->envImpDev;
+    S.envTextured = 1;
+    S.envW = W; S.envH = H;
+    S.envImpDim = dim;
+    S.envImpBaseMip = mips - 1;
+    S.envImpInvDim = 1.f / (float)dim;
+    for (int k = 0; k < kMaxImpMips; k++) S.envImpOff[k] = k < mips ? off[k] : 0;
+    for (int k = 0; k < 3; k++) S.envScale[k] = d->intensity * d->tint[k];
+    envRotation(d->rotationDegrees, h->envCur);
+    h->envLast = h->envCur;
+    return uploadScene(h);
+}
+
+// EnvMap::setRotation: takes effect at the next frame; the frame after sees this
+// orientation as the previous frame's (Scene::updateEnvMap, SC/Scene.cpp:761-775).
+extern "C" int32_t vrestir_set_env_rotation(vrestir_handle* h, const float degreesXYZ[3]) {
+    if (!h || !degreesXYZ) return VRESTIR_EINVAL;
+    envRotation(degreesXYZ, h->envCur);
+    return VRESTIR_OK;
+}
+
+// EnvMap::setIntensity / setTint.
+extern "C" int32_t vrestir_set_env_intensity(vrestir_handle* h, float intensity, const float tint[3]) {
+    if (!h || !tint) return VRESTIR_EINVAL;
+    Scene& S = h->sceneHost;
+    for (int k = 0; k < 3; k++) S.envScale[k] = intensity * tint[k];
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipDeviceSynchronize());
     return uploadScene(h);
 }
 
@@ -524,6 +703,8 @@ extern "C" int32_t vrestir_prepare_frame(vrestir_handle* h, int32_t W, int32_t H
     F.spatialRounds = p.spatialReuseRounds;
     F.cam = h->cam;
     F.prevCam = h->prevCam;
+    F.env[0] = h->envCur;
+    F.env[1] = h->haveEnvLast ? h->envLast : h->envCur;
     std::memcpy(F.prevView, h->prevView, sizeof(F.prevView));
     std::memcpy(F.prevProj, h->prevProj, sizeof(F.prevProj));
     fillOptions(F.initialOpt, VRESTIR_ANALYTIC_TRACKING, p.initialLightingTrackingMethod, p.initialLightSamples,
@@ -545,6 +726,8 @@ extern "C" int32_t vrestir_prepare_frame(vrestir_handle* h, int32_t W, int32_t H
         return fail(h, VRESTIR_EINVAL, "an option references a mip level the bound volume does not have");
     if (p.useAnalyticLights && h->sceneHost.numLights == 0)
         return fail(h, VRESTIR_EINVAL, "useAnalyticLights is set but no point light is bound");
+    if (h->sceneHost.hasEmission && h->sceneHost.lutRows == 0)
+        return fail(h, VRESTIR_ESTATE, "an emission grid is bound but no blackbody table (vrestir_set_blackbody_lut)");
     h->totalRoundId = 0;
     h->frameOpen = true;
     return VRESTIR_OK;
@@ -686,6 +869,8 @@ extern "C" int32_t vrestir_end_frame(vrestir_handle* h) {
     if (!h) return VRESTIR_EINVAL;
     h->temporalAccumulated = true;
     h->prevCam = h->cam;
+    h->envLast = h->envCur;
+    h->haveEnvLast = true;
     std::memcpy(h->prevView, h->view, sizeof(h->view));
     std::memcpy(h->prevProj, h->proj, sizeof(h->proj));
     h->frameCount++;
@@ -843,13 +1028,15 @@ extern "C" int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t 
                                        const float* rays, const uint32_t* seeds, float* out) {
     if (!h) return VRESTIR_EINVAL;
     if (!h->volumeSet) return fail(h, VRESTIR_ESTATE, "no volume bound");
-    if (mode < 0 || mode > VRESTIR_TRACK_RNG_STREAM) return fail(h, VRESTIR_EINVAL, "unknown tracking mode");
+    if (mode < 0 || mode > VRESTIR_TRACK_EMISSION) return fail(h, VRESTIR_EINVAL, "unknown tracking mode");
+    if (mode == VRESTIR_TRACK_EMISSION && h->sceneHost.hasEmission && h->sceneHost.lutRows == 0)
+        return fail(h, VRESTIR_ESTATE, "no blackbody table bound");
     if (mip < 0 || mip >= kMaxSlots || !h->meta.slots[mip].valid) return fail(h, VRESTIR_EINVAL, "grid slot not bound");
     if (n < 0 || (n > 0 && (!rays || !seeds || !out))) return fail(h, VRESTIR_EINVAL, "bad ray batch");
     HIPCHK(h, hipSetDevice(h->device));
     int occSlots[2];
     const int occBytes = occPlan(h, mip, -1, occSlots);
-    HIPCHK(h, launchTrack((hipStream_t)stream, h->sceneDev, mode, mip, useLinearSampler, tStepScale, numSamples, n,
+    HIPCHK(h, launchTrack((hipStream_t)stream, h->sceneDev, h->envCur, mode, mip, useLinearSampler, tStepScale, numSamples, n,
                           rays, seeds, out, occSlots[0], occBytes));
     return VRESTIR_OK;
 }

@@ -20,7 +20,8 @@ __global__ __launch_bounds__(256) void kFeatures(const Scene* __restrict__ sc, c
 
 
 // Batched tracking primitive: one lane per ray.
-__global__ __launch_bounds__(256) void kTrack(const Scene* __restrict__ sc, int mode, int mip, int linear,
+__global__ __launch_bounds__(256) void kTrack(const Scene* __restrict__ sc, const EnvXf ex, int mode, int mip,
+                                              int linear,
                                               float tStepScale, int numSamples, int n, const float* __restrict__ rays,
                                               const uint32_t* __restrict__ seeds, float* __restrict__ out,
                                               int occSlot) {
@@ -66,6 +67,27 @@ __global__ __launch_bounds__(256) void kTrack(const Scene* __restrict__ sc, int 
 #pragma unroll
             for (int k = 0; k < 8; k++) o[k] = __uint_as_float(rngNext(rng));
             break;
+        case 11: {  // environment radiance towards ray.d
+            v3 L = envEval(S, ex, ray.d);
+            o[0] = L.x; o[1] = L.y; o[2] = L.z;
+            break;
+        }
+        case 12: {  // environment importance sample at (origin.x, origin.y)
+            float pdf;
+            v3 d = envSample(S, ex, r[0], r[1], pdf);
+            v3 L = envEval(S, ex, d);
+            int id; float u, v;
+            encodeLightUV(S, ex, d, id, u, v);
+            o[0] = d.x; o[1] = d.y; o[2] = d.z; o[3] = pdf;
+            o[4] = L.x; o[5] = L.y; o[6] = L.z; o[7] = (float)id;
+            break;
+        }
+        case 13: {  // emission and temperature at the point origin
+            v3 E = emissionWorld(S, ray.o);
+            o[0] = E.x; o[1] = E.y; o[2] = E.z;
+            o[3] = S.hasEmission ? temperatureWorld(S, ray.o) : 0.f;
+            break;
+        }
         default: break;
     }
     float* d = out + 8 * (size_t)i;
@@ -89,11 +111,11 @@ hipError_t launchStage(const LaunchArgs& a) {
     return hipGetLastError();
 }
 
-hipError_t launchTrack(hipStream_t stream, const Scene* scene, int mode, int mip, int linear, float tStepScale,
-                       int numSamples, int n, const float* rays, const uint32_t* seeds, float* out, int occSlot,
-                       int occBytes) {
+hipError_t launchTrack(hipStream_t stream, const Scene* scene, const EnvXf& env, int mode, int mip, int linear,
+                       float tStepScale, int numSamples, int n, const float* rays, const uint32_t* seeds, float* out,
+                       int occSlot, int occBytes) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(kTrack, dim3((n + 255) / 256), dim3(256), occBytes, stream, scene, mode, mip, linear,
+    hipLaunchKernelGGL(kTrack, dim3((n + 255) / 256), dim3(256), occBytes, stream, scene, env, mode, mip, linear,
                        tStepScale, numSamples, n, rays, seeds, out, occSlot);
     return hipGetLastError();
 }

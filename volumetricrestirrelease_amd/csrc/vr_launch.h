@@ -45,8 +45,11 @@ hipError_t launchAccumulate(hipStream_t st, const float4* frame, double4* sum, i
 hipError_t launchResolve(hipStream_t st, const double4* sum, double inv, float4* out, int n);
 int relMsePartialCount(int n);
 hipError_t launchRelMse(hipStream_t st, const float4* img, const float4* ref, int n, double* partial, double* out);
-hipError_t launchTrack(hipStream_t stream, const Scene* scene, int mode, int mip, int linear, float tStepScale,
-                       int numSamples, int n, const float* rays, const uint32_t* seeds, float* out, int occSlot,
-                       int occBytes);
+hipError_t launchTrack(hipStream_t stream, const Scene* scene, const EnvXf& env, int mode, int mip, int linear,
+                       float tStepScale, int numSamples, int n, const float* rays, const uint32_t* seeds, float* out,
+                       int occSlot, int occBytes);
+// Importance mip chain of a lat-long map: level k of dim >> k texels at imp + off[k].
+hipError_t launchEnvImportance(hipStream_t st, const float* tex, int W, int H, float* imp, const int* off, int dim,
+                               int spp);
 
 }  // namespace vr

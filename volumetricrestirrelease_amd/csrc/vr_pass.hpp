@@ -96,18 +96,111 @@ VR_DEV v3 octToDir(float px, float py) {
     float f = r * sqrtf(2.f - r * r);
     return {f * sgnf(px) * cosf(phi), f * sgnf(py) * sinf(phi), sgnf(d) * (1.f - r * r)};
 }
+// ------------------------------------------------------------------ environment map
+// A constant environment (envTextured == 0) has no orientation: light UVs hold
+// the world direction. A textured one follows EnvMap / EnvMapSampler.
+VR_DEV v3 mat3v(const float* M, v3 d) {
+    return V3(M[0] * d.x + M[1] * d.y + M[2] * d.z, M[3] * d.x + M[4] * d.y + M[5] * d.z,
+              M[6] * d.x + M[7] * d.y + M[8] * d.z);
+}
+// world_to_latlong_map (UT/Math/MathHelpers.slang:92-99); p.y is clamped to
+// [-1, 1] so acos stays defined for directions normalised to just above 1.
+VR_DEV void latlong(v3 dir, float& u, float& v) {
+    v3 p = norm3(dir);
+    u = atan2f(p.x, -p.z) * (0.5f / kPiF) + 0.5f;
+    v = acosf(fminf(fmaxf(p.y, -1.f), 1.f)) * (1.f / kPiF);
+}
+// envMap.SampleLevel(envSampler, uv, 0): bilinear over texel centres, wrap in u,
+// clamp in v (LT/EnvMap.cpp:113-121). A NaN coordinate reads black.
+VR_DEV v3 texBilinear(const float* tex, int W, int H, float u, float v) {
+    float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
+    if (!(fabsf(x) < 1e30f) || !(fabsf(y) < 1e30f)) return V3(0.f);
+    float x0f = floorf(x), y0f = floorf(y);
+    float fx = x - x0f, fy = y - y0f;
+    int x0 = (int)fmodf(x0f, (float)W);
+    if (x0 < 0) x0 += W;
+    int x1 = x0 + 1 == W ? 0 : x0 + 1;
+    const int yi = (int)fminf(fmaxf(y0f, -1.f), (float)H);
+    int y0 = min(max(yi, 0), H - 1);
+    int y1 = min(max(yi + 1, 0), H - 1);
+    gptr<float> t = gbl<float>(tex);
+    const size_t r0 = (size_t)y0 * W, r1 = (size_t)y1 * W;
+    float a[3], b[3], c[3], d[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        a[k] = t[4 * (r0 + x0) + k]; b[k] = t[4 * (r0 + x1) + k];
+        c[k] = t[4 * (r1 + x0) + k]; d[k] = t[4 * (r1 + x1) + k];
+    }
+    v3 top = V3(a[0] + (b[0] - a[0]) * fx, a[1] + (b[1] - a[1]) * fx, a[2] + (b[2] - a[2]) * fx);
+    v3 bot = V3(c[0] + (d[0] - c[0]) * fx, c[1] + (d[1] - c[1]) * fx, c[2] + (d[2] - c[2]) * fx);
+    return top + (bot - top) * fy;
+}
+VR_DEV v3 envBilinear(const Scene& S, float u, float v) { return texBilinear(S.envTex, S.envW, S.envH, u, v); }
+// EnvMap::evalLocal / eval (LT/EnvMap.slang:45-57): intensity * tint * texel.
+VR_DEV v3 envEvalLocal(const Scene& S, v3 dLocal) {
+    float u, v;
+    latlong(dLocal, u, v);
+    return ld3(S.envScale) * envBilinear(S, u, v);
+}
+VR_DEV v3 envEval(const Scene& S, const EnvXf& X, v3 dWorld) {
+    if (!S.envTextured) return ld3(S.env);
+    return envEvalLocal(S, mat3v(X.toLocal, dWorld));
+}
+// EnvMapSampler::encodeLightUV / decodeLightUV (LT/EnvMapSampler.slang:75-90).
+VR_DEV void encodeLightUV(const Scene& S, const EnvXf& X, v3 dWorld, int& id, float& u, float& v) {
+    v3 d = S.envTextured ? mat3v(X.toLocal, dWorld) : dWorld;
+    id = d.z < 0 ? -2 : -1;
+    u = d.x; v = d.y;
+}
 VR_DEV v3 decodeLightUV(float u, float v, int id) {
     v3 d{u, v, sqrtf(1 - u * u - v * v)};
     if (isnan(d.z)) d.z = 0.f;
     if (id == -2) d.z = -d.z;
     return d;
 }
+VR_DEV v3 decodeLightUV(const Scene& S, const EnvXf& X, float u, float v, int id) {
+    v3 d = decodeLightUV(u, v, id);
+    return S.envTextured ? mat3v(X.toWorld, d) : d;
+}
+// EnvMapSampler::sample (LT/EnvMapSampler.slang:94-146): hierarchical 2x2 warping
+// of (r0, r1) down the importance mip chain, then the equal-area octahedral map.
+// Returns the world direction; pdf is per solid angle.
+VR_DEV v3 envSample(const Scene& S, const EnvXf& X, float r0, float r1, float& pdf) {
+    if (!S.envTextured) {
+        pdf = kInv4PiF;
+        return octToDir(r0, r1);
+    }
+    gptr<float> imp = gbl<float>(S.envImp);
+    int px = 0, py = 0;
+    for (int mip = S.envImpBaseMip - 1; mip >= 0; mip--) {
+        px *= 2; py *= 2;
+        const int dk = S.envImpDim >> mip;
+        const size_t o = (size_t)S.envImpOff[mip] + (size_t)py * dk + px;
+        float w0 = imp[o], w1 = imp[o + 1], w2 = imp[o + dk], w3 = imp[o + dk + 1];
+        float q0 = w0 + w2, q1 = w1 + w3;
+        float d = q0 / (q0 + q1);
+        int ox;
+        if (r0 < d) { ox = 0; r0 = r0 / d; }
+        else { ox = 1; r0 = (r0 - d) / (1.f - d); }
+        float e = ox ? w1 / q1 : w0 / q0;
+        int oy;
+        if (r1 < e) { oy = 0; r1 = r1 / e; }
+        else { oy = 1; r1 = (r1 - e) / (1.f - e); }
+        px += ox; py += oy;
+    }
+    float u = ((float)px + r0) * S.envImpInvDim, v = ((float)py + r1) * S.envImpInvDim;
+    v3 dir = octToDir(u, v);
+    float avg = imp[S.envImpOff[S.envImpBaseMip]];
+    pdf = imp[(size_t)S.envImpOff[0] + (size_t)py * S.envImpDim + px] / avg * kInv4PiF;
+    return mat3v(X.toWorld, dir);
+}
+
 struct LightSample { v3 dir, Li, rayDir; float pdf, pdfArea, rayDistance; };
 
-// sampleSceneLights (VR/VolumeUtils.slang:12-149) for a constant environment
-// (uniform octahedral warp, pdf 1/(4 pi)) and point lights.
-VR_DEV bool sampleLights(const Scene& S, v3 origin, bool useEnv, bool useAnalytic, bool useEmissive, Rng& rng,
-                         LightSample& ls, int& lightIndex, float& uvx, float& uvy) {
+// sampleSceneLights (VR/VolumeUtils.slang:12-149) for the environment (constant
+// or textured) and point lights.
+VR_DEV bool sampleLights(const Scene& S, const EnvXf& X, v3 origin, bool useEnv, bool useAnalytic, bool useEmissive,
+                         Rng& rng, LightSample& ls, int& lightIndex, float& uvx, float& uvy) {
     if (!useEnv && !useAnalytic && !useEmissive) return false;
     float p0 = useEnv ? 1.f : 0.f, p1 = useAnalytic ? 1.f : 0.f, p2 = useEmissive ? 1.f : 0.f;
     float sum = p0 + p1 + p2;
@@ -119,15 +212,16 @@ VR_DEV bool sampleLights(const Scene& S, v3 origin, bool useEnv, bool useAnalyti
         if (u < p0) {
             float r0 = u01(rng);
             float r1 = u01(rng);
-            v3 dir = octToDir(r0, r1);
-            float pdf = p0 * (1.f * kInv4PiF);
+            float lpdf;
+            v3 dir = envSample(S, X, r0, r1, lpdf);
+            v3 Le = envEval(S, X, dir);
+            float pdf = p0 * lpdf;
             ls.rayDir = ls.dir = dir;
             ls.rayDistance = kTMax;
             ls.pdf = pdf;
             ls.pdfArea = pdf;
-            ls.Li = pdf > 0.f ? ld3(S.env) / pdf : V3(0.f);
-            lightIndex = dir.z < 0 ? -2 : -1;
-            uvx = dir.x; uvy = dir.y;
+            ls.Li = pdf > 0.f ? Le / pdf : V3(0.f);
+            encodeLightUV(S, X, dir, lightIndex, uvx, uvy);
             return !(isnan(dir.x) || isnan(dir.y) || isnan(dir.z));
         }
         u -= p0;
@@ -157,14 +251,49 @@ VR_DEV bool sampleLights(const Scene& S, v3 origin, bool useEnv, bool useAnalyti
     return false;
 }
 
+// ------------------------------------------------------------------ volume emission
+// TemperatureWorldSpace (VR/VolumeBase.slang:196-209): trilinear lookup of the
+// temperature grid with no bounds test beyond the grid's bricks.
+VR_DEV float temperatureWorld(const Scene& S, v3 pW) {
+    const Grid& G = S.grid[kTemperatureSlot];
+    v3 p = xformP(G.w2i, pW);
+    if (!(p.x >= 0.f && p.y >= 0.f && p.z >= 0.f)) return 0.f;
+    if (!(p.x < (float)(G.nbx * 8) && p.y < (float)(G.nby * 8) && p.z < (float)(G.nbz * 8))) return 0.f;
+    int bx = (int)p.x / 8, by = (int)p.y / 8, bz = (int)p.z / 8;
+    uint32_t b = brickLookup(G, kTemperatureSlot, bx, by, bz);
+    if (b == kEmpty) return 0.f;
+    v3 q = p - V3((float)(bx * 8), (float)(by * 8), (float)(bz * 8));
+    return sampleBrick(G, b, q, true);
+}
+// ConvertTempToColor + EmissionWorldSpace (VR/VolumeBase.slang:211-232): the
+// blackbody table is a 1-D texture over [25 K, 6425 K) sampled with a linear
+// filter and border colour 0.
+VR_DEV v3 blackbody(const Scene& S, float temp) {
+    temp = fminf(6400.f, (temp - S.tempCutoff) * S.tempScale);
+    float q = (temp - 25.f) / 6400.f;
+    const int N = S.lutRows;
+    float x = q * (float)N - 0.5f;
+    if (!(x > -1.f && x < (float)N)) return V3(0.f);  // both taps on the border (or NaN)
+    float x0f = floorf(x);
+    float f = x - x0f;
+    int i0 = (int)x0f, i1 = i0 + 1;
+    v3 a = i0 >= 0 ? ld3(S.lut + 3 * i0) : V3(0.f);
+    v3 b = i1 < N ? ld3(S.lut + 3 * i1) : V3(0.f);
+    return S.LeScale * (a + (b - a) * f);
+}
+VR_DEV v3 emissionWorld(const Scene& S, v3 pW) {
+    if (!S.hasEmission) return V3(0.f);
+    return blackbody(S, temperatureWorld(S, pW));
+}
+
 // SampleDirectLighting (VR/VolumeUtils.slang:454-492).
 template <int LM>
-VR_DEV v3 sampleDirect(const Scene& S, Rng& rng, float& pdf, const MediumInteraction& mi, const Options& o,
-                       bool enableShadow, int& lightIndex, float& uvx, float& uvy) {
+VR_DEV v3 sampleDirect(const Scene& S, const EnvXf& X, Rng& rng, float& pdf, const MediumInteraction& mi,
+                       const Options& o, bool enableShadow, int& lightIndex, float& uvx, float& uvy) {
     pdf = 0.f;
     LightSample ls;
     ls.pdfArea = 0.f;
-    bool valid = sampleLights(S, mi.p, o.useEnv, o.useAnalytic, o.useEmissive, rng, ls, lightIndex, uvx, uvy);
+    bool valid = sampleLights(S, X, mi.p, o.useEnv, o.useAnalytic, o.useEmissive, rng, ls, lightIndex, uvx, uvy);
     pdf = ls.pdfArea;
     if (!valid) { pdf = 0.f; return V3(0.f); }
     Ray sh{mi.p, ls.rayDir, 0.f, ls.rayDistance};
@@ -178,13 +307,13 @@ VR_DEV v3 sampleDirect(const Scene& S, Rng& rng, float& pdf, const MediumInterac
 }
 
 // directLighting (VR/VolumeUtils.slang:416-450): NEE of the reference path tracer.
-VR_DEV v3 directLightingPT(const Scene& S, Rng& rng, const MediumInteraction& mi, int nSamples, const Options& o,
-                           int mip, uint32_t method) {
+VR_DEV v3 directLightingPT(const Scene& S, const EnvXf& X, Rng& rng, const MediumInteraction& mi, int nSamples,
+                           const Options& o, int mip, uint32_t method) {
     v3 Ld = V3(0.f);
     for (int i = 0; i < nSamples; i++) {
         LightSample ls;
         int idx; float ux, uy;
-        bool valid = sampleLights(S, mi.p, o.useEnv, o.useAnalytic, o.useEmissive, rng, ls, idx, ux, uy);
+        bool valid = sampleLights(S, X, mi.p, o.useEnv, o.useAnalytic, o.useEmissive, rng, ls, idx, ux, uy);
         if (!valid) continue;
         Ray sh{mi.p, ls.rayDir, 0.f, ls.rayDistance};
         float vis = visibilityT<3>(S, sh, rng, 1, mip, true, method, 1.f);
@@ -254,10 +383,12 @@ struct ReplayTrace {
 };
 
 // evaluate_F_ (VR/ReSTIRHelper.slang:91-423) + evaluate_L_in_volume (:443-496).
-// `extra` points at the extra-bounce records (3 floats each) the tap indexes.
+// `extra` points at the extra-bounce records (3 floats each) the tap indexes;
+// X is the environment orientation of the frame evaluated (isLastFrame picks the
+// previous frame's).
 template <int B, class TP>
-VR_DEV v3 evaluateF(const Scene& S, Res tap, const float* extra, Ray ray, Rng& rng, const Options& opt, bool noReuse,
-                    bool isFinal, TP& tp) {
+VR_DEV v3 evaluateF(const Scene& S, const EnvXf& X, Res tap, const float* extra, Ray ray, Rng& rng, const Options& opt,
+                    bool noReuse, bool isFinal, TP& tp) {
     bool isBg = tap.depth == kTMax;
     ray.tMax = tap.depth;
     v3 F = V3(1.f), Fn = V3(1.f);
@@ -278,9 +409,9 @@ VR_DEV v3 evaluateF(const Scene& S, Res tap, const float* extra, Ray ray, Rng& r
     }
     if (anyPos(D())) {
         if (isBg) {
-            F = F * ld3(S.env);
+            F = F * envEval(S, X, ray.d);
         } else if (isSelfEmission) {
-            F = F * V3(0.f);  // emission grids are not bound for static density volumes
+            F = F * emissionWorld(S, pW);
         } else {
             bool scatterEmission = false;
             if (B > 1 && maxInd > 0) {
@@ -333,16 +464,16 @@ VR_DEV v3 evaluateF(const Scene& S, Res tap, const float* extra, Ray ray, Rng& r
                     mi.p = pW;
                     if (allZero(D())) return V3(0.f);
                 }
-                if (scatterEmission) F = F * V3(0.f);
+                if (scatterEmission) F = F * emissionWorld(S, pW);
                 else mi = {at(sr, sr.tMax), -sr.d, S.g, true};
             }
             if (!scatterEmission && anyPos(D())) {
                 Ray sh;
                 v3 Ld;
                 if (tap.lightID < 0) {
-                    v3 wi = decodeLightUV(tap.uvx, tap.uvy, tap.lightID);
+                    v3 wi = decodeLightUV(S, X, tap.uvx, tap.uvy, tap.lightID);
                     sh = {mi.p, wi, 0.f, kTMax};
-                    Ld = ld3(S.env) * phaseHG(dot3(mi.wo, wi), mi.g);
+                    Ld = envEval(S, X, wi) * phaseHG(dot3(mi.wo, wi), mi.g);
                 } else if (tap.lightID < S.numLights) {
                     const float* L = S.lights + 6 * tap.lightID;
                     v3 toL = ld3(L) - mi.p;
@@ -364,10 +495,10 @@ VR_DEV v3 evaluateF(const Scene& S, Res tap, const float* extra, Ray ray, Rng& r
 }
 // The direct form: traces every transmittance ray where evaluateF needs it.
 template <int B, int VM, int LM>
-VR_DEV v3 evaluateF(const Scene& S, const Res& tap, const float* extra, const Ray& ray, Rng& rng, const Options& opt,
-                    bool noReuse, bool isFinal) {
+VR_DEV v3 evaluateF(const Scene& S, const EnvXf& X, const Res& tap, const float* extra, const Ray& ray, Rng& rng,
+                    const Options& opt, bool noReuse, bool isFinal) {
     DirectTrace<VM, LM> tp;
-    return evaluateF<B>(S, tap, extra, ray, rng, opt, noReuse, isFinal, tp);
+    return evaluateF<B>(S, X, tap, extra, ray, rng, opt, noReuse, isFinal, tp);
 }
 
 // ComputeInitialSample (VR/ComputeInitialSample.slang:4-395). ex[] receives the
@@ -435,10 +566,11 @@ VR_DEV Res computeInitialSample(const Scene& S, const Frame& Fr, Ray ray, float 
                 out.lightID = -1;
                 out.uvx = 0.f; out.uvy = 0.f;
                 float outLightPdf = 0.f;
-                v3 Le = V3(0.f);
+                v3 Le = actualDensity > 0.f ? emissionWorld(S, mi.p) : V3(0.f);
                 v3 oma = V3(1.f) - albedo;
                 bool shadow = opt.lightSamples == 0 ? false : true;
-                v3 Ld = sampleDirect<IM>(S, rng, outLightPdf, mi, opt, shadow, out.lightID, out.uvx, out.uvy);
+                v3 Ld = sampleDirect<IM>(S, Fr.env[0], rng, outLightPdf, mi, opt, shadow, out.lightID, out.uvx,
+                                         out.uvy);
                 v3 wi = V3(0.f);
                 if (B > 1) {
                     float u0 = u01(rng);
@@ -485,7 +617,7 @@ VR_DEV Res computeInitialSample(const Scene& S, const Frame& Fr, Ray ray, float 
                 }
             } else {
                 pathPHat *= Tr;
-                float p_y = pathPHat * lum(ld3(S.env));
+                float p_y = pathPHat * lum(envEval(S, Fr.env[0], ray.d));
                 out.runningSum = out.p_y == 0.f ? 0.f : p_y / out.p_y;
                 out.p_y = p_y;
                 hitEmpty = true;

@@ -51,8 +51,9 @@ VR_DEV v3 pathTrace(const Scene& S, const Frame& Fr, Ray ray, Rng& rng, int ligh
         sampleDecomposition(S, ray, rng, mi, 0);
         if (mi.valid) {
             v3 albedo = ld3(S.sigma_s) / S.sigma_t;
+            if (S.hasEmission) L = L + emissionWorld(S, mi.p) * (V3(1.f) - albedo) * beta;
             beta = beta * albedo;
-            v3 Ld = directLightingPT(S, rng, mi, lightSamples, Fr.initialOpt, 0, 3u);
+            v3 Ld = directLightingPT(S, Fr.env[0], rng, mi, lightSamples, Fr.initialOpt, 0, 3u);
             L = L + beta * Ld;
             v3 wi = V3(0.f);
             if (maxBounces > 1) {
@@ -68,7 +69,7 @@ VR_DEV v3 pathTrace(const Scene& S, const Frame& Fr, Ray ray, Rng& rng, int ligh
                 }
             }
         } else {
-            if (bounce == 0) L = L + beta * ld3(S.env);
+            if (bounce == 0) L = L + beta * envEval(S, Fr.env[0], ray.d);
             bounce = maxBounces;
         }
     }
@@ -148,7 +149,7 @@ VR_TRACING_KERNEL void kTrace(const Scene* __restrict__ sc, const Frame fr,
             }
     }
     fin.exStart = exStart;
-    float ph = lum(evaluateF<B, VM, LM>(S, fin, outExtra, ray, rng, F.spatialOpt, false, false));
+    float ph = lum(evaluateF<B, VM, LM>(S, F.env[0], fin, outExtra, ray, rng, F.spatialOpt, false, false));
     if (fin.runningSum > 0.f) {
         fin.runningSum *= fin.p_y == 0.f ? 0.f : ph / fin.p_y;
         fin.p_y = ph;
@@ -259,7 +260,7 @@ VR_TRACING_KERNEL void kTemporal(const Scene* __restrict__ sc, const Frame fr,
             nPy = ti.p_y;
             if (isnan(ti.runningSum) || isinf(ti.runningSum)) ti.runningSum = 0.f;
             if (i > 0 && ti.runningSum != 0.f) {
-                float ph = lum(evaluateF<B, VM, LM>(S, ti, temporalExtra, ray, rng, F.spatialOpt, false, false));
+                float ph = lum(evaluateF<B, VM, LM>(S, F.env[0], ti, temporalExtra, ray, rng, F.spatialOpt, false, false));
                 float w = ph / ti.p_y;
                 if (isinf(w) || isnan(w)) w = 0.f;
                 ti.runningSum *= w;
@@ -286,7 +287,7 @@ VR_TRACING_KERNEL void kTemporal(const Scene* __restrict__ sc, const Frame fr,
                     Ray nr{ld3(F.prevCam.pos), norm3(rayDirNN(F.prevCam, rsx, rsy, W, H)), 0.f, centerPrevDepth};
                     Res tmp = ti;
                     tmp.depth = centerPrevDepth;
-                    float py = lum(evaluateF<B, VM, LM>(S, tmp, curExtra, nr, rng, F.spatialOpt, false, false));
+                    float py = lum(evaluateF<B, VM, LM>(S, F.env[1], tmp, curExtra, nr, rng, F.spatialOpt, false, false));
                     if (isinf(py) || isnan(py)) py = 0.f;
                     psum += py * cM;
                 }
@@ -379,7 +380,7 @@ VR_TRACING_KERNEL void kSpatial(const Scene* __restrict__ sc, const Frame fr, in
         Res tap = loadRes(in, ty * W + tx, B);
         float misW = 1.f;
         if (s > 0 && tap.runningSum != 0.f) {
-            float ph = lum(evaluateF<B>(S, tap, inExtra, ray, rng, F.spatialOpt, false, false, tp));
+            float ph = lum(evaluateF<B>(S, F.env[0], tap, inExtra, ray, rng, F.spatialOpt, false, false, tp));
             float w = ph / tap.p_y;
             if (isinf(w) || isnan(w)) w = 0.f;
             tap.runningSum *= w;
@@ -406,7 +407,7 @@ VR_TRACING_KERNEL void kSpatial(const Scene* __restrict__ sc, const Frame fr, in
                     psum += py2 * m2;
                 } else {
                     Ray nr{ray.o, rayDir(F.cam, t2x, t2y, W, H), 0.f, tap.depth};
-                    float py = lum(evaluateF<B>(S, tap, inExtra, nr, rng, F.spatialOpt, false, false, tp));
+                    float py = lum(evaluateF<B>(S, F.env[0], tap, inExtra, nr, rng, F.spatialOpt, false, false, tp));
                     if (isinf(py) || isnan(py)) py = 0.f;
                     psum += py * m2;
                 }
@@ -480,7 +481,7 @@ __global__ __launch_bounds__(256) void kFinal(const Scene* __restrict__ sc, cons
         Res r = loadRes(cur, px.id, B);
         if (r.runningSum > 0.f) {
             Ray ray{ld3(F.cam.pos), rayDir(F.cam, px.x, px.y, F.width, F.height), 0.f, r.depth};
-            v3 f = evaluateF<B, VM, LM>(S, r, curExtra, ray, rng, F.finalOpt, F.noReuse != 0, true);
+            v3 f = evaluateF<B, VM, LM>(S, F.env[0], r, curExtra, ray, rng, F.finalOpt, F.noReuse != 0, true);
             float Wt = r.p_y == 0.0f ? 1.f : r.runningSum / (r.p_y * r.M);
             c = c + f * Wt;
         }

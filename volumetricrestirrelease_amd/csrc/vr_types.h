@@ -8,12 +8,16 @@
 namespace vr {
 
 static const uint32_t kEmpty = 0xFFFFFFFFu;
-static const int kMaxSlots = 16;    // mips 0..7, conservative 8..15 (VR/HostDeviceSharedConstants.slang:19-20)
+static const int kMaxSlots = 17;    // mips 0..7, conservative 8..15, temperature 16
+                                    // (VR/HostDeviceSharedConstants.slang:19-23)
+static const int kTemperatureSlot = 16;
 static const int kBrick = 8;        // voxels per brick side
 static const int kApron = 10;       // stored brick side (one-voxel apron on each face)
 static const int kBrickVals = 1000; // kApron^3 values per stored brick
 static const int kNodeBricks = 16;  // bricks per level-1 node side (128 voxels)
 static const int kMaxLights = 256;
+static const int kMaxLutRows = 256;  // blackbody table rows (the reference's table has 128)
+static const int kMaxImpMips = 12;   // importance map levels: 2048^2 .. 1^2
 
 // One grid slot in HBM: a flat brick table over the grid's brick extent, an
 // occupancy table over 128^3-voxel blocks (the hierarchical-DDA level above
@@ -50,9 +54,29 @@ struct Scene {
     float superVoxelDiag;
     int32_t numMips;
     int32_t numLights;
-    float env[3];
+    float env[3];                 // constant environment radiance (used when envTextured == 0)
     float pad_;
     float lights[kMaxLights * 6];  // pos.xyz, intensity.xyz
+    // Textured lat-long environment map (LT/EnvMap.slang:34-80) and its hierarchical
+    // importance map (LT/EnvMapSampler.slang:94-167): level k (envImpDim >> k texels
+    // square, row-major) starts at envImp + envImpOff[k]; level envImpBaseMip is 1x1.
+    const float* envTex;          // envW * envH RGBA texels, row 0 at v = 0
+    const float* envImp;
+    int32_t envTextured, envW, envH, envImpDim, envImpBaseMip;
+    float envImpInvDim;
+    float envScale[3];            // EnvMapData intensity * tint
+    int32_t envImpOff[kMaxImpMips];
+    // Volume emission (VR/VolumeBase.slang:196-232): the temperature grid is slot
+    // kTemperatureSlot; lut holds lutRows RGB rows of the blackbody table.
+    int32_t hasEmission, lutRows;
+    float LeScale, tempCutoff, tempScale, pad2_;
+    float lut[kMaxLutRows * 3];
+};
+
+// Orientation of the environment map for one frame (EnvMapData::transform and
+// invTransform, LT/EnvMapData.slang:35-41): world = toWorld * local, row-major.
+struct EnvXf {
+    float toWorld[9], toLocal[9];
 };
 
 // SamplingOptions (VR/HostDeviceSharedDefinitions.h:100-138).
@@ -98,6 +122,7 @@ struct Frame {
     int32_t occSlot[2];           // grid slots whose occupancy tables the kernel keeps in LDS (-1: none)
     CameraFrame cam, prevCam;
     float prevView[16], prevProj[16];
+    EnvXf env[2];                 // environment orientation: [0] this frame, [1] the previous frame
     Options initialOpt, spatialOpt, finalOpt;
 };
 // A transmittance ray recorded by the record pass of a reuse kernel

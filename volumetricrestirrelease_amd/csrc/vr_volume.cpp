@@ -259,10 +259,42 @@ int buildVolume(const vrestir_volume_desc& d, HostVolume& out, std::string& err)
         out.slots[k].valid = out.slots[8 + k].valid = true;
     }
     out.numMips = d.numMips;
+    if (d.temperature) {
+        int rc = buildTemperatureSlot(d.temperature, d.nx, d.ny, d.nz, d.indexToWorld, fp16,
+                                      out.slots[kTemperatureSlot], err);
+        if (rc) return rc;
+    }
     const float* X = d.indexToWorld;
     auto rl2 = [&](int r) { return X[r * 4] * X[r * 4] + X[r * 4 + 1] * X[r * 4 + 1] + X[r * 4 + 2] * X[r * 4 + 2]; };
     out.tStep = (std::sqrt(rl2(0)) + std::sqrt(rl2(1)) + std::sqrt(rl2(2))) / 3.f;
     out.superVoxelDiag = 8.f * std::sqrt(rl2(0) + rl2(1) + rl2(2));
+    return VRESTIR_OK;
+}
+
+// The temperature grid of an emissive volume (slot kTemperatureSlot, loaded from
+// <name>_temperature.vbx at SC/Scene.cpp:2808-2811): one full-resolution slot with
+// its own index -> world transform. Values <= 0 are stored as 0 (they leave no
+// brick); dimensions are padded with zeros to whole bricks.
+int buildTemperatureSlot(const float* t, int nx, int ny, int nz, const float* indexToWorld, bool fp16,
+                         HostGrid& out, std::string& err) {
+    if (!t || nx <= 0 || ny <= 0 || nz <= 0 || nx > 4096 || ny > 4096 || nz > 4096) {
+        err = "temperature grid dimensions must be in [1, 4096]";
+        return VRESTIR_EINVAL;
+    }
+    DenseF g;
+    g.nx = (nx + 7) / 8 * 8; g.ny = (ny + 7) / 8 * 8; g.nz = (nz + 7) / 8 * 8;
+    g.v.assign((size_t)g.nx * g.ny * g.nz, 0.f);
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                float v = t[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)];
+                if (!std::isfinite(v)) { err = "temperature grid holds a non-finite value"; return VRESTIR_EINVAL; }
+                g.v[g.idx(x, y, z)] = v > 0.f ? v : 0.f;
+            }
+    out = HostGrid();
+    buildSlot(out, g, fp16);
+    mipTransforms(indexToWorld, 1, out.w2i);
+    out.valid = true;
     return VRESTIR_OK;
 }
 

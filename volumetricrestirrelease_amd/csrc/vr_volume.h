@@ -45,6 +45,8 @@ struct HostVolume {
 uint16_t floatToHalfBits(float f);
 float halfBitsToFloat(uint16_t h);
 int buildVolume(const vrestir_volume_desc& d, HostVolume& out, std::string& err);
+int buildTemperatureSlot(const float* t, int nx, int ny, int nz, const float* indexToWorld, bool fp16,
+                         HostGrid& out, std::string& err);
 int synthFbm(uint64_t seed, int nx, int ny, int nz, int octaves, float baseFreq, float lacunarity, float gain,
              int sphereMask, float threshold, float amplitude, float* out);
 

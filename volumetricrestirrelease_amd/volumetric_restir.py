@@ -90,14 +90,49 @@ class VolumetricReSTIR:
     def setScene(self, spec, frame: int = 0):
         d = spec.volume_desc()
         self._check(self.lib.vrestir_set_volume(self.h, ctypes.byref(d)), "set_volume")
+        if spec.temperature is not None:
+            lut, rows = spec.blackbody_array()
+            self._check(self.lib.vrestir_set_blackbody_lut(self.h, lut.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                                           rows), "set_blackbody_lut")
         arr, n, env = spec.light_array()
         self._check(self.lib.vrestir_set_lights(self.h, arr, n, env), "set_lights")
+        em = spec.env_map_desc()
+        self._check(self.lib.vrestir_set_env_map(self.h, ctypes.byref(em) if em is not None else None),
+                    "set_env_map")
         self.setCamera(spec, frame)
         self.setParams(spec.make_params())
 
     def setCamera(self, spec, frame: int = 0):
+        """Camera (and env rotation, for animated environments) of a frame."""
         c = spec.camera_struct(frame)
         self._check(self.lib.vrestir_set_camera(self.h, ctypes.byref(c)), "set_camera")
+        if spec.env_map is not None:
+            rot = (ctypes.c_float * 3)(*spec.env_rotation_at(frame))
+            self._check(self.lib.vrestir_set_env_rotation(self.h, rot), "set_env_rotation")
+
+    def setEnvMap(self, rgb, intensity=1.0, tint=(1.0, 1.0, 1.0), rotation=(0.0, 0.0, 0.0), importance_dim=0,
+                  importance_spp=0):
+        """Scene::setEnvMap + setEnvMapIntensity + setEnvMapRotation; rgb: (H, W, 3) or None."""
+        if rgb is None:
+            self._check(self.lib.vrestir_set_env_map(self.h, None), "set_env_map")
+            return
+        d = abi.EnvMapDesc()
+        arr = np.ascontiguousarray(rgb, dtype=np.float32)
+        d.height, d.width = int(arr.shape[0]), int(arr.shape[1])
+        d.rgb = arr.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        d.intensity = intensity
+        for k in range(3):
+            d.tint[k] = tint[k]
+            d.rotationDegrees[k] = rotation[k]
+        d.importanceDim, d.importanceSpp = importance_dim, importance_spp
+        self._check(self.lib.vrestir_set_env_map(self.h, ctypes.byref(d)), "set_env_map")
+
+    def setEnvMapRotation(self, degrees_xyz):
+        self._check(self.lib.vrestir_set_env_rotation(self.h, (ctypes.c_float * 3)(*degrees_xyz)), "set_env_rotation")
+
+    def setEnvMapIntensity(self, intensity, tint=(1.0, 1.0, 1.0)):
+        self._check(self.lib.vrestir_set_env_intensity(self.h, intensity, (ctypes.c_float * 3)(*tint)),
+                    "set_env_intensity")
 
     def setParams(self, p):
         st = p.struct if isinstance(p, VolumetricReSTIRParams) else p
