@@ -127,6 +127,10 @@ typedef struct vrestir_volume_desc {
                                     temperatureScale)) (VR/VolumeBase.slang:211-232; needs
                                     vrestir_set_blackbody_lut) */
     float LeScale, temperatureCutOff, temperatureScale;
+    const float* velocity;       /* optional velocity grid (same dims and transform) or NULL: 3 floats per voxel,
+                                    world-space displacement per frame, used by temporal reprojection while a
+                                    sequence animates (VR/TemporalReuse.cs.slang:161-166) */
+    float velocityScale;         /* VolumeDesc::velocityScale */
 } vrestir_volume_desc;
 
 /* A lat-long environment map (Scene::setEnvMap / setEnvMapIntensity / setEnvMapRotation
@@ -190,6 +194,21 @@ int32_t vrestir_set_blackbody_lut(vrestir_handle* h, const float* rgb, int32_t r
  * volume (addGVDBVolume hasEmission=True, SC/Scene.cpp:2789-2811). */
 int32_t vrestir_load_vbx_emission(vrestir_handle* h, const char* pathPrefix, float LeScale, float temperatureCutOff,
                                   float temperatureScale);
+/* Animated volumes (addGVDBVolumeSequence, SC/Scene.cpp:3329-3355 and the per-frame
+ * update of SC/Scene.cpp:825-863). vrestir_set_volume binds frame 0 of a new
+ * sequence; each vrestir_add_volume_frame keeps one more frame resident in HBM
+ * (material parameters stay the first frame's). vrestir_set_volume_frame makes
+ * frame `index` current with the frame before it (in sequence order, the first
+ * frame's previous is the last) in the previous-frame slots; animating != 0 is
+ * the reference's hasAnimation: temporal reuse then reprojects with the velocity
+ * grid and, with usePrevVolumeForReproj, evaluates the previous frame's samples
+ * on the previous frame's grids. */
+int32_t vrestir_add_volume_frame(vrestir_handle* h, const vrestir_volume_desc* desc, int32_t* index);
+int32_t vrestir_set_volume_frame(vrestir_handle* h, int32_t index, int32_t animating);
+int32_t vrestir_volume_frame_count(const vrestir_handle* h, int32_t* count);
+/* mFreezeFrame (VR/VolumetricReSTIR.cpp:499-780): while set, execute only runs final
+ * shading on the last reservoirs and does not advance the frame count. */
+int32_t vrestir_set_freeze_frame(vrestir_handle* h, int32_t freeze);
 
 /* ---- options: updateDict / getScriptingDictionary (VR/VolumetricReSTIR.cpp:1280-1347).
  *      Any change resets the temporal history and the frame counter, like mOptionsChanged. ---- */
@@ -254,7 +273,8 @@ enum {
     VRESTIR_TRACK_RNG_STREAM = 10,         /* first 8 uint32 of the sample generator (bit-cast to float) */
     VRESTIR_TRACK_ENV_EVAL = 11,           /* environment radiance towards dir: out rgb */
     VRESTIR_TRACK_ENV_SAMPLE = 12,         /* importance sample at u = origin.xy: out dir, pdf, Le, lightID */
-    VRESTIR_TRACK_EMISSION = 13            /* emission at the point origin: out rgb, temperature */
+    VRESTIR_TRACK_EMISSION = 13,           /* emission at the point origin: out rgb, temperature */
+    VRESTIR_TRACK_VELOCITY = 14            /* velocity grid at the point origin: out xyz */
 };
 int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t mode, int32_t mipLevel,
                             int32_t useLinearSampler, float tStepScale, int32_t numSamples, int32_t n,
@@ -290,6 +310,26 @@ int32_t vrestir_synth_fbm(uint64_t seed, int32_t nx, int32_t ny, int32_t nz, int
 /* Camera orbit about a centre (camera animation mode 2, VR/VolumetricReSTIR.cpp:879-904). */
 int32_t vrestir_orbit_camera(const vrestir_camera* base, const float center[3], int32_t frame,
                              float degreesPerFrame, vrestir_camera* out);
+/* Camera animation (VR/VolumetricReSTIR.cpp:801-905, fields as the pass's mCamera* members,
+ * defaults in VR/VolumetricReSTIR.h:228-240): the camera and freeze state after `updates`
+ * end-of-frame updates from the toggle. mode 0: sideways shake with pauses; 1: shake,
+ * forward moves and pauses; 2: rotation about the point of the view ray closest to
+ * volumeCenter. A frozen state maps to vrestir_set_freeze_frame(1). */
+typedef struct vrestir_camera_animation_desc {
+    int32_t mode;                  /* 0, 1, 2 (mCameraAnimationMode) */
+    int32_t frameInterval;         /* 50 */
+    float moveScale;               /* 0.3 */
+    float forwardScale;            /* 1 */
+    int32_t pauseInterval;         /* 150 */
+    int32_t shakeRoundsBeforePause;/* 3 */
+    int32_t shakeTotalRounds;      /* 1 */
+    int32_t rotationFrames;        /* 360 */
+    float rotationSpeed;           /* 1 degree per frame */
+    float rotationDistance;        /* <= 0: the initial distance */
+    float volumeCenter[3];         /* Scene::getSceneVolumeCenter */
+} vrestir_camera_animation_desc;
+int32_t vrestir_camera_animation(const vrestir_camera* base, const vrestir_camera_animation_desc* anim, int32_t updates,
+                                 vrestir_camera* out, int32_t* freeze);
 /* Brick statistics of a dense grid: number of 8^3 bricks with any non-zero voxel. */
 int32_t vrestir_count_bricks(const float* density, int32_t nx, int32_t ny, int32_t nz, int64_t* out);
 /* Write / read the .vbx container (gvdb-voxel-src/GVDB_FILESPEC.txt, minor version 12). */

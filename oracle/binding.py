@@ -73,6 +73,9 @@ def lib():
         "orc_set_temperature": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_float, c_float,
                                         c_float]),
         "orc_set_blackbody_lut": (c_int, [c_void_p, c_void_p, c_int]),
+        "orc_add_volume_frame": (c_int, [c_void_p, POINTER(abi.VolumeDesc)]),
+        "orc_set_volume_frame": (c_int, [c_void_p, c_int, c_int]),
+        "orc_set_freeze_frame": (c_int, [c_void_p, c_int]),
         "orc_set_env_map": (c_int, [c_void_p, c_int, c_int, c_void_p, c_float, c_void_p, c_void_p, c_int, c_int]),
         "orc_set_env_rotation": (c_int, [c_void_p, c_void_p]),
         "orc_set_env_intensity": (c_int, [c_void_p, c_float, c_void_p]),
@@ -137,6 +140,19 @@ class Oracle:
         assert self.L.orc_set_camera(self.h, ctypes.byref(c), spec.width, spec.height) == 0
         if spec.env_map is not None:
             self.set_env_rotation(spec.env_rotation_at(frame))
+
+    # ---- animated volumes
+    def add_volume_frame(self, desc):
+        d = desc.volume_desc() if hasattr(desc, "volume_desc") else desc
+        idx = self.L.orc_add_volume_frame(self.h, ctypes.byref(d))
+        assert idx > 0, idx
+        return idx
+
+    def set_volume_frame(self, index, animating=True):
+        assert self.L.orc_set_volume_frame(self.h, index, 1 if animating else 0) == 0
+
+    def set_freeze_frame(self, freeze):
+        assert self.L.orc_set_freeze_frame(self.h, 1 if freeze else 0) == 0
 
     # ---- environment map and emission
     def set_env_map(self, rgb, intensity=1.0, tint=(1.0, 1.0, 1.0), rotation=(0.0, 0.0, 0.0), dim=0, spp=0):

@@ -62,6 +62,18 @@ struct Oracle {
     // environment orientation: set now, used by the last frame, and this frame's [current, previous]
     EnvXf envCur{}, envLast{}, frameEnv[2]{};
     bool haveEnvLast = false;
+    // volume sequence: frames of slots 0..17 and the velocity field; the bound frame
+    struct VolFrame {
+        GridSlot grids[18];
+        VelocityField velocity;
+        int numMips = 0;
+        float tStep = 1.f, superVoxelDiag = 1.f;
+    };
+    std::vector<VolFrame> frames;
+    int curFrame = 0;
+    bool hasAnimation = false;
+    int prevGridOffset = 0;
+    bool freezeFrame = false;
 };
 
 static int maxB(const Oracle& O) { return O.P.maxBounces; }
@@ -86,10 +98,11 @@ static void forRows(Oracle& O, F&& fn) {
 
 // ---------------------------------------------------------------------------
 // evaluate_F_ (VR/ReSTIRHelper.slang:91-423), no surface scene, no vertex reuse.
+// mipOff = kPrevDensityOffset evaluates on the previous frame's grids
+// (useLastFrameGrid, VR/ReSTIRHelper.slang:95-96), else 0.
 static f3 evaluateF(const Oracle& O, const EnvXf& X, Reservoir tap, const f3* extraBuf, Ray ray, SampleGen& sg,
-                    const SamplingOptions& opt, bool noReuse, bool isFinalShading) {
+                    const SamplingOptions& opt, bool noReuse, bool isFinalShading, int mipOff = 0) {
     const Medium& m = O.S.med;
-    int mipOff = 0;  // previous-frame grids are not bound (static volumes)
     bool isBg = tap.depth == kRayTMax;
     ray.tMax = tap.depth;
     f3 F = mk3(1.f);
@@ -116,7 +129,7 @@ static f3 evaluateF(const Oracle& O, const EnvXf& X, Reservoir tap, const f3* ex
         if (isBg) {
             F = F * envEval(O.S, X, ray.dir);
         } else if (isSelfEmission) {
-            F = F * emissionWorld(m, pW);
+            F = F * emissionWorld(m, pW, mipOff != 0);
         } else {
             bool isScatterSelfEmission = false;
             if (maxB(O) > 1 && maxInd > 0) {
@@ -167,7 +180,7 @@ static f3 evaluateF(const Oracle& O, const EnvXf& X, Reservoir tap, const f3* ex
                     mi.p = pW;
                     if (allZero(F)) return mk3(0.f);
                 }
-                if (isScatterSelfEmission) F = F * emissionWorld(m, pW);
+                if (isScatterSelfEmission) F = F * emissionWorld(m, pW, mipOff != 0);
                 else mi = {scatterRay.at(scatterRay.tMax), -scatterRay.dir, m.g, true};
             }
             if (!isScatterSelfEmission && anyPositive(F)) {
@@ -197,8 +210,8 @@ static f3 evaluateF(const Oracle& O, const EnvXf& X, Reservoir tap, const f3* ex
 }
 
 static float evaluatePHat(const Oracle& O, const EnvXf& X, const Ray& ray, SampleGen& sg, const f3* extraBuf,
-                          const SamplingOptions& opt, const Reservoir& tap) {
-    f3 F = evaluateF(O, X, tap, extraBuf, ray, sg, opt, false, false);
+                          const SamplingOptions& opt, const Reservoir& tap, int mipOff = 0) {
+    f3 F = evaluateF(O, X, tap, extraBuf, ray, sg, opt, false, false, mipOff);
     return luminance(F);
 }
 
@@ -502,6 +515,8 @@ static void stageTemporal(Oracle& O, int curIdx) {
             if (reprojDepth == kRayTMax && O.P.temporalReprojectionMode != 2 && !isBgRes)
                 reprojDepth = rejectionSampleDepth(O.S.med, ray, sg, 8 + O.P.temporalReprojectionMipLevel);
             f3 p = ray.origin + ray.dir * reprojDepth;
+            // VR/TemporalReuse.cs.slang:161-166
+            if (O.S.med.velocity.valid && O.hasAnimation) p = p - velocityWorld(O.S.med, p) * O.S.med.velocityScale;
             float sx, sy;
             int sxi, syi;
             if (reprojDepth == kRayTMax) {
@@ -593,7 +608,7 @@ static void stageTemporal(Oracle& O, int curIdx) {
                         // isLastFrame = j > 0 (VR/TemporalReuse.cs.slang:335-338)
                         float py = evaluatePHat(O, O.frameEnv[j > 0 ? 1 : 0], nr, sg,
                                                 i == 0 ? curExtra.data() : O.temporalExtra.data(), O.spatialOpt,
-                                                taps[i]);
+                                                taps[i], j > 0 ? O.prevGridOffset : 0);
                         taps[i].depth = backup;
                         if (std::isinf(py) || std::isnan(py)) py = 0.f;
                         psum += py * cM;
@@ -824,6 +839,7 @@ static void prepareFrame(Oracle& O, int W, int H) {
                   p.useEnvironmentLights != 0, p.useAnalyticLights != 0, p.useEmissiveLights != 0,
                   p.vertexReuseStartBounce};
     O.totalRoundId = 0;
+    O.prevGridOffset = (p.usePrevVolumeForReproj && O.hasAnimation) ? kPrevDensityOffset : 0;
     // EnvMap data / prevData of this frame (SC/Scene.cpp:761-775)
     O.frameEnv[0] = O.envCur;
     O.frameEnv[1] = O.haveEnvLast ? O.envLast : O.envCur;
@@ -872,6 +888,19 @@ static void runStage(Oracle& O, int stage, int round) {
 static void executeFrame(Oracle& O, int W, int H) {
     const vrestir_params& p = O.P;
     prepareFrame(O, W, H);
+    if (O.freezeFrame) {
+        // mFreezeFrame (VR/VolumetricReSTIR.cpp:499-780): only final shading runs, on the
+        // reservoirs of totalRoundId = [spatial], with the previous frame count
+        O.totalRoundId = p.enableSpatialReuse ? 1 : 0;
+        O.frameCount--;
+        runStage(O, VRESTIR_STAGE_FINAL_SHADING, 0);
+        O.frameCount++;
+        O.temporalAccumulated = true;
+        O.S.prevCam = O.S.cam;
+        O.envLast = O.envCur;
+        O.haveEnvLast = true;
+        return;
+    }
     runStage(O, VRESTIR_STAGE_GENERATE_FEATURES, 0);
     runStage(O, VRESTIR_STAGE_TRACE_RAYS, 0);
     if (!p.useReference && p.enableTemporalReuse) runStage(O, VRESTIR_STAGE_TEMPORAL_REUSE, 0);
@@ -964,13 +993,12 @@ int orc_set_params(void* h, const vrestir_params* p) {
     return VRESTIR_OK;
 }
 
-int orc_set_volume(void* h, const vrestir_volume_desc* d) {
-    Oracle* O = (Oracle*)h;
+// One frame of a volume: mips, conservative mips, temperature and velocity.
+static int buildFrame(const vrestir_volume_desc* d, Oracle::VolFrame& f) {
     if (d->numMips < 1 || d->numMips > 8) return VRESTIR_EINVAL;
     int align = 8 << (d->numMips - 1);
     if (d->nx % align || d->ny % align || d->nz % align) return VRESTIR_EINVAL;
-    Medium& m = O->S.med;
-    for (auto& g : m.grid) g = GridSlot();
+    f = Oracle::VolFrame();
     DenseGrid base;
     base.nx = d->nx; base.ny = d->ny; base.nz = d->nz;
     base.v.assign(d->density, d->density + (size_t)d->nx * d->ny * d->nz);
@@ -980,38 +1008,117 @@ int orc_set_volume(void* h, const vrestir_volume_desc* d) {
     DenseGrid curN = base, curC = cons;
     for (int k = 0; k < d->numMips; k++) {
         if (k > 0) { curN = downsample2(curN); curC = downsample2(curC); }
-        buildSlot(m.grid[k], curN, d->indexToWorld, 1 << k, fp16);
-        buildSlot(m.grid[8 + k], curC, d->indexToWorld, 1 << k, fp16);
+        buildSlot(f.grids[k], curN, d->indexToWorld, 1 << k, fp16);
+        buildSlot(f.grids[8 + k], curC, d->indexToWorld, 1 << k, fp16);
     }
-    m.numMips = d->numMips;
+    f.numMips = d->numMips;
+    const float* X = f.grids[0].xform;
+    auto rl = [&](int r) { return std::sqrt(X[r * 4] * X[r * 4] + X[r * 4 + 1] * X[r * 4 + 1] + X[r * 4 + 2] * X[r * 4 + 2]); };
+    f.tStep = (rl(0) + rl(1) + rl(2)) / 3.f;
+    auto rd = [&](int r) { return X[r * 4] * X[r * 4] + X[r * 4 + 1] * X[r * 4 + 1] + X[r * 4 + 2] * X[r * 4 + 2]; };
+    f.superVoxelDiag = 8.f * std::sqrt(rd(0) + rd(1) + rd(2));
+    if (d->temperature) {
+        DenseGrid g;
+        g.nx = d->nx; g.ny = d->ny; g.nz = d->nz;
+        g.v.assign(d->temperature, d->temperature + (size_t)d->nx * d->ny * d->nz);
+        for (float& v : g.v) v = v > 0.f ? v : 0.f;
+        buildSlot(f.grids[kTemperatureSlot], g, d->indexToWorld, 1, fp16);
+    }
+    if (d->velocity) {
+        VelocityField& V = f.velocity;
+        V.valid = true;
+        V.nx = d->nx; V.ny = d->ny; V.nz = d->nz;
+        V.v.assign(d->velocity, d->velocity + 3 * (size_t)d->nx * d->ny * d->nz);
+        // world -> index of the grid: the inverse of indexToWorld (mip 0)
+        std::memcpy(V.w2i, f.grids[0].invxform, sizeof(V.w2i));
+        const int bnx = (V.nx + 7) / 8, bny = (V.ny + 7) / 8, bnz = (V.nz + 7) / 8;
+        V.brickOcc.assign((size_t)bnx * bny * bnz, 0);
+        for (int z = 0; z < V.nz; z++)
+            for (int y = 0; y < V.ny; y++)
+                for (int x = 0; x < V.nx; x++) {
+                    const float* q = &V.v[3 * ((size_t)x + (size_t)V.nx * ((size_t)y + (size_t)V.ny * z))];
+                    if (q[0] != 0.f || q[1] != 0.f || q[2] != 0.f)
+                        V.brickOcc[(size_t)(x / 8) + (size_t)bnx * ((size_t)(y / 8) + (size_t)bny * (z / 8))] = 1;
+                }
+    }
+    return VRESTIR_OK;
+}
+
+// Binds frame `cur` and, in the previous-frame slots, the frame before it in
+// sequence order (the first frame's previous is the last, SC/Scene.cpp:3342-3350).
+static void bindFrame(Oracle& O, int cur, bool animating) {
+    const int n = (int)O.frames.size();
+    const Oracle::VolFrame& c = O.frames[cur];
+    const Oracle::VolFrame& pv = O.frames[(cur + n - 1) % n];
+    Medium& m = O.S.med;
+    for (auto& g : m.grid) g = GridSlot();
+    for (int s = 0; s < 18; s++) m.grid[s] = c.grids[s];
+    for (int k = 0; k < 8; k++) m.grid[kPrevDensityOffset + k] = pv.grids[k];
+    m.grid[kTemperatureSlot + kPrevExtraOffset] = pv.grids[kTemperatureSlot];
+    m.velocity = c.velocity;
+    m.numMips = c.numMips;
+    m.tStep = c.tStep;
+    m.superVoxelDiag = c.superVoxelDiag;
+    m.hasEmission = c.grids[kTemperatureSlot].valid;
+    m.prevHasEmission = pv.grids[kTemperatureSlot].valid;
+    O.curFrame = cur;
+    O.hasAnimation = animating;
+}
+
+int orc_set_volume(void* h, const vrestir_volume_desc* d) {
+    Oracle* O = (Oracle*)h;
+    Oracle::VolFrame f;
+    int rc = buildFrame(d, f);
+    if (rc) return rc;
+    O->frames.clear();
+    O->frames.push_back(std::move(f));
+    Medium& m = O->S.med;
     m.sigma_s = {d->sigmaS[0], d->sigmaS[1], d->sigmaS[2]};
     m.sigma_a = {d->sigmaA[0], d->sigmaA[1], d->sigmaA[2]};
     m.sigma_t = d->sigmaS[0] + d->sigmaA[0];
     m.g = d->g;
     m.densityScale = d->densityScale;
-    const float* X = m.grid[0].xform;
-    auto rl = [&](int r) { return std::sqrt(X[r * 4] * X[r * 4] + X[r * 4 + 1] * X[r * 4 + 1] + X[r * 4 + 2] * X[r * 4 + 2]); };
-    m.tStep = (rl(0) + rl(1) + rl(2)) / 3.f;
-    auto rd = [&](int r) { return X[r * 4] * X[r * 4] + X[r * 4 + 1] * X[r * 4 + 1] + X[r * 4 + 2] * X[r * 4 + 2]; };
-    m.superVoxelDiag = 8.f * std::sqrt(rd(0) + rd(1) + rd(2));
     m.worldScaling = 1.f;
-    m.hasEmission = false;
-    if (d->temperature) {
-        int rc = orc_set_temperature(h, d->temperature, d->nx, d->ny, d->nz, d->indexToWorld, fp16 ? 1 : 0,
-                                     d->LeScale, d->temperatureCutOff, d->temperatureScale);
-        if (rc) return rc;
-    }
+    m.LeScale = d->LeScale;
+    m.tempCutoff = d->temperatureCutOff;
+    m.tempScale = d->temperatureScale;
+    m.velocityScale = d->velocityScale;
+    bindFrame(*O, 0, false);
     O->volumeSet = true;
     O->optionsChanged = true;
     return VRESTIR_OK;
 }
 
-// Temperature grid (slot 16) of an emissive volume; values <= 0 are stored as 0 and
+// addGVDBVolumeSequence: one more frame (material parameters stay the first frame's).
+int orc_add_volume_frame(void* h, const vrestir_volume_desc* d) {
+    Oracle* O = (Oracle*)h;
+    if (!O->volumeSet || d->numMips != O->frames[0].numMips) return VRESTIR_EINVAL;
+    Oracle::VolFrame f;
+    int rc = buildFrame(d, f);
+    if (rc) return rc;
+    O->frames.push_back(std::move(f));
+    bindFrame(*O, O->curFrame, O->hasAnimation);
+    return (int)O->frames.size() - 1;
+}
+
+int orc_set_volume_frame(void* h, int index, int animating) {
+    Oracle* O = (Oracle*)h;
+    if (index < 0 || index >= (int)O->frames.size()) return VRESTIR_EINVAL;
+    bindFrame(*O, index, animating != 0);
+    return VRESTIR_OK;
+}
+
+int orc_set_freeze_frame(void* h, int freeze) {
+    ((Oracle*)h)->freezeFrame = freeze != 0;
+    return VRESTIR_OK;
+}
+
+// Temperature grid (slot 16) of the current frame; values <= 0 are stored as 0 and
 // the dimensions padded with zeros to whole bricks.
 int orc_set_temperature(void* h, const float* t, int nx, int ny, int nz, const float* indexToWorld, int fp16,
                         float LeScale, float cutoff, float scale) {
     Oracle* O = (Oracle*)h;
-    if (!t || nx <= 0 || ny <= 0 || nz <= 0) return VRESTIR_EINVAL;
+    if (!t || nx <= 0 || ny <= 0 || nz <= 0 || O->frames.empty()) return VRESTIR_EINVAL;
     Medium& m = O->S.med;
     DenseGrid g;
     g.nx = (nx + 7) / 8 * 8; g.ny = (ny + 7) / 8 * 8; g.nz = (nz + 7) / 8 * 8;
@@ -1022,11 +1129,11 @@ int orc_set_temperature(void* h, const float* t, int nx, int ny, int nz, const f
                 float v = t[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)];
                 g.v[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] = v > 0.f ? v : 0.f;
             }
-    buildSlot(m.grid[kTemperatureSlot], g, indexToWorld, 1, fp16 != 0);
-    m.hasEmission = true;
+    buildSlot(O->frames[O->curFrame].grids[kTemperatureSlot], g, indexToWorld, 1, fp16 != 0);
     m.LeScale = LeScale;
     m.tempCutoff = cutoff;
     m.tempScale = scale;
+    bindFrame(*O, O->curFrame, O->hasAnimation);
     return VRESTIR_OK;
 }
 
@@ -1259,6 +1366,11 @@ int orc_track_batch(void* h, int mode, int mip, int linear, float tStepScale, in
             case VRESTIR_TRACK_ENV_EVAL: orc_env_eval(h, r + 3, o); break;
             case VRESTIR_TRACK_ENV_SAMPLE: orc_env_sample(h, r[0], r[1], o); break;
             case VRESTIR_TRACK_EMISSION: orc_emission(h, r, o); break;
+            case VRESTIR_TRACK_VELOCITY: {
+                f3 v = velocityWorld(m, ray.origin);
+                o[0] = v.x; o[1] = v.y; o[2] = v.z;
+                break;
+            }
             default: return VRESTIR_EINVAL;
         }
     }

@@ -281,8 +281,8 @@ static inline float envEvalPdf(const SceneState& S, const EnvXf& X, f3 dWorld) {
 // (VR/VolumeBase.slang:196-232): trilinear temperature lookup in the temperature
 // grid, T' = min(6400, (T - cutoff) * scale), and the blackbody table as a 1-D
 // texture over u = (T' - 25) / 6400 with a linear filter and border colour 0.
-static inline float temperatureWorld(const Medium& m, f3 pW) {
-    return valueAtPoint(m.grid[kTemperatureSlot], worldToMediumP(m, pW, kTemperatureSlot), true);
+static inline float temperatureWorld(const Medium& m, f3 pW, int slot = kTemperatureSlot) {
+    return valueAtPoint(m.grid[slot], worldToMediumP(m, pW, slot), true);
 }
 static inline f3 blackbody(const Medium& m, float temp) {
     temp = std::min(6400.f, (temp - m.tempCutoff) * m.tempScale);
@@ -296,9 +296,44 @@ static inline f3 blackbody(const Medium& m, float temp) {
     f3 a = i0 >= 0 ? m.lut[i0] : mk3(0.f), b = i1 < N ? m.lut[i1] : mk3(0.f);
     return m.LeScale * (a + (b - a) * f);
 }
-static inline f3 emissionWorld(const Medium& m, f3 pW) {
-    if (!m.hasEmission) return mk3(0.f);
-    return blackbody(m, temperatureWorld(m, pW));
+// last: the previous frame's temperature grid (useLastFrameGrid).
+static inline f3 emissionWorld(const Medium& m, f3 pW, bool last = false) {
+    if (last ? !m.prevHasEmission : !m.hasEmission) return mk3(0.f);
+    return blackbody(m, temperatureWorld(m, pW, last ? kTemperatureSlot + kPrevExtraOffset : kTemperatureSlot));
+}
+
+// VelocityWorld (VR/VolumeBase.slang:183-194) on the dense field: trilinear over
+// voxel centres with zero outside the grid, zero in a brick with no velocity.
+static inline f3 velocityWorld(const Medium& m, f3 pW) {
+    const VelocityField& V = m.velocity;
+    if (!V.valid) return mk3(0.f);
+    f4 q = mulRow({pW.x, pW.y, pW.z, 1.f}, V.w2i);
+    f3 p{q.x / q.w, q.y / q.w, q.z / q.w};
+    const int bnx = (V.nx + 7) / 8, bny = (V.ny + 7) / 8, bnz = (V.nz + 7) / 8;
+    if (!(p.x >= 0.f && p.y >= 0.f && p.z >= 0.f) || !(p.x < bnx * 8.f && p.y < bny * 8.f && p.z < bnz * 8.f))
+        return mk3(0.f);
+    int bx = (int)p.x / 8, by = (int)p.y / 8, bz = (int)p.z / 8;
+    if (!V.brickOcc[(size_t)bx + (size_t)bnx * ((size_t)by + (size_t)bny * bz)]) return mk3(0.f);
+    auto at = [&](int x, int y, int z, int c) -> float {
+        if (x < 0 || y < 0 || z < 0 || x >= V.nx || y >= V.ny || z >= V.nz) return 0.f;
+        return V.v[3 * ((size_t)x + (size_t)V.nx * ((size_t)y + (size_t)V.ny * z)) + c];
+    };
+    f3 x = p - 0.5f;
+    f3 i0 = f3floor(x);
+    f3 f = x - i0;
+    int ix = (int)i0.x, iy = (int)i0.y, iz = (int)i0.z;
+    float out[3];
+    for (int c = 0; c < 3; c++) {
+        float v000 = at(ix, iy, iz, c), v100 = at(ix + 1, iy, iz, c);
+        float v010 = at(ix, iy + 1, iz, c), v110 = at(ix + 1, iy + 1, iz, c);
+        float v001 = at(ix, iy, iz + 1, c), v101 = at(ix + 1, iy, iz + 1, c);
+        float v011 = at(ix, iy + 1, iz + 1, c), v111 = at(ix + 1, iy + 1, iz + 1, c);
+        float a0 = v000 + (v100 - v000) * f.x, a1 = v010 + (v110 - v010) * f.x;
+        float a2 = v001 + (v101 - v001) * f.x, a3 = v011 + (v111 - v011) * f.x;
+        float b0 = a0 + (a1 - a0) * f.y, b1 = a2 + (a3 - a2) * f.y;
+        out[c] = b0 + (b1 - b0) * f.z;
+    }
+    return {out[0], out[1], out[2]};
 }
 
 struct SceneLightSample { f3 dir; float distance; f3 Li; float pdf, pdfArea; f3 rayDir; float rayDistance; };

@@ -14,10 +14,27 @@ namespace orc {
 struct Ray { f3 origin, dir; float tMin, tMax; f3 at(float t) const { return origin + dir * t; } };
 
 // Medium parameters (SC/SceneTypes.slang:85-110) plus the grid slots.
-static const int kTemperatureSlot = 16;  // VR/HostDeviceSharedConstants.slang:23
+// Grid slots (VR/HostDeviceSharedConstants.slang:19-28).
+static const int kTemperatureSlot = 16;
+static const int kPrevDensityOffset = 19;  // previous frame's mips at 19 + k
+static const int kPrevExtraOffset = 11;    // previous frame's temperature at 27
+
+// A velocity field as dense voxel data (3 floats per voxel, world units per
+// frame) with the brick occupancy GVDB would store: a lookup inside a brick that
+// holds no non-zero voxel reads 0 (getNodeAtPoint finds no node).
+struct VelocityField {
+    bool valid = false;
+    int nx = 0, ny = 0, nz = 0;
+    float w2i[16];
+    std::vector<float> v;           // 3 per voxel, x fastest
+    std::vector<uint8_t> brickOcc;  // per 8^3 brick
+};
 
 struct Medium {
-    GridSlot grid[17];                      // mips 0..7, conservative 8..15, temperature 16
+    GridSlot grid[29];                      // mips 0..7, conservative 8..15, temperature 16, previous frame 19..27
+    VelocityField velocity;
+    float velocityScale = 1.f;
+    bool prevHasEmission = false;
     int numMips = 0;
     float sigma_t = 1.f;
     f3 sigma_s{1, 1, 1}, sigma_a{0, 0, 0};

@@ -16,6 +16,9 @@ int orc_set_params(void* h, const vrestir_params* p);
 int orc_set_volume(void* h, const vrestir_volume_desc* d);
 int orc_set_lights(void* h, const vrestir_point_light* lights, int n, const float env[3]);
 int orc_set_camera(void* h, const vrestir_camera* c, int width, int height);
+int orc_add_volume_frame(void* h, const vrestir_volume_desc* d);
+int orc_set_volume_frame(void* h, int index, int animating);
+int orc_set_freeze_frame(void* h, int freeze);
 int orc_set_temperature(void* h, const float* t, int nx, int ny, int nz, const float* indexToWorld, int fp16,
                         float LeScale, float cutoff, float scale);
 int orc_set_blackbody_lut(void* h, const float* rgb, int rows);

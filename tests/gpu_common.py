@@ -106,11 +106,15 @@ STAGE_MATCH = 0.995
 RADIANCE_RTOL = 1e-4
 
 
-def run_injected_frames(pair, frames=3):
+def run_injected_frames(pair, frames=3, before_frame=None):
+    """before_frame(pair, frame), when given, runs before each frame's set-up (e.g. to
+    advance an animated volume on both sides)."""
     B = pair.B
     report = []
     for frame in range(frames):
         say(f"  frame {frame}")
+        if before_frame is not None:
+            before_frame(pair, frame)
         pair.begin(frame)
         if frame > 0:
             pair.sync_history()

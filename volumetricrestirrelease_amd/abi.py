@@ -38,7 +38,7 @@ BUF_FEATURES, BUF_TEMPORAL_FEATURES, BUF_RESERVOIRS0, BUF_RESERVOIRS1, BUF_TEMPO
 TRACK_TR_ANALYTIC, TRACK_TR_RAY_MARCHING, TRACK_TR_RATIO, TRACK_TR_RESIDUAL_RATIO, \
     TRACK_TR_ANALOG_RESIDUAL_RATIO, TRACK_SAMPLE_ANALYTIC, TRACK_SAMPLE_DECOMPOSITION, \
     TRACK_SAMPLE_CELL_BY_DENSITY, TRACK_FEATURE_MARCH, TRACK_DENSITY_AT_POINT, TRACK_RNG_STREAM, \
-    TRACK_ENV_EVAL, TRACK_ENV_SAMPLE, TRACK_EMISSION = range(14)
+    TRACK_ENV_EVAL, TRACK_ENV_SAMPLE, TRACK_EMISSION, TRACK_VELOCITY = range(15)
 
 # (name, ctype) in declaration order of vrestir_params
 PARAM_FIELDS = [
@@ -80,6 +80,7 @@ class VolumeDesc(ctypes.Structure):
         ("g", c_float), ("densityScale", c_float),
         ("temperature", POINTER(c_float)),
         ("LeScale", c_float), ("temperatureCutOff", c_float), ("temperatureScale", c_float),
+        ("velocity", POINTER(c_float)), ("velocityScale", c_float),
     ]
 
 
@@ -90,6 +91,29 @@ class EnvMapDesc(ctypes.Structure):
         ("intensity", c_float), ("tint", c_float * 3), ("rotationDegrees", c_float * 3),
         ("importanceDim", c_int32), ("importanceSpp", c_int32),
     ]
+
+
+class CameraAnimationDesc(ctypes.Structure):
+    _fields_ = [
+        ("mode", c_int32), ("frameInterval", c_int32), ("moveScale", c_float), ("forwardScale", c_float),
+        ("pauseInterval", c_int32), ("shakeRoundsBeforePause", c_int32), ("shakeTotalRounds", c_int32),
+        ("rotationFrames", c_int32), ("rotationSpeed", c_float), ("rotationDistance", c_float),
+        ("volumeCenter", c_float * 3),
+    ]
+
+
+def camera_animation_desc(mode=0, **kw) -> CameraAnimationDesc:
+    """The pass's camera-animation members with their defaults (VR/VolumetricReSTIR.h:228-240)."""
+    d = CameraAnimationDesc(mode=mode, frameInterval=50, moveScale=0.3, forwardScale=1.0, pauseInterval=150,
+                            shakeRoundsBeforePause=3, shakeTotalRounds=1, rotationFrames=360, rotationSpeed=1.0,
+                            rotationDistance=0.0)
+    for k, v in kw.items():
+        if k == "volumeCenter":
+            for i in range(3):
+                d.volumeCenter[i] = v[i]
+        else:
+            setattr(d, k, v)
+    return d
 
 
 class PointLight(ctypes.Structure):
@@ -142,6 +166,12 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "vrestir_set_env_intensity": (i32, [vp, f32, POINTER(f32)]),
         "vrestir_set_blackbody_lut": (i32, [vp, POINTER(f32), i32]),
         "vrestir_load_vbx_emission": (i32, [vp, c_char_p, f32, f32, f32]),
+        "vrestir_add_volume_frame": (i32, [vp, POINTER(VolumeDesc), POINTER(i32)]),
+        "vrestir_set_volume_frame": (i32, [vp, i32, i32]),
+        "vrestir_volume_frame_count": (i32, [vp, POINTER(i32)]),
+        "vrestir_set_freeze_frame": (i32, [vp, i32]),
+        "vrestir_camera_animation": (i32, [POINTER(Camera), POINTER(CameraAnimationDesc), i32, POINTER(Camera),
+                                           POINTER(i32)]),
         "vrestir_set_params": (i32, [vp, POINTER(Params)]),
         "vrestir_get_params": (i32, [vp, POINTER(Params)]),
         "vrestir_reset_history": (i32, [vp]),
@@ -175,6 +205,8 @@ EXPORTED_SYMBOLS = [
     "vrestir_version", "vrestir_default_params", "vrestir_create", "vrestir_destroy", "vrestir_last_error",
     "vrestir_set_volume", "vrestir_load_vbx", "vrestir_set_lights", "vrestir_set_camera", "vrestir_set_env_map",
     "vrestir_set_env_rotation", "vrestir_set_env_intensity", "vrestir_set_blackbody_lut", "vrestir_load_vbx_emission",
+    "vrestir_add_volume_frame", "vrestir_set_volume_frame", "vrestir_volume_frame_count", "vrestir_set_freeze_frame",
+    "vrestir_camera_animation",
     "vrestir_set_params",
     "vrestir_get_params", "vrestir_reset_history", "vrestir_execute", "vrestir_frame_count",
     "vrestir_prepare_frame", "vrestir_run_stage", "vrestir_end_frame", "vrestir_read_buffer",

@@ -22,6 +22,15 @@ using namespace vr;
 
 static thread_local std::string g_createError;
 
+// One frame of a volume sequence: the grid descriptors of slots 0..17 and their allocations.
+struct VolumeFrame {
+    Grid grids[kFrameSlots];
+    HostGrid meta[kFrameSlots];  // metadata only (pools are released after upload)
+    std::vector<void*> allocs;
+    int numMips = 0;
+    float tStep = 1.f, superVoxelDiag = 1.f;
+};
+
 struct vrestir_handle {
     int device = 0;
     std::string err;
@@ -31,7 +40,6 @@ struct vrestir_handle {
     // scene
     Scene sceneHost;
     Scene* sceneDev = nullptr;
-    std::vector<void*> volumeAllocs;
     HostVolume meta;  // slot metadata only (pools are released after upload)
     bool volumeSet = false, camSet = false;
     vrestir_camera camDesc;
@@ -71,8 +79,11 @@ struct vrestir_handle {
     float* envImpDev = nullptr;
     EnvXf envCur{}, envLast{};
     bool haveEnvLast = false;
-    // temperature grid bound after the volume (vrestir_load_vbx_emission)
-    std::vector<void*> tempAllocs;
+    // the volume: frames of a sequence (one for a static volume), each resident in HBM
+    std::vector<VolumeFrame> frames;
+    int curFrame = 0;
+    bool hasAnimation = false;
+    bool freezeFrame = false;
 };
 
 static int fail(vrestir_handle* h, int code, const std::string& msg) {
@@ -105,10 +116,11 @@ static void freeFrameBuffers(vrestir_handle* h) {
     h->W = h->H = 0;
 }
 static void freeVolume(vrestir_handle* h) {
-    for (void* p : h->volumeAllocs) (void)hipFree(p);
-    h->volumeAllocs.clear();
-    for (void* p : h->tempAllocs) (void)hipFree(p);
-    h->tempAllocs.clear();
+    for (VolumeFrame& f : h->frames)
+        for (void* p : f.allocs) (void)hipFree(p);
+    h->frames.clear();
+    h->curFrame = 0;
+    h->hasAnimation = false;
 }
 static void freeEnv(vrestir_handle* h) {
     dfree(h->envTexDev);
@@ -294,11 +306,10 @@ static int uploadScene(vrestir_handle* h) {
     return VRESTIR_OK;
 }
 
-// Uploads one built slot into S.grid[s]; device allocations are recorded in allocs.
-static int uploadSlot(vrestir_handle* h, int s, HostGrid& hg, std::vector<void*>& allocs) {
-    Grid& G = h->sceneHost.grid[s];
+// Uploads one built slot into G (metadata into m); device allocations are recorded in allocs.
+static int uploadGrid(vrestir_handle* h, Grid& G, HostGrid& m, HostGrid& hg, std::vector<void*>& allocs) {
     std::memset(&G, 0, sizeof(Grid));
-    h->meta.slots[s] = HostGrid();
+    m = HostGrid();
     if (!hg.valid) return VRESTIR_OK;
     auto up = [&](const void* src, size_t bytes, const void** dst) -> int {
         void* p = nullptr;
@@ -329,13 +340,55 @@ static int uploadSlot(vrestir_handle* h, int s, HostGrid& hg, std::vector<void*>
     std::memcpy(G.w2i, hg.w2i, sizeof(G.w2i));
     G.maxValue = hg.maxValue;
     G.fp16 = hg.fp16 ? 1 : 0;
+    G.channels = hg.channels;
     G.valid = 1;
     // keep metadata, drop the host copies of the pools
-    HostGrid& m = h->meta.slots[s];
     m.valid = true; m.nx = hg.nx; m.ny = hg.ny; m.nz = hg.nz; m.numBricks = hg.numBricks;
-    m.maxValue = hg.maxValue; m.fp16 = hg.fp16;
+    m.maxValue = hg.maxValue; m.fp16 = hg.fp16; m.channels = hg.channels;
     m.nbx = hg.nbx; m.nby = hg.nby; m.nbz = hg.nbz;
     return VRESTIR_OK;
+}
+
+static int uploadFrame(vrestir_handle* h, HostVolume& hv, VolumeFrame& f) {
+    f.numMips = hv.numMips;
+    f.tStep = hv.tStep;
+    f.superVoxelDiag = hv.superVoxelDiag;
+    for (int s = 0; s < kFrameSlots; s++) {
+        int rc = uploadGrid(h, f.grids[s], f.meta[s], hv.slots[s], f.allocs);
+        if (rc) return rc;
+    }
+    return VRESTIR_OK;
+}
+
+// Binds frame `cur` of the sequence and, in the previous-frame slots, the frame
+// before it in sequence order (the first frame's previous is the last,
+// SC/Scene.cpp:3342-3350). animating: the sequence advanced since the last frame
+// (VolumeDesc::hasAnimation, SC/Scene.cpp:825-863).
+static int bindFrame(vrestir_handle* h, int cur, bool animating) {
+    const int n = (int)h->frames.size();
+    const VolumeFrame& c = h->frames[cur];
+    const VolumeFrame& pv = h->frames[(cur + n - 1) % n];
+    Scene& S = h->sceneHost;
+    for (int s = 0; s < kMaxSlots; s++) { std::memset(&S.grid[s], 0, sizeof(Grid)); h->meta.slots[s] = HostGrid(); }
+    for (int s = 0; s < kFrameSlots; s++) { S.grid[s] = c.grids[s]; h->meta.slots[s] = c.meta[s]; }
+    for (int k = 0; k < 8; k++) {
+        S.grid[kPrevDensityOffset + k] = pv.grids[k];
+        h->meta.slots[kPrevDensityOffset + k] = pv.meta[k];
+    }
+    for (int s : {kTemperatureSlot, kVelocitySlot}) {
+        S.grid[s + kPrevExtraOffset] = pv.grids[s];
+        h->meta.slots[s + kPrevExtraOffset] = pv.meta[s];
+    }
+    h->meta.numMips = c.numMips;
+    S.numMips = c.numMips;
+    S.tStep = c.tStep;
+    S.superVoxelDiag = c.superVoxelDiag;
+    S.hasEmission = c.grids[kTemperatureSlot].valid ? 1 : 0;
+    S.prevHasEmission = pv.grids[kTemperatureSlot].valid ? 1 : 0;
+    S.hasVelocity = c.grids[kVelocitySlot].valid ? 1 : 0;
+    h->curFrame = cur;
+    h->hasAnimation = animating;
+    return uploadScene(h);
 }
 
 extern "C" int32_t vrestir_set_volume(vrestir_handle* h, const vrestir_volume_desc* d) {
@@ -346,29 +399,61 @@ extern "C" int32_t vrestir_set_volume(vrestir_handle* h, const vrestir_volume_de
     HIPCHK(h, hipSetDevice(h->device));
     HIPCHK(h, hipDeviceSynchronize());
     freeVolume(h);
+    h->frames.emplace_back();
+    if ((rc = uploadFrame(h, hv, h->frames.back()))) return rc;
     Scene& S = h->sceneHost;
-    for (int s = 0; s < kMaxSlots; s++)
-        if ((rc = uploadSlot(h, s, hv.slots[s], h->volumeAllocs))) return rc;
-    h->meta.numMips = hv.numMips;
-    S.numMips = hv.numMips;
     S.sigma_t = d->sigmaS[0] + d->sigmaA[0];
     for (int i = 0; i < 3; i++) { S.sigma_s[i] = d->sigmaS[i]; S.sigma_a[i] = d->sigmaA[i]; }
     S.g = d->g;
     S.densityScale = d->densityScale;
-    S.tStep = hv.tStep;
     S.worldScaling = 1.f;
-    S.superVoxelDiag = hv.superVoxelDiag;
     // addGVDBVolume(hasEmission, LeScale, temperatureCutOff, temperatureScale), SC/Scene.cpp:2754-2790
-    S.hasEmission = hv.slots[kTemperatureSlot].valid ? 1 : 0;
     S.LeScale = d->LeScale;
     S.tempCutoff = d->temperatureCutOff;
     S.tempScale = d->temperatureScale;
+    S.velocityScale = d->velocityScale;
     h->volumeSet = true;
     h->optionsChanged = true;
-    return uploadScene(h);
+    return bindFrame(h, 0, false);
 }
 
-// <prefix>_temperature.vbx next to <prefix>_mip0.vbx (SC/Scene.cpp:2789-2791, 2808-2811).
+// addGVDBVolumeSequence (SC/Scene.cpp:3329-3355): one more frame of the sequence,
+// kept resident in HBM. Its material (sigma, g, density and emission scales)
+// is the first frame's, as the reference keeps them across frames (SC/Scene.cpp:829-855).
+extern "C" int32_t vrestir_add_volume_frame(vrestir_handle* h, const vrestir_volume_desc* d, int32_t* index) {
+    if (!h || !d) return VRESTIR_EINVAL;
+    if (!h->volumeSet) return fail(h, VRESTIR_ESTATE, "bind the first frame with vrestir_set_volume");
+    if (d->numMips != h->frames[0].numMips) return fail(h, VRESTIR_EINVAL, "every frame needs the first frame's numMips");
+    HostVolume hv;
+    int rc = buildVolume(*d, hv, h->err);
+    if (rc) return rc;
+    HIPCHK(h, hipSetDevice(h->device));
+    h->frames.emplace_back();
+    if ((rc = uploadFrame(h, hv, h->frames.back()))) return rc;
+    if (index) *index = (int32_t)h->frames.size() - 1;
+    return bindFrame(h, h->curFrame, h->hasAnimation);
+}
+
+// The scene's per-frame VDB update (SC/Scene.cpp:825-863): frame `index` becomes
+// current; animating = 0 is the paused state (mPauseVDBAnimation), in which the
+// previous-frame grids and the velocity grid are not used.
+extern "C" int32_t vrestir_set_volume_frame(vrestir_handle* h, int32_t index, int32_t animating) {
+    if (!h) return VRESTIR_EINVAL;
+    if (!h->volumeSet) return fail(h, VRESTIR_ESTATE, "no volume bound");
+    if (index < 0 || index >= (int)h->frames.size()) return fail(h, VRESTIR_EINVAL, "frame index out of range");
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipDeviceSynchronize());
+    return bindFrame(h, index, animating != 0);
+}
+
+extern "C" int32_t vrestir_volume_frame_count(const vrestir_handle* h, int32_t* count) {
+    if (!h || !count) return VRESTIR_EINVAL;
+    *count = (int32_t)h->frames.size();
+    return VRESTIR_OK;
+}
+
+// <prefix>_temperature.vbx next to <prefix>_mip0.vbx (SC/Scene.cpp:2789-2791, 2808-2811),
+// attached to the current frame.
 extern "C" int32_t vrestir_load_vbx_emission(vrestir_handle* h, const char* pathPrefix, float LeScale,
                                              float temperatureCutOff, float temperatureScale) {
     if (!h || !pathPrefix) return VRESTIR_EINVAL;
@@ -382,19 +467,17 @@ extern "C" int32_t vrestir_load_vbx_emission(vrestir_handle* h, const char* path
     rc = vrestir_read_vbx(path.c_str(), t.data(), &nx, &ny, &nz, i2w);
     if (rc) return fail(h, rc, "cannot read " + path);
     HostGrid hg;
-    rc = buildTemperatureSlot(t.data(), nx, ny, nz, i2w, h->meta.slots[0].fp16, hg, h->err);
+    VolumeFrame& f = h->frames[h->curFrame];
+    rc = buildTemperatureSlot(t.data(), nx, ny, nz, i2w, f.meta[0].fp16, hg, h->err);
     if (rc) return rc;
     HIPCHK(h, hipSetDevice(h->device));
     HIPCHK(h, hipDeviceSynchronize());
-    for (void* p : h->tempAllocs) (void)hipFree(p);
-    h->tempAllocs.clear();
-    if ((rc = uploadSlot(h, kTemperatureSlot, hg, h->tempAllocs))) return rc;
+    if ((rc = uploadGrid(h, f.grids[kTemperatureSlot], f.meta[kTemperatureSlot], hg, f.allocs))) return rc;
     Scene& S = h->sceneHost;
-    S.hasEmission = 1;
     S.LeScale = LeScale;
     S.tempCutoff = temperatureCutOff;
     S.tempScale = temperatureScale;
-    return uploadScene(h);
+    return bindFrame(h, h->curFrame, h->hasAnimation);
 }
 
 // The blackbody table of SC/Scene.cpp:2877-2893 (LUT/BlackBodyRadiationRGB_50K-6400K.txt,
@@ -705,6 +788,8 @@ extern "C" int32_t vrestir_prepare_frame(vrestir_handle* h, int32_t W, int32_t H
     F.prevCam = h->prevCam;
     F.env[0] = h->envCur;
     F.env[1] = h->haveEnvLast ? h->envLast : h->envCur;
+    F.hasAnimation = h->hasAnimation ? 1 : 0;
+    F.prevGridOffset = (p.usePrevVolumeForReproj && h->hasAnimation) ? kPrevDensityOffset : 0;
     std::memcpy(F.prevView, h->prevView, sizeof(F.prevView));
     std::memcpy(F.prevProj, h->prevProj, sizeof(F.prevProj));
     fillOptions(F.initialOpt, VRESTIR_ANALYTIC_TRACKING, p.initialLightingTrackingMethod, p.initialLightSamples,
@@ -885,6 +970,15 @@ extern "C" int32_t vrestir_execute(vrestir_handle* h, void* stream, int32_t W, i
     int rc = vrestir_prepare_frame(h, W, H);
     if (rc) return rc;
     const vrestir_params& p = h->P;
+    if (h->freezeFrame) {
+        // mFreezeFrame (VR/VolumetricReSTIR.cpp:499-780): only final shading runs, on the
+        // reservoirs of totalRoundId = [spatial], with the previous frame count
+        h->frame.frameCount = h->frameCount - 1;
+        h->totalRoundId = p.enableSpatialReuse ? 1 : 0;
+        if ((rc = vrestir_run_stage(h, stream, VRESTIR_STAGE_FINAL_SHADING, 0, outRGBA))) return rc;
+        h->frameCount--;  // end_frame advances it; a frozen frame does not
+        return vrestir_end_frame(h);
+    }
     if ((rc = vrestir_run_stage(h, stream, VRESTIR_STAGE_GENERATE_FEATURES, 0, nullptr))) return rc;
     if ((rc = vrestir_run_stage(h, stream, VRESTIR_STAGE_TRACE_RAYS, 0, nullptr))) return rc;
     if (!p.useReference && p.enableTemporalReuse)
@@ -896,6 +990,14 @@ extern "C" int32_t vrestir_execute(vrestir_handle* h, void* stream, int32_t W, i
         if ((rc = vrestir_run_stage(h, stream, VRESTIR_STAGE_COPY_RESERVOIRS, 0, nullptr))) return rc;
     if ((rc = vrestir_run_stage(h, stream, VRESTIR_STAGE_FINAL_SHADING, 0, outRGBA))) return rc;
     return vrestir_end_frame(h);
+}
+
+// mFreezeFrame: later executes only re-shade the last reservoirs (the camera
+// animation modes pause the pass this way).
+extern "C" int32_t vrestir_set_freeze_frame(vrestir_handle* h, int32_t freeze) {
+    if (!h) return VRESTIR_EINVAL;
+    h->freezeFrame = freeze != 0;
+    return VRESTIR_OK;
 }
 
 // ------------------------------------------------------------------ buffers (exchange format)
@@ -1028,7 +1130,7 @@ extern "C" int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t 
                                        const float* rays, const uint32_t* seeds, float* out) {
     if (!h) return VRESTIR_EINVAL;
     if (!h->volumeSet) return fail(h, VRESTIR_ESTATE, "no volume bound");
-    if (mode < 0 || mode > VRESTIR_TRACK_EMISSION) return fail(h, VRESTIR_EINVAL, "unknown tracking mode");
+    if (mode < 0 || mode > VRESTIR_TRACK_VELOCITY) return fail(h, VRESTIR_EINVAL, "unknown tracking mode");
     if (mode == VRESTIR_TRACK_EMISSION && h->sceneHost.hasEmission && h->sceneHost.lutRows == 0)
         return fail(h, VRESTIR_ESTATE, "no blackbody table bound");
     if (mip < 0 || mip >= kMaxSlots || !h->meta.slots[mip].valid) return fail(h, VRESTIR_EINVAL, "grid slot not bound");
@@ -1069,6 +1171,106 @@ extern "C" int32_t vrestir_count_bricks(const float* d, int32_t nx, int32_t ny, 
 // Orbit about `center` (camera animation mode 2, VR/VolumetricReSTIR.cpp:879-904):
 // rotate the start position about the vertical axis through the point of the
 // initial view ray closest to the centre, looking at that point.
+// The camera animation modes of VR/VolumetricReSTIR.cpp:801-905, run as the
+// reference's per-frame state machine from the toggle (framesMoved = 1, camera
+// = base, not frozen) through `updates` end-of-frame updates.
+namespace {
+struct V3d { double x, y, z; };
+V3d v3d(const float* a) { return {a[0], a[1], a[2]}; }
+V3d operator+(V3d a, V3d b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+V3d operator-(V3d a, V3d b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+V3d operator*(double s, V3d a) { return {s * a.x, s * a.y, s * a.z}; }
+double dotd(V3d a, V3d b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+V3d crossd(V3d a, V3d b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+V3d nrm(V3d a) { double l = std::sqrt(dotd(a, a)); return l > 0 ? (1.0 / l) * a : a; }
+}  // namespace
+
+extern "C" int32_t vrestir_camera_animation(const vrestir_camera* base, const vrestir_camera_animation_desc* a,
+                                            int32_t updates, vrestir_camera* out, int32_t* freeze) {
+    if (!base || !a || !out || updates < 0 || a->frameInterval <= 0) return VRESTIR_EINVAL;
+    const double kPi = 3.14159265358979323846;
+    V3d pos = v3d(base->posW), target = v3d(base->target), up = v3d(base->up);
+    const V3d bpos = pos, btarget = target;
+    V3d anchor2 = pos;
+    bool frozen = false;
+    int moved = 1;
+    const int I = a->frameInterval;
+    auto moveRight = [&](double dist, V3d anchor) {  // moveCameraRight (:256-263)
+        V3d view = nrm(target - pos);
+        V3d right = nrm(crossd(view, up));
+        pos = anchor + dist * right;
+        target = pos + view;
+    };
+    auto forward = [&](double dist, V3d anchor) {  // _forwardCameraInterval (:265-272)
+        V3d view = nrm(target - anchor);
+        V3d center = anchor + (double)a->forwardScale * view;
+        pos = center - dist * view;
+        target = pos + view;
+    };
+    for (int u = 0; u < updates; u++) {
+        if (a->mode == 0) {
+            const int roundFrames = a->shakeRoundsBeforePause * I + a->pauseInterval;
+            const int total = a->shakeTotalRounds * roundFrames;
+            const double speed = 2 * kPi / I;
+            if (moved++ < total) {
+                if (moved % roundFrames < a->shakeRoundsBeforePause * I) {
+                    frozen = false;
+                    moveRight(std::sin(speed * moved) * a->moveScale, bpos);
+                } else {
+                    frozen = true;
+                }
+            }
+        } else if (a->mode == 1) {
+            const double speed = 2 * kPi / I;
+            if (moved++ < 6 * I) {
+                if (moved < I) {
+                    moveRight(std::sin(speed * moved) * a->moveScale, bpos);
+                } else if (moved < 2 * I) {
+                    frozen = true;
+                } else if (moved < 3 * I) {
+                    frozen = false;
+                    moveRight(std::sin(speed * (moved - I)) * a->moveScale, bpos);
+                    anchor2 = pos;
+                } else if (moved < 3 * I + I / 4) {
+                    frozen = false;
+                    forward(a->forwardScale * std::cos(speed * (moved - 3 * I)), anchor2);
+                } else if (moved < 4 * I + I / 4) {
+                    frozen = true;
+                } else if (moved < 4 * I + 3 * I / 4) {
+                    frozen = false;
+                    forward(a->forwardScale * std::cos(speed * (moved - 4 * I)), anchor2);
+                } else if (moved < 5 * I + 3 * I / 4) {
+                    frozen = true;
+                } else {
+                    frozen = false;
+                    forward(a->forwardScale * std::cos(speed * (moved - 5 * I)), anchor2);
+                }
+            }
+        } else {
+            // mode 2: rotation about the point of the initial view ray closest to the volume centre
+            V3d vdir = nrm(btarget - bpos);
+            V3d rotCenter = bpos + dotd(v3d(a->volumeCenter) - bpos, vdir) * vdir;
+            if (moved++ < a->rotationFrames) {
+                frozen = false;
+                const double ang = moved * (double)a->rotationSpeed * kPi / 180.0;
+                V3d X = nrm(bpos - rotCenter);
+                V3d Y = nrm(crossd({0, 1, 0}, X));
+                X = nrm(crossd(Y, {0, 1, 0}));
+                V3d actual = rotCenter + (bpos - rotCenter) - dotd(bpos - rotCenter, X) * X;
+                V3d d = bpos - actual;
+                double dist = a->rotationDistance <= 0 ? std::sqrt(dotd(d, d)) : (double)a->rotationDistance;
+                pos = actual + dist * (std::cos(ang) * X + std::sin(ang) * Y);
+                target = rotCenter;
+            }
+        }
+    }
+    *out = *base;
+    out->posW[0] = (float)pos.x; out->posW[1] = (float)pos.y; out->posW[2] = (float)pos.z;
+    out->target[0] = (float)target.x; out->target[1] = (float)target.y; out->target[2] = (float)target.z;
+    if (freeze) *freeze = frozen ? 1 : 0;
+    return VRESTIR_OK;
+}
+
 extern "C" int32_t vrestir_orbit_camera(const vrestir_camera* base, const float center[3], int32_t frame,
                                         float degreesPerFrame, vrestir_camera* out) {
     if (!base || !center || !out) return VRESTIR_EINVAL;

@@ -82,6 +82,11 @@ __global__ __launch_bounds__(256) void kTrack(const Scene* __restrict__ sc, cons
             o[4] = L.x; o[5] = L.y; o[6] = L.z; o[7] = (float)id;
             break;
         }
+        case 14: {  // velocity at the point origin
+            v3 v = S.hasVelocity ? velocityWorld(S, ray.o) : V3(0.f);
+            o[0] = v.x; o[1] = v.y; o[2] = v.z;
+            break;
+        }
         case 13: {  // emission and temperature at the point origin
             v3 E = emissionWorld(S, ray.o);
             o[0] = E.x; o[1] = E.y; o[2] = E.z;

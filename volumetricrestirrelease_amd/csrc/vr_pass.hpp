@@ -252,18 +252,34 @@ VR_DEV bool sampleLights(const Scene& S, const EnvXf& X, v3 origin, bool useEnv,
 }
 
 // ------------------------------------------------------------------ volume emission
-// TemperatureWorldSpace (VR/VolumeBase.slang:196-209): trilinear lookup of the
-// temperature grid with no bounds test beyond the grid's bricks.
-VR_DEV float temperatureWorld(const Scene& S, v3 pW) {
-    const Grid& G = S.grid[kTemperatureSlot];
+// getValueAtPoint (GVS/gvdb.slang:6-29) in world space: trilinear lookup of a
+// grid slot with no bounds test beyond the grid's bricks. Returns the brick id
+// (kEmpty outside every brick) and the brick-local point.
+VR_DEV uint32_t brickAtWorld(const Scene& S, int slot, v3 pW, v3& q) {
+    const Grid& G = S.grid[slot];
     v3 p = xformP(G.w2i, pW);
-    if (!(p.x >= 0.f && p.y >= 0.f && p.z >= 0.f)) return 0.f;
-    if (!(p.x < (float)(G.nbx * 8) && p.y < (float)(G.nby * 8) && p.z < (float)(G.nbz * 8))) return 0.f;
+    if (!(p.x >= 0.f && p.y >= 0.f && p.z >= 0.f)) return kEmpty;
+    if (!(p.x < (float)(G.nbx * 8) && p.y < (float)(G.nby * 8) && p.z < (float)(G.nbz * 8))) return kEmpty;
     int bx = (int)p.x / 8, by = (int)p.y / 8, bz = (int)p.z / 8;
-    uint32_t b = brickLookup(G, kTemperatureSlot, bx, by, bz);
-    if (b == kEmpty) return 0.f;
-    v3 q = p - V3((float)(bx * 8), (float)(by * 8), (float)(bz * 8));
-    return sampleBrick(G, b, q, true);
+    q = p - V3((float)(bx * 8), (float)(by * 8), (float)(bz * 8));
+    return brickLookup(G, slot, bx, by, bz);
+}
+// TemperatureWorldSpace (VR/VolumeBase.slang:196-209); slot 16, or 27 for the
+// previous frame of a sequence.
+VR_DEV float temperatureWorld(const Scene& S, v3 pW, int slot = kTemperatureSlot) {
+    v3 q;
+    uint32_t b = brickAtWorld(S, slot, pW, q);
+    return b == kEmpty ? 0.f : sampleBrick(S.grid[slot], b, q, true);
+}
+// VelocityWorld (VR/VolumeBase.slang:183-194): trilinear lookup of the three
+// channels of the velocity grid. The grid holds world-space displacement per
+// frame (the reference's model-space vector times the volume's model-to-world).
+VR_DEV v3 velocityWorld(const Scene& S, v3 pW) {
+    v3 q;
+    uint32_t b = brickAtWorld(S, kVelocitySlot, pW, q);
+    if (b == kEmpty) return V3(0.f);
+    const Grid& G = S.grid[kVelocitySlot];
+    return V3(sampleBrick(G, 3 * b, q, true), sampleBrick(G, 3 * b + 1, q, true), sampleBrick(G, 3 * b + 2, q, true));
 }
 // ConvertTempToColor + EmissionWorldSpace (VR/VolumeBase.slang:211-232): the
 // blackbody table is a 1-D texture over [25 K, 6425 K) sampled with a linear
@@ -281,9 +297,10 @@ VR_DEV v3 blackbody(const Scene& S, float temp) {
     v3 b = i1 < N ? ld3(S.lut + 3 * i1) : V3(0.f);
     return S.LeScale * (a + (b - a) * f);
 }
-VR_DEV v3 emissionWorld(const Scene& S, v3 pW) {
-    if (!S.hasEmission) return V3(0.f);
-    return blackbody(S, temperatureWorld(S, pW));
+// last: the previous frame's temperature grid (useLastFrameGrid).
+VR_DEV v3 emissionWorld(const Scene& S, v3 pW, bool last = false) {
+    if (last ? !S.prevHasEmission : !S.hasEmission) return V3(0.f);
+    return blackbody(S, temperatureWorld(S, pW, last ? kTemperatureSlot + kPrevExtraOffset : kTemperatureSlot));
 }
 
 // SampleDirectLighting (VR/VolumeUtils.slang:454-492).
@@ -341,13 +358,14 @@ VR_DEV v3 directLightingPT(const Scene& S, const EnvXf& X, Rng& rng, const Mediu
 template <int VM, int LM>
 struct DirectTrace {
     static constexpr bool kSplit = false;
-    // kind 0: visibility options; 1: lighting options (grid 0 in final shading)
-    VR_DEV float vis(const Scene& S, const Ray& r, Rng& rng, const Options& o, int kind, bool isFinal) {
+    // kind 0: visibility options; 1: lighting options (grid 0 in final shading).
+    // off: kPrevDensityOffset for the previous frame's grids, else 0.
+    VR_DEV float vis(const Scene& S, const Ray& r, Rng& rng, const Options& o, int kind, bool isFinal, int off) {
         if (kind == 0)
-            return visibilityT<VM>(S, r, rng, o.visibilitySamples, o.visibilityMipLevel,
+            return visibilityT<VM>(S, r, rng, o.visibilitySamples, o.visibilityMipLevel + off,
                                    o.visibilityUseLinearSampler != 0, o.visibilityTrackingMethod,
                                    o.visibilityTStepScale);
-        return visibilityT<LM>(S, r, rng, o.lightSamples, isFinal ? 0 : o.lightingMipLevel,
+        return visibilityT<LM>(S, r, rng, o.lightSamples, isFinal ? 0 : o.lightingMipLevel + off,
                                o.lightingUseLinearSampler != 0, o.lightingTrackingMethod, o.lightingTStepScale);
     }
 };
@@ -361,7 +379,7 @@ struct RecordTrace {
     static constexpr bool kSplit = true;
     TraceRec* out;  // this pixel's record slots
     int n;          // records written
-    VR_DEV float vis(const Scene&, const Ray& r, Rng&, const Options& o, int kind, bool) {
+    VR_DEV float vis(const Scene&, const Ray& r, Rng&, const Options& o, int kind, bool, int) {
         const int samples = kind == 0 ? o.visibilitySamples : o.lightSamples;
         if (samples <= 0) return sameSamples(1.f, samples);
         TraceRec rec;
@@ -375,7 +393,7 @@ struct ReplayTrace {
     static constexpr bool kSplit = true;
     const float* tr;  // this pixel's transmittances, in record order
     int n;
-    VR_DEV float vis(const Scene&, const Ray&, Rng&, const Options& o, int kind, bool) {
+    VR_DEV float vis(const Scene&, const Ray&, Rng&, const Options& o, int kind, bool, int) {
         const int samples = kind == 0 ? o.visibilitySamples : o.lightSamples;
         if (samples <= 0) return sameSamples(1.f, samples);
         return sameSamples(tr[n++], samples);
@@ -385,10 +403,11 @@ struct ReplayTrace {
 // evaluate_F_ (VR/ReSTIRHelper.slang:91-423) + evaluate_L_in_volume (:443-496).
 // `extra` points at the extra-bounce records (3 floats each) the tap indexes;
 // X is the environment orientation of the frame evaluated (isLastFrame picks the
-// previous frame's).
+// previous frame's); gridOff = kPrevDensityOffset evaluates on the previous
+// frame's grids (useLastFrameGrid, VR/ReSTIRHelper.slang:95-96), else 0.
 template <int B, class TP>
 VR_DEV v3 evaluateF(const Scene& S, const EnvXf& X, Res tap, const float* extra, Ray ray, Rng& rng, const Options& opt,
-                    bool noReuse, bool isFinal, TP& tp) {
+                    bool noReuse, bool isFinal, TP& tp, int gridOff = 0) {
     bool isBg = tap.depth == kTMax;
     ray.tMax = tap.depth;
     v3 F = V3(1.f), Fn = V3(1.f);
@@ -399,9 +418,9 @@ VR_DEV v3 evaluateF(const Scene& S, const EnvXf& X, Res tap, const float* extra,
     v3 pW = at(ray, ray.tMax);
     MediumInteraction mi{pW, -ray.d, S.g, true};
     {
-        float density = (isBg || noReuse) ? 1.f : densityWorld(S, 0, pW);
+        float density = (isBg || noReuse) ? 1.f : densityWorld(S, gridOff, pW);
         if (density == 0.f) return V3(0.f);
-        if (!noReuse) vis = tp.vis(S, ray, rng, opt, 0, isFinal);
+        if (!noReuse) vis = tp.vis(S, ray, rng, opt, 0, isFinal, gridOff);
         v3 ss = isBg ? V3(1.f) : (isSelfEmission ? ld3(S.sigma_a) : ld3(S.sigma_s));
         if (noReuse && !isBg) ss = ss / S.sigma_t;
         F = F * (V3(vis * density) * ss);
@@ -411,7 +430,7 @@ VR_DEV v3 evaluateF(const Scene& S, const EnvXf& X, Res tap, const float* extra,
         if (isBg) {
             F = F * envEval(S, X, ray.d);
         } else if (isSelfEmission) {
-            F = F * emissionWorld(S, pW);
+            F = F * emissionWorld(S, pW, gridOff != 0);
         } else {
             bool scatterEmission = false;
             if (B > 1 && maxInd > 0) {
@@ -445,7 +464,7 @@ VR_DEV v3 evaluateF(const Scene& S, const EnvXf& X, Res tap, const float* extra,
                     float sd;
                     if (!noReuse) {
                         ss = curEmissive ? ld3(S.sigma_a) : ld3(S.sigma_s);
-                        sd = fmaxf(0.f, densityWorld(S, 0, pW));
+                        sd = fmaxf(0.f, densityWorld(S, gridOff, pW));
                     } else {
                         ss = curEmissive ? ld3(S.sigma_a) / S.sigma_t : ld3(S.sigma_s) / S.sigma_t;
                         sd = 1.f;
@@ -458,13 +477,13 @@ VR_DEV v3 evaluateF(const Scene& S, const EnvXf& X, Res tap, const float* extra,
                     }
                     if (allZero(D())) return V3(0.f);
                     float sv = 1.f;
-                    if (!noReuse) sv = tp.vis(S, sr, rng, opt, 0, isFinal);
+                    if (!noReuse) sv = tp.vis(S, sr, rng, opt, 0, isFinal, gridOff);
                     F = F * sv;
                     mi.wo = -sr.d;
                     mi.p = pW;
                     if (allZero(D())) return V3(0.f);
                 }
-                if (scatterEmission) F = F * emissionWorld(S, pW);
+                if (scatterEmission) F = F * emissionWorld(S, pW, gridOff != 0);
                 else mi = {at(sr, sr.tMax), -sr.d, S.g, true};
             }
             if (!scatterEmission && anyPos(D())) {
@@ -486,7 +505,7 @@ VR_DEV v3 evaluateF(const Scene& S, const EnvXf& X, Res tap, const float* extra,
                 } else {
                     return V3(0.f);
                 }
-                float Tr = tp.vis(S, sh, rng, opt, 1, isFinal);
+                float Tr = tp.vis(S, sh, rng, opt, 1, isFinal, gridOff);
                 F = F * (Ld * Tr);
             }
         }
@@ -496,9 +515,9 @@ VR_DEV v3 evaluateF(const Scene& S, const EnvXf& X, Res tap, const float* extra,
 // The direct form: traces every transmittance ray where evaluateF needs it.
 template <int B, int VM, int LM>
 VR_DEV v3 evaluateF(const Scene& S, const EnvXf& X, const Res& tap, const float* extra, const Ray& ray, Rng& rng,
-                    const Options& opt, bool noReuse, bool isFinal) {
+                    const Options& opt, bool noReuse, bool isFinal, int gridOff = 0) {
     DirectTrace<VM, LM> tp;
-    return evaluateF<B>(S, X, tap, extra, ray, rng, opt, noReuse, isFinal, tp);
+    return evaluateF<B>(S, X, tap, extra, ray, rng, opt, noReuse, isFinal, tp, gridOff);
 }
 
 // ComputeInitialSample (VR/ComputeInitialSample.slang:4-395). ex[] receives the

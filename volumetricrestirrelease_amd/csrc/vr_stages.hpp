@@ -197,6 +197,8 @@ VR_TRACING_KERNEL void kTemporal(const Scene* __restrict__ sc, const Frame fr,
         if (reprojDepth == kTMax && F.reprojectionMode != 2 && !isBgRes)
             reprojDepth = rejectionSampleDepth(S, ray, rng, 8 + F.reprojectionMip);
         v3 p = ray.o + ray.d * reprojDepth;
+        // last frame's position of a moving medium (VR/TemporalReuse.cs.slang:161-166)
+        if (S.hasVelocity && F.hasAnimation) p = p - velocityWorld(S, p) * S.velocityScale;
         const float* V = F.prevView;
         const float* P = F.prevProj;
         float vx = p.x * V[0] + p.y * V[4] + p.z * V[8] + V[12];
@@ -287,7 +289,9 @@ VR_TRACING_KERNEL void kTemporal(const Scene* __restrict__ sc, const Frame fr,
                     Ray nr{ld3(F.prevCam.pos), norm3(rayDirNN(F.prevCam, rsx, rsy, W, H)), 0.f, centerPrevDepth};
                     Res tmp = ti;
                     tmp.depth = centerPrevDepth;
-                    float py = lum(evaluateF<B, VM, LM>(S, F.env[1], tmp, curExtra, nr, rng, F.spatialOpt, false, false));
+                    // isLastFrame (VR/TemporalReuse.cs.slang:335-338): previous orientation and grids
+                    float py = lum(evaluateF<B, VM, LM>(S, F.env[1], tmp, curExtra, nr, rng, F.spatialOpt, false, false,
+                                                        F.prevGridOffset));
                     if (isinf(py) || isnan(py)) py = 0.f;
                     psum += py * cM;
                 }

@@ -8,9 +8,16 @@
 namespace vr {
 
 static const uint32_t kEmpty = 0xFFFFFFFFu;
-static const int kMaxSlots = 17;    // mips 0..7, conservative 8..15, temperature 16
-                                    // (VR/HostDeviceSharedConstants.slang:19-23)
+// Grid slots (VR/HostDeviceSharedConstants.slang:19-28): mips 0..7, conservative
+// mips 8..15, temperature 16, velocity 17 (18, the super-voxel grid, is not used),
+// and the previous frame of an animated sequence: mips at 19 + k, temperature at
+// 16 + 11 = 27, velocity at 28.
+static const int kMaxSlots = 29;
 static const int kTemperatureSlot = 16;
+static const int kVelocitySlot = 17;
+static const int kFrameSlots = 18;           // slots one frame of a sequence owns
+static const int kPrevDensityOffset = 19;    // kPrevFrameDensityGridOffset
+static const int kPrevExtraOffset = 11;      // kPrevFrameExtraGridOffset
 static const int kBrick = 8;        // voxels per brick side
 static const int kApron = 10;       // stored brick side (one-voxel apron on each face)
 static const int kBrickVals = 1000; // kApron^3 values per stored brick
@@ -36,6 +43,8 @@ struct Grid {
     int32_t valid;
     int32_t occWords;             // entries of occ
     uint32_t numBricks;
+    int32_t channels;             // values per voxel: 1, or 3 for the velocity grid (pool: [brick][channel][1000])
+    int32_t pad_;
     // Brick occupancy as 32-bit words over the brick-table index, each with the
     // number of occupied bricks in all earlier words: brick ids are assigned in
     // table order, so id = prefix + popcount(word bits below the brick's bit).
@@ -70,6 +79,9 @@ struct Scene {
     // kTemperatureSlot; lut holds lutRows RGB rows of the blackbody table.
     int32_t hasEmission, lutRows;
     float LeScale, tempCutoff, tempScale, pad2_;
+    // animated sequences: the previous frame's emission flag, a velocity grid (world units per frame)
+    int32_t prevHasEmission, hasVelocity;
+    float velocityScale, pad3_;
     float lut[kMaxLutRows * 3];
 };
 
@@ -123,6 +135,8 @@ struct Frame {
     CameraFrame cam, prevCam;
     float prevView[16], prevProj[16];
     EnvXf env[2];                 // environment orientation: [0] this frame, [1] the previous frame
+    int32_t hasAnimation;         // the volume advanced a frame of its sequence since the last frame
+    int32_t prevGridOffset;       // kPrevDensityOffset when previous-frame grids serve isLastFrame lookups, else 0
     Options initialOpt, spatialOpt, finalOpt;
 };
 // A transmittance ray recorded by the record pass of a reuse kernel

@@ -264,6 +264,10 @@ int buildVolume(const vrestir_volume_desc& d, HostVolume& out, std::string& err)
                                       out.slots[kTemperatureSlot], err);
         if (rc) return rc;
     }
+    if (d.velocity) {
+        int rc = buildVelocitySlot(d.velocity, d.nx, d.ny, d.nz, d.indexToWorld, out.slots[kVelocitySlot], err);
+        if (rc) return rc;
+    }
     const float* X = d.indexToWorld;
     auto rl2 = [&](int r) { return X[r * 4] * X[r * 4] + X[r * 4 + 1] * X[r * 4 + 1] + X[r * 4 + 2] * X[r * 4 + 2]; };
     out.tStep = (std::sqrt(rl2(0)) + std::sqrt(rl2(1)) + std::sqrt(rl2(2))) / 3.f;
@@ -295,6 +299,78 @@ int buildTemperatureSlot(const float* t, int nx, int ny, int nz, const float* in
     buildSlot(out, g, fp16);
     mipTransforms(indexToWorld, 1, out.w2i);
     out.valid = true;
+    return VRESTIR_OK;
+}
+
+// The velocity grid of an animated volume (slot kVelocitySlot; the reference loads
+// <name>_velocity_{x,y,z}.vbx, SC/Scene.cpp:2812-2820): three fp32 channels per
+// voxel sharing one brick topology (a brick exists where any component is
+// non-zero), pool laid out [brick][channel][10^3 apron values].
+int buildVelocitySlot(const float* v, int nx, int ny, int nz, const float* indexToWorld, HostGrid& g,
+                      std::string& err) {
+    if (!v || nx <= 0 || ny <= 0 || nz <= 0 || nx > 4096 || ny > 4096 || nz > 4096) {
+        err = "velocity grid dimensions must be in [1, 4096]";
+        return VRESTIR_EINVAL;
+    }
+    g = HostGrid();
+    const int px = (nx + 7) / 8 * 8, py = (ny + 7) / 8 * 8, pz = (nz + 7) / 8 * 8;
+    auto at = [&](int x, int y, int z, int c) -> float {
+        if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) return 0.f;
+        return v[3 * ((size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)) + c];
+    };
+    for (size_t i = 0; i < 3 * (size_t)nx * ny * nz; i++)
+        if (!std::isfinite(v[i])) { err = "velocity grid holds a non-finite value"; return VRESTIR_EINVAL; }
+    g.nx = px; g.ny = py; g.nz = pz;
+    g.fp16 = false;
+    g.nbx = px / 8; g.nby = py / 8; g.nbz = pz / 8;
+    g.nnx = (g.nbx + 15) / 16; g.nny = (g.nby + 15) / 16; g.nnz = (g.nbz + 15) / 16;
+    const size_t nbTot = (size_t)g.nbx * g.nby * g.nbz;
+    g.brickTable.assign(nbTot, kEmpty);
+    g.nodeTable.assign((size_t)g.nnx * g.nny * g.nnz, 0u);
+    std::vector<int32_t> brickPos;
+    uint32_t nb = 0;
+    for (int bz = 0; bz < g.nbz; bz++)
+        for (int by = 0; by < g.nby; by++)
+            for (int bx = 0; bx < g.nbx; bx++) {
+                bool any = false;
+                for (int z = 0; z < 8 && !any; z++)
+                    for (int y = 0; y < 8 && !any; y++)
+                        for (int x = 0; x < 8 && !any; x++)
+                            for (int c = 0; c < 3 && !any; c++)
+                                if (at(bx * 8 + x, by * 8 + y, bz * 8 + z, c) != 0.f) any = true;
+                if (!any) continue;
+                g.brickTable[bx + (size_t)g.nbx * (by + (size_t)g.nby * bz)] = nb++;
+                brickPos.push_back(bx); brickPos.push_back(by); brickPos.push_back(bz);
+                g.nodeTable[(bx / 16) + (size_t)g.nnx * ((by / 16) + (size_t)g.nny * (bz / 16))] = 1u;
+            }
+    g.numBricks = nb;
+    const size_t words = (nbTot + 31) / 32;
+    g.occ.assign(2 * words, 0u);
+    uint32_t before = 0;
+    for (size_t w = 0; w < words; w++) {
+        uint32_t bits = 0;
+        for (int b = 0; b < 32; b++) {
+            size_t c = w * 32 + b;
+            if (c < nbTot && g.brickTable[c] != kEmpty) bits |= 1u << b;
+        }
+        g.occ[2 * w] = bits;
+        g.occ[2 * w + 1] = before;
+        before += (uint32_t)__builtin_popcount(bits);
+    }
+    g.bounds.assign((size_t)nb * 4, 0.f);
+    g.pool32.assign((size_t)nb * 3 * kBrickVals, 0.f);
+    parallelFor((int)nb, [&](int id) {
+        int x0 = brickPos[3 * id] * 8, y0 = brickPos[3 * id + 1] * 8, z0 = brickPos[3 * id + 2] * 8;
+        for (int c = 0; c < 3; c++)
+            for (int k = -1; k <= 8; k++)
+                for (int j = -1; j <= 8; j++)
+                    for (int i = -1; i <= 8; i++)
+                        g.pool32[((size_t)id * 3 + c) * kBrickVals + (size_t)(i + 1) +
+                                 kApron * ((size_t)(j + 1) + kApron * (size_t)(k + 1))] = at(x0 + i, y0 + j, z0 + k, c);
+    });
+    g.channels = 3;
+    mipTransforms(indexToWorld, 1, g.w2i);
+    g.valid = true;
     return VRESTIR_OK;
 }
 

@@ -28,6 +28,7 @@ struct HostGrid {
     float maxValue = 0.f;
     float w2i[16];
     uint32_t numBricks = 0;
+    int channels = 1;
     std::vector<uint32_t> brickTable, nodeTable;
     std::vector<uint32_t> occ;    // {word, prefix} pairs over the brick-table index
     std::vector<float> bounds;
@@ -47,6 +48,8 @@ float halfBitsToFloat(uint16_t h);
 int buildVolume(const vrestir_volume_desc& d, HostVolume& out, std::string& err);
 int buildTemperatureSlot(const float* t, int nx, int ny, int nz, const float* indexToWorld, bool fp16,
                          HostGrid& out, std::string& err);
+int buildVelocitySlot(const float* v, int nx, int ny, int nz, const float* indexToWorld, HostGrid& out,
+                      std::string& err);
 int synthFbm(uint64_t seed, int nx, int ny, int nz, int octaves, float baseFreq, float lacunarity, float gain,
              int sphereMask, float threshold, float amplitude, float* out);
 

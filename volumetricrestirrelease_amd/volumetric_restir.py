@@ -127,6 +127,20 @@ class VolumetricReSTIR:
         d.importanceDim, d.importanceSpp = importance_dim, importance_spp
         self._check(self.lib.vrestir_set_env_map(self.h, ctypes.byref(d)), "set_env_map")
 
+    # ---- animated volumes (addGVDBVolumeSequence, SC/Scene.cpp:3329-3355)
+    def addVolumeFrame(self, desc) -> int:
+        """Append one frame (an abi.VolumeDesc, or a SceneSpec) to the bound sequence."""
+        d = desc.volume_desc() if hasattr(desc, "volume_desc") else desc
+        idx = ctypes.c_int32()
+        self._check(self.lib.vrestir_add_volume_frame(self.h, ctypes.byref(d), ctypes.byref(idx)), "add_volume_frame")
+        return idx.value
+
+    def setVolumeFrame(self, index: int, animating: bool = True):
+        self._check(self.lib.vrestir_set_volume_frame(self.h, index, 1 if animating else 0), "set_volume_frame")
+
+    def setFreezeFrame(self, freeze: bool):
+        self._check(self.lib.vrestir_set_freeze_frame(self.h, 1 if freeze else 0), "set_freeze_frame")
+
     def setEnvMapRotation(self, degrees_xyz):
         self._check(self.lib.vrestir_set_env_rotation(self.h, (ctypes.c_float * 3)(*degrees_xyz)), "set_env_rotation")
 
