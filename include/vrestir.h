@@ -281,11 +281,12 @@ int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t mode, int32
                             const float* raysDev, const uint32_t* seedsDev, float* outDev);
 
 /* ---- implementation choices (no counterpart in the reference) ----
- * VRESTIR_TUNE_SPLIT_REUSE (default 0): 1 runs spatial reuse as three launches
- * -- record its transmittance rays, trace them from per-wave balanced queues,
- * replay the reuse with their values -- when its ray kinds use ray marching;
- * 0 runs it as one kernel. Results are the same either way; on the 1080p
- * cloud workload both take the same time. */
+ * VRESTIR_TUNE_SPLIT_REUSE (default 2): 1 and 2 run spatial reuse as three
+ * launches -- record its transmittance rays, trace them from per-wave balanced
+ * queues, replay the reuse with their values -- when its ray kinds use ray
+ * marching; 1 traces each ray in one lane, 2 walks rays and marches their brick
+ * segments in separate phases; 0 runs it as one kernel. Results agree to float
+ * rounding (2 sums each ray's optical depth per brick). */
 enum { VRESTIR_TUNE_SPLIT_REUSE = 1 };
 int32_t vrestir_set_tuning(vrestir_handle* h, int32_t key, int32_t value);
 

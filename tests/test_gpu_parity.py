@@ -84,15 +84,19 @@ def test_whole_frames_uninjected():
         assert abs(co[:, :3].mean() - cg[:, :3].mean()) <= 2e-3 * co[:, :3].mean()
 
 
-def test_split_spatial_reuse_matches_single_kernel():
-    say("case: spatial reuse as record / balanced trace / replay vs one kernel, fp16 b=3, 2 rounds, 3 frames")
+@pytest.mark.parametrize("mode,rtol,frac_min", [(1, 1e-5, 0.999), (2, 1e-4, 0.99)])
+def test_split_spatial_reuse_matches_single_kernel(mode, rtol, frac_min):
+    """Mode 1 traces each recorded ray in one lane: the same arithmetic as the
+    single kernel. Mode 2 sums each ray's optical depth per brick segment, so its
+    transmittances differ by float rounding and a few reservoir selections flip."""
+    say(f"case: spatial reuse as record / trace (mode {mode}) / replay vs one kernel, fp16 b=3, 2 rounds, 3 frames")
     import torch
     from volumetricrestirrelease_amd.volumetric_restir import VolumetricReSTIR
     spec = small_scene(fp16=True, maxBounces=3, initialM=8, temporalReuseMThreshold=2.5, spatialReuseRounds=2,
                        spatialSampleCount=6)
     spec.orbit_degrees_per_frame = 2.0
     outs = []
-    for split in (0, 1):
+    for split in (0, mode):
         g = VolumetricReSTIR()
         g.setScene(spec)
         assert abi.load_library().vrestir_set_tuning(g.h, abi.TUNE_SPLIT_REUSE, split) == 0
@@ -106,6 +110,6 @@ def test_split_spatial_reuse_matches_single_kernel():
         outs.append(frames)
     for f in range(3):
         a, b = outs[0][f][..., :3], outs[1][f][..., :3]
-        frac = float((rel(a.reshape(-1, 3), b.reshape(-1, 3)).max(axis=1) <= 1e-5).mean())
-        say(f"   frame {f}: pixels equal within 1e-5: {frac:.4f}")
-        assert frac >= 0.999
+        frac = float((rel(a.reshape(-1, 3), b.reshape(-1, 3)).max(axis=1) <= rtol).mean())
+        say(f"   frame {f}: pixels equal within {rtol}: {frac:.4f}")
+        assert frac >= frac_min

@@ -71,7 +71,7 @@ struct vrestir_handle {
     int relPartialN = 0;
     double* relOut = nullptr;
     // record / trace / replay buffers of the reuse stages
-    bool splitReuse = false;
+    int splitReuse = 2;  // 0: one kernel; 1: balanced walker queue; 2: brick-segment trace
     SplitBufs split{};
     size_t splitPixels = 0;
     // textured environment map: texels, importance mip chain, orientation now and at the last frame
@@ -839,6 +839,7 @@ static int occPlan(const vrestir_handle* h, int a, int b, int slots[2]) {
 static const size_t kSplitBudget = (size_t)48 << 30;
 static int splitPlan(vrestir_handle* h, LaunchArgs& a) {
     a.splitTK = 0;
+    a.splitMode = 0;
     const vrestir_params& p = h->P;
     const Options& so = h->frame.spatialOpt;
     if (!h->splitReuse || so.visibilityTrackingMethod != 2u || so.lightingTrackingMethod != 2u) return VRESTIR_OK;
@@ -859,6 +860,7 @@ static int splitPlan(vrestir_handle* h, LaunchArgs& a) {
     }
     a.split = h->split;
     a.splitTK = 2;
+    a.splitMode = h->splitReuse;
     return VRESTIR_OK;
 }
 
@@ -866,7 +868,10 @@ static int splitPlan(vrestir_handle* h, LaunchArgs& a) {
 extern "C" int32_t vrestir_set_tuning(vrestir_handle* h, int32_t key, int32_t value) {
     if (!h) return VRESTIR_EINVAL;
     switch (key) {
-        case VRESTIR_TUNE_SPLIT_REUSE: h->splitReuse = value != 0; return VRESTIR_OK;
+        case VRESTIR_TUNE_SPLIT_REUSE:
+            if (value < 0 || value > 2) return fail(h, VRESTIR_EINVAL, "split reuse mode must be 0, 1 or 2");
+            h->splitReuse = value;
+            return VRESTIR_OK;
         default: return fail(h, VRESTIR_EINVAL, "unknown tuning key");
     }
 }

@@ -79,6 +79,280 @@ __global__ __launch_bounds__(256) void kTraceQueue(const Scene* __restrict__ sc,
     }
 }
 
+// ------------------------------------------------------------------ brick-segment trace
+// The same queue of recorded rays, traced in two alternating phases so that the
+// lanes of a wave run one code path at a time:
+//   * walk: every lane steps the hierarchical DDA of its own ray, one cell per
+//     iteration, and pushes each occupied brick it enters (with the march's first
+//     sample) to a per-wave segment queue in LDS; a lane whose walk is over takes
+//     the next ray of the queue;
+//   * march: lanes take segments from the queue, a new one as soon as theirs is
+//     done, and ray-march them (TrMarch's brick loop); each finished segment adds
+//     its optical-depth sum to its ray's slot with an LDS float atomic.
+// After each march phase the rays whose walk ended are complete: Tr = exp(sum).
+// A ray waits in one of its lane's two slots for the march that completes it;
+// a lane with both slots waiting pauses its walks until then.
+static constexpr int kRaySlots = 128;           // two per lane
+static constexpr int kSegCap = 128;             // segment queue entries per wave
+static constexpr int kSegHigh = kSegCap - 64;   // the walk phase stops above this fill (<= 64 pushes per step)
+static constexpr int kWalkSteps = 32;           // walk-phase iterations before marching
+static constexpr int kNodeWords = 64;           // block-occupancy bits kept in LDS (2048 blocks)
+static constexpr int kRayRefill = 16;           // lanes that must want a ray before a refill round
+
+struct SegQueue {
+    int rec[kRaySlots];
+    float sum[kRaySlots];
+    float dq[3][kRaySlots];
+    float tStep[kRaySlots], tFar[kRaySlots];
+    int gs[kRaySlots];                          // grid slot | linear << 8
+    int owner[kSegCap];                         // ray slot
+    uint32_t brick[kSegCap];
+    float t[kSegCap], q[3][kSegCap];
+};
+
+VR_DEV void waveSync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// A brick sample with the storage format and filter fixed at compile time
+// (FP16, LIN >= 0) or read per lane (-1): sampleBrick's arithmetic.
+template <int FP16, int LIN>
+VR_DEV float sampleSeg(const Grid& G, uint32_t brick, v3 q, bool linear) {
+    if constexpr (FP16 < 0 || LIN < 0) {
+        return sampleBrick(G, brick, q, linear);
+    } else if constexpr (LIN == 0) {
+        int i = (int)floorf(q.x) + 1, j = (int)floorf(q.y) + 1, k = (int)floorf(q.z) + 1;
+        if ((unsigned)i >= 10u || (unsigned)j >= 10u || (unsigned)k >= 10u) return 0.f;
+        const size_t o = (size_t)brick * kBrickVals + i + kApron * (j + kApron * k);
+        if constexpr (FP16) return (float)gbl<_Float16>(G.pool)[o];
+        else return gbl<float>(G.pool)[o];
+    } else {
+        v3 x = q + 0.5f;
+        v3 i0 = floor3(x);
+        v3 f = x - i0;
+        int ix = (int)i0.x, iy = (int)i0.y, iz = (int)i0.z;
+        if ((unsigned)ix >= 9u || (unsigned)iy >= 9u || (unsigned)iz >= 9u) return sampleBrick(G, brick, q, true);
+        const size_t o = (size_t)brick * kBrickVals + ix + kApron * (iy + kApron * iz);
+        float v[8];
+        if constexpr (FP16) {
+            gptr<_Float16> p = gbl<_Float16>(G.pool) + o;
+            v[0] = (float)p[0]; v[1] = (float)p[1]; v[2] = (float)p[10]; v[3] = (float)p[11];
+            v[4] = (float)p[100]; v[5] = (float)p[101]; v[6] = (float)p[110]; v[7] = (float)p[111];
+        } else {
+            gptr<float> p = gbl<float>(G.pool) + o;
+            v[0] = p[0]; v[1] = p[1]; v[2] = p[10]; v[3] = p[11];
+            v[4] = p[100]; v[5] = p[101]; v[6] = p[110]; v[7] = p[111];
+        }
+        return trilerp(v, f);
+    }
+}
+
+// Drains the wave's segment queue: each lane marches a segment (TrMarch's brick
+// loop: up to 128 samples while inside the brick and before tFar) and takes the
+// next one as soon as it is done; the segment's sum goes to its ray's slot.
+// FP16 / LIN >= 0: every segment uses grid G with that format and filter.
+template <int FP16, int LIN>
+VR_DEV void marchPhase(const Scene& S, const Grid& Gu, SegQueue& Q, int nSeg, uint64_t below) {
+    int mHead = 0;  // wave-uniform
+    bool has = false;
+    int owner = 0, it = 0, gs = 0;
+    uint32_t brick = 0;
+    float t = 0.f, part = 0.f, tStep = 0.f, tFar = 0.f;
+    v3 q = V3(0.f), dq = V3(0.f);
+    for (;;) {
+        const uint64_t nm = __ballot(!has);
+        if (nm && mHead < nSeg) {
+            if (!has) {
+                const int e = mHead + __popcll(nm & below);
+                if (e < nSeg) {
+                    owner = Q.owner[e];
+                    brick = Q.brick[e];
+                    t = Q.t[e];
+                    q = V3(Q.q[0][e], Q.q[1][e], Q.q[2][e]);
+                    dq = V3(Q.dq[0][owner], Q.dq[1][owner], Q.dq[2][owner]);
+                    tStep = Q.tStep[owner];
+                    tFar = Q.tFar[owner];
+                    gs = Q.gs[owner];
+                    it = 0;
+                    part = 0.f;
+                    has = true;
+                }
+            }
+            mHead += __popcll(nm);
+        }
+        if (!__ballot(has)) break;
+        if (has) {
+            const bool done = !(it < 128 && inBrick(q)) || t >= tFar;
+            if (!done) {
+                const Grid& G = (FP16 >= 0 && LIN >= 0) ? Gu : S.grid[gs & 255];
+                const float st = sampleSeg<FP16, LIN>(G, brick, q, (gs & 256) != 0) * S.densityScale * S.sigma_t;
+                part += -st * 1.f * tStep;
+                q = q + dq;
+                t += 1.f * tStep;
+                it++;
+            } else {
+                atomicAdd(&Q.sum[owner], part);
+                has = false;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ sc, const Frame fr, Options opt,
+                                                      int isFinal, const TraceRec* __restrict__ recs,
+                                                      const int* __restrict__ counts, float* __restrict__ trs,
+                                                      int cap, int numPixels) {
+    const Scene& S = *sc;
+    occCacheLoad(S, fr.occSlot[0], fr.occSlot[1]);
+    __shared__ int pre[4][65];
+    __shared__ SegQueue queues[4];
+    __shared__ uint32_t nodeBits[kNodeWords];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const int p0 = (blockIdx.x * 4 + wv) * 64;
+    const int cnt = p0 + lane < numPixels ? counts[p0 + lane] : 0;
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    pre[wv][lane + 1] = incl;
+    if (lane == 0) pre[wv][0] = 0;
+    __syncthreads();
+    const int total = pre[wv][64];
+    SegQueue& Q = queues[wv];
+
+    const int vs = opt.visibilityMipLevel, ls = isFinal ? 0 : opt.lightingMipLevel;
+    const bool oneSlot = vs == ls;
+    const bool uniformLinear = opt.visibilityUseLinearSampler != 0;
+    const bool uniformMarch = oneSlot && uniformLinear == (opt.lightingUseLinearSampler != 0);
+    // the walk grid's block occupancy as bits in LDS, when it fits
+    const Grid& Gw = S.grid[vs];
+    const int nNodes = Gw.nnx * Gw.nny * Gw.nnz;
+    const bool nodeLds = oneSlot && nNodes <= kNodeWords * 32;
+    if (nodeLds) {
+        for (int i = threadIdx.x; i < kNodeWords; i += blockDim.x) {
+            uint32_t bits = 0;
+            for (int b = 0; b < 32; b++) {
+                const int n = i * 32 + b;
+                if (n < nNodes && gbl<uint32_t>(Gw.nodeTable)[n] != 0u) bits |= 1u << b;
+            }
+            nodeBits[i] = bits;
+        }
+    }
+    __syncthreads();
+    const uint32_t* nb = nodeLds ? nodeBits : nullptr;
+    Walker<2> w;
+    w.active = false;
+    int bank = 0;      // the current ray's slot is 2 * lane + bank
+    int waitMask = 0;  // bit b: slot 2 * lane + b holds a walked ray awaiting its march
+    int head = 0;      // next ray of the wave's queue (wave-uniform)
+    for (;;) {
+        // ---- walk phase
+        int nSeg = 0;  // wave-uniform
+        for (int step = 0; step < kWalkSteps && nSeg <= kSegHigh; step++) {
+            const bool want = !w.active && !((waitMask >> bank) & 1) && head < total;
+            const uint64_t wm = __ballot(want);
+            // refill in rounds of at least kRayRefill lanes (or when no walk is left), so the
+            // wave waits for record loads less often
+            if (wm && (__popcll(wm) >= kRayRefill || !__ballot(w.active))) {
+                if (want) {
+                    const int r = head + __popcll(wm & below);
+                    if (r < total) {
+                        int lo = 0, hi = 64;  // the pixel q with pre[q] <= r < pre[q + 1]
+                        while (hi - lo > 1) {
+                            const int mid = (lo + hi) >> 1;
+                            if (pre[wv][mid] <= r) lo = mid; else hi = mid;
+                        }
+                        const int rec = (p0 + lo) * cap + (r - pre[wv][lo]);
+                        const TraceRec R = recs[rec];
+                        const int kind = R.kind;
+                        const int slot = kind == 0 ? vs : ls;
+                        const bool linear =
+                            kind == 0 ? opt.visibilityUseLinearSampler != 0 : opt.lightingUseLinearSampler != 0;
+                        const float tss = kind == 0 ? opt.visibilityTStepScale : opt.lightingTStepScale;
+                        walkBegin(w, S, oneSlot, vs, slot,
+                                  Ray{V3(R.ox, R.oy, R.oz), V3(R.dx, R.dy, R.dz), 0.f, R.tMax}, linear, tss);
+                        if (w.active) {
+                            const int rs = 2 * lane + bank;
+                            Q.rec[rs] = rec;
+                            Q.sum[rs] = 0.f;
+                            const v3 dq = 1.f * w.tStep * w.dda.dir;
+                            Q.dq[0][rs] = dq.x; Q.dq[1][rs] = dq.y; Q.dq[2][rs] = dq.z;
+                            Q.tStep[rs] = w.tStep;
+                            Q.tFar[rs] = w.tFar;
+                            Q.gs[rs] = slot | (linear ? 256 : 0);
+                        } else {
+                            trs[rec] = w.Tr;  // the ray misses the grid
+                        }
+                    }
+                }
+                head += __popcll(wm);
+            }
+            if (!__ballot(w.active)) {
+                if (head >= total || !__ballot(!((waitMask >> bank) & 1))) break;
+                continue;
+            }
+            bool push = false;
+            uint32_t bk = kEmpty;
+            float t0 = 0.f;
+            v3 q0 = V3(0.f);
+            const int rs = 2 * lane + bank;
+            if (w.active) {
+                push = oneSlot ? w.ddaSegment(S.grid[vs], nb, bk, t0, q0)
+                               : w.ddaSegment(S.grid[w.slot], nullptr, bk, t0, q0);
+                if (!w.active) {
+                    waitMask |= 1 << bank;
+                    bank ^= 1;
+                }
+            }
+            const uint64_t pm = __ballot(push);
+            if (push) {
+                const int e = nSeg + __popcll(pm & below);
+                Q.owner[e] = rs;
+                Q.brick[e] = bk;
+                Q.t[e] = t0;
+                Q.q[0][e] = q0.x; Q.q[1][e] = q0.y; Q.q[2][e] = q0.z;
+            }
+            nSeg += __popcll(pm);
+        }
+        waveSync();
+        // ---- march phase: drain the segment queue
+        if (uniformMarch) {
+            const Grid& G = S.grid[vs];
+            if (G.fp16) {
+                if (uniformLinear) marchPhase<1, 1>(S, G, Q, nSeg, below);
+                else marchPhase<1, 0>(S, G, Q, nSeg, below);
+            } else {
+                if (uniformLinear) marchPhase<0, 1>(S, G, Q, nSeg, below);
+                else marchPhase<0, 0>(S, G, Q, nSeg, below);
+            }
+        } else {
+            marchPhase<-1, -1>(S, S.grid[vs], Q, nSeg, below);
+        }
+        waveSync();
+        // ---- rays whose walk ended are complete
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+            if ((waitMask >> b) & 1) {
+                const int rs = 2 * lane + b;
+                trs[Q.rec[rs]] = expf(Q.sum[rs]);
+            }
+        waitMask = 0;
+        waveSync();
+        if (head >= total && !__ballot(w.active)) break;
+    }
+}
+
+hipError_t launchTraceSegments(const LaunchArgs& a, const Options& opt, bool isFinal) {
+    const int n = a.width * a.height;
+    hipLaunchKernelGGL(kTraceSegments, dim3((n + 255) / 256), dim3(256), a.occBytes, a.stream, a.scene, a.frame, opt,
+                       isFinal ? 1 : 0, a.split.recs, a.split.counts, a.split.trs, a.split.cap, n);
+    return hipGetLastError();
+}
+
 hipError_t launchTraceQueue(const LaunchArgs& a, const Options& opt, bool isFinal, int tk) {
     const int n = a.width * a.height;
     const dim3 g((n + 255) / 256), blk(256);

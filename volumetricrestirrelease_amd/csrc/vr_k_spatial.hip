@@ -15,7 +15,8 @@ hipError_t launchSpatialReuse(const LaunchArgs& a) {
         })
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        if ((e = launchTraceQueue(a, so, false, 2)) != hipSuccess) return e;
+        e = a.splitMode == 2 ? launchTraceSegments(a, so, false) : launchTraceQueue(a, so, false, 2);
+        if (e != hipSuccess) return e;
         VR_DISPATCH_B(a, {
             hipLaunchKernelGGL((kSpatial<B, 2, 2, kReplay>), g, blk, a.occBytes, a.stream, a.scene, a.frame, a.round,
                                a.offsets, a.res0, a.res1, a.extra0, a.extra1, a.features, a.split);

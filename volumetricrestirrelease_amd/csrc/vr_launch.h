@@ -15,6 +15,7 @@ struct LaunchArgs {
     int stage, round, maxBounces, width, height;
     int occBytes;  // dynamic LDS of the occupancy tables named in frame.occSlot
     int splitTK;   // > 0: run the reuse stage as record / trace / replay with tracker kind splitTK
+    int splitMode; // the trace of that split: 1 = walker queue (kTraceQueue), 2 = brick segments (kTraceSegments)
     SplitBufs split;
     const Scene* scene;
     Frame frame;
@@ -40,6 +41,7 @@ hipError_t launchSpatialReuse(const LaunchArgs& a);
 hipError_t launchFinalShading(const LaunchArgs& a);
 // Balanced trace of the rays recorded in a.split (vr_k_queue.hip); tk: 2 ray marching, 1 analytic.
 hipError_t launchTraceQueue(const LaunchArgs& a, const Options& opt, bool isFinal, int tk);
+hipError_t launchTraceSegments(const LaunchArgs& a, const Options& opt, bool isFinal);
 // Ground-truth accumulation and relMSE (vr_k_accum.hip).
 hipError_t launchAccumulate(hipStream_t st, const float4* frame, double4* sum, int n, bool reset);
 hipError_t launchResolve(hipStream_t st, const double4* sum, double inv, float4* out, int n);

@@ -98,6 +98,75 @@ struct Walker {
         levelUp();
         iter++;
     }
+    // One hierarchical-DDA cell of a ray-marching walk (TK = 2) whose bricks are
+    // marched elsewhere (kTraceSegments). When the cell is an occupied brick it
+    // returns true with the brick and the march's first sample (t0 and the
+    // brick-local point q0, as advance() would start it) and walks on past the
+    // brick, as advance() does after the march leaves it. A march started at
+    // one brick never depends on another, and a walk's samples at t >= tFar add
+    // nothing (the walk finishes at the first such sample inside a brick), so the
+    // walk ends once its cells lie beyond tFar.
+    // nodeBits: the grid's block occupancy as a bit mask (in LDS), or null to read the block table.
+    VR_DEV bool ddaSegment(const Grid& G, const uint32_t* nodeBits, uint32_t& bk, float& t0, v3& q0) {
+        const float eps = 0.01f;
+        if (!(iter < 4096 && lev > 0 && lev <= 2 && !outside(lev == 2 ? 32 : 16)) || dda.tx >= tFar) {
+            active = false;
+            return false;
+        }
+        dda.next();
+        bool occupied;
+        int cx, cy, cz;
+        if (lev == 2) {
+            int b = (((dda.p.z << 5) + dda.p.y) << 5) + dda.p.x;
+            if (nodeBits) {
+                const int nx = b & 31, ny = (b >> 5) & 31, nz = b >> 10;
+                const bool in = b < 32768 && (unsigned)nx < (unsigned)G.nnx && (unsigned)ny < (unsigned)G.nny &&
+                                (unsigned)nz < (unsigned)G.nnz;
+                const int i = in ? nx + G.nnx * (ny + G.nny * nz) : 0;
+                occupied = in && ((nodeBits[i >> 5] >> (i & 31)) & 1u);
+            } else {
+                occupied = b < 32768 && nodeAt(G, b & 31, (b >> 5) & 31, b >> 10);
+            }
+            cx = b & 31; cy = (b >> 5) & 31; cz = b >> 10;
+        } else {
+            int b = (((dda.p.z << 4) + dda.p.y) << 4) + dda.p.x;
+            cx = (node & 255) * 16 + (b & 15);
+            cy = ((node >> 8) & 255) * 16 + ((b >> 4) & 15);
+            cz = (node >> 16) * 16 + (b >> 8);
+            occupied = b < 4096;
+        }
+        bk = kEmpty;
+        if (occupied && lev == 1) {
+            bk = brickLookup(G, slot, cx, cy, cz);
+            occupied = bk != kEmpty;
+        }
+        if (occupied && lev == 1) {
+            const v3 bmin = V3((float)(cx * 8), (float)(cy * 8), (float)(cz * 8));
+            float t = dda.tx - eps;
+            t = tNear + (floorf((t - tNear) / tStep) + 0.5f) * tStep;
+            if (t < dda.tx) t += tStep;
+            t0 = t;
+            q0 = at(Ray{dda.pos, dda.dir, 0.f, 0.f}, t) - bmin;
+            dda.next();
+            dda.step();
+            dda.tx += eps;
+            levelUp();
+            iter++;
+            return true;
+        }
+        if (occupied) {
+            lev = 1;
+            node = cx | (cy << 8) | (cz << 16);
+            tMax1 = dda.ty;
+            dda.prepare(V3((float)(cx * 128), (float)(cy * 128), (float)(cz * 128)), 8.f);
+        } else {
+            dda.step();
+            dda.tx += eps;
+        }
+        levelUp();
+        iter++;
+        return false;
+    }
     // One hierarchical-DDA cell; when it enters an occupied brick, also the
     // brick's first tracker step. G must be the grid begin() was given.
     VR_DEV void advance(const Scene& S, const Grid& G) {
