@@ -256,6 +256,7 @@ VR_DEV bool intersectBounds(const Grid& G, const Ray& r, float& t0o, float& t1o,
 // ------------------------------------------------------------------ hierarchical DDA (GVS/gvdbDda.slang:86-157)
 struct Dda {
     v3 pos, dir;
+    v3 iad;  // |1 / dir|: vdel / dir for a power-of-two cell size vdel is exactly vdel * (1 / dir)
     i3v pStep;
     v3 tDel;
     float tx, ty;
@@ -264,18 +265,21 @@ struct Dda {
     i3v mask;
     VR_DEV void setFromRay(v3 o, v3 d, float t0) {
         pos = o; dir = d;
+        iad = abs3(V3(1.f) / d);
         pStep = {d.x >= 0 ? 1 : -1, d.y >= 0 ? 1 : -1, d.z >= 0 ? 1 : -1};
         tx = t0;
     }
+    // vdel: the level's cell size, a power of two (128 or 8), so tDel = |vdel / dir|
+    // and the division by vdel below are exact scalings.
     VR_DEV void prepare(v3 vmin, float vdel) {
-        tDel = abs3(V3(vdel) / dir);
-        v3 pf = (pos + dir * tx - vmin) / V3(vdel);
+        tDel = V3(vdel) * iad;
+        v3 pf = (pos + dir * tx - vmin) * V3(1.f / vdel);
         v3 fl = floor3(pf);
         tSide = ((fl - pf + 0.5f) * tof(pStep) + 0.5f) * tDel + tx;
         p = {(int)fl.x, (int)fl.y, (int)fl.z};
     }
     VR_DEV void prepareLeaf(v3 vmin) {
-        tDel = abs3(V3(1.f) / dir);
+        tDel = iad;
         v3 pf = pos + dir * tx - vmin;
         v3 fl = floor3(pf);
         tSide = ((fl - pf + 0.5f) * tof(pStep) + 0.5f) * tDel + tx;
