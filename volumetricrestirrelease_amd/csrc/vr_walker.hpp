@@ -140,32 +140,29 @@ struct Walker {
             bk = brickLookup(G, slot, cx, cy, cz);
             occupied = bk != kEmpty;
         }
-        if (occupied && lev == 1) {
+        const bool seg = occupied && lev == 1;
+        if (seg) {
             const v3 bmin = V3((float)(cx * 8), (float)(cy * 8), (float)(cz * 8));
             float t = dda.tx - eps;
             t = tNear + (floorf((t - tNear) / tStep) + 0.5f) * tStep;
             if (t < dda.tx) t += tStep;
             t0 = t;
             q0 = at(Ray{dda.pos, dda.dir, 0.f, 0.f}, t) - bmin;
-            dda.next();
-            dda.step();
-            dda.tx += eps;
-            levelUp();
-            iter++;
-            return true;
         }
-        if (occupied) {
+        if (occupied && lev == 2) {
             lev = 1;
             node = cx | (cy << 8) | (cz << 16);
             tMax1 = dda.ty;
             dda.prepare(V3((float)(cx * 128), (float)(cy * 128), (float)(cz * 128)), 8.f);
         } else {
+            // an empty cell, or past the brick: advance()'s leaveBrick() repeats next(), which
+            // recomputes the same axis mask from the unchanged side distances
             dda.step();
             dda.tx += eps;
         }
         levelUp();
         iter++;
-        return false;
+        return seg;
     }
     // One hierarchical-DDA cell; when it enters an occupied brick, also the
     // brick's first tracker step. G must be the grid begin() was given.
