@@ -82,14 +82,18 @@ __global__ __launch_bounds__(256) void kTraceQueue(const Scene* __restrict__ sc,
 // ------------------------------------------------------------------ brick-segment trace
 // The same queue of recorded rays, traced in two alternating phases so that the
 // lanes of a wave run one code path at a time:
-//   * walk: every lane steps the hierarchical DDA of its own ray, one cell per
-//     iteration, and pushes each occupied brick it enters (with the march's first
-//     sample) to a per-wave segment queue in LDS; a lane whose walk is over takes
-//     the next ray of the queue;
+//   * walk: every lane steps the DDA of its own ray, one cell per iteration (at
+//     brick level only when the grid's block bits fit LDS, Walker::flatSegment,
+//     else the hierarchical walk), and pushes each occupied brick it enters (with
+//     the march's first sample) to a per-wave segment queue in LDS; a lane whose
+//     walk is over takes the next ray of the queue;
 //   * march: lanes take segments from the queue, a new one as soon as theirs is
 //     done, and ray-march them (TrMarch's brick loop); each finished segment adds
-//     its optical-depth sum to its ray's slot with an LDS float atomic.
-// After each march phase the rays whose walk ended are complete: Tr = exp(sum).
+//     its optical-depth sum to its ray's slot with an LDS float atomic. While
+//     walks remain, the march's tail (at most kCarryLanes lanes busy and the
+//     queue empty) goes back to the queue for the next march.
+// After each march phase the rays whose walk ended and that have no pending
+// segment are complete: Tr = exp(sum).
 // A ray waits in one of its lane's two slots for the march that completes it;
 // a lane with both slots waiting pauses its walks until then.
 static constexpr int kRaySlots = 128;           // two per lane
@@ -98,6 +102,7 @@ static constexpr int kSegHigh = kSegCap - 64;   // the walk phase stops above th
 static constexpr int kWalkSteps = 32;           // walk-phase iterations before marching
 static constexpr int kNodeWords = 64;           // block-occupancy bits kept in LDS (2048 blocks)
 static constexpr int kRayRefill = 16;           // lanes that must want a ray before a refill round
+static constexpr int kCarryLanes = 32;          // march lanes left when the rest of the march is carried over
 
 struct SegQueue {
     int rec[kRaySlots];
@@ -105,7 +110,8 @@ struct SegQueue {
     float dq[3][kRaySlots];
     float tStep[kRaySlots], tFar[kRaySlots];
     int gs[kRaySlots];                          // grid slot | linear << 8
-    int owner[kSegCap];                         // ray slot
+    int pend[kRaySlots];                        // 1: the ray has a carried-over segment
+    int owner[kSegCap];                         // ray slot | samples already taken << 8
     uint32_t brick[kSegCap];
     float t[kSegCap], q[3][kSegCap];
 };
@@ -151,9 +157,14 @@ VR_DEV float sampleSeg(const Grid& G, uint32_t brick, v3 q, bool linear) {
 // Drains the wave's segment queue: each lane marches a segment (TrMarch's brick
 // loop: up to 128 samples while inside the brick and before tFar) and takes the
 // next one as soon as it is done; the segment's sum goes to its ray's slot.
+// When the queue is empty and at most kCarryLanes lanes are still marching (and
+// mayCarry holds), those lanes add their partial sums, write their segments back
+// to the front of the queue, flag their rays as pending, and the phase ends; the
+// count of such segments is returned. Each segment taken advances at least one
+// sample before it can be carried, so every segment ends.
 // FP16 / LIN >= 0: every segment uses grid G with that format and filter.
 template <int FP16, int LIN>
-VR_DEV void marchPhase(const Scene& S, const Grid& Gu, SegQueue& Q, int nSeg, uint64_t below) {
+VR_DEV int marchPhase(const Scene& S, const Grid& Gu, SegQueue& Q, int nSeg, uint64_t below, bool mayCarry) {
     int mHead = 0;  // wave-uniform
     bool has = false;
     int owner = 0, it = 0, gs = 0;
@@ -166,7 +177,9 @@ VR_DEV void marchPhase(const Scene& S, const Grid& Gu, SegQueue& Q, int nSeg, ui
             if (!has) {
                 const int e = mHead + __popcll(nm & below);
                 if (e < nSeg) {
-                    owner = Q.owner[e];
+                    const int ow = Q.owner[e];
+                    owner = ow & 255;
+                    it = ow >> 8;
                     brick = Q.brick[e];
                     t = Q.t[e];
                     q = V3(Q.q[0][e], Q.q[1][e], Q.q[2][e]);
@@ -174,14 +187,14 @@ VR_DEV void marchPhase(const Scene& S, const Grid& Gu, SegQueue& Q, int nSeg, ui
                     tStep = Q.tStep[owner];
                     tFar = Q.tFar[owner];
                     gs = Q.gs[owner];
-                    it = 0;
                     part = 0.f;
                     has = true;
                 }
             }
             mHead += __popcll(nm);
         }
-        if (!__ballot(has)) break;
+        const uint64_t hm = __ballot(has);
+        if (!hm) break;
         if (has) {
             const bool done = !(it < 128 && inBrick(q)) || t >= tFar;
             if (!done) {
@@ -196,7 +209,21 @@ VR_DEV void marchPhase(const Scene& S, const Grid& Gu, SegQueue& Q, int nSeg, ui
                 has = false;
             }
         }
+        if (mayCarry && mHead >= nSeg && __popcll(hm) <= kCarryLanes) {
+            const uint64_t cm = __ballot(has);
+            if (has) {
+                atomicAdd(&Q.sum[owner], part);
+                const int e = __popcll(cm & below);
+                Q.owner[e] = owner | (it << 8);
+                Q.brick[e] = brick;
+                Q.t[e] = t;
+                Q.q[0][e] = q.x; Q.q[1][e] = q.y; Q.q[2][e] = q.z;
+                Q.pend[owner] = 1;
+            }
+            return __popcll(cm);
+        }
     }
+    return 0;
 }
 
 __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ sc, const Frame fr, Options opt,
@@ -244,14 +271,23 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
     }
     __syncthreads();
     const uint32_t* nb = nodeLds ? nodeBits : nullptr;
+    // with the block bits in LDS the walk runs over bricks only (flatSegment),
+    // jumping empty blocks when the grid has any
+    const bool flat = nodeLds;
+    int nOcc = nodeLds ? __popc(nodeBits[lane]) : 0;  // kNodeWords == 64: one word per lane
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nOcc += __shfl_xor(nOcc, o);
+    const bool skipBlocks = flat && nOcc < nNodes;
     Walker<2> w;
     w.active = false;
     int bank = 0;      // the current ray's slot is 2 * lane + bank
     int waitMask = 0;  // bit b: slot 2 * lane + b holds a walked ray awaiting its march
     int head = 0;      // next ray of the wave's queue (wave-uniform)
+    int nSeg = 0;      // segments in the queue (wave-uniform)
+    Q.pend[2 * lane] = 0;
+    Q.pend[2 * lane + 1] = 0;
     for (;;) {
         // ---- walk phase
-        int nSeg = 0;  // wave-uniform
         for (int step = 0; step < kWalkSteps && nSeg <= kSegHigh; step++) {
             const bool want = !w.active && !((waitMask >> bank) & 1) && head < total;
             const uint64_t wm = __ballot(want);
@@ -275,6 +311,7 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
                         const float tss = kind == 0 ? opt.visibilityTStepScale : opt.lightingTStepScale;
                         walkBegin(w, S, oneSlot, vs, slot,
                                   Ray{V3(R.ox, R.oy, R.oz), V3(R.dx, R.dy, R.dz), 0.f, R.tMax}, linear, tss);
+                        if (flat && w.active) w.flatBegin();
                         if (w.active) {
                             const int rs = 2 * lane + bank;
                             Q.rec[rs] = rec;
@@ -301,8 +338,9 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
             v3 q0 = V3(0.f);
             const int rs = 2 * lane + bank;
             if (w.active) {
-                push = oneSlot ? w.ddaSegment(S.grid[vs], nb, bk, t0, q0)
-                               : w.ddaSegment(S.grid[w.slot], nullptr, bk, t0, q0);
+                if (flat) push = w.flatSegment(S.grid[vs], nb, skipBlocks, bk, t0, q0);
+                else if (oneSlot) push = w.ddaSegment(S.grid[vs], nb, bk, t0, q0);
+                else push = w.ddaSegment(S.grid[w.slot], nullptr, bk, t0, q0);
                 if (!w.active) {
                     waitMask |= 1 << bank;
                     bank ^= 1;
@@ -319,30 +357,33 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
             nSeg += __popcll(pm);
         }
         waveSync();
-        // ---- march phase: drain the segment queue
+        // ---- march phase: drain the segment queue, or all but a short tail when walks remain
+        const bool mayCarry = head < total || __ballot(w.active) != 0ull;
         if (uniformMarch) {
             const Grid& G = S.grid[vs];
             if (G.fp16) {
-                if (uniformLinear) marchPhase<1, 1>(S, G, Q, nSeg, below);
-                else marchPhase<1, 0>(S, G, Q, nSeg, below);
+                if (uniformLinear) nSeg = marchPhase<1, 1>(S, G, Q, nSeg, below, mayCarry);
+                else nSeg = marchPhase<1, 0>(S, G, Q, nSeg, below, mayCarry);
             } else {
-                if (uniformLinear) marchPhase<0, 1>(S, G, Q, nSeg, below);
-                else marchPhase<0, 0>(S, G, Q, nSeg, below);
+                if (uniformLinear) nSeg = marchPhase<0, 1>(S, G, Q, nSeg, below, mayCarry);
+                else nSeg = marchPhase<0, 0>(S, G, Q, nSeg, below, mayCarry);
             }
         } else {
-            marchPhase<-1, -1>(S, S.grid[vs], Q, nSeg, below);
+            nSeg = marchPhase<-1, -1>(S, S.grid[vs], Q, nSeg, below, mayCarry);
         }
         waveSync();
-        // ---- rays whose walk ended are complete
+        // ---- rays whose walk ended and that have no carried segment are complete
 #pragma unroll
-        for (int b = 0; b < 2; b++)
-            if ((waitMask >> b) & 1) {
-                const int rs = 2 * lane + b;
+        for (int b = 0; b < 2; b++) {
+            const int rs = 2 * lane + b;
+            if (((waitMask >> b) & 1) && !Q.pend[rs]) {
                 trs[Q.rec[rs]] = expf(Q.sum[rs]);
+                waitMask &= ~(1 << b);
             }
-        waitMask = 0;
+            Q.pend[rs] = 0;
+        }
         waveSync();
-        if (head >= total && !__ballot(w.active)) break;
+        if (head >= total && !__ballot(w.active) && !__ballot(waitMask != 0)) break;
     }
 }
 

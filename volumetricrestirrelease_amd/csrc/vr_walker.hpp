@@ -164,6 +164,57 @@ struct Walker {
         iter++;
         return seg;
     }
+    // Single-level form of ddaSegment: the DDA runs over the grid's 8^3 bricks
+    // from the ray's entry on (flatBegin), so a ray crossing occupied 128^3
+    // blocks takes no block-level steps and no re-preparation at block borders.
+    // It visits the bricks the hierarchical walk visits: a prepare from the same
+    // distance gives bit-identical side distances (the block origin only shifts
+    // the exact cell coordinate), and along the walk the side distances differ
+    // from the per-block re-prepared ones by rounding only, which moves a
+    // segment's first sample only when the distance lands within that rounding
+    // of a step boundary. With skipBlocks, a cell of an empty block (bit clear
+    // in nodeBits) jumps to the block's exit, as the block-level step does.
+    VR_DEV void flatBegin() { dda.prepare(V3(0.f), 8.f); }
+    VR_DEV bool flatSegment(const Grid& G, const uint32_t* nodeBits, bool skipBlocks, uint32_t& bk, float& t0,
+                            v3& q0) {
+        const float eps = 0.01f;
+        if (!(iter < 4096) || dda.tx >= tFar) {
+            active = false;
+            return false;
+        }
+        dda.next();
+        const int cx = dda.p.x, cy = dda.p.y, cz = dda.p.z;
+        bk = brickLookup(G, slot, cx, cy, cz);
+        const bool seg = bk != kEmpty;
+        if (seg) {
+            const v3 bmin = V3((float)(cx * 8), (float)(cy * 8), (float)(cz * 8));
+            float t = dda.tx - eps;
+            t = tNear + (floorf((t - tNear) / tStep) + 0.5f) * tStep;
+            if (t < dda.tx) t += tStep;
+            t0 = t;
+            q0 = at(Ray{dda.pos, dda.dir, 0.f, 0.f}, t) - bmin;
+        }
+        bool jump = false;
+        if (skipBlocks && !seg) {
+            const int nx = cx >> 4, ny = cy >> 4, nz = cz >> 4;
+            const bool in = (unsigned)nx < (unsigned)G.nnx && (unsigned)ny < (unsigned)G.nny &&
+                            (unsigned)nz < (unsigned)G.nnz;
+            const int i = in ? nx + G.nnx * (ny + G.nny * nz) : 0;
+            jump = !(in && ((nodeBits[i >> 5] >> (i & 31)) & 1u));
+        }
+        if (jump) {
+            Dda nd = dda;
+            nd.prepare(V3(0.f), 128.f);
+            nd.next();
+            dda.tx = nd.ty + eps;
+            dda.prepare(V3(0.f), 8.f);
+        } else {
+            dda.step();
+            dda.tx += eps;
+        }
+        iter++;
+        return seg;
+    }
     // One hierarchical-DDA cell; when it enters an occupied brick, also the
     // brick's first tracker step. G must be the grid begin() was given.
     VR_DEV void advance(const Scene& S, const Grid& G) {
