@@ -121,35 +121,27 @@ VR_DEV void waveSync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// A brick sample with the storage format and filter fixed at compile time
-// (FP16, LIN >= 0) or read per lane (-1): sampleBrick's arithmetic.
+// A sample at a point q inside the brick ([0, 8)^3), so every cell of the
+// footprint lies in the 10^3 apron brick: sampleBrick's arithmetic without its
+// range checks, addressed by a 32-bit byte offset from the pool's base (the
+// launch checks that the pool is below 4 GiB).
 template <int FP16, int LIN>
-VR_DEV float sampleSeg(const Grid& G, uint32_t brick, v3 q, bool linear) {
-    if constexpr (FP16 < 0 || LIN < 0) {
-        return sampleBrick(G, brick, q, linear);
-    } else if constexpr (LIN == 0) {
-        int i = (int)floorf(q.x) + 1, j = (int)floorf(q.y) + 1, k = (int)floorf(q.z) + 1;
-        if ((unsigned)i >= 10u || (unsigned)j >= 10u || (unsigned)k >= 10u) return 0.f;
-        const size_t o = (size_t)brick * kBrickVals + i + kApron * (j + kApron * k);
-        if constexpr (FP16) return (float)gbl<_Float16>(G.pool)[o];
-        else return gbl<float>(G.pool)[o];
+VR_DEV float sampleInside(gptr<char> pool, uint32_t brick, v3 q) {
+    using E = typename std::conditional<FP16 != 0, _Float16, float>::type;
+    constexpr uint32_t es = sizeof(E);
+    if constexpr (LIN == 0) {
+        const uint32_t i = (uint32_t)((int)floorf(q.x) + 1), j = (uint32_t)((int)floorf(q.y) + 1),
+                       k = (uint32_t)((int)floorf(q.z) + 1);
+        return (float)*(gptr<E>)(pool + (brick * (uint32_t)kBrickVals + i + kApron * (j + kApron * k)) * es);
     } else {
         v3 x = q + 0.5f;
         v3 i0 = floor3(x);
         v3 f = x - i0;
-        int ix = (int)i0.x, iy = (int)i0.y, iz = (int)i0.z;
-        if ((unsigned)ix >= 9u || (unsigned)iy >= 9u || (unsigned)iz >= 9u) return sampleBrick(G, brick, q, true);
-        const size_t o = (size_t)brick * kBrickVals + ix + kApron * (iy + kApron * iz);
+        const uint32_t ix = (uint32_t)(int)i0.x, iy = (uint32_t)(int)i0.y, iz = (uint32_t)(int)i0.z;
+        gptr<E> p = (gptr<E>)(pool + (brick * (uint32_t)kBrickVals + ix + kApron * (iy + kApron * iz)) * es);
         float v[8];
-        if constexpr (FP16) {
-            gptr<_Float16> p = gbl<_Float16>(G.pool) + o;
-            v[0] = (float)p[0]; v[1] = (float)p[1]; v[2] = (float)p[10]; v[3] = (float)p[11];
-            v[4] = (float)p[100]; v[5] = (float)p[101]; v[6] = (float)p[110]; v[7] = (float)p[111];
-        } else {
-            gptr<float> p = gbl<float>(G.pool) + o;
-            v[0] = p[0]; v[1] = p[1]; v[2] = p[10]; v[3] = p[11];
-            v[4] = p[100]; v[5] = p[101]; v[6] = p[110]; v[7] = p[111];
-        }
+        v[0] = (float)p[0]; v[1] = (float)p[1]; v[2] = (float)p[10]; v[3] = (float)p[11];
+        v[4] = (float)p[100]; v[5] = (float)p[101]; v[6] = (float)p[110]; v[7] = (float)p[111];
         return trilerp(v, f);
     }
 }
@@ -165,6 +157,8 @@ VR_DEV float sampleSeg(const Grid& G, uint32_t brick, v3 q, bool linear) {
 // FP16 / LIN >= 0: every segment uses grid G with that format and filter.
 template <int FP16, int LIN>
 VR_DEV int marchPhase(const Scene& S, const Grid& Gu, SegQueue& Q, int nSeg, uint64_t below, bool mayCarry) {
+    const gptr<char> pool = gbl<char>(Gu.pool);
+    const float dScale = S.densityScale, sigT = S.sigma_t;
     int mHead = 0;  // wave-uniform
     bool has = false;
     int owner = 0, it = 0, gs = 0;
@@ -198,8 +192,10 @@ VR_DEV int marchPhase(const Scene& S, const Grid& Gu, SegQueue& Q, int nSeg, uin
         if (has) {
             const bool done = !(it < 128 && inBrick(q)) || t >= tFar;
             if (!done) {
-                const Grid& G = (FP16 >= 0 && LIN >= 0) ? Gu : S.grid[gs & 255];
-                const float st = sampleSeg<FP16, LIN>(G, brick, q, (gs & 256) != 0) * S.densityScale * S.sigma_t;
+                float v;
+                if constexpr (FP16 >= 0 && LIN >= 0) v = sampleInside<FP16, LIN>(pool, brick, q);
+                else v = sampleBrick(S.grid[gs & 255], brick, q, (gs & 256) != 0);
+                const float st = v * dScale * sigT;
                 part += -st * 1.f * tStep;
                 q = q + dq;
                 t += 1.f * tStep;
@@ -224,6 +220,33 @@ VR_DEV int marchPhase(const Scene& S, const Grid& Gu, SegQueue& Q, int nSeg, uin
         }
     }
     return 0;
+}
+
+// The two walk forms of kTraceSegments behind one interface: the brick-level
+// FlatWalk when the walk grid is uniform and its block bits fit LDS, else the
+// hierarchical Walker<2>.
+struct WalkCtx {
+    const Scene* S;
+    int vs;                   // the walk grid slot when oneSlot
+    bool oneSlot;
+    const uint32_t* nb;       // block bits in LDS, or null
+    bool skipBlocks;          // the grid has empty blocks (FlatWalk)
+    FlatGrid g;               // FlatWalk's grid view
+};
+VR_DEV void segBegin(Walker<2>& w, const WalkCtx& c, int slot, const Ray& r, bool linear, float tss) {
+    walkBegin(w, *c.S, c.oneSlot, c.vs, slot, r, linear, tss);
+}
+VR_DEV void segBegin(FlatWalk& w, const WalkCtx& c, int, const Ray& r, bool, float tss) {
+    w.begin(*c.S, c.S->grid[c.vs], c.vs, r, tss);
+}
+VR_DEV float segMissTr(const Walker<2>& w) { return w.Tr; }
+VR_DEV float segMissTr(const FlatWalk&) { return 1.f; }
+VR_DEV bool segStep(Walker<2>& w, const WalkCtx& c, uint32_t& bk, float& t0, v3& q0) {
+    if (c.oneSlot) return w.ddaSegment(c.S->grid[c.vs], c.nb, bk, t0, q0);
+    return w.ddaSegment(c.S->grid[w.slot], nullptr, bk, t0, q0);
+}
+VR_DEV bool segStep(FlatWalk& w, const WalkCtx& c, uint32_t& bk, float& t0, v3& q0) {
+    return w.step(c.g, c.nb, c.skipBlocks, bk, t0, q0);
 }
 
 __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ sc, const Frame fr, Options opt,
@@ -254,7 +277,10 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
     const int vs = opt.visibilityMipLevel, ls = isFinal ? 0 : opt.lightingMipLevel;
     const bool oneSlot = vs == ls;
     const bool uniformLinear = opt.visibilityUseLinearSampler != 0;
-    const bool uniformMarch = oneSlot && uniformLinear == (opt.lightingUseLinearSampler != 0);
+    // one grid, format and filter for every segment, and byte offsets into its pool fit 32 bits
+    const bool uniformMarch = oneSlot && uniformLinear == (opt.lightingUseLinearSampler != 0) &&
+                              (uint64_t)S.grid[vs].numBricks * kBrickVals * (S.grid[vs].fp16 ? 2u : 4u) <=
+                                  0xffffffffull;
     // the walk grid's block occupancy as bits in LDS, when it fits
     const Grid& Gw = S.grid[vs];
     const int nNodes = Gw.nnx * Gw.nny * Gw.nnz;
@@ -271,119 +297,126 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
     }
     __syncthreads();
     const uint32_t* nb = nodeLds ? nodeBits : nullptr;
-    // with the block bits in LDS the walk runs over bricks only (flatSegment),
+    // with the block bits in LDS the walk runs over bricks only (FlatWalk),
     // jumping empty blocks when the grid has any
     const bool flat = nodeLds;
     int nOcc = nodeLds ? __popc(nodeBits[lane]) : 0;  // kNodeWords == 64: one word per lane
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) nOcc += __shfl_xor(nOcc, o);
     const bool skipBlocks = flat && nOcc < nNodes;
-    Walker<2> w;
-    w.active = false;
-    int bank = 0;      // the current ray's slot is 2 * lane + bank
-    int waitMask = 0;  // bit b: slot 2 * lane + b holds a walked ray awaiting its march
-    int head = 0;      // next ray of the wave's queue (wave-uniform)
-    int nSeg = 0;      // segments in the queue (wave-uniform)
-    Q.pend[2 * lane] = 0;
-    Q.pend[2 * lane + 1] = 0;
-    for (;;) {
-        // ---- walk phase
-        for (int step = 0; step < kWalkSteps && nSeg <= kSegHigh; step++) {
-            const bool want = !w.active && !((waitMask >> bank) & 1) && head < total;
-            const uint64_t wm = __ballot(want);
-            // refill in rounds of at least kRayRefill lanes (or when no walk is left), so the
-            // wave waits for record loads less often
-            if (wm && (__popcll(wm) >= kRayRefill || !__ballot(w.active))) {
-                if (want) {
-                    const int r = head + __popcll(wm & below);
-                    if (r < total) {
-                        int lo = 0, hi = 64;  // the pixel q with pre[q] <= r < pre[q + 1]
-                        while (hi - lo > 1) {
-                            const int mid = (lo + hi) >> 1;
-                            if (pre[wv][mid] <= r) lo = mid; else hi = mid;
-                        }
-                        const int rec = (p0 + lo) * cap + (r - pre[wv][lo]);
-                        const TraceRec R = recs[rec];
-                        const int kind = R.kind;
-                        const int slot = kind == 0 ? vs : ls;
-                        const bool linear =
-                            kind == 0 ? opt.visibilityUseLinearSampler != 0 : opt.lightingUseLinearSampler != 0;
-                        const float tss = kind == 0 ? opt.visibilityTStepScale : opt.lightingTStepScale;
-                        walkBegin(w, S, oneSlot, vs, slot,
-                                  Ray{V3(R.ox, R.oy, R.oz), V3(R.dx, R.dy, R.dz), 0.f, R.tMax}, linear, tss);
-                        if (flat && w.active) w.flatBegin();
-                        if (w.active) {
-                            const int rs = 2 * lane + bank;
-                            Q.rec[rs] = rec;
-                            Q.sum[rs] = 0.f;
-                            const v3 dq = 1.f * w.tStep * w.dda.dir;
-                            Q.dq[0][rs] = dq.x; Q.dq[1][rs] = dq.y; Q.dq[2][rs] = dq.z;
-                            Q.tStep[rs] = w.tStep;
-                            Q.tFar[rs] = w.tFar;
-                            Q.gs[rs] = slot | (linear ? 256 : 0);
-                        } else {
-                            trs[rec] = w.Tr;  // the ray misses the grid
+    WalkCtx wc{&S, vs, oneSlot, nb, skipBlocks, {}};
+    if (flat) wc.g.init(S.grid[vs], vs);
+    auto trace = [&](auto& w) {
+        w.active = false;
+        int bank = 0;      // the current ray's slot is 2 * lane + bank
+        int waitMask = 0;  // bit b: slot 2 * lane + b holds a walked ray awaiting its march
+        int head = 0;      // next ray of the wave's queue (wave-uniform)
+        int nSeg = 0;      // segments in the queue (wave-uniform)
+        Q.pend[2 * lane] = 0;
+        Q.pend[2 * lane + 1] = 0;
+        for (;;) {
+            // ---- walk phase
+            for (int step = 0; step < kWalkSteps && nSeg <= kSegHigh; step++) {
+                const bool want = !w.active && !((waitMask >> bank) & 1) && head < total;
+                const uint64_t wm = __ballot(want);
+                // refill in rounds of at least kRayRefill lanes (or when no walk is left), so the
+                // wave waits for record loads less often
+                if (wm && (__popcll(wm) >= kRayRefill || !__ballot(w.active))) {
+                    if (want) {
+                        const int r = head + __popcll(wm & below);
+                        if (r < total) {
+                            int lo = 0, hi = 64;  // the pixel q with pre[q] <= r < pre[q + 1]
+                            while (hi - lo > 1) {
+                                const int mid = (lo + hi) >> 1;
+                                if (pre[wv][mid] <= r) lo = mid; else hi = mid;
+                            }
+                            const int rec = (p0 + lo) * cap + (r - pre[wv][lo]);
+                            const TraceRec R = recs[rec];
+                            const int kind = R.kind;
+                            const int slot = kind == 0 ? vs : ls;
+                            const bool linear =
+                                kind == 0 ? opt.visibilityUseLinearSampler != 0 : opt.lightingUseLinearSampler != 0;
+                            const float tss = kind == 0 ? opt.visibilityTStepScale : opt.lightingTStepScale;
+                            segBegin(w, wc, slot, Ray{V3(R.ox, R.oy, R.oz), V3(R.dx, R.dy, R.dz), 0.f, R.tMax},
+                                     linear, tss);
+                            if (w.active) {
+                                const int rs = 2 * lane + bank;
+                                Q.rec[rs] = rec;
+                                Q.sum[rs] = 0.f;
+                                const v3 dq = 1.f * w.tStep * w.dda.dir;
+                                Q.dq[0][rs] = dq.x; Q.dq[1][rs] = dq.y; Q.dq[2][rs] = dq.z;
+                                Q.tStep[rs] = w.tStep;
+                                Q.tFar[rs] = w.tFar;
+                                Q.gs[rs] = slot | (linear ? 256 : 0);
+                            } else {
+                                trs[rec] = segMissTr(w);  // the ray misses the grid
+                            }
                         }
                     }
+                    head += __popcll(wm);
                 }
-                head += __popcll(wm);
-            }
-            if (!__ballot(w.active)) {
-                if (head >= total || !__ballot(!((waitMask >> bank) & 1))) break;
-                continue;
-            }
-            bool push = false;
-            uint32_t bk = kEmpty;
-            float t0 = 0.f;
-            v3 q0 = V3(0.f);
-            const int rs = 2 * lane + bank;
-            if (w.active) {
-                if (flat) push = w.flatSegment(S.grid[vs], nb, skipBlocks, bk, t0, q0);
-                else if (oneSlot) push = w.ddaSegment(S.grid[vs], nb, bk, t0, q0);
-                else push = w.ddaSegment(S.grid[w.slot], nullptr, bk, t0, q0);
-                if (!w.active) {
-                    waitMask |= 1 << bank;
-                    bank ^= 1;
+                if (!__ballot(w.active)) {
+                    if (head >= total || !__ballot(!((waitMask >> bank) & 1))) break;
+                    continue;
                 }
+                bool push = false;
+                uint32_t bk = kEmpty;
+                float t0 = 0.f;
+                v3 q0 = V3(0.f);
+                const int rs = 2 * lane + bank;
+                if (w.active) {
+                    push = segStep(w, wc, bk, t0, q0);
+                    if (!w.active) {
+                        waitMask |= 1 << bank;
+                        bank ^= 1;
+                    }
+                }
+                const uint64_t pm = __ballot(push);
+                if (push) {
+                    const int e = nSeg + __popcll(pm & below);
+                    Q.owner[e] = rs;
+                    Q.brick[e] = bk;
+                    Q.t[e] = t0;
+                    Q.q[0][e] = q0.x; Q.q[1][e] = q0.y; Q.q[2][e] = q0.z;
+                }
+                nSeg += __popcll(pm);
             }
-            const uint64_t pm = __ballot(push);
-            if (push) {
-                const int e = nSeg + __popcll(pm & below);
-                Q.owner[e] = rs;
-                Q.brick[e] = bk;
-                Q.t[e] = t0;
-                Q.q[0][e] = q0.x; Q.q[1][e] = q0.y; Q.q[2][e] = q0.z;
-            }
-            nSeg += __popcll(pm);
-        }
-        waveSync();
-        // ---- march phase: drain the segment queue, or all but a short tail when walks remain
-        const bool mayCarry = head < total || __ballot(w.active) != 0ull;
-        if (uniformMarch) {
-            const Grid& G = S.grid[vs];
-            if (G.fp16) {
-                if (uniformLinear) nSeg = marchPhase<1, 1>(S, G, Q, nSeg, below, mayCarry);
-                else nSeg = marchPhase<1, 0>(S, G, Q, nSeg, below, mayCarry);
+            waveSync();
+            // ---- march phase: drain the segment queue, or all but a short tail when walks remain
+            const bool mayCarry = head < total || __ballot(w.active) != 0ull;
+            if (uniformMarch) {
+                const Grid& G = S.grid[vs];
+                if (G.fp16) {
+                    if (uniformLinear) nSeg = marchPhase<1, 1>(S, G, Q, nSeg, below, mayCarry);
+                    else nSeg = marchPhase<1, 0>(S, G, Q, nSeg, below, mayCarry);
+                } else {
+                    if (uniformLinear) nSeg = marchPhase<0, 1>(S, G, Q, nSeg, below, mayCarry);
+                    else nSeg = marchPhase<0, 0>(S, G, Q, nSeg, below, mayCarry);
+                }
             } else {
-                if (uniformLinear) nSeg = marchPhase<0, 1>(S, G, Q, nSeg, below, mayCarry);
-                else nSeg = marchPhase<0, 0>(S, G, Q, nSeg, below, mayCarry);
+                nSeg = marchPhase<-1, -1>(S, S.grid[vs], Q, nSeg, below, mayCarry);
             }
-        } else {
-            nSeg = marchPhase<-1, -1>(S, S.grid[vs], Q, nSeg, below, mayCarry);
-        }
-        waveSync();
-        // ---- rays whose walk ended and that have no carried segment are complete
+            waveSync();
+            // ---- rays whose walk ended and that have no carried segment are complete
 #pragma unroll
-        for (int b = 0; b < 2; b++) {
-            const int rs = 2 * lane + b;
-            if (((waitMask >> b) & 1) && !Q.pend[rs]) {
-                trs[Q.rec[rs]] = expf(Q.sum[rs]);
-                waitMask &= ~(1 << b);
+            for (int b = 0; b < 2; b++) {
+                const int rs = 2 * lane + b;
+                if (((waitMask >> b) & 1) && !Q.pend[rs]) {
+                    trs[Q.rec[rs]] = expf(Q.sum[rs]);
+                    waitMask &= ~(1 << b);
+                }
+                Q.pend[rs] = 0;
             }
-            Q.pend[rs] = 0;
+            waveSync();
+            if (head >= total && !__ballot(w.active) && !__ballot(waitMask != 0)) break;
         }
-        waveSync();
-        if (head >= total && !__ballot(w.active) && !__ballot(waitMask != 0)) break;
+    };
+    if (flat) {
+        FlatWalk w;
+        trace(w);
+    } else {
+        Walker<2> w;
+        trace(w);
     }
 }
 

@@ -105,11 +105,15 @@ struct Walker {
     // brick, as advance() does after the march leaves it. A march started at
     // one brick never depends on another, and a walk's samples at t >= tFar add
     // nothing (the walk finishes at the first such sample inside a brick), so the
-    // walk ends once its cells lie beyond tFar.
+    // walk ends once its cells lie beyond tFar: a cell entered at tx - eps puts
+    // its first sample after that entry (the snapped start moves up by tStep when
+    // it falls before tx), so no sample of it can precede tFar once tx - eps >= tFar.
+    // (tx itself may exceed tFar while samples of the cell still precede it: eps
+    // is a world-space length, larger than tStep at fine mips.)
     // nodeBits: the grid's block occupancy as a bit mask (in LDS), or null to read the block table.
     VR_DEV bool ddaSegment(const Grid& G, const uint32_t* nodeBits, uint32_t& bk, float& t0, v3& q0) {
         const float eps = 0.01f;
-        if (!(iter < 4096 && lev > 0 && lev <= 2 && !outside(lev == 2 ? 32 : 16)) || dda.tx >= tFar) {
+        if (!(iter < 4096 && lev > 0 && lev <= 2 && !outside(lev == 2 ? 32 : 16)) || dda.tx - eps >= tFar) {
             active = false;
             return false;
         }
@@ -161,57 +165,6 @@ struct Walker {
             dda.tx += eps;
         }
         levelUp();
-        iter++;
-        return seg;
-    }
-    // Single-level form of ddaSegment: the DDA runs over the grid's 8^3 bricks
-    // from the ray's entry on (flatBegin), so a ray crossing occupied 128^3
-    // blocks takes no block-level steps and no re-preparation at block borders.
-    // It visits the bricks the hierarchical walk visits: a prepare from the same
-    // distance gives bit-identical side distances (the block origin only shifts
-    // the exact cell coordinate), and along the walk the side distances differ
-    // from the per-block re-prepared ones by rounding only, which moves a
-    // segment's first sample only when the distance lands within that rounding
-    // of a step boundary. With skipBlocks, a cell of an empty block (bit clear
-    // in nodeBits) jumps to the block's exit, as the block-level step does.
-    VR_DEV void flatBegin() { dda.prepare(V3(0.f), 8.f); }
-    VR_DEV bool flatSegment(const Grid& G, const uint32_t* nodeBits, bool skipBlocks, uint32_t& bk, float& t0,
-                            v3& q0) {
-        const float eps = 0.01f;
-        if (!(iter < 4096) || dda.tx >= tFar) {
-            active = false;
-            return false;
-        }
-        dda.next();
-        const int cx = dda.p.x, cy = dda.p.y, cz = dda.p.z;
-        bk = brickLookup(G, slot, cx, cy, cz);
-        const bool seg = bk != kEmpty;
-        if (seg) {
-            const v3 bmin = V3((float)(cx * 8), (float)(cy * 8), (float)(cz * 8));
-            float t = dda.tx - eps;
-            t = tNear + (floorf((t - tNear) / tStep) + 0.5f) * tStep;
-            if (t < dda.tx) t += tStep;
-            t0 = t;
-            q0 = at(Ray{dda.pos, dda.dir, 0.f, 0.f}, t) - bmin;
-        }
-        bool jump = false;
-        if (skipBlocks && !seg) {
-            const int nx = cx >> 4, ny = cy >> 4, nz = cz >> 4;
-            const bool in = (unsigned)nx < (unsigned)G.nnx && (unsigned)ny < (unsigned)G.nny &&
-                            (unsigned)nz < (unsigned)G.nnz;
-            const int i = in ? nx + G.nnx * (ny + G.nny * nz) : 0;
-            jump = !(in && ((nodeBits[i >> 5] >> (i & 31)) & 1u));
-        }
-        if (jump) {
-            Dda nd = dda;
-            nd.prepare(V3(0.f), 128.f);
-            nd.next();
-            dda.tx = nd.ty + eps;
-            dda.prepare(V3(0.f), 8.f);
-        } else {
-            dda.step();
-            dda.tx += eps;
-        }
         iter++;
         return seg;
     }
@@ -314,6 +267,120 @@ struct Walker {
 // Starts / advances walker w on the grid of its ray kind. When both kinds use
 // the same grid (uniform flag `oneSlot`), the grid descriptor is read once per
 // wave through scalar loads instead of per lane.
+// ------------------------------------------------------------------ brick-level walk
+// Single-level form of Walker<2>::ddaSegment for kTraceSegments: the DDA runs
+// over the grid's 8^3 bricks from the ray's entry on, so a ray crossing
+// occupied 128^3 blocks takes no block-level steps and no re-preparation at
+// block borders. It visits the bricks the hierarchical walk visits: a prepare
+// from the same distance gives bit-identical side distances (the block origin
+// shifts the floor and its argument by the same exact amount), and along the
+// walk the side distances differ from the per-block re-prepared ones by
+// rounding only, which moves a segment's first sample only when the distance
+// lands within that rounding of a step boundary. With skipBlocks, a cell of an
+// empty block jumps to the block's exit, as the block-level step does.
+
+// The walk grid's brick occupancy, with its sizes held in registers.
+struct FlatGrid {
+    int nbx, nby, nbz, nnx, nny, nnz;
+    uint32_t numBricks;
+    const uint2* occ;  // the grid's LDS occupancy table (occCacheLoad), or null: read brickTable
+    const uint32_t* brickTable;
+    VR_DEV void init(const Grid& G, int slot) {
+        nbx = G.nbx; nby = G.nby; nbz = G.nbz;
+        nnx = G.nnx; nny = G.nny; nnz = G.nnz;
+        numBricks = G.numBricks;
+        brickTable = G.brickTable;
+        const int k = slot == sOccSlot[0] ? 0 : (slot == sOccSlot[1] ? 1 : -1);
+        occ = k < 0 ? nullptr : sOccTab + sOccOff[k];
+    }
+    // brickLookup's answer
+    VR_DEV uint32_t lookup(int bx, int by, int bz) const {
+        if ((unsigned)bx >= (unsigned)nbx || (unsigned)by >= (unsigned)nby || (unsigned)bz >= (unsigned)nbz)
+            return kEmpty;
+        const uint32_t idx = (uint32_t)(bx + nbx * (by + nby * bz));
+        if (!occ) return gbl<uint32_t>(brickTable)[idx];
+        const uint2 e = occ[idx >> 5];
+        const uint32_t bit = 1u << (idx & 31u);
+        if (!(e.x & bit)) return kEmpty;
+        const uint32_t id = e.y + (uint32_t)__popc(e.x & (bit - 1u));
+        return id < numBricks ? id : kEmpty;
+    }
+};
+
+struct FlatWalk {
+    Dda dda;
+    float tNear, tFar, tStep, invStep;
+    int iter;
+    bool active;
+    // Walker<2>::begin for a ray-marching walk of grid G (slot `slot`), then the
+    // brick-level prepare. A ray that misses the grid is complete (Tr = 1).
+    VR_DEV void begin(const Scene& S, const Grid& G, int slot, const Ray& rWorld, float tStepScale) {
+        const float eps = 0.01f;
+        Ray ray{xformP(G.w2i, rWorld.o), xformV(G.w2i, rWorld.d), rWorld.tMin, rWorld.tMax};
+        int eff = slot >= 19 ? slot - 19 : slot;
+        eff = eff >= 8 ? eff - 8 : eff;
+        tStep = S.tStep * S.worldScaling * tStepScale * (float)(eff + 1);
+        invStep = 1.f / tStep;
+        if (!intersectBounds(G, ray, tNear, tFar, false)) {
+            active = false;
+            return;
+        }
+        dda.setFromRay(ray.o, ray.d, tNear + eps);
+        dda.prepare(V3(0.f), 8.f);
+        iter = 0;
+        active = true;
+    }
+    // One brick cell; true with the brick and the march's first sample (t0, q0)
+    // when the cell holds an occupied brick. The walk ends once its cells lie
+    // beyond tFar (the test of Walker::ddaSegment).
+    VR_DEV bool step(const FlatGrid& g, const uint32_t* nodeBits, bool skipBlocks, uint32_t& bk, float& t0,
+                     v3& q0) {
+        const float eps = 0.01f;
+        if (!(iter < 4096) || dda.tx - eps >= tFar) {
+            active = false;
+            return false;
+        }
+        dda.next();
+        const int cx = dda.p.x, cy = dda.p.y, cz = dda.p.z;
+        bk = g.lookup(cx, cy, cz);
+        const bool seg = bk != kEmpty;
+        if (seg) {
+            const v3 bmin = V3((float)(cx * 8), (float)(cy * 8), (float)(cz * 8));
+            float t = dda.tx - eps;
+            // (t - tNear) / tStep through the reciprocal: the product is within 3 ulp
+            // of the quotient, so only a product that close to an integer can floor
+            // differently, and those take the division
+            const float x = t - tNear;
+            float qt = x * invStep;
+            if (fabsf(qt - rintf(qt)) <= fabsf(qt) * 4.8e-7f) qt = x / tStep;
+            t = tNear + (floorf(qt) + 0.5f) * tStep;
+            if (t < dda.tx) t += tStep;
+            t0 = t;
+            q0 = at(Ray{dda.pos, dda.dir, 0.f, 0.f}, t) - bmin;
+        }
+        bool jump = false;
+        if (skipBlocks && !seg) {
+            const int nx = cx >> 4, ny = cy >> 4, nz = cz >> 4;
+            const bool in = (unsigned)nx < (unsigned)g.nnx && (unsigned)ny < (unsigned)g.nny &&
+                            (unsigned)nz < (unsigned)g.nnz;
+            const int i = in ? nx + g.nnx * (ny + g.nny * nz) : 0;
+            jump = !(in && ((nodeBits[i >> 5] >> (i & 31)) & 1u));
+        }
+        if (jump) {
+            Dda nd = dda;
+            nd.prepare(V3(0.f), 128.f);
+            nd.next();
+            dda.tx = nd.ty + eps;
+            dda.prepare(V3(0.f), 8.f);
+        } else {
+            dda.step();
+            dda.tx += eps;
+        }
+        iter++;
+        return seg;
+    }
+};
+
 template <int TK>
 VR_DEV void walkBegin(Walker<TK>& w, const Scene& S, bool oneSlot, int uniformSlot, int laneSlot, const Ray& r,
                       bool linear, float tss) {
