@@ -121,29 +121,157 @@ VR_DEV void waveSync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// A sample at a point q inside the brick ([0, 8)^3), so every cell of the
-// footprint lies in the 10^3 apron brick: sampleBrick's arithmetic without its
-// range checks, addressed by a 32-bit byte offset from the pool's base (the
-// launch checks that the pool is below 4 GiB).
+// The footprint of a sample at a point q inside the brick ([0, 8)^3), so every
+// cell of it lies in the 10^3 apron brick: sampleBrick's loads without its range
+// checks, addressed by a 32-bit byte offset from the pool's base (the launch
+// checks that the pool is below 4 GiB). The stored values stay unconverted until
+// footprintValue, so the loads can be in flight across other work.
 template <int FP16, int LIN>
-VR_DEV float sampleInside(gptr<char> pool, uint32_t brick, v3 q) {
+struct Footprint {
+    // fp16 + linear: the four x-pairs as 32-bit words (low half = lower x), left packed
+    // so that no unpacking waits for the loads; otherwise the values as stored
     using E = typename std::conditional<FP16 != 0, _Float16, float>::type;
+    using W = typename std::conditional<FP16 != 0 && LIN != 0, uint32_t, E>::type;
+    W v[LIN ? (FP16 ? 4 : 8) : 1];
+    v3 f;  // trilinear weights (LIN)
+};
+typedef uint32_t __attribute__((aligned(2))) u32pair;  // two fp16 values at a 2-byte-aligned address
+// valid == false loads the first footprint of brick 0 instead (the pool holds at
+// least one brick), so every lane issues the same loads and the wait for them is
+// counted exactly.
+template <int FP16, int LIN>
+VR_DEV void fetchInside(gptr<char> pool, bool valid, uint32_t brick, v3 q, Footprint<FP16, LIN>& fp) {
+    using E = typename Footprint<FP16, LIN>::E;
     constexpr uint32_t es = sizeof(E);
     if constexpr (LIN == 0) {
         const uint32_t i = (uint32_t)((int)floorf(q.x) + 1), j = (uint32_t)((int)floorf(q.y) + 1),
                        k = (uint32_t)((int)floorf(q.z) + 1);
-        return (float)*(gptr<E>)(pool + (brick * (uint32_t)kBrickVals + i + kApron * (j + kApron * k)) * es);
+        const uint32_t o = valid ? (brick * (uint32_t)kBrickVals + i + kApron * (j + kApron * k)) * es : 0u;
+        fp.v[0] = *(gptr<E>)(pool + o);
     } else {
         v3 x = q + 0.5f;
         v3 i0 = floor3(x);
-        v3 f = x - i0;
+        fp.f = x - i0;
         const uint32_t ix = (uint32_t)(int)i0.x, iy = (uint32_t)(int)i0.y, iz = (uint32_t)(int)i0.z;
-        gptr<E> p = (gptr<E>)(pool + (brick * (uint32_t)kBrickVals + ix + kApron * (iy + kApron * iz)) * es);
-        float v[8];
-        v[0] = (float)p[0]; v[1] = (float)p[1]; v[2] = (float)p[10]; v[3] = (float)p[11];
-        v[4] = (float)p[100]; v[5] = (float)p[101]; v[6] = (float)p[110]; v[7] = (float)p[111];
-        return trilerp(v, f);
+        const uint32_t o = valid ? (brick * (uint32_t)kBrickVals + ix + kApron * (iy + kApron * iz)) * es : 0u;
+        const gptr<char> b = pool + o;
+        if constexpr (FP16 != 0) {
+            fp.v[0] = *(gptr<u32pair>)(b);
+            fp.v[1] = *(gptr<u32pair>)(b + 10 * es);
+            fp.v[2] = *(gptr<u32pair>)(b + 100 * es);
+            fp.v[3] = *(gptr<u32pair>)(b + 110 * es);
+        } else {
+            gptr<E> p = (gptr<E>)b;
+            fp.v[0] = p[0]; fp.v[1] = p[1]; fp.v[2] = p[10]; fp.v[3] = p[11];
+            fp.v[4] = p[100]; fp.v[5] = p[101]; fp.v[6] = p[110]; fp.v[7] = p[111];
+        }
     }
+}
+VR_DEV float halfLo(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu)); }
+VR_DEV float halfHi(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)); }
+// sampleBrick's value from a fetched footprint
+template <int FP16, int LIN>
+VR_DEV float footprintValue(const Footprint<FP16, LIN>& fp) {
+    if constexpr (LIN == 0) {
+        return (float)fp.v[0];
+    } else {
+        float v[8];
+        if constexpr (FP16 != 0) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                v[2 * c] = halfLo(fp.v[c]);
+                v[2 * c + 1] = halfHi(fp.v[c]);
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 8; c++) v[c] = fp.v[c];
+        }
+        return trilerp(v, fp.f);
+    }
+}
+
+// Drains the wave's segment queue when every segment uses grid Gu with storage
+// format FP16 and filter LIN, software-pipelined: each iteration a lane fetches
+// the footprint of its segment's next sample (TrMarch's brick loop: up to 128
+// samples while inside the brick and before tFar) and steps past it, then adds
+// the sample it fetched the iteration before to its running sum, so a sample's
+// loads are in flight across a whole iteration. A lane whose segment has no next
+// sample takes the next segment of the queue in the same iteration. Samples of
+// one ray slot that follow each other in a lane add up in order; the sum goes to
+// the slot with an LDS float atomic when the lane moves to another slot.
+template <int FP16, int LIN>
+VR_DEV void marchPipelined(const Scene& S, const Grid& Gu, SegQueue& Q, int nSeg, uint64_t below) {
+    const gptr<char> pool = gbl<char>(Gu.pool);
+    const float dScale = S.densityScale, sigT = S.sigma_t;
+    int mHead = 0;  // wave-uniform
+    // issue side: the segment whose next sample is fetched
+    bool iHas = false;
+    int iOwner = 0, iIt = 0;
+    uint32_t iBrick = 0;
+    float iT = 0.f, iTStep = 0.f, iTFar = 0.f;
+    v3 iQ = V3(0.f), iDq = V3(0.f);
+    // the sample fetched in the previous iteration (its footprint alternates
+    // between fpA and fpB, so a fetch never waits to be copied)
+    bool fHas = false;
+    int fOwner = 0;
+    float fStep = 0.f;
+    Footprint<FP16, LIN> fpA{}, fpB{};
+    // compute side: the running sum of ray slot cOwner
+    int cOwner = -1;
+    float part = 0.f;
+    // one iteration: fetch the next sample into nxt, add the one in cur; false when all is done
+    auto iteration = [&](const Footprint<FP16, LIN>& cur, Footprint<FP16, LIN>& nxt) -> bool {
+        if (iHas && !(iIt < 128 && inBrick(iQ) && iT < iTFar)) iHas = false;
+        const uint64_t nm = __ballot(!iHas);
+        if (nm && mHead < nSeg) {
+            if (!iHas) {
+                const int e = mHead + __popcll(nm & below);
+                if (e < nSeg) {
+                    const int ow = Q.owner[e];
+                    iOwner = ow & 255;
+                    iIt = ow >> 8;
+                    iBrick = Q.brick[e];
+                    iT = Q.t[e];
+                    iQ = V3(Q.q[0][e], Q.q[1][e], Q.q[2][e]);
+                    iDq = V3(Q.dq[0][iOwner], Q.dq[1][iOwner], Q.dq[2][iOwner]);
+                    iTStep = Q.tStep[iOwner];
+                    iTFar = Q.tFar[iOwner];
+                    // a segment without a sample adds nothing
+                    iHas = iIt < 128 && inBrick(iQ) && iT < iTFar;
+                }
+            }
+            mHead += __popcll(nm);
+        }
+        // fetch the issue side's next sample and step past it
+        const bool nHas = iHas;
+        const int nOwner = iOwner;
+        const float nStep = iTStep;
+        fetchInside<FP16, LIN>(pool, iHas, iBrick, iQ, nxt);
+        if (iHas) {
+            iQ = iQ + iDq;
+            iT += 1.f * iTStep;
+            iIt++;
+        }
+        // add the sample fetched last iteration; its value is formed in every lane (from
+        // stale registers where nothing was fetched) so that the wait for cur's loads is
+        // not inside a branch: after it no path leaves a load pending on cur's registers
+        const float st = footprintValue<FP16, LIN>(cur) * dScale * sigT;
+        if (fHas && fOwner != cOwner) {
+            if (cOwner >= 0) atomicAdd(&Q.sum[cOwner], part);
+            cOwner = fOwner;
+            part = 0.f;
+        }
+        part = fHas ? part + -st * 1.f * fStep : part;
+        fHas = nHas;
+        fOwner = nOwner;
+        fStep = nStep;
+        return __ballot(fHas || iHas) != 0ull || mHead < nSeg;
+    };
+    for (;;) {
+        if (!iteration(fpA, fpB)) break;
+        if (!iteration(fpB, fpA)) break;
+    }
+    if (cOwner >= 0) atomicAdd(&Q.sum[cOwner], part);
 }
 
 // Drains the wave's segment queue: each lane marches a segment (TrMarch's brick
@@ -154,10 +282,8 @@ VR_DEV float sampleInside(gptr<char> pool, uint32_t brick, v3 q) {
 // to the front of the queue, flag their rays as pending, and the phase ends; the
 // count of such segments is returned. Each segment taken advances at least one
 // sample before it can be carried, so every segment ends.
-// FP16 / LIN >= 0: every segment uses grid G with that format and filter.
-template <int FP16, int LIN>
-VR_DEV int marchPhase(const Scene& S, const Grid& Gu, SegQueue& Q, int nSeg, uint64_t below, bool mayCarry) {
-    const gptr<char> pool = gbl<char>(Gu.pool);
+// Each segment names its grid slot and filter (any storage format).
+VR_DEV int marchPhase(const Scene& S, SegQueue& Q, int nSeg, uint64_t below, bool mayCarry) {
     const float dScale = S.densityScale, sigT = S.sigma_t;
     int mHead = 0;  // wave-uniform
     bool has = false;
@@ -192,10 +318,7 @@ VR_DEV int marchPhase(const Scene& S, const Grid& Gu, SegQueue& Q, int nSeg, uin
         if (has) {
             const bool done = !(it < 128 && inBrick(q)) || t >= tFar;
             if (!done) {
-                float v;
-                if constexpr (FP16 >= 0 && LIN >= 0) v = sampleInside<FP16, LIN>(pool, brick, q);
-                else v = sampleBrick(S.grid[gs & 255], brick, q, (gs & 256) != 0);
-                const float st = v * dScale * sigT;
+                const float st = sampleBrick(S.grid[gs & 255], brick, q, (gs & 256) != 0) * dScale * sigT;
                 part += -st * 1.f * tStep;
                 q = q + dq;
                 t += 1.f * tStep;
@@ -277,8 +400,10 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
     const int vs = opt.visibilityMipLevel, ls = isFinal ? 0 : opt.lightingMipLevel;
     const bool oneSlot = vs == ls;
     const bool uniformLinear = opt.visibilityUseLinearSampler != 0;
-    // one grid, format and filter for every segment, and byte offsets into its pool fit 32 bits
+    // one grid, format and filter for every segment; the grid has bricks and byte offsets
+    // into its pool fit 32 bits
     const bool uniformMarch = oneSlot && uniformLinear == (opt.lightingUseLinearSampler != 0) &&
+                              S.grid[vs].numBricks > 0 &&
                               (uint64_t)S.grid[vs].numBricks * kBrickVals * (S.grid[vs].fp16 ? 2u : 4u) <=
                                   0xffffffffull;
     // the walk grid's block occupancy as bits in LDS, when it fits
@@ -383,18 +508,19 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
             }
             waveSync();
             // ---- march phase: drain the segment queue, or all but a short tail when walks remain
-            const bool mayCarry = head < total || __ballot(w.active) != 0ull;
             if (uniformMarch) {
                 const Grid& G = S.grid[vs];
                 if (G.fp16) {
-                    if (uniformLinear) nSeg = marchPhase<1, 1>(S, G, Q, nSeg, below, mayCarry);
-                    else nSeg = marchPhase<1, 0>(S, G, Q, nSeg, below, mayCarry);
+                    if (uniformLinear) marchPipelined<1, 1>(S, G, Q, nSeg, below);
+                    else marchPipelined<1, 0>(S, G, Q, nSeg, below);
                 } else {
-                    if (uniformLinear) nSeg = marchPhase<0, 1>(S, G, Q, nSeg, below, mayCarry);
-                    else nSeg = marchPhase<0, 0>(S, G, Q, nSeg, below, mayCarry);
+                    if (uniformLinear) marchPipelined<0, 1>(S, G, Q, nSeg, below);
+                    else marchPipelined<0, 0>(S, G, Q, nSeg, below);
                 }
+                nSeg = 0;
             } else {
-                nSeg = marchPhase<-1, -1>(S, S.grid[vs], Q, nSeg, below, mayCarry);
+                const bool mayCarry = head < total || __ballot(w.active) != 0ull;
+                nSeg = marchPhase(S, Q, nSeg, below, mayCarry);
             }
             waveSync();
             // ---- rays whose walk ended and that have no carried segment are complete
