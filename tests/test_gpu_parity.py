@@ -117,22 +117,28 @@ def test_split_spatial_reuse_matches_single_kernel(mode, rtol, frac_min):
         assert frac >= frac_min
 
 
-def test_split_spatial_reuse_sparse_blocks():
-    """A 256^3 grid (2 x 2 x 2 blocks of 128^3) holding two clouds in opposite
-    blocks, traced at mip 0: the brick-segment walk jumps the six empty blocks
-    between them. At mip 0 the march step is shorter than the DDA's 0.01
-    entry offset, so a ray's last brick can hold samples before tFar although
-    its offset entry lies past tFar; a walk that stopped at the offset entry
-    dropped them (6 % of pixels off by more than 1e-3)."""
-    say("case: spatial reuse at mip 0, brick segments vs one kernel on a 256^3 grid with 6 empty 128^3 blocks")
+@pytest.mark.parametrize("layout", ["sparse", "dense"])
+def test_split_spatial_reuse_block_borders(layout):
+    """A 256^3 grid (2 x 2 x 2 blocks of 128^3) traced at mip 0, brick segments
+    against the single kernel. sparse: two clouds in opposite blocks, so the walk
+    steps over the six empty blocks between them; dense: one cloud through all
+    eight blocks, so walks cross block borders inside it, where the hierarchical
+    walk re-prepares 0.01 past the border (skipping a corner brick the ray clips
+    for less than that; a brick walk ignoring borders differed on 3 % of rays).
+    At mip 0 the march step is also shorter than that offset, so a ray's last
+    brick can hold samples before tFar although its offset entry lies past tFar
+    (a walk that stopped at the offset entry put 6 % of sparse pixels off by more
+    than 1e-3)."""
+    say(f"case: spatial reuse at mip 0, brick segments vs one kernel on a {layout} 256^3 grid of 2 x 2 x 2 blocks")
     import torch
     from volumetricrestirrelease_amd.volumetric_restir import VolumetricReSTIR
     spec = small_scene(res=256, width=64, height=48, fp16=True, maxBounces=3, initialM=6, spatialReuseRounds=2,
                        spatialSampleCount=5, spatialVisibilityMipLevel=0, spatialLightingMipLevel=0)
-    d = np.zeros_like(spec.density)
-    d[16:112, 16:112, 16:112] = spec.density[16:112, 16:112, 16:112]
-    d[144:240, 144:240, 144:240] = spec.density[144:240, 144:240, 144:240]
-    spec.density = d
+    if layout == "sparse":
+        d = np.zeros_like(spec.density)
+        d[16:112, 16:112, 16:112] = spec.density[16:112, 16:112, 16:112]
+        d[144:240, 144:240, 144:240] = spec.density[144:240, 144:240, 144:240]
+        spec.density = d
     outs = []
     for split in (0, 2):
         g = VolumetricReSTIR()

@@ -103,6 +103,7 @@ static constexpr int kWalkSteps = 32;           // walk-phase iterations before 
 static constexpr int kNodeWords = 64;           // block-occupancy bits kept in LDS (2048 blocks)
 static constexpr int kRayRefill = 16;           // lanes that must want a ray before a refill round
 static constexpr int kCarryLanes = 32;          // march lanes left when the rest of the march is carried over
+static constexpr int kBlockBatch = 16;          // lanes at a block border before their block entries run
 
 struct SegQueue {
     int rec[kRaySlots];
@@ -353,14 +354,13 @@ struct WalkCtx {
     int vs;                   // the walk grid slot when oneSlot
     bool oneSlot;
     const uint32_t* nb;       // block bits in LDS, or null
-    bool skipBlocks;          // the grid has empty blocks (FlatWalk)
     FlatGrid g;               // FlatWalk's grid view
 };
 VR_DEV void segBegin(Walker<2>& w, const WalkCtx& c, int slot, const Ray& r, bool linear, float tss) {
     walkBegin(w, *c.S, c.oneSlot, c.vs, slot, r, linear, tss);
 }
 VR_DEV void segBegin(FlatWalk& w, const WalkCtx& c, int, const Ray& r, bool, float tss) {
-    w.begin(*c.S, c.S->grid[c.vs], c.vs, r, tss);
+    w.begin(*c.S, c.S->grid[c.vs], c.vs, r, tss, c.g, c.nb);
 }
 VR_DEV float segMissTr(const Walker<2>& w) { return w.Tr; }
 VR_DEV float segMissTr(const FlatWalk&) { return 1.f; }
@@ -369,8 +369,13 @@ VR_DEV bool segStep(Walker<2>& w, const WalkCtx& c, uint32_t& bk, float& t0, v3&
     return w.ddaSegment(c.S->grid[w.slot], nullptr, bk, t0, q0);
 }
 VR_DEV bool segStep(FlatWalk& w, const WalkCtx& c, uint32_t& bk, float& t0, v3& q0) {
-    return w.step(c.g, c.nb, c.skipBlocks, bk, t0, q0);
+    return w.step(c.g, bk, t0, q0);
 }
+// FlatWalk's pending block entries (the hierarchical walker has none)
+VR_DEV bool segBlockEntry(const Walker<2>&) { return false; }
+VR_DEV bool segBlockEntry(const FlatWalk& w) { return w.blockEntry; }
+VR_DEV void segEnterBlock(Walker<2>&, const WalkCtx&) {}
+VR_DEV void segEnterBlock(FlatWalk& w, const WalkCtx& c) { w.enterBlock(c.g, c.nb); }
 
 __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ sc, const Frame fr, Options opt,
                                                       int isFinal, const TraceRec* __restrict__ recs,
@@ -422,14 +427,9 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
     }
     __syncthreads();
     const uint32_t* nb = nodeLds ? nodeBits : nullptr;
-    // with the block bits in LDS the walk runs over bricks only (FlatWalk),
-    // jumping empty blocks when the grid has any
+    // with the block bits in LDS the walk iterates over bricks only (FlatWalk)
     const bool flat = nodeLds;
-    int nOcc = nodeLds ? __popc(nodeBits[lane]) : 0;  // kNodeWords == 64: one word per lane
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) nOcc += __shfl_xor(nOcc, o);
-    const bool skipBlocks = flat && nOcc < nNodes;
-    WalkCtx wc{&S, vs, oneSlot, nb, skipBlocks, {}};
+    WalkCtx wc{&S, vs, oneSlot, nb, {}};
     if (flat) wc.g.init(S.grid[vs], vs);
     auto trace = [&](auto& w) {
         w.active = false;
@@ -484,17 +484,22 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
                     if (head >= total || !__ballot(!((waitMask >> bank) & 1))) break;
                     continue;
                 }
+                // block entries (FlatWalk), batched: a lane whose walk reached a block
+                // border waits until kBlockBatch lanes (or all walking lanes) do
+                const bool wasActive = w.active;
+                const uint64_t bm = __ballot(w.active && segBlockEntry(w));
+                if (bm && (__popcll(bm) >= kBlockBatch || bm == __ballot(w.active))) {
+                    if (w.active && segBlockEntry(w)) segEnterBlock(w, wc);
+                }
                 bool push = false;
                 uint32_t bk = kEmpty;
                 float t0 = 0.f;
                 v3 q0 = V3(0.f);
                 const int rs = 2 * lane + bank;
-                if (w.active) {
-                    push = segStep(w, wc, bk, t0, q0);
-                    if (!w.active) {
-                        waitMask |= 1 << bank;
-                        bank ^= 1;
-                    }
+                if (w.active && !segBlockEntry(w)) push = segStep(w, wc, bk, t0, q0);
+                if (wasActive && !w.active) {
+                    waitMask |= 1 << bank;
+                    bank ^= 1;
                 }
                 const uint64_t pm = __ballot(push);
                 if (push) {
