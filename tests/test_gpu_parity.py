@@ -38,6 +38,8 @@ def test_stage_parity_fp16_three_bounces_two_rounds():
     dict(enableSpatialReuse=0),
     dict(useEnvironmentLights=0),
     dict(useAnalyticLights=0, initialUseCoarserGridForIndirectBounce=0),
+    # spatial rays on two grids and filters: the segment trace's hierarchical walk and per-segment march
+    dict(spatialVisibilityMipLevel=0, spatialLightingMipLevel=1, spatialVisibilityUseLinearSampler=0),
 ])
 def test_stage_parity_option_surface(variant):
     say("case: option variant", variant)
@@ -84,8 +86,9 @@ def test_whole_frames_uninjected():
         assert abs(co[:, :3].mean() - cg[:, :3].mean()) <= 2e-3 * co[:, :3].mean()
 
 
-@pytest.mark.parametrize("mode,rtol,frac_min", [(1, 1e-5, 0.999), (2, 1e-4, 0.999)])
-def test_split_spatial_reuse_matches_single_kernel(mode, rtol, frac_min):
+@pytest.mark.parametrize("mode,rtol,frac_min,mixed", [(1, 1e-5, 0.999, False), (2, 1e-4, 0.999, False),
+                                                     (2, 1e-4, 0.999, True)])
+def test_split_spatial_reuse_matches_single_kernel(mode, rtol, frac_min, mixed):
     """Mode 1 traces each recorded ray in one lane: the same arithmetic as the
     single kernel. Mode 2 sums each ray's optical depth per brick segment, so its
     transmittances differ by float summation order (a few 1e-6 relative) and a
@@ -96,6 +99,8 @@ def test_split_spatial_reuse_matches_single_kernel(mode, rtol, frac_min):
     from volumetricrestirrelease_amd.volumetric_restir import VolumetricReSTIR
     spec = small_scene(fp16=True, maxBounces=3, initialM=8, temporalReuseMThreshold=2.5, spatialReuseRounds=2,
                        spatialSampleCount=6)
+    if mixed:  # visibility rays point-sampled at mip 0, lighting rays trilinear at mip 1
+        spec.params.update(spatialVisibilityMipLevel=0, spatialLightingMipLevel=1, spatialVisibilityUseLinearSampler=0)
     spec.orbit_degrees_per_frame = 2.0
     outs = []
     for split in (0, mode):
