@@ -284,9 +284,12 @@ int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t mode, int32
  * VRESTIR_TUNE_SPLIT_REUSE (default 2): 1 and 2 run spatial reuse as three
  * launches -- record its transmittance rays, trace them from per-wave balanced
  * queues, replay the reuse with their values -- when its ray kinds use ray
- * marching; 1 traces each ray in one lane, 2 walks rays and marches their brick
- * segments in separate phases; 0 runs it as one kernel. Results agree to float
- * rounding (2 sums each ray's optical depth per brick). */
+ * marching; 1 traces each ray in one lane, 2 walks rays and marches their
+ * brick segments in separate phases. Mode 2 also splits final shading when its
+ * ray kinds use analytic tracking on a mip-0 grid of at most 2048 128^3 blocks
+ * (the segments are integrated cell by cell); otherwise it runs as one kernel.
+ * 0 runs each stage as one kernel. Results agree to float rounding (2 sums each
+ * ray's optical depth per brick). */
 enum { VRESTIR_TUNE_SPLIT_REUSE = 1 };
 int32_t vrestir_set_tuning(vrestir_handle* h, int32_t key, int32_t value);
 

@@ -834,32 +834,59 @@ static int occPlan(const vrestir_handle* h, int a, int b, int slots[2]) {
 }
 
 // The spatial stage runs as record / trace / replay (balanced tracing of its
-// transmittance rays) when both of its ray kinds use ray marching and the ray
-// records of a frame fit kSplitBudget; otherwise it runs in one kernel.
+// transmittance rays) when both of its ray kinds use ray marching, and final
+// shading when both of its ray kinds use analytic tracking (segment form only,
+// when grid 0's blocks fit the trace's LDS bit mask), and the ray records of a
+// frame fit kSplitBudget; otherwise each runs in one kernel. The record buffers
+// hold the larger per-pixel capacity; each stage strides them by its own.
 static const size_t kSplitBudget = (size_t)48 << 30;
-static int splitPlan(vrestir_handle* h, LaunchArgs& a) {
+static int splitEnsure(vrestir_handle* h, int cap, size_t n) {
+    if (h->split.cap >= cap && h->splitPixels == n) return VRESTIR_OK;
+    freeSplit(h);
+    HIPCHK(h, hipMalloc((void**)&h->split.recs, n * (size_t)cap * sizeof(TraceRec)));
+    HIPCHK(h, hipMalloc((void**)&h->split.trs, n * (size_t)cap * sizeof(float)));
+    HIPCHK(h, hipMalloc((void**)&h->split.counts, n * sizeof(int32_t)));
+    HIPCHK(h, hipMalloc((void**)&h->split.plan, n * sizeof(uint32_t)));
+    h->split.cap = cap;
+    h->splitPixels = n;
+    return VRESTIR_OK;
+}
+static int splitPlan(vrestir_handle* h, LaunchArgs& a, int stage) {
     a.splitTK = 0;
     a.splitMode = 0;
+    if (!h->splitReuse) return VRESTIR_OK;
     const vrestir_params& p = h->P;
-    const Options& so = h->frame.spatialOpt;
-    if (!h->splitReuse || so.visibilityTrackingMethod != 2u || so.lightingTrackingMethod != 2u) return VRESTIR_OK;
-    const int K = p.spatialSampleCount;
-    const int evals = p.spatialMISMethod == VRESTIR_MIS_TALBOT ? K * (K - 1) : K - 1;
-    const int cap = std::max(1, evals * (p.maxBounces + 1));
     const size_t n = (size_t)h->W * h->H;
-    const size_t bytes = n * (size_t)cap * (sizeof(TraceRec) + sizeof(float)) + n * 8;
-    if (bytes > kSplitBudget) return VRESTIR_OK;
-    if (h->split.cap != cap || h->splitPixels != n) {
-        freeSplit(h);
-        HIPCHK(h, hipMalloc((void**)&h->split.recs, n * (size_t)cap * sizeof(TraceRec)));
-        HIPCHK(h, hipMalloc((void**)&h->split.trs, n * (size_t)cap * sizeof(float)));
-        HIPCHK(h, hipMalloc((void**)&h->split.counts, n * sizeof(int32_t)));
-        HIPCHK(h, hipMalloc((void**)&h->split.plan, n * sizeof(uint32_t)));
-        h->split.cap = cap;
-        h->splitPixels = n;
+    int cap = 0, tk = 0;
+    if (stage == VRESTIR_STAGE_SPATIAL_REUSE) {
+        const Options& so = h->frame.spatialOpt;
+        if (so.visibilityTrackingMethod != 2u || so.lightingTrackingMethod != 2u) return VRESTIR_OK;
+        const int K = p.spatialSampleCount;
+        const int evals = p.spatialMISMethod == VRESTIR_MIS_TALBOT ? K * (K - 1) : K - 1;
+        cap = std::max(1, evals * (p.maxBounces + 1));
+        tk = 2;
+    } else if (stage == VRESTIR_STAGE_FINAL_SHADING) {
+        const Options& fo = h->frame.finalOpt;
+        if (fo.visibilityTrackingMethod != 1u || fo.lightingTrackingMethod != 1u || p.useReference ||
+            h->frame.visualizeTransmittance)
+            return VRESTIR_OK;
+        // the analytic segment trace only (mode 2): one grid whose blocks fit its LDS bit mask, one filter
+        const Grid& g0 = h->sceneHost.grid[fo.visibilityMipLevel];
+        const bool uniformFilter = fo.visibilityUseLinearSampler == fo.lightingUseLinearSampler;
+        if (h->splitReuse != 2 || fo.visibilityMipLevel != 0 || !g0.valid ||
+            (size_t)g0.nnx * g0.nny * g0.nnz > 2048 || !uniformFilter)
+            return VRESTIR_OK;
+        cap = p.maxBounces + 1;  // the primary ray, one per extra bounce, the light ray
+        tk = 1;
+    } else {
+        return VRESTIR_OK;
     }
+    if (n * (size_t)cap * (sizeof(TraceRec) + sizeof(float)) + n * 8 > kSplitBudget) return VRESTIR_OK;
+    int rc = splitEnsure(h, cap, n);
+    if (rc) return rc;
     a.split = h->split;
-    a.splitTK = 2;
+    a.split.cap = cap;
+    a.splitTK = tk;
     a.splitMode = h->splitReuse;
     return VRESTIR_OK;
 }
@@ -938,8 +965,8 @@ extern "C" int32_t vrestir_run_stage(vrestir_handle* h, void* stream, int32_t st
         default: break;
     }
     a.occBytes = occPlan(h, oa, ob, a.frame.occSlot);
-    if (stage == VRESTIR_STAGE_SPATIAL_REUSE) {
-        int rc = splitPlan(h, a);
+    if (stage == VRESTIR_STAGE_SPATIAL_REUSE || stage == VRESTIR_STAGE_FINAL_SHADING) {
+        int rc = splitPlan(h, a, stage);
         if (rc) return rc;
     }
     HIPCHK(h, launchStage(a));

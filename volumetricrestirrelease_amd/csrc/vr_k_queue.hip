@@ -346,6 +346,93 @@ VR_DEV int marchPhase(const Scene& S, SegQueue& Q, int nSeg, uint64_t below, boo
     return 0;
 }
 
+// ------------------------------------------------------------------ analytic segments
+// The segment trace for analytic tracking (TrAnalytic): a segment is an occupied
+// brick and the distance tx its cell walk starts from; the march integrates the
+// brick cell by cell (the trilinear cell polynomial or the voxel's constant
+// density) up to the cell that reaches tFar.
+struct CellQueue {
+    int rec[kRaySlots];
+    float sum[kRaySlots];
+    float pos[3][kRaySlots], dir[3][kRaySlots];  // the index-space ray (vertex-shifted when trilinear)
+    float tFar[kRaySlots];
+    int pend[kRaySlots];                         // unused: the analytic march carries nothing over
+    int owner[kSegCap];                          // ray slot
+    uint32_t brick[kSegCap];
+    float tx[kSegCap];
+    int cell[kSegCap];                           // brick coordinates, x | y << 10 | z << 20
+};
+
+// Drains a wave's analytic segment queue: each lane integrates a segment's cells
+// (TrAnalytic::brickStep from the segment's tx: the leaf DDA prepared there, t =
+// tx - 0.01, at most 128 cells while inside the brick, ending at the cell that
+// reaches tFar) and takes the next segment as soon as it is done; the segment's
+// sum goes to its ray's slot with an LDS float atomic. LIN: trilinear cells.
+template <bool LIN>
+VR_DEV void marchCells(const Scene& S, const Grid& G, CellQueue& Q, int nSeg, uint64_t below) {
+    const float eps = 0.01f;
+    int mHead = 0;  // wave-uniform
+    bool has = false;
+    int owner = 0, it = 0;
+    BrickCtx bc;
+    bc.brick = 0;
+    bc.vmin = V3(0.f);
+    float t = 0.f, tFar = 0.f, part = 0.f;
+    Dda dl;
+    for (;;) {
+        const uint64_t nm = __ballot(!has);
+        if (nm && mHead < nSeg) {
+            if (!has) {
+                const int e = mHead + __popcll(nm & below);
+                if (e < nSeg) {
+                    owner = Q.owner[e];
+                    bc.brick = Q.brick[e];
+                    const float tx = Q.tx[e];
+                    const int c = Q.cell[e];
+                    bc.vmin = V3((float)((c & 1023) * 8), (float)(((c >> 10) & 1023) * 8), (float)((c >> 20) * 8));
+                    dl.setFromRay(V3(Q.pos[0][owner], Q.pos[1][owner], Q.pos[2][owner]),
+                                  V3(Q.dir[0][owner], Q.dir[1][owner], Q.dir[2][owner]), tx);
+                    dl.prepareLeaf(bc.vmin);
+                    tFar = Q.tFar[owner];
+                    t = tx - eps;
+                    it = 0;
+                    part = 0.f;
+                    has = true;
+                }
+            }
+            mHead += __popcll(nm);
+        }
+        if (!__ballot(has)) break;
+        if (has) {
+            bool done = !(it < 128 && leafInside(dl));
+            if (!done) {
+                dl.next();
+                const float maxDeltaT = dl.ty - t;
+                if constexpr (LIN) {
+                    const Poly c = cellPoly(S, G, dl, bc, dl.dir);
+                    const float td = fminf(tFar - t, maxDeltaT);
+                    const float t2 = td * td, t3 = t2 * td, t4 = t2 * t2;
+                    part += -(c.c3 * t4 / 4 + c.c2 * t3 / 3 + c.c1 * t2 / 2 + c.c0 * td);
+                } else {
+                    const float st = voxelDensity(S, G, bc, dl.p) * S.sigma_t;
+                    part += -fminf(tFar - t, maxDeltaT) * st;
+                }
+                if (t + maxDeltaT >= tFar) {
+                    done = true;
+                } else {
+                    t += maxDeltaT;
+                    dl.step();
+                    it++;
+                }
+            }
+            if (done) {
+                atomicAdd(&Q.sum[owner], part);
+                has = false;
+            }
+        }
+    }
+}
+
 // The two walk forms of kTraceSegments behind one interface: the brick-level
 // FlatWalk when the walk grid is uniform and its block bits fit LDS, else the
 // hierarchical Walker<2>.
@@ -355,12 +442,13 @@ struct WalkCtx {
     bool oneSlot;
     const uint32_t* nb;       // block bits in LDS, or null
     FlatGrid g;               // FlatWalk's grid view
+    bool cells;               // analytic tracking (the ray is vertex-shifted when trilinear)
 };
 VR_DEV void segBegin(Walker<2>& w, const WalkCtx& c, int slot, const Ray& r, bool linear, float tss) {
     walkBegin(w, *c.S, c.oneSlot, c.vs, slot, r, linear, tss);
 }
-VR_DEV void segBegin(FlatWalk& w, const WalkCtx& c, int, const Ray& r, bool, float tss) {
-    w.begin(*c.S, c.S->grid[c.vs], c.vs, r, tss, c.g, c.nb);
+VR_DEV void segBegin(FlatWalk& w, const WalkCtx& c, int, const Ray& r, bool linear, float tss) {
+    w.begin(*c.S, c.S->grid[c.vs], c.vs, r, tss, c.g, c.nb, c.cells && linear);
 }
 VR_DEV float segMissTr(const Walker<2>& w) { return w.Tr; }
 VR_DEV float segMissTr(const FlatWalk&) { return 1.f; }
@@ -371,12 +459,32 @@ VR_DEV bool segStep(Walker<2>& w, const WalkCtx& c, uint32_t& bk, float& t0, v3&
 VR_DEV bool segStep(FlatWalk& w, const WalkCtx& c, uint32_t& bk, float& t0, v3& q0) {
     return w.step(c.g, bk, t0, q0);
 }
+// A walked ray's slot data, by queue kind
+template <class W>
+VR_DEV void storeRay(SegQueue& Q, int rs, const W& w, int slot, bool linear) {
+    const v3 dq = 1.f * w.tStep * w.dda.dir;
+    Q.dq[0][rs] = dq.x; Q.dq[1][rs] = dq.y; Q.dq[2][rs] = dq.z;
+    Q.tStep[rs] = w.tStep;
+    Q.tFar[rs] = w.tFar;
+    Q.gs[rs] = slot | (linear ? 256 : 0);
+}
+template <class W>
+VR_DEV void storeRay(CellQueue& Q, int rs, const W& w, int, bool) {
+    Q.pos[0][rs] = w.dda.pos.x; Q.pos[1][rs] = w.dda.pos.y; Q.pos[2][rs] = w.dda.pos.z;
+    Q.dir[0][rs] = w.dda.dir.x; Q.dir[1][rs] = w.dda.dir.y; Q.dir[2][rs] = w.dda.dir.z;
+    Q.tFar[rs] = w.tFar;
+}
+
 // FlatWalk's pending block entries (the hierarchical walker has none)
 VR_DEV bool segBlockEntry(const Walker<2>&) { return false; }
 VR_DEV bool segBlockEntry(const FlatWalk& w) { return w.blockEntry; }
 VR_DEV void segEnterBlock(Walker<2>&, const WalkCtx&) {}
 VR_DEV void segEnterBlock(FlatWalk& w, const WalkCtx& c) { w.enterBlock(c.g, c.nb); }
 
+// TK = 2: ray marching (SegQueue, brick-sample march); TK = 1: analytic tracking
+// (CellQueue, cell integration; launched only on a grid whose block bits fit LDS,
+// one grid and one filter for both ray kinds).
+template <int TK>
 __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ sc, const Frame fr, Options opt,
                                                       int isFinal, const TraceRec* __restrict__ recs,
                                                       const int* __restrict__ counts, float* __restrict__ trs,
@@ -384,7 +492,8 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
     const Scene& S = *sc;
     occCacheLoad(S, fr.occSlot[0], fr.occSlot[1]);
     __shared__ int pre[4][65];
-    __shared__ SegQueue queues[4];
+    using Queue = std::conditional_t<TK == 2, SegQueue, CellQueue>;
+    __shared__ Queue queues[4];
     __shared__ uint32_t nodeBits[kNodeWords];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint64_t below = (1ull << lane) - 1ull;
@@ -400,7 +509,7 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
     if (lane == 0) pre[wv][0] = 0;
     __syncthreads();
     const int total = pre[wv][64];
-    SegQueue& Q = queues[wv];
+    Queue& Q = queues[wv];
 
     const int vs = opt.visibilityMipLevel, ls = isFinal ? 0 : opt.lightingMipLevel;
     const bool oneSlot = vs == ls;
@@ -429,7 +538,7 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
     const uint32_t* nb = nodeLds ? nodeBits : nullptr;
     // with the block bits in LDS the walk iterates over bricks only (FlatWalk)
     const bool flat = nodeLds;
-    WalkCtx wc{&S, vs, oneSlot, nb, {}};
+    WalkCtx wc{&S, vs, oneSlot, nb, {}, TK == 1};
     if (flat) wc.g.init(S.grid[vs], vs);
     auto trace = [&](auto& w) {
         w.active = false;
@@ -468,11 +577,7 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
                                 const int rs = 2 * lane + bank;
                                 Q.rec[rs] = rec;
                                 Q.sum[rs] = 0.f;
-                                const v3 dq = 1.f * w.tStep * w.dda.dir;
-                                Q.dq[0][rs] = dq.x; Q.dq[1][rs] = dq.y; Q.dq[2][rs] = dq.z;
-                                Q.tStep[rs] = w.tStep;
-                                Q.tFar[rs] = w.tFar;
-                                Q.gs[rs] = slot | (linear ? 256 : 0);
+                                storeRay(Q, rs, w, slot, linear);
                             } else {
                                 trs[rec] = segMissTr(w);  // the ray misses the grid
                             }
@@ -495,8 +600,12 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
                 uint32_t bk = kEmpty;
                 float t0 = 0.f;
                 v3 q0 = V3(0.f);
+                i3v cell = {0, 0, 0};
                 const int rs = 2 * lane + bank;
-                if (w.active && !segBlockEntry(w)) push = segStep(w, wc, bk, t0, q0);
+                if (w.active && !segBlockEntry(w)) {
+                    if constexpr (TK == 2) push = segStep(w, wc, bk, t0, q0);
+                    else push = w.stepCells(wc.g, bk, t0, cell);
+                }
                 if (wasActive && !w.active) {
                     waitMask |= 1 << bank;
                     bank ^= 1;
@@ -506,14 +615,23 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
                     const int e = nSeg + __popcll(pm & below);
                     Q.owner[e] = rs;
                     Q.brick[e] = bk;
-                    Q.t[e] = t0;
-                    Q.q[0][e] = q0.x; Q.q[1][e] = q0.y; Q.q[2][e] = q0.z;
+                    if constexpr (TK == 2) {
+                        Q.t[e] = t0;
+                        Q.q[0][e] = q0.x; Q.q[1][e] = q0.y; Q.q[2][e] = q0.z;
+                    } else {
+                        Q.tx[e] = t0;
+                        Q.cell[e] = cell.x | (cell.y << 10) | (cell.z << 20);
+                    }
                 }
                 nSeg += __popcll(pm);
             }
             waveSync();
             // ---- march phase: drain the segment queue, or all but a short tail when walks remain
-            if (uniformMarch) {
+            if constexpr (TK == 1) {
+                if (uniformLinear) marchCells<true>(S, S.grid[vs], Q, nSeg, below);
+                else marchCells<false>(S, S.grid[vs], Q, nSeg, below);
+                nSeg = 0;
+            } else if (uniformMarch) {
                 const Grid& G = S.grid[vs];
                 if (G.fp16) {
                     if (uniformLinear) marchPipelined<1, 1>(S, G, Q, nSeg, below);
@@ -545,7 +663,7 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
     if (flat) {
         FlatWalk w;
         trace(w);
-    } else {
+    } else if constexpr (TK == 2) {
         Walker<2> w;
         trace(w);
     }
@@ -553,22 +671,25 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
 
 hipError_t launchTraceSegments(const LaunchArgs& a, const Options& opt, bool isFinal) {
     const int n = a.width * a.height;
-    hipLaunchKernelGGL(kTraceSegments, dim3((n + 255) / 256), dim3(256), a.occBytes, a.stream, a.scene, a.frame, opt,
-                       isFinal ? 1 : 0, a.split.recs, a.split.counts, a.split.trs, a.split.cap, n);
+    hipLaunchKernelGGL(kTraceSegments<2>, dim3((n + 255) / 256), dim3(256), a.occBytes, a.stream, a.scene, a.frame,
+                       opt, isFinal ? 1 : 0, a.split.recs, a.split.counts, a.split.trs, a.split.cap, n);
+    return hipGetLastError();
+}
+
+hipError_t launchTraceCells(const LaunchArgs& a, const Options& opt, bool isFinal) {
+    const int n = a.width * a.height;
+    hipLaunchKernelGGL(kTraceSegments<1>, dim3((n + 255) / 256), dim3(256), a.occBytes, a.stream, a.scene, a.frame,
+                       opt, isFinal ? 1 : 0, a.split.recs, a.split.counts, a.split.trs, a.split.cap, n);
     return hipGetLastError();
 }
 
 hipError_t launchTraceQueue(const LaunchArgs& a, const Options& opt, bool isFinal, int tk) {
     const int n = a.width * a.height;
     const dim3 g((n + 255) / 256), blk(256);
-    if (tk == 2)
-        hipLaunchKernelGGL(kTraceQueue<2>, g, blk, a.occBytes, a.stream, a.scene, a.frame, opt, isFinal ? 1 : 0,
-                           a.split.recs, a.split.counts, a.split.trs, a.split.cap, n);
-    else if (tk == 1)
-        hipLaunchKernelGGL(kTraceQueue<1>, g, blk, a.occBytes, a.stream, a.scene, a.frame, opt, isFinal ? 1 : 0,
-                           a.split.recs, a.split.counts, a.split.trs, a.split.cap, n);
-    else
-        return hipErrorInvalidValue;
+    // ray marching only: the analytic walker queue has no parity coverage and is not launched
+    if (tk != 2) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(kTraceQueue<2>, g, blk, a.occBytes, a.stream, a.scene, a.frame, opt, isFinal ? 1 : 0,
+                       a.split.recs, a.split.counts, a.split.trs, a.split.cap, n);
     return hipGetLastError();
 }
 

@@ -42,6 +42,8 @@ hipError_t launchFinalShading(const LaunchArgs& a);
 // Balanced trace of the rays recorded in a.split (vr_k_queue.hip); tk: 2 ray marching, 1 analytic.
 hipError_t launchTraceQueue(const LaunchArgs& a, const Options& opt, bool isFinal, int tk);
 hipError_t launchTraceSegments(const LaunchArgs& a, const Options& opt, bool isFinal);
+// The same for analytic tracking (kTraceSegments<1>): one grid whose block bits fit LDS, one filter.
+hipError_t launchTraceCells(const LaunchArgs& a, const Options& opt, bool isFinal);
 // Ground-truth accumulation and relMSE (vr_k_accum.hip).
 hipError_t launchAccumulate(hipStream_t st, const float4* frame, double4* sum, int n, bool reset);
 hipError_t launchResolve(hipStream_t st, const double4* sum, double inv, float4* out, int n);

@@ -463,18 +463,23 @@ __global__ __launch_bounds__(256) void kCopy(const Frame fr, const ResStore* __r
     }
 }
 
-// FinalShading (VR/FinalShading.cs.slang:71-141).
-template <int B, int VM, int LM>
+// FinalShading (VR/FinalShading.cs.slang:71-141). MODE: kDirect traces each
+// transmittance ray on the spot; kRecord writes them to sb (and nothing else);
+// kReplay reads their transmittances back in the same order.
+template <int B, int VM, int LM, int MODE = kDirect>
 __global__ __launch_bounds__(256) void kFinal(const Scene* __restrict__ sc, const Frame fr,
                                               const ResStore* __restrict__ cur, const float* __restrict__ curExtra,
                                               const float2* __restrict__ features, const float4* __restrict__ colorIn,
-                                              float4* __restrict__ out) {
+                                              float4* __restrict__ out, SplitBufs sb) {
     const Scene& S = *sc;
     const Frame& F = fr;
     occCacheLoad(S, F.occSlot[0], F.occSlot[1]);
     PixelIdx px = pixelOf(F);
     if (!px.ok) return;
     Rng rng = rngCreate(px.x, px.y, (uint32_t)(F.numTotalRounds * F.frameCount + F.numTotalRounds - 1));
+    TracePolicy<MODE, VM, LM> tp;
+    if constexpr (MODE == kRecord) { tp.out = sb.recs + (size_t)px.id * sb.cap; tp.n = 0; }
+    if constexpr (MODE == kReplay) { tp.tr = sb.trs + (size_t)px.id * sb.cap; tp.n = 0; }
     v3 c = V3(0.f);
     if (F.useReference) {
         float4 v = colorIn[px.id];
@@ -485,10 +490,14 @@ __global__ __launch_bounds__(256) void kFinal(const Scene* __restrict__ sc, cons
         Res r = loadRes(cur, px.id, B);
         if (r.runningSum > 0.f) {
             Ray ray{ld3(F.cam.pos), rayDir(F.cam, px.x, px.y, F.width, F.height), 0.f, r.depth};
-            v3 f = evaluateF<B, VM, LM>(S, F.env[0], r, curExtra, ray, rng, F.finalOpt, F.noReuse != 0, true);
+            v3 f = evaluateF<B>(S, F.env[0], r, curExtra, ray, rng, F.finalOpt, F.noReuse != 0, true, tp);
             float Wt = r.p_y == 0.0f ? 1.f : r.runningSum / (r.p_y * r.M);
             c = c + f * Wt;
         }
+    }
+    if constexpr (MODE == kRecord) {
+        sb.counts[px.id] = tp.n;
+        return;
     }
     float4 col = make_float4(c.x, c.y, c.z, 1.f);
     if (isnan(c.x) || isnan(c.y) || isnan(c.z) || isinf(c.x) || isinf(c.y) || isinf(c.z))

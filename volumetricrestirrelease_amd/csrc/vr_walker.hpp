@@ -321,31 +321,44 @@ struct FlatWalk {
     int iter;
     bool active;
     bool blockEntry;   // tx passed tBlock: enterBlock() runs before the next step
-    // Walker<2>::begin for a ray-marching walk of grid G (slot `slot`) and the entry
-    // of its first block. A ray that misses the grid is complete (Tr = 1).
+    // Walker<TK>::begin for a walk of grid G (slot `slot`) and the entry of its first
+    // block: vc (analytic tracking with the trilinear filter) shifts the ray by half
+    // a voxel and takes the hierarchical walk's up to three block-level steps while
+    // outside the block range. A ray that misses the grid is complete (Tr = 1).
     VR_DEV void begin(const Scene& S, const Grid& G, int slot, const Ray& rWorld, float tStepScale,
-                      const FlatGrid& g, const uint32_t* nodeBits) {
+                      const FlatGrid& g, const uint32_t* nodeBits, bool vc = false) {
         const float eps = 0.01f;
         Ray ray{xformP(G.w2i, rWorld.o), xformV(G.w2i, rWorld.d), rWorld.tMin, rWorld.tMax};
+        if (vc) ray.o = ray.o - 0.5f;
         int eff = slot >= 19 ? slot - 19 : slot;
         eff = eff >= 8 ? eff - 8 : eff;
         tStep = S.tStep * S.worldScaling * tStepScale * (float)(eff + 1);
         invStep = 1.f / tStep;
-        if (!intersectBounds(G, ray, tNear, tFar, false)) {
+        if (!intersectBounds(G, ray, tNear, tFar, vc)) {
             active = false;
             return;
         }
         dda.setFromRay(ray.o, ray.d, tNear + eps);
         iter = 0;
         active = true;
-        enterBlock(g, nodeBits);
+        blockEntry = false;
+        Dda nd = dda;
+        nd.prepare(V3(0.f), 128.f);
+        if (vc) {
+            int k = 0;
+            while (k++ < 3 && (nd.p.x < 0 || nd.p.y < 0 || nd.p.z < 0 || nd.p.x > 32 || nd.p.y > 32 || nd.p.z > 32)) {
+                nd.next();
+                nd.step();
+                nd.tx += eps;
+            }
+        }
+        enterBlockFrom(nd, g, nodeBits);
     }
     // The hierarchical walk's level-up at tx and its block-level iterations up to
     // the next occupied block, whose brick level it prepares at the same tx (the
     // walk ends when tx passes tFar, leaves the 32^3-block index range or runs out
     // of iterations). nodeBits: the grid's block bits (in LDS).
     VR_DEV void enterBlock(const FlatGrid& g, const uint32_t* nodeBits) {
-        const float eps = 0.01f;
         blockEntry = false;
         if (dda.tx > tFar) {
             active = false;
@@ -353,6 +366,11 @@ struct FlatWalk {
         }
         Dda nd = dda;
         nd.prepare(V3(0.f), 128.f);
+        enterBlockFrom(nd, g, nodeBits);
+    }
+    // enterBlock's block-level iterations from the prepared block-level DDA nd
+    VR_DEV void enterBlockFrom(Dda nd, const FlatGrid& g, const uint32_t* nodeBits) {
+        const float eps = 0.01f;
         for (;;) {
             if (iter >= 4096 || nd.p.x < 0 || nd.p.y < 0 || nd.p.z < 0 || nd.p.x > 32 || nd.p.y > 32 ||
                 nd.p.z > 32) {
@@ -410,6 +428,42 @@ struct FlatWalk {
         dda.tx += eps;
         iter++;
         blockEntry = dda.tx > tBlock;
+        return seg;
+    }
+    // One brick cell of an analytic-tracking walk: true with the brick, the
+    // distance tx its cell walk starts from and its brick coordinates when the
+    // cell holds an occupied brick. The tracker integrates each occupied brick
+    // cell by cell until a cell reaches tFar, where the traversal ends, and walks
+    // on past tFar through empty cells; so the walk ends after the first occupied
+    // brick whose exit lies at or past tFar and whose cell walk has a cell (or at
+    // a block border past tFar). The cell walk starts at the voxel containing tx
+    // = entry + 0.01, which lies past a brick the ray clips for less than that:
+    // such a brick adds nothing and the tracker goes on to the next one, where
+    // its first cell, already past tFar, adds the clamped term and ends it.
+    VR_DEV bool stepCells(const FlatGrid& g, uint32_t& bk, float& txOut, i3v& cell) {
+        const float eps = 0.01f;
+        if (!(iter < 4096)) {
+            active = false;
+            return false;
+        }
+        dda.next();
+        cell = dda.p;
+        bk = g.lookup(dda.p.x, dda.p.y, dda.p.z);
+        const bool seg = bk != kEmpty;
+        txOut = dda.tx;
+        bool last = false;
+        if (seg && dda.ty >= tFar) {
+            // Dda::prepareLeaf's first cell, inside the brick?
+            const v3 pf = dda.pos + dda.dir * dda.tx - V3((float)(cell.x * 8), (float)(cell.y * 8), (float)(cell.z * 8));
+            const v3 fl = floor3(pf);
+            const i3v lp = {(int)fl.x, (int)fl.y, (int)fl.z};
+            last = (unsigned)lp.x < 8u && (unsigned)lp.y < 8u && (unsigned)lp.z < 8u;
+        }
+        dda.step();
+        dda.tx += eps;
+        iter++;
+        blockEntry = dda.tx > tBlock;
+        if (last) active = false;
         return seg;
     }
 };
