@@ -285,9 +285,11 @@ int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t mode, int32
  * launches -- record its transmittance rays, trace them from per-wave balanced
  * queues, replay the reuse with their values -- when its ray kinds use ray
  * marching; 1 traces each ray in one lane, 2 walks rays and marches their
- * brick segments in separate phases. Mode 2 also splits final shading when its
- * ray kinds use analytic tracking on a mip-0 grid of at most 2048 128^3 blocks
- * (the segments are integrated cell by cell); otherwise it runs as one kernel.
+ * brick segments in separate phases. Mode 2 also splits temporal reuse (same
+ * ray kinds, not on an animated volume's previous-frame grids) and final
+ * shading when its ray kinds use analytic tracking on a mip-0 grid of at most
+ * 2048 128^3 blocks (the segments are integrated cell by cell); otherwise they
+ * run as one kernel.
  * 0 runs each stage as one kernel. Results agree to float rounding (2 sums each
  * ray's optical depth per brick). */
 enum { VRESTIR_TUNE_SPLIT_REUSE = 1 };

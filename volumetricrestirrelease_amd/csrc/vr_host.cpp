@@ -834,7 +834,8 @@ static int occPlan(const vrestir_handle* h, int a, int b, int slots[2]) {
 }
 
 // The spatial stage runs as record / trace / replay (balanced tracing of its
-// transmittance rays) when both of its ray kinds use ray marching, and final
+// transmittance rays) when both of its ray kinds use ray marching, temporal reuse
+// likewise (segment form only, and not on the previous frame's grids), and final
 // shading when both of its ray kinds use analytic tracking (segment form only,
 // when grid 0's blocks fit the trace's LDS bit mask), and the ray records of a
 // frame fit kSplitBudget; otherwise each runs in one kernel. The record buffers
@@ -864,6 +865,14 @@ static int splitPlan(vrestir_handle* h, LaunchArgs& a, int stage) {
         const int K = p.spatialSampleCount;
         const int evals = p.spatialMISMethod == VRESTIR_MIS_TALBOT ? K * (K - 1) : K - 1;
         cap = std::max(1, evals * (p.maxBounces + 1));
+        tk = 2;
+    } else if (stage == VRESTIR_STAGE_TEMPORAL_REUSE) {
+        // the segment trace only; rays on the previous frame's grids (animated volumes) run in one kernel
+        const Options& so = h->frame.spatialOpt;
+        if (h->splitReuse != 2 || so.visibilityTrackingMethod != 2u || so.lightingTrackingMethod != 2u ||
+            h->frame.prevGridOffset != 0)
+            return VRESTIR_OK;
+        cap = 2 * (p.maxBounces + 1);  // two target evaluations of b + 1 rays each
         tk = 2;
     } else if (stage == VRESTIR_STAGE_FINAL_SHADING) {
         const Options& fo = h->frame.finalOpt;
@@ -965,7 +974,8 @@ extern "C" int32_t vrestir_run_stage(vrestir_handle* h, void* stream, int32_t st
         default: break;
     }
     a.occBytes = occPlan(h, oa, ob, a.frame.occSlot);
-    if (stage == VRESTIR_STAGE_SPATIAL_REUSE || stage == VRESTIR_STAGE_FINAL_SHADING) {
+    if (stage == VRESTIR_STAGE_TEMPORAL_REUSE || stage == VRESTIR_STAGE_SPATIAL_REUSE ||
+        stage == VRESTIR_STAGE_FINAL_SHADING) {
         int rc = splitPlan(h, a, stage);
         if (rc) return rc;
     }

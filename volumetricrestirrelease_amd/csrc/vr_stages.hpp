@@ -163,20 +163,36 @@ VR_DEV float2 featuresAt(const float2* f, int id, int n) {
 }
 VR_DEV bool isBackground(float2 f) { return f.y == 1.f && __float_as_int(f.x) != 0; }
 
-// TemporalReuse (VR/TemporalReuse.cs.slang:80-377).
-template <int B, int VM, int LM>
+// How a stage kernel obtains transmittances: traced on the spot (kDirect), or
+// split around a balanced trace of the recorded rays (kRecord, kReplay).
+enum { kDirect = 0, kRecord = 1, kReplay = 2 };
+template <int MODE, int VM, int LM>
+using TracePolicy = std::conditional_t<MODE == kDirect, DirectTrace<VM, LM>,
+                                       std::conditional_t<MODE == kRecord, RecordTrace, ReplayTrace>>;
+
+// TemporalReuse (VR/TemporalReuse.cs.slang:80-377). MODE: kDirect traces each
+// transmittance ray on the spot; kRecord writes them to sb (and nothing else:
+// which rays are traced depends on no transmittance, and the ray-marching
+// trackers draw no random numbers, so resampling can be left out); kReplay reads
+// their transmittances back in the same order.
+template <int B, int VM, int LM, int MODE = kDirect>
 VR_TRACING_KERNEL void kTemporal(const Scene* __restrict__ sc, const Frame fr,
                                                  ResStore* __restrict__ cur, float* __restrict__ curExtra,
                                                  const ResStore* __restrict__ temporal,
                                                  const float* __restrict__ temporalExtra,
                                                  const float2* __restrict__ features,
                                                  const float2* __restrict__ temporalFeatures,
-                                                 float2* __restrict__ mvec) {
+                                                 float2* __restrict__ mvec, SplitBufs sb) {
     const Scene& S = *sc;
     const Frame& F = fr;
     occCacheLoad(S, F.occSlot[0], F.occSlot[1]);
     PixelIdx px = pixelOf(F);
-    if (!px.ok || F.isFirstFrame) return;
+    if (!px.ok) return;
+    TracePolicy<MODE, VM, LM> tp;
+    if constexpr (MODE == kRecord) { tp.out = sb.recs + (size_t)px.id * sb.cap; tp.n = 0; sb.counts[px.id] = 0; }
+    if constexpr (MODE == kReplay) { tp.tr = sb.trs + (size_t)px.id * sb.cap; tp.n = 0; }
+    constexpr bool kWrites = MODE != kRecord;
+    if (F.isFirstFrame) return;
     const int W = F.width, H = F.height, N = W * H;
     Rng rng = rngCreate(px.x, px.y, (uint32_t)(F.numTotalRounds * F.frameCount + 1));
     int selectedId = -1;
@@ -222,7 +238,8 @@ VR_TRACING_KERNEL void kTemporal(const Scene* __restrict__ sc, const Frame fr,
         {
             float2 tf = featuresAt(temporalFeatures, syi * W + sxi, N);
             if (isBgRes && !isBackground(tf)) {
-                if (F.outputMotionVec) mvec[px.id] = make_float2((float)(rsx - px.x) / W, (float)(rsy - px.y) / H);
+                if (kWrites && F.outputMotionVec)
+                    mvec[px.id] = make_float2((float)(rsx - px.x) / W, (float)(rsy - px.y) / H);
                 return;
             }
         }
@@ -239,7 +256,7 @@ VR_TRACING_KERNEL void kTemporal(const Scene* __restrict__ sc, const Frame fr,
         rsx = px.x; rsy = px.y;
         taps[1] = loadRes(temporal, px.id, B);
     }
-    if (F.outputMotionVec) mvec[px.id] = make_float2((float)(rsx - px.x) / W, (float)(rsy - px.y) / H);
+    if (kWrites && F.outputMotionVec) mvec[px.id] = make_float2((float)(rsx - px.x) / W, (float)(rsy - px.y) / H);
     float maxPrevM = F.temporalThreshold * taps[0].M;
     if (numUsed == 2) {
         temporalOriginalDepth = taps[1].depth;
@@ -262,7 +279,7 @@ VR_TRACING_KERNEL void kTemporal(const Scene* __restrict__ sc, const Frame fr,
             nPy = ti.p_y;
             if (isnan(ti.runningSum) || isinf(ti.runningSum)) ti.runningSum = 0.f;
             if (i > 0 && ti.runningSum != 0.f) {
-                float ph = lum(evaluateF<B, VM, LM>(S, F.env[0], ti, temporalExtra, ray, rng, F.spatialOpt, false, false));
+                float ph = lum(evaluateF<B>(S, F.env[0], ti, temporalExtra, ray, rng, F.spatialOpt, false, false, tp));
                 float w = ph / ti.p_y;
                 if (isinf(w) || isnan(w)) w = 0.f;
                 ti.runningSum *= w;
@@ -290,18 +307,24 @@ VR_TRACING_KERNEL void kTemporal(const Scene* __restrict__ sc, const Frame fr,
                     Res tmp = ti;
                     tmp.depth = centerPrevDepth;
                     // isLastFrame (VR/TemporalReuse.cs.slang:335-338): previous orientation and grids
-                    float py = lum(evaluateF<B, VM, LM>(S, F.env[1], tmp, curExtra, nr, rng, F.spatialOpt, false, false,
-                                                        F.prevGridOffset));
+                    float py = lum(evaluateF<B>(S, F.env[1], tmp, curExtra, nr, rng, F.spatialOpt, false, false, tp,
+                                                F.prevGridOffset));
                     if (isinf(py) || isnan(py)) py = 0.f;
                     psum += py * cM;
                 }
             }
             if (psum > 0) talbotW = pqi * k / psum;
         }
-        ti.runningSum *= talbotW;
-        bool curSel = resampleMaxM(ti, maxPrevM, output, rng);
-        hasSel |= curSel;
-        if (curSel) selectedId = i;
+        if constexpr (kWrites) {
+            ti.runningSum *= talbotW;
+            bool curSel = resampleMaxM(ti, maxPrevM, output, rng);
+            hasSel |= curSel;
+            if (curSel) selectedId = i;
+        }
+    }
+    if constexpr (MODE == kRecord) {
+        sb.counts[px.id] = tp.n;
+        return;
     }
     if (B > 1) {
         if (hasSel && selectedId > 0) {
@@ -317,11 +340,6 @@ VR_TRACING_KERNEL void kTemporal(const Scene* __restrict__ sc, const Frame fr,
     }
     storeRes(cur, px.id, output);
 }
-
-enum { kDirect = 0, kRecord = 1, kReplay = 2 };
-template <int MODE, int VM, int LM>
-using TracePolicy = std::conditional_t<MODE == kDirect, DirectTrace<VM, LM>,
-                                       std::conditional_t<MODE == kRecord, RecordTrace, ReplayTrace>>;
 
 // SpatialReuse (VR/SpatialReuse.cs.slang:94-265) for one round. MODE: the
 // direct form, or the record / replay passes around a balanced trace of the
