@@ -21,12 +21,12 @@ sys.path.insert(0, ROOT)
 
 STAGE_NAMES = ["GenerateFeatures", "TraceRays", "TemporalReuse", "SpatialReuse", "CopyReservoirs", "FinalShading"]
 HBM_PEAK_GBS = 8000.0
-# HBM traffic per launch of each stage at config1 (SpatialReuse and FinalShading:
-# their record, trace and replay kernels), from rocprofv3 --pmc FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00
+# HBM traffic per launch of each stage at config1 (SpatialReuse, TemporalReuse and
+# FinalShading: their record, trace and replay kernels), from rocprofv3 --pmc FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00
 # (factors calibrated on a 256 MiB copy in the same process); commands and figures
 # in profiles/SpatialReuse_config1_1080p.md.
-MEASURED_TRAFFIC_BYTES_CONFIG1 = {"SpatialReuse": 33.199e9, "TraceRays": 14.728e9, "FinalShading": 4.483e9,
-                                  "TemporalReuse": 1.660e9, "GenerateFeatures": 0.180e9, "CopyReservoirs": 0.156e9}
+MEASURED_TRAFFIC_BYTES_CONFIG1 = {"SpatialReuse": 33.161e9, "TraceRays": 14.718e9, "FinalShading": 4.477e9,
+                                  "TemporalReuse": 3.593e9, "GenerateFeatures": 0.180e9, "CopyReservoirs": 0.156e9}
 
 
 def stage_bytes_per_pixel(stage, b, k):
