@@ -312,6 +312,14 @@ struct BrickCtx {
 
 // VolumeTrackingGVDB (VR/VolumeUtils.slang:171-282): walks 128^3 blocks, then
 // bricks, and hands every occupied brick to the tracker with the DDA state.
+// The walk runs at brick level only; it keeps the hierarchical walk's block
+// semantics: when tx passes the current block's exit it re-prepares at the
+// block level at tx, steps over empty blocks, and prepares the brick level at
+// the same tx in the next occupied block (a prepare from the same distance
+// gives bit-identical side distances at either level), and it ends where the
+// hierarchical walk leaves its top level (tx past tFar at a block border, or
+// outside the 32^3-block index range). The trackers use the brick-level DDA
+// only through pos, dir, |1/dir|, tx and the brick's minimum.
 template <class A>
 VR_DEV void traverse(const Scene& S, const Ray& rWorld, int slot, Rng& rng, A& ad, bool vertexCenter) {
     const Grid& G = S.grid[slot];
@@ -322,62 +330,61 @@ VR_DEV void traverse(const Scene& S, const Ray& rWorld, int slot, Rng& rng, A& a
     if (!intersectBounds(G, ray, tNear, tFar, vertexCenter)) { ad.end(); return; }
     ad.setRayInfo(tNear, tFar, ray);
     ad.start();
-    int lev = 2;
-    float tMax2 = tFar, tMax1 = 0.f;
-    i3v node = {0, 0, 0};  // current 128^3 block when lev == 1
     Dda dda;
     dda.setFromRay(ray.o, ray.d, tNear + eps);
-    dda.prepare(V3(0.f), 128.f);
-    auto outside = [&](int res) {
-        return dda.p.x < 0 || dda.p.y < 0 || dda.p.z < 0 || dda.p.x > res || dda.p.y > res || dda.p.z > res;
+    auto outsideBlocks = [](const Dda& d) {
+        return d.p.x < 0 || d.p.y < 0 || d.p.z < 0 || d.p.x > 32 || d.p.y > 32 || d.p.z > 32;
     };
-    if (vertexCenter) {
-        int it = 0;
-        while (it++ < 3 && outside(32)) { dda.next(); dda.step(); dda.tx += eps; }
+    int iter = 0;
+    float tBlock = 0.f;
+    // block-level iterations from the prepared block-level DDA nd up to the next
+    // occupied block, whose brick level is then prepared at the same tx
+    auto enterFrom = [&](Dda nd) -> bool {
+        for (;;) {
+            if (iter >= 4096 || outsideBlocks(nd)) return false;
+            nd.next();
+            const int b = (((nd.p.z << 5) + nd.p.y) << 5) + nd.p.x;
+            iter++;
+            if (b < 32768 && nodeAt(G, b & 31, (b >> 5) & 31, b >> 10)) break;
+            nd.step();
+            nd.tx += eps;
+            if (nd.tx > tFar) return false;
+        }
+        tBlock = nd.ty;
+        dda.tx = nd.tx;
+        dda.prepare(V3(0.f), 8.f);
+        return true;
+    };
+    {
+        Dda nd = dda;
+        nd.prepare(V3(0.f), 128.f);
+        if (vertexCenter) {
+            int k = 0;
+            while (k++ < 3 && outsideBlocks(nd)) { nd.next(); nd.step(); nd.tx += eps; }
+        }
+        if (!enterFrom(nd)) { ad.end(); return; }
     }
-    for (int iter = 0; iter < 4096 && lev > 0 && lev <= 2 && !outside(lev == 2 ? 32 : 16); iter++) {
+    while (iter < 4096) {
         dda.next();
-        bool occupied;
-        int cx = 0, cy = 0, cz = 0;
-        if (lev == 2) {
-            int b = (((dda.p.z << 5) + dda.p.y) << 5) + dda.p.x;
-            occupied = b < 32768 && nodeAt(G, b & 31, (b >> 5) & 31, b >> 10);
-            cx = b & 31; cy = (b >> 5) & 31; cz = b >> 10;
-        } else {
-            int b = (((dda.p.z << 4) + dda.p.y) << 4) + dda.p.x;
-            cx = node.x * 16 + (b & 15); cy = node.y * 16 + ((b >> 4) & 15); cz = node.z * 16 + (b >> 8);
-            occupied = b < 4096;
+        const uint32_t brick = brickLookup(G, slot, dda.p.x, dda.p.y, dda.p.z);
+        if (brick != kEmpty) {
+            float t = dda.tx - eps;
+            BrickCtx bc;
+            bc.brick = brick;
+            bc.vmin = V3((float)(dda.p.x * 8), (float)(dda.p.y * 8), (float)(dda.p.z * 8));
+            gptr<float> bd = gbl<float>(G.bounds) + 4 * (size_t)brick;
+            bc.bounds[0] = bd[0] * S.densityScale; bc.bounds[1] = bd[1] * S.densityScale;
+            bc.bounds[2] = bd[2] * S.densityScale; bc.bounds[3] = bd[3] * S.densityScale;
+            if (ad.brickStep(dda, bc, slot, t, rng)) return;
         }
-        uint32_t brick = kEmpty;
-        if (occupied && lev == 1) {
-            brick = brickLookup(G, slot, cx, cy, cz);
-            occupied = brick != kEmpty;
-        }
-        if (occupied) {
-            if (lev == 1) {
-                float t = dda.tx - eps;
-                BrickCtx bc;
-                bc.brick = brick;
-                bc.vmin = V3((float)(cx * 8), (float)(cy * 8), (float)(cz * 8));
-                gptr<float> bd = gbl<float>(G.bounds) + 4 * (size_t)brick;
-                bc.bounds[0] = bd[0] * S.densityScale; bc.bounds[1] = bd[1] * S.densityScale;
-                bc.bounds[2] = bd[2] * S.densityScale; bc.bounds[3] = bd[3] * S.densityScale;
-                if (ad.brickStep(dda, bc, slot, t, rng)) return;
-                dda.step();
-                dda.tx += eps;
-            } else {
-                lev = 1;
-                node = {cx, cy, cz};
-                tMax1 = dda.ty;
-                dda.prepare(V3((float)(cx * 128), (float)(cy * 128), (float)(cz * 128)), 8.f);
-            }
-        } else {
-            dda.step();
-            dda.tx += eps;
-        }
-        while (lev <= 2 && dda.tx > (lev == 1 ? tMax1 : tMax2)) {
-            lev++;
-            if (lev <= 2) dda.prepare(V3(0.f), 128.f);
+        dda.step();
+        dda.tx += eps;
+        iter++;
+        if (dda.tx > tBlock) {
+            if (dda.tx > tFar) break;
+            Dda nd = dda;
+            nd.prepare(V3(0.f), 128.f);
+            if (!enterFrom(nd)) break;
         }
     }
     ad.end();
