@@ -1,0 +1,51 @@
+"""The HIP pass's expectation against the HIP volume path tracer
+(IntegrateByVolumePathTracing, VR/VolumePathTracingFunctions.slang:3-132): the
+same statistical check the oracle passes on the CPU, run through the product
+library at a larger image and more frames. Final shading uses the path
+tracer's transmittance estimator (residual ratio tracking) so both estimate
+the same integral; consecutive frames with temporal reuse are batched."""
+import numpy as np
+import pytest
+
+from gpu_common import say
+from test_oracle_unbiased import CASES, compare
+from volumetricrestirrelease_amd import scenes
+from volumetricrestirrelease_amd.volumetric_restir import VolumetricReSTIR
+
+pytestmark = pytest.mark.gpu
+
+W, H = 48, 36
+
+
+def gpu_frames(params, n, reference):
+    import torch
+    spec = scenes.config0(res=32, width=W, height=H)
+    spec.params.update(finalVisibilityTrackingMethod=3, finalLightTrackingMethod=3)
+    spec.params.update(params)
+    if reference:
+        spec.params.update(useReference=1, baselineSamplePerPixel=4)
+    r = VolumetricReSTIR()
+    r.setScene(spec)
+    img = torch.zeros((H, W, 4), device="cuda")
+    out = torch.empty((n, H, W, 3), dtype=torch.float64, device="cuda")
+    for k in range(n):
+        r.execute(img)
+        out[k] = img[..., :3].double()
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_gpu_pass_expectation_equals_path_tracer(case):
+    params, batch = CASES[case]
+    n = 4000
+    say(f"case: {case} {W}x{H}, {n} pass frames against {n // 4} path-traced frames at 4 spp")
+    shared = {k: v for k, v in params.items() if k == "maxBounces"}
+    pt = gpu_frames(shared, n // 4, reference=True)
+    ps = gpu_frames(params, n, reference=False)
+    z, zi, rel = compare(pt, ps, batch)
+    chi2 = float((z ** 2).mean())
+    say(f"   image z {zi:.2f} (rel {rel:+.4f}), chi2/dof {chi2:.3f}, max|z| {np.abs(z).max():.2f}")
+    assert abs(zi) < 4.5
+    assert chi2 < 1.4
+    assert np.abs(z).max() < 6.5
