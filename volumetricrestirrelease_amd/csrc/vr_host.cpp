@@ -105,7 +105,7 @@ static void dfree(T*& p) {
 }
 
 static void freeSplit(vrestir_handle* h) {
-    dfree(h->split.recs); dfree(h->split.trs); dfree(h->split.counts); dfree(h->split.plan);
+    dfree(h->split.recs); dfree(h->split.trs); dfree(h->split.dens); dfree(h->split.counts); dfree(h->split.plan);
     h->split.cap = 0;
     h->splitPixels = 0;
 }
@@ -846,6 +846,7 @@ static int splitEnsure(vrestir_handle* h, int cap, size_t n) {
     freeSplit(h);
     HIPCHK(h, hipMalloc((void**)&h->split.recs, n * (size_t)cap * sizeof(TraceRec)));
     HIPCHK(h, hipMalloc((void**)&h->split.trs, n * (size_t)cap * sizeof(float)));
+    HIPCHK(h, hipMalloc((void**)&h->split.dens, n * (size_t)cap * sizeof(float)));
     HIPCHK(h, hipMalloc((void**)&h->split.counts, n * sizeof(int32_t)));
     HIPCHK(h, hipMalloc((void**)&h->split.plan, n * sizeof(uint32_t)));
     h->split.cap = cap;
@@ -890,7 +891,7 @@ static int splitPlan(vrestir_handle* h, LaunchArgs& a, int stage) {
     } else {
         return VRESTIR_OK;
     }
-    if (n * (size_t)cap * (sizeof(TraceRec) + sizeof(float)) + n * 8 > kSplitBudget) return VRESTIR_OK;
+    if (n * (size_t)cap * (sizeof(TraceRec) + 2 * sizeof(float)) + n * 8 > kSplitBudget) return VRESTIR_OK;
     int rc = splitEnsure(h, cap, n);
     if (rc) return rc;
     a.split = h->split;

@@ -42,11 +42,11 @@ __global__ __launch_bounds__(256) void kTraceQueue(const Scene* __restrict__ sc,
     Walker<TK> w;
     w.active = false;
     bool busy = false;
-    int rec = 0, head = 0;
+    int tri = 0, head = 0;  // tri: the ray's transmittance slot
     for (;;) {
         if (busy) {
             walkAdvance(w, S, oneSlot, vs);
-            if (!w.active) { trs[rec] = w.Tr; busy = false; }
+            if (!w.active) { trs[tri] = w.Tr; busy = false; }
         }
         const uint64_t idle = __ballot(!busy);
         const int nIdle = __popcll(idle);
@@ -63,7 +63,8 @@ __global__ __launch_bounds__(256) void kTraceQueue(const Scene* __restrict__ sc,
                     const int mid = (lo + hi) >> 1;
                     if (pre[wv][mid] <= r) lo = mid; else hi = mid;
                 }
-                rec = (p0 + lo) * cap + (r - pre[wv][lo]);
+                const int rec = (p0 + lo) * cap + (r - pre[wv][lo]);
+                tri = (r - pre[wv][lo]) * numPixels + p0 + lo;
                 const TraceRec R = recs[rec];
                 const int kind = R.kind;
                 const int slot = kind == 0 ? vs : ls;
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(256) void kTraceQueue(const Scene* __restrict__ sc,
                 walkBegin(w, S, oneSlot, vs, slot, Ray{V3(R.ox, R.oy, R.oz), V3(R.dx, R.dy, R.dz), 0.f, R.tMax},
                           linear, tss);
                 if (w.active) busy = true;
-                else trs[rec] = w.Tr;
+                else trs[tri] = w.Tr;
             }
         }
         head += nIdle;
@@ -106,7 +107,7 @@ static constexpr int kCarryLanes = 32;          // march lanes left when the res
 static constexpr int kBlockBatch = 16;          // lanes at a block border before their block entries run
 
 struct SegQueue {
-    int rec[kRaySlots];
+    int rec[kRaySlots];  // the ray's transmittance slot
     float sum[kRaySlots];
     float dq[3][kRaySlots];
     float tStep[kRaySlots], tFar[kRaySlots];
@@ -352,7 +353,7 @@ VR_DEV int marchPhase(const Scene& S, SegQueue& Q, int nSeg, uint64_t below, boo
 // brick cell by cell (the trilinear cell polynomial or the voxel's constant
 // density) up to the cell that reaches tFar.
 struct CellQueue {
-    int rec[kRaySlots];
+    int rec[kRaySlots];  // the ray's transmittance slot
     float sum[kRaySlots];
     float pos[3][kRaySlots], dir[3][kRaySlots];  // the index-space ray (vertex-shifted when trilinear)
     float tFar[kRaySlots];
@@ -565,6 +566,7 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
                                 if (pre[wv][mid] <= r) lo = mid; else hi = mid;
                             }
                             const int rec = (p0 + lo) * cap + (r - pre[wv][lo]);
+                            const int tri = (r - pre[wv][lo]) * numPixels + p0 + lo;
                             const TraceRec R = recs[rec];
                             const int kind = R.kind;
                             const int slot = kind == 0 ? vs : ls;
@@ -575,11 +577,11 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
                                      linear, tss);
                             if (w.active) {
                                 const int rs = 2 * lane + bank;
-                                Q.rec[rs] = rec;
+                                Q.rec[rs] = tri;
                                 Q.sum[rs] = 0.f;
                                 storeRay(Q, rs, w, slot, linear);
                             } else {
-                                trs[rec] = segMissTr(w);  // the ray misses the grid
+                                trs[tri] = segMissTr(w);  // the ray misses the grid
                             }
                         }
                     }

@@ -355,6 +355,11 @@ VR_DEV v3 directLightingPT(const Scene& S, const EnvXf& X, Rng& rng, const Mediu
 // only at the transmittance-free product Fn (sample, phase, density and light
 // factors), never at a visibility: a zero visibility makes F zero and every
 // later factor is finite, so continuing changes the result by nothing.
+//
+// The record pass also keeps every density lookup of its target evaluations,
+// and the replay reads them back in the same order instead of fetching the
+// volume again. An evaluation makes at most b lookups and may record b + 1
+// rays, so a pixel's ray capacity bounds its lookups too.
 template <int VM, int LM>
 struct DirectTrace {
     static constexpr bool kSplit = false;
@@ -368,6 +373,7 @@ struct DirectTrace {
         return visibilityT<LM>(S, r, rng, o.lightSamples, isFinal ? 0 : o.lightingMipLevel + off,
                                o.lightingUseLinearSampler != 0, o.lightingTrackingMethod, o.lightingTStepScale);
     }
+    VR_DEV float density(const Scene& S, int off, v3 p) { return densityWorld(S, off, p); }
 };
 // visibilityT's average of `samples` identical transmittances (0/0 for none).
 VR_DEV float sameSamples(float tr, int samples) {
@@ -388,16 +394,29 @@ struct RecordTrace {
         out[n++] = rec;
         return sameSamples(1.f, samples);
     }
+    float* dens;       // this pixel's first density slot; slot i is i * dstride further
+    size_t dstride;
+    int nd;
+    VR_DEV float density(const Scene& S, int off, v3 p) {
+        const float d = densityWorld(S, off, p);
+        dens[(size_t)nd++ * dstride] = d;
+        return d;
+    }
 };
 struct ReplayTrace {
     static constexpr bool kSplit = true;
-    const float* tr;  // this pixel's transmittances, in record order
+    const float* tr;  // this pixel's first transmittance; the i-th recorded ray's is i * trStride further
+    size_t trStride;
     int n;
     VR_DEV float vis(const Scene&, const Ray&, Rng&, const Options& o, int kind, bool, int) {
         const int samples = kind == 0 ? o.visibilitySamples : o.lightSamples;
         if (samples <= 0) return sameSamples(1.f, samples);
-        return sameSamples(tr[n++], samples);
+        return sameSamples(tr[(size_t)n++ * trStride], samples);
     }
+    const float* dens;  // the record pass's density lookups, in the same order and layout
+    size_t dstride;
+    int nd;
+    VR_DEV float density(const Scene&, int, v3) { return dens[(size_t)nd++ * dstride]; }
 };
 
 // evaluate_F_ (VR/ReSTIRHelper.slang:91-423) + evaluate_L_in_volume (:443-496).
@@ -418,7 +437,7 @@ VR_DEV v3 evaluateF(const Scene& S, const EnvXf& X, Res tap, const float* extra,
     v3 pW = at(ray, ray.tMax);
     MediumInteraction mi{pW, -ray.d, S.g, true};
     {
-        float density = (isBg || noReuse) ? 1.f : densityWorld(S, gridOff, pW);
+        float density = (isBg || noReuse) ? 1.f : tp.density(S, gridOff, pW);
         if (density == 0.f) return V3(0.f);
         if (!noReuse) vis = tp.vis(S, ray, rng, opt, 0, isFinal, gridOff);
         v3 ss = isBg ? V3(1.f) : (isSelfEmission ? ld3(S.sigma_a) : ld3(S.sigma_s));
@@ -464,7 +483,7 @@ VR_DEV v3 evaluateF(const Scene& S, const EnvXf& X, Res tap, const float* extra,
                     float sd;
                     if (!noReuse) {
                         ss = curEmissive ? ld3(S.sigma_a) : ld3(S.sigma_s);
-                        sd = fmaxf(0.f, densityWorld(S, gridOff, pW));
+                        sd = fmaxf(0.f, tp.density(S, gridOff, pW));
                     } else {
                         ss = curEmissive ? ld3(S.sigma_a) / S.sigma_t : ld3(S.sigma_s) / S.sigma_t;
                         sd = 1.f;

@@ -189,8 +189,8 @@ VR_TRACING_KERNEL void kTemporal(const Scene* __restrict__ sc, const Frame fr,
     PixelIdx px = pixelOf(F);
     if (!px.ok) return;
     TracePolicy<MODE, VM, LM> tp;
-    if constexpr (MODE == kRecord) { tp.out = sb.recs + (size_t)px.id * sb.cap; tp.n = 0; sb.counts[px.id] = 0; }
-    if constexpr (MODE == kReplay) { tp.tr = sb.trs + (size_t)px.id * sb.cap; tp.n = 0; }
+    if constexpr (MODE == kRecord) { tp.out = sb.recs + (size_t)px.id * sb.cap; tp.n = 0; tp.dens = sb.dens + px.id; tp.dstride = (size_t)F.width * F.height; tp.nd = 0; sb.counts[px.id] = 0; }
+    if constexpr (MODE == kReplay) { tp.tr = sb.trs + px.id; tp.trStride = (size_t)F.width * F.height; tp.n = 0; tp.dens = sb.dens + px.id; tp.dstride = (size_t)F.width * F.height; tp.nd = 0; }
     constexpr bool kWrites = MODE != kRecord;
     if (F.isFirstFrame) return;
     const int W = F.width, H = F.height, N = W * H;
@@ -365,8 +365,8 @@ VR_TRACING_KERNEL void kSpatial(const Scene* __restrict__ sc, const Frame fr, in
     const int K = F.spatialSampleCount;
     const bool talbot = F.spatialMIS == 1;
     TracePolicy<MODE, VM, LM> tp;
-    if constexpr (MODE == kRecord) { tp.out = sb.recs + (size_t)px.id * sb.cap; tp.n = 0; }
-    if constexpr (MODE == kReplay) { tp.tr = sb.trs + (size_t)px.id * sb.cap; tp.n = 0; }
+    if constexpr (MODE == kRecord) { tp.out = sb.recs + (size_t)px.id * sb.cap; tp.n = 0; tp.dens = sb.dens + px.id; tp.dstride = (size_t)F.width * F.height; tp.nd = 0; }
+    if constexpr (MODE == kReplay) { tp.tr = sb.trs + px.id; tp.trStride = (size_t)F.width * F.height; tp.n = 0; tp.dens = sb.dens + px.id; tp.dstride = (size_t)F.width * F.height; tp.nd = 0; }
     uint32_t plan = 0;
     if constexpr (MODE == kReplay) plan = sb.plan[px.id];
     Rng rng = rngCreate(px.x, px.y, (uint32_t)(numRounds * F.frameCount + rid));
@@ -496,8 +496,8 @@ __global__ __launch_bounds__(256) void kFinal(const Scene* __restrict__ sc, cons
     if (!px.ok) return;
     Rng rng = rngCreate(px.x, px.y, (uint32_t)(F.numTotalRounds * F.frameCount + F.numTotalRounds - 1));
     TracePolicy<MODE, VM, LM> tp;
-    if constexpr (MODE == kRecord) { tp.out = sb.recs + (size_t)px.id * sb.cap; tp.n = 0; }
-    if constexpr (MODE == kReplay) { tp.tr = sb.trs + (size_t)px.id * sb.cap; tp.n = 0; }
+    if constexpr (MODE == kRecord) { tp.out = sb.recs + (size_t)px.id * sb.cap; tp.n = 0; tp.dens = sb.dens + px.id; tp.dstride = (size_t)F.width * F.height; tp.nd = 0; }
+    if constexpr (MODE == kReplay) { tp.tr = sb.trs + px.id; tp.trStride = (size_t)F.width * F.height; tp.n = 0; tp.dens = sb.dens + px.id; tp.dstride = (size_t)F.width * F.height; tp.nd = 0; }
     v3 c = V3(0.f);
     if (F.useReference) {
         float4 v = colorIn[px.id];

@@ -150,7 +150,8 @@ struct alignas(16) TraceRec {
 // Buffers of the record / trace / replay form of a reuse kernel.
 struct SplitBufs {
     TraceRec* recs;   // cap records per pixel, in evaluation order
-    float* trs;       // their transmittances
+    float* trs;       // their transmittances: ray i of pixel p at i * pixels + p
+    float* dens;      // the density lookups of the same evaluations: slot i of pixel p at i * pixels + p
     int32_t* counts;  // records per pixel
     uint32_t* plan;   // per pixel: bit s = candidate s's Talbot evaluations were recorded
     int32_t cap;
