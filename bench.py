@@ -25,8 +25,8 @@ HBM_PEAK_GBS = 8000.0
 # FinalShading: their record, trace and replay kernels), from rocprofv3 --pmc FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00
 # (factors calibrated on a 256 MiB copy in the same process); commands and figures
 # in profiles/SpatialReuse_config1_1080p.md.
-MEASURED_TRAFFIC_BYTES_CONFIG1 = {"SpatialReuse": 33.154e9, "TraceRays": 12.897e9, "FinalShading": 4.480e9,
-                                  "TemporalReuse": 3.597e9, "GenerateFeatures": 0.183e9, "CopyReservoirs": 0.156e9}
+MEASURED_TRAFFIC_BYTES_CONFIG1 = {"SpatialReuse": 32.743e9, "TraceRays": 12.905e9, "FinalShading": 4.322e9,
+                                  "TemporalReuse": 3.363e9, "GenerateFeatures": 0.183e9, "CopyReservoirs": 0.156e9}
 
 
 def stage_bytes_per_pixel(stage, b, k):
