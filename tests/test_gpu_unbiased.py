@@ -8,8 +8,7 @@ import numpy as np
 import pytest
 
 from gpu_common import say
-from test_oracle_unbiased import CASES, compare
-from volumetricrestirrelease_amd import scenes
+from test_oracle_unbiased import CASES, compare, scene
 from volumetricrestirrelease_amd.volumetric_restir import VolumetricReSTIR
 
 pytestmark = pytest.mark.gpu
@@ -17,9 +16,9 @@ pytestmark = pytest.mark.gpu
 W, H = 48, 36
 
 
-def gpu_frames(params, n, reference):
+def gpu_frames(params, n, reference, scene_name="config0"):
     import torch
-    spec = scenes.config0(res=32, width=W, height=H)
+    spec = scene(scene_name, W, H)
     spec.params.update(finalVisibilityTrackingMethod=3, finalLightTrackingMethod=3)
     spec.params.update(params)
     if reference:
@@ -38,11 +37,13 @@ def gpu_frames(params, n, reference):
 @pytest.mark.parametrize("case", list(CASES))
 def test_gpu_pass_expectation_equals_path_tracer(case):
     params, batch = CASES[case]
+    params = dict(params)
+    name = params.pop("scene", "config0")
     n = 4000
     say(f"case: {case} {W}x{H}, {n} pass frames against {n // 4} path-traced frames at 4 spp")
     shared = {k: v for k, v in params.items() if k == "maxBounces"}
-    pt = gpu_frames(shared, n // 4, reference=True)
-    ps = gpu_frames(params, n, reference=False)
+    pt = gpu_frames(shared, n // 4, reference=True, scene_name=name)
+    ps = gpu_frames(params, n, reference=False, scene_name=name)
     z, zi, rel = compare(pt, ps, batch)
     chi2 = float((z ** 2).mean())
     say(f"   image z {zi:.2f} (rel {rel:+.4f}), chi2/dof {chi2:.3f}, max|z| {np.abs(z).max():.2f}")

@@ -7,14 +7,17 @@ What is pinned is the product of all the pieces at once: the candidates' source
 pdfs and RIS weights (VR/ComputeInitialSample.slang:4-395, VR/TraceRays.cs.slang:64-201),
 the Talbot MIS weights and neighbour resampling of spatial reuse
 (VR/SpatialReuse.cs.slang:94-265) and temporal reuse (VR/TemporalReuse.cs.slang:80-377),
-the extra-bounce records and Russian roulette of longer paths, and the
+the extra-bounce records and Russian roulette of longer paths, light
+selection between the point light and the importance-sampled environment map,
+emission of the medium (the path tag of an emissive vertex), and the
 unbiased contribution weight that final shading multiplies F by
 (VR/FinalShading.cs.slang:71-141). A target function that is merely
 different only changes the variance; a wrong weight, pdf, MIS term or index
 changes the mean. Three mistakes planted in a copy of the oracle on the CPU
 (the Talbot sum evaluating the wrong neighbour's ray, an inverted neighbour
 resampling weight, the phase pdf dropped from a candidate's source pdf) raise
-chi^2 per degree of freedom from about 1.05 to 23-190.
+chi^2 per degree of freedom from about 1.05 to 23-190; dropping the 1/dist^2 of
+an emissive vertex raises it to 11 in the emissive case.
 
 Final shading here uses the path tracer's own transmittance estimator (residual
 ratio tracking). The analytic tracker that final shading uses by default walks
@@ -36,8 +39,18 @@ W, H = 16, 12
 THREADS = 8
 
 
-def frames(params, n, reference):
-    spec = scenes.config0(res=32, width=W, height=H)
+def scene(name, width=W, height=H):
+    if name == "emissive":
+        spec = scenes.config_emissive(res=32, width=width, height=height)
+        spec.env_importance_dim = 64
+        spec.env_importance_spp = 16
+        spec.le_scale = 40.0  # emission about a quarter of the radiance (0.7 % at LeScale 1)
+        return spec
+    return scenes.config0(res=32, width=width, height=height)
+
+
+def frames(params, n, reference, scene_name="config0"):
+    spec = scene(scene_name)
     spec.params.update(finalVisibilityTrackingMethod=3, finalLightTrackingMethod=3)
     spec.params.update(params)
     if reference:
@@ -75,16 +88,20 @@ CASES = {
     "spatial_b2": (dict(enableTemporalReuse=0), 1),
     # path length 3 (extra-bounce records, Russian roulette), temporal + 2 spatial rounds
     "temporal_spatial2_b3": (dict(maxBounces=3, spatialReuseRounds=2), 50),
+    # textured sky (importance-sampled environment), emissive volume, point light, b = 3
+    "emissive_env_spatial_b3": (dict(scene="emissive", enableTemporalReuse=0), 1),
 }
 
 
 @pytest.mark.parametrize("case", list(CASES))
 def test_pass_expectation_equals_path_tracer(case):
     params, batch = CASES[case]
+    params = dict(params)
+    name = params.pop("scene", "config0")
     n = 3000
     shared = {k: v for k, v in params.items() if k == "maxBounces"}
-    pt = frames(shared, n // 4, reference=True)
-    ps = frames(params, n, reference=False)
+    pt = frames(shared, n // 4, reference=True, scene_name=name)
+    ps = frames(params, n, reference=False, scene_name=name)
     z, zi, rel = compare(pt, ps, batch)
     chi2 = float((z ** 2).mean())
     print(f"{case}: image z {zi:.2f} (rel {rel:+.4f}), chi2/dof {chi2:.3f}, max|z| {np.abs(z).max():.2f}")
