@@ -17,7 +17,8 @@ changes the mean. Three mistakes planted in a copy of the oracle on the CPU
 (the Talbot sum evaluating the wrong neighbour's ray, an inverted neighbour
 resampling weight, the phase pdf dropped from a candidate's source pdf) raise
 chi^2 per degree of freedom from about 1.05 to 23-190; dropping the 1/dist^2 of
-an emissive vertex raises it to 11 in the emissive case.
+an emissive vertex raises it to 11 in the emissive case; in temporal reuse, a
+Talbot term without its capped M, or a normaliser over uncapped M, to 5e4 and 4e3.
 
 Final shading here uses the path tracer's own transmittance estimator (residual
 ratio tracking). The analytic tracker that final shading uses by default walks
