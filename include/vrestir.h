@@ -232,7 +232,12 @@ enum {
     VRESTIR_STAGE_TEMPORAL_REUSE = 2,     /* VR/TemporalReuse.cs.slang:80-377 */
     VRESTIR_STAGE_SPATIAL_REUSE = 3,      /* VR/SpatialReuse.cs.slang:94-265 (round = argument) */
     VRESTIR_STAGE_COPY_RESERVOIRS = 4,    /* VR/CopyReservoirs.cs.slang:84-95 */
-    VRESTIR_STAGE_FINAL_SHADING = 5       /* VR/FinalShading.cs.slang:71-141 */
+    VRESTIR_STAGE_FINAL_SHADING = 5,      /* VR/FinalShading.cs.slang:71-141 */
+    /* the learned proposal's two dispatches (no counterpart in the reference; see
+     * vrestir_learned_desc): the network's CDF for every camera ray, run before
+     * TraceRays, and the Adam steps on TraceRays' training rays, run after it */
+    VRESTIR_STAGE_LEARNED_PROPOSAL = 6,
+    VRESTIR_STAGE_LEARNED_TRAIN = 7
 };
 /* Buffers in the reference's own identities (VR/VolumetricReSTIR.cpp:419-450).
  * Host exchange format per pixel: features {int32 noReflectiveSurface; float transmittance};
@@ -247,7 +252,13 @@ enum {
     VRESTIR_BUF_EXTRA0 = 5,
     VRESTIR_BUF_EXTRA1 = 6,
     VRESTIR_BUF_TEMPORAL_EXTRA = 7,
-    VRESTIR_BUF_COLOR = 8
+    VRESTIR_BUF_COLOR = 8,
+    /* learned proposal: uint16 cdf[32] per pixel (cdf[0] = 0, cdf[32] = 65536 implicit) */
+    VRESTIR_BUF_LEARNED_CDF = 9,
+    /* learned proposal: float[72] per training ray (VRESTIR_LEARNED_RECORD_FLOATS): raw inputs
+     * (entry point in mip 0's normalised box, direction, segment length / box diagonal),
+     * the summed weight W, then per bin the summed weights and summed weight^2 * g_used */
+    VRESTIR_BUF_TRAIN_RECORDS = 10
 };
 int32_t vrestir_prepare_frame(vrestir_handle* h, int32_t width, int32_t height);
 int32_t vrestir_run_stage(vrestir_handle* h, void* stream, int32_t stage, int32_t round, float* outRGBA);
@@ -279,6 +290,55 @@ enum {
 int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t mode, int32_t mipLevel,
                             int32_t useLinearSampler, float tStepScale, int32_t numSamples, int32_t n,
                             const float* raysDev, const uint32_t* seedsDev, float* outDev);
+
+/* ---- learned free-flight proposal (an addition of this project: no counterpart
+ * in the reference). Per camera ray a 36 -> 64 -> 64 -> 64 -> 32 ReLU network on a
+ * frequency encoding of (entry point, direction, segment length) predicts a 32-bin
+ * pdf g over the analytic tracker's CDF coordinate c = 1 - exp(-tau). The initial
+ * candidates' distances are the tracker's, driven by optical-depth targets
+ * -log(1 - c) with c drawn from g, so their source pdf -- pdfDist of
+ * VR/ComputeInitialSample.slang:86,107 -- becomes g(c) sigma_t Tr (an escape: the
+ * mass of g above 1 - Tr), evaluated exactly from the same 16-bit CDF it samples.
+ * g mixes the network's softmax with the uniform pdf (defensiveFraction), so the
+ * analytic proposal stays in every frame; with judge on, a frame samples with the
+ * network only while the network's estimated second moment of the RIS weights
+ * (importance-reweighted from the training rays) is below the analytic one's.
+ * Training is online: one ray per trainBlockX x trainBlockY block records its
+ * candidates' weights, and hand-written BF16 MFMA kernels (FP32 accumulation)
+ * take Adam steps on the cross-entropy of g against those weights. ---- */
+enum { VRESTIR_LEARNED_PARAMS = 12768, VRESTIR_LEARNED_RECORD_FLOATS = 72, VRESTIR_LEARNED_RAW_FLOATS = 8 };
+typedef struct vrestir_learned_desc {
+    int32_t enable;              /* 0: the reference's analytic proposal only */
+    int32_t train;               /* online training on the pass's own candidates */
+    int32_t trainBlockX, trainBlockY; /* one training ray per block of pixels (16 x 8) */
+    int32_t trainStepsPerFrame;  /* Adam steps per frame, the frame's training rays split evenly (1) */
+    float learningRate;          /* Adam step size (2e-3) */
+    float defensiveFraction;     /* weight of the uniform pdf in g, in [1/64, 1] (0.25) */
+    int32_t judge;               /* 1: the analytic proposal decides whether the network samples (1) */
+    uint64_t seed;               /* initial weights; the output layer starts at zero (g = 1) */
+} vrestir_learned_desc;
+void vrestir_default_learned(vrestir_learned_desc* out);
+/* Sets the options and (re)initialises the network from seed, the Adam state and the judge. */
+int32_t vrestir_set_learned(vrestir_handle* h, const vrestir_learned_desc* desc);
+int32_t vrestir_get_learned(const vrestir_handle* h, vrestir_learned_desc* out);
+/* After the stream drains: [0] loss of the last Adam step, [1] the analytic proposal's and
+ * [2] the network's second moment of the per-ray normalised RIS weights over the last
+ * step's rays, [3] 1 if the network samples the next frame, [4] Adam steps taken. */
+int32_t vrestir_learned_stats(vrestir_handle* h, float stats[5]);
+/* FP32 master weights (VRESTIR_LEARNED_PARAMS floats: W1[36][64], b1, W2[64][64], b2,
+ * W3[64][64], b3, W4[64][32], b4). Setting them resets the Adam state. */
+int32_t vrestir_learned_get_weights(vrestir_handle* h, float* out, int32_t n);
+int32_t vrestir_learned_set_weights(vrestir_handle* h, const float* in, int32_t n);
+/* The network on n raw inputs (device float[8 n]): the 16-bit CDF (device uint16[32 n])
+ * and optionally the logits (device float[32 n] or NULL). Ignores the judge. */
+int32_t vrestir_learned_forward(vrestir_handle* h, void* stream, const float* rawDev, int32_t n, uint16_t* cdfDev,
+                                float* logitsDev);
+/* Sum over n training records (device float[72 n]) of scale x the cross-entropy gradient:
+ * device float[VRESTIR_LEARNED_PARAMS + 3] = gradient, scale x loss, and the two second moments. */
+int32_t vrestir_learned_gradient(vrestir_handle* h, void* stream, const float* recsDev, int32_t n, float scale,
+                                 float* outDev);
+/* One Adam step on the mean cross-entropy of n training records (device float[72 n]). */
+int32_t vrestir_learned_train(vrestir_handle* h, void* stream, const float* recsDev, int32_t n);
 
 /* ---- implementation choices (no counterpart in the reference) ----
  * VRESTIR_TUNE_SPLIT_REUSE (default 2): 1 and 2 run spatial reuse as three

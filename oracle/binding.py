@@ -88,6 +88,19 @@ def lib():
         "orc_dir_to_oct": (None, [c_void_p, c_void_p]),
         "orc_emission": (None, [c_void_p, c_void_p, c_void_p]),
         "orc_blackbody": (None, [c_void_p, c_float, c_void_p]),
+        "orc_set_learned": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_float, c_float, c_int]),
+        "orc_learned_set_weights": (c_int, [c_void_p, c_void_p, c_int]),
+        "orc_learned_get_weights": (c_int, [c_void_p, c_void_p, c_int]),
+        "orc_learned_stats": (None, [c_void_p, c_void_p]),
+        "orc_learned_raw": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p]),
+        "orc_mlp_encode": (None, [c_void_p, c_void_p]),
+        "orc_mlp_forward": (None, [c_void_p, c_void_p, c_int, c_float, c_int, c_void_p, c_void_p]),
+        "orc_mlp_loss": (c_double, [c_void_p, c_void_p, c_int, c_float, c_int]),
+        "orc_mlp_backward": (None, [c_void_p, c_void_p, c_int, c_float, c_float, c_int, c_void_p, c_void_p]),
+        "orc_mlp_adam": (None, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_float]),
+        "orc_mlp_sample": (c_float, [c_void_p, c_float, POINTER(c_int), POINTER(c_float)]),
+        "orc_mlp_mass_above": (c_float, [c_void_p, c_float]),
+        "orc_mlp_train_slot": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -259,6 +272,30 @@ class Oracle:
         self.L.orc_stage_seconds(self.h, out)
         return list(out)
 
+    # ---- learned proposal
+    def set_learned(self, enable=True, train=True, block=(16, 8), steps=1, lr=2e-3, alpha=0.25, judge=True):
+        rc = self.L.orc_set_learned(self.h, int(enable), int(train), block[0], block[1], steps, lr, alpha, int(judge))
+        assert rc == 0, rc
+
+    def set_learned_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert self.L.orc_learned_set_weights(self.h, w.ctypes.data, w.size) == 0
+
+    def learned_weights(self):
+        w = np.zeros(MLP_PARAMS, dtype=np.float32)
+        assert self.L.orc_learned_get_weights(self.h, w.ctypes.data, w.size) == 0
+        return w
+
+    def learned_stats(self):
+        out = np.zeros(5, dtype=np.float32)
+        self.L.orc_learned_stats(self.h, out.ctypes.data)
+        return out
+
+    def learned_raw(self, px, py, W, H):
+        out = np.zeros(MLP_RAW, dtype=np.float32)
+        assert self.L.orc_learned_raw(self.h, px, py, W, H, out.ctypes.data) == 0
+        return out
+
     def grid_info(self, slot):
         dims = (c_int * 3)()
         mx = c_float()
@@ -305,3 +342,86 @@ def ref_xoshiro128ss(seed, n: int):
     s = np.ascontiguousarray(seed, dtype=np.uint32)
     L.ref_xoshiro128ss_seed(s.ctypes.data)
     return np.array([L.ref_xoshiro128ss_next() for _ in range(n)], dtype=np.uint32)
+
+
+# ---- the learned proposal's network (oracle/oracle_mlp.hpp)
+MLP_IN, MLP_HID, MLP_OUT, MLP_RAW, MLP_REC = 36, 64, 32, 8, 72
+MLP_PARAMS = 12768
+MLP_OFFSETS = {"W1": (0, (36, 64)), "b1": (2304, (64,)), "W2": (2368, (64, 64)), "b2": (6464, (64,)),
+               "W3": (6528, (64, 64)), "b3": (10624, (64,)), "W4": (10688, (64, 32)), "b4": (12736, (32,))}
+
+
+def mlp_unpack(w):
+    return {k: np.asarray(w[o:o + int(np.prod(sh))]).reshape(sh) for k, (o, sh) in MLP_OFFSETS.items()}
+
+
+def mlp_encode(raw):
+    raw = np.ascontiguousarray(raw, dtype=np.float32)
+    out = np.zeros(MLP_IN, dtype=np.float32)
+    lib().orc_mlp_encode(raw.ctypes.data, out.ctypes.data)
+    return out
+
+
+def mlp_forward(w, raw, alpha=0.25, emulate=True):
+    """Logits [n, 32] and 16-bit CDF [n, 32] of raw inputs [n, 8]."""
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    raw = np.ascontiguousarray(raw, dtype=np.float32).reshape(-1, MLP_RAW)
+    n = raw.shape[0]
+    z = np.zeros((n, MLP_OUT), dtype=np.float32)
+    cdf = np.zeros((n, MLP_OUT), dtype=np.uint16)
+    lib().orc_mlp_forward(w.ctypes.data, raw.ctypes.data, n, alpha, int(emulate), z.ctypes.data, cdf.ctypes.data)
+    return z, cdf
+
+
+def mlp_loss(w, recs, alpha=0.25, emulate=False):
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    recs = np.ascontiguousarray(recs, dtype=np.float32).reshape(-1, MLP_REC)
+    return lib().orc_mlp_loss(w.ctypes.data, recs.ctypes.data, recs.shape[0], alpha, int(emulate))
+
+
+def mlp_backward(w, recs, alpha=0.25, scale=1.0, emulate=False):
+    """Gradient of scale * sum of record losses, and [scale * loss, s_plain, s_net]."""
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    recs = np.ascontiguousarray(recs, dtype=np.float32).reshape(-1, MLP_REC)
+    g = np.zeros(MLP_PARAMS, dtype=np.float64)
+    st = np.zeros(3, dtype=np.float64)
+    lib().orc_mlp_backward(w.ctypes.data, recs.ctypes.data, recs.shape[0], alpha, scale, int(emulate), g.ctypes.data,
+                           st.ctypes.data)
+    return g, st
+
+
+def mlp_adam(w, m, v, g, t, lr):
+    """One Adam step in place on float32 arrays."""
+    g = np.ascontiguousarray(g, dtype=np.float32)
+    lib().orc_mlp_adam(w.ctypes.data, m.ctypes.data, v.ctypes.data, g.ctypes.data, w.size, t, lr)
+
+
+def mlp_sample(cdf, u):
+    cdf = np.ascontiguousarray(cdf, dtype=np.uint16)
+    b = c_int()
+    g = c_float()
+    c = lib().orc_mlp_sample(cdf.ctypes.data, u, ctypes.byref(b), ctypes.byref(g))
+    return c, b.value, g.value
+
+
+def mlp_mass_above(cdf, x):
+    cdf = np.ascontiguousarray(cdf, dtype=np.uint16)
+    return lib().orc_mlp_mass_above(cdf.ctypes.data, x)
+
+
+def mlp_train_slot(x, y, W, H, bx, by, frame):
+    return lib().orc_mlp_train_slot(x, y, W, H, bx, by, frame)
+
+
+def mlp_random_weights(seed, scale=1.0, last=1.0):
+    """He-uniform weights (output layer times `last`) and small random biases."""
+    rng = np.random.default_rng(seed)
+    w = np.zeros(MLP_PARAMS, dtype=np.float32)
+    for k, (o, sh) in MLP_OFFSETS.items():
+        n = int(np.prod(sh))
+        if k.startswith("W"):
+            lim = scale * np.sqrt(6.0 / sh[0]) * (last if k == "W4" else 1.0)
+            w[o:o + n] = rng.uniform(-lim, lim, size=n)
+        else:
+            w[o:o + n] = rng.uniform(-0.1, 0.1, size=n) * scale
+    return w

@@ -21,6 +21,7 @@
 #include <thread>
 
 #include "oracle_pass.hpp"
+#include "oracle_mlp.hpp"
 
 namespace orc {
 
@@ -74,6 +75,18 @@ struct Oracle {
     bool hasAnimation = false;
     int prevGridOffset = 0;
     bool freezeFrame = false;
+    // learned free-flight proposal: per-pixel 16-bit CDF over c, training records
+    bool learned = false;
+    float learnedAlpha = 0.25f;
+    int trainBX = 16, trainBY = 8;
+    std::vector<uint16_t> learnedCdf;
+    std::vector<float> trainRecs;
+    bool learnedTrain = true, learnedJudge = true, learnedInUse = true;
+    int learnedSteps = 1, adamStep = 0;
+    float learnedLr = 2e-3f;
+    std::vector<float> mlpParams = std::vector<float>(mlp::kParams, 0.f), mlpM = std::vector<float>(mlp::kParams, 0.f),
+                       mlpV = std::vector<float>(mlp::kParams, 0.f);
+    float learnedStats[5] = {0, 0, 0, 1, 0};
 };
 
 static int maxB(const Oracle& O) { return O.P.maxBounces; }
@@ -410,6 +423,24 @@ static void stageFeatures(Oracle& O) {
     });
 }
 
+// Inputs of the learned proposal for a camera ray: entry point into mip 0's
+// index box normalised to [0,1]^3, the world direction, the segment length over
+// the box diagonal (a ray that misses: centre, length 0).
+static void learnedRaw(const Medium& m, const Ray& rw, float* raw) {
+    Ray ri = worldToMedium(m, rw, 0);
+    const GridSlot& g = m.grid[0];
+    float t0 = 0.f, t1 = 0.f;
+    raw[3] = rw.dir.x; raw[4] = rw.dir.y; raw[5] = rw.dir.z; raw[7] = 0.f;
+    if (!intersectVolumeBound(m, ri, t0, t1, 0, false)) {
+        raw[0] = raw[1] = raw[2] = 0.5f;
+        raw[6] = 0.f;
+        return;
+    }
+    f3 e = ri.origin + t0 * ri.dir;
+    raw[0] = e.x / g.bmax.x; raw[1] = e.y / g.bmax.y; raw[2] = e.z / g.bmax.z;
+    raw[6] = (t1 - t0) * length(ri.dir) / length(g.bmax);
+}
+
 // TraceRays (VR/TraceRays.cs.slang:64-201).
 static void stageTrace(Oracle& O) {
     const int b = maxB(O);
@@ -433,17 +464,57 @@ static void stageTrace(Oracle& O) {
         int M = O.P.initialM;
         int rounds = (M + 3) / 4;
         bool noReuse = !O.P.enableSpatialReuse && !O.P.enableTemporalReuse;
+        // Learned proposal (no counterpart in the reference): the distances are the
+        // analytic tracker's, driven by optical-depth targets -log(1 - c) with c
+        // drawn from the pixel's 32-bin CDF, so the source pdf is g(c) times the
+        // tracker's sigma_t Tr (and an escape has the CDF mass above 1 - Tr).
+        const bool learned = O.learned && !noReuse;
+        const uint16_t* cdf = learned ? &O.learnedCdf[(size_t)rid * mlp::kOut] : nullptr;
+        float* rec = nullptr;
+        if (learned) {
+            int slot = mlp::trainSlot(x, y, O.W, O.H, O.trainBX, O.trainBY, O.frameCount);
+            if (slot >= 0) {
+                rec = &O.trainRecs[(size_t)slot * mlp::kRecFloats];
+                for (int k = 0; k < mlp::kRecFloats; k++) rec[k] = 0.f;
+                learnedRaw(O.S.med, ray, rec);
+                if (!(rec[6] > 0.f)) rec = nullptr;  // a ray that misses the volume does not train
+            }
+        }
         for (int r = 0; r < rounds; r++) {
             float h[4] = {0, 0, 0, 0}, pd[4] = {0, 0, 0, 0}, tr[4] = {0, 0, 0, 0};
             int rs = r == rounds - 1 ? (M - 4 * (rounds - 1)) : 4;
-            if (!noReuse)
+            float gc[4] = {1, 1, 1, 1};
+            int bins[4] = {0, 0, 0, 0};
+            if (learned) {
+                float targets[4] = {0, 0, 0, 0};
+                for (int s = 0; s < rs; s++) {
+                    float c = mlp::sampleC(cdf, next1D(sg), bins[s], gc[s]);
+                    targets[s] = -std::log(1 - c);
+                }
+                sampleAnalyticTargets(O.S.med, ray, sg, O.initialOpt.visibilityUseLinearSampler,
+                                      O.initialOpt.visibilityMipLevel, targets, rs, h, pd, tr);
+            } else if (!noReuse)
                 sampleAnalytic(O.S.med, ray, sg, O.initialOpt.visibilityUseLinearSampler, h,
                                O.initialOpt.visibilityMipLevel, pd, tr, rs);
             for (int s = 0; s < rs; s++) {
                 f3 ex[8] = {};
-                Reservoir o = computeInitialSample(O, ray, h[s], pd[s], tr[s], b, sg, O.initialOpt,
+                // ratio of the learned source pdf to the analytic one
+                float ratio = 1.f;
+                if (learned) {
+                    if (h[s] == kRayTMax) ratio = tr[s] > 0.f ? mlp::massAbove(cdf, 1.f - tr[s]) / tr[s] : gc[s];
+                    else ratio = gc[s];
+                }
+                Reservoir o = computeInitialSample(O, ray, h[s], pd[s] * ratio, tr[s], b, sg, O.initialOpt,
                                                    O.P.initialUseCoarserGridForIndirectBounce != 0,
                                                    O.P.initialUseRussianRoulette != 0, noReuse, ex);
+                if (rec) {
+                    // the sample's weight against the analytic proposal over g(c): its
+                    // contribution to the c-space target density
+                    float wt = o.runningSum * (ratio / gc[s]);
+                    rec[7] += wt;
+                    rec[mlp::kRecHist + bins[s]] += wt;
+                    rec[mlp::kRecSq + bins[s]] += wt * wt * gc[s];
+                }
                 bool sel = resampleStep(o, fin, sg);
                 if (sel && b > 1) {
                     int mi = decodeMaxIndirect(fin.sampledPixel);
@@ -817,6 +888,13 @@ static void prepareFrame(Oracle& O, int W, int H) {
         O.output.assign(n, {0, 0, 0, 0});
         O.mvec.assign(n, {0, 0});
         O.temporalAccumulated = false;
+        O.learnedCdf.assign(n * mlp::kOut, 0);
+        for (size_t i = 0; i < n; i++)
+            for (int j = 0; j < mlp::kOut; j++) O.learnedCdf[i * mlp::kOut + j] = (uint16_t)(2048 * j);
+    }
+    {
+        size_t nr = (size_t)mlp::trainSlots(W, H, O.trainBX, O.trainBY) * mlp::kRecFloats;
+        if (O.trainRecs.size() != nr) O.trainRecs.assign(nr, 0.f);
     }
     O.rowBegin = std::max(0, std::min(O.rowBegin, H));
     O.rowEnd = O.rowEnd <= 0 ? H : std::min(O.rowEnd, H);
@@ -843,6 +921,55 @@ static void prepareFrame(Oracle& O, int W, int H) {
     // EnvMap data / prevData of this frame (SC/Scene.cpp:761-775)
     O.frameEnv[0] = O.envCur;
     O.frameEnv[1] = O.haveEnvLast ? O.envLast : O.envCur;
+}
+
+// The learned proposal's CDF for every camera ray (stage 6): the network in its
+// bf16-operand form, or the uniform CDF while the judge keeps it out.
+static void stageLearnedProposal(Oracle& O) {
+    forRows(O, [&](int x, int y) {
+        uint16_t* cdf = &O.learnedCdf[((size_t)y * O.W + x) * mlp::kOut];
+        if (!O.learnedInUse) {
+            for (int j = 0; j < mlp::kOut; j++) cdf[j] = (uint16_t)(2048 * j);
+            return;
+        }
+        Ray ray{O.S.cam.pos, rayDir(O.S.cam, x, y, O.W, O.H), 0, kRayTMax};
+        float raw[mlp::kRaw];
+        learnedRaw(O.S.med, ray, raw);
+        mlp::Acts<float> a;
+        mlp::forward<float>(O.mlpParams.data(), raw, O.learnedAlpha, true, a);
+        mlp::quantize(a.p, cdf);
+    });
+}
+
+// Adam steps on this frame's training rays (stage 7): the records split into
+// learnedSteps consecutive minibatches, each step on the mean cross-entropy of
+// its minibatch; the judge compares the two second moments of the last step.
+// The records are consumed (their weight cleared).
+static void stageLearnedTrain(Oracle& O) {
+    if (!O.learnedTrain) return;
+    const int n = mlp::trainSlots(O.W, O.H, O.trainBX, O.trainBY);
+    const int K = std::max(1, O.learnedSteps);
+    for (int k = 0; k < K; k++) {
+        const int b0 = (int)((int64_t)n * k / K), b1 = (int)((int64_t)n * (k + 1) / K);
+        if (b1 <= b0) continue;
+        std::vector<double> g(mlp::kParams, 0.0);
+        double st[3] = {0, 0, 0};
+        const float scale = 1.f / (float)(b1 - b0);
+        for (int r = b0; r < b1; r++)
+            mlp::backward<float>(O.mlpParams.data(), &O.trainRecs[(size_t)r * mlp::kRecFloats], O.learnedAlpha, scale,
+                                 true, g.data(), st);
+        std::vector<float> gf(g.begin(), g.end());
+        O.adamStep++;
+        mlp::adam(O.mlpParams.data(), O.mlpM.data(), O.mlpV.data(), gf.data(), mlp::kParams, O.adamStep, O.learnedLr,
+                  mlp::kAdamB1, mlp::kAdamB2, mlp::kAdamEps);
+        O.learnedStats[0] = (float)st[0];
+        O.learnedStats[1] = (float)st[1];
+        O.learnedStats[2] = (float)st[2];
+    }
+    O.learnedInUse = !O.learnedJudge || O.learnedStats[2] < O.learnedStats[1];
+    O.learnedStats[3] = O.learnedInUse ? 1.f : 0.f;
+    O.learnedStats[4] = (float)O.adamStep;
+    for (int r = 0; r < n; r++) O.trainRecs[(size_t)r * mlp::kRecFloats + 7] = 0.f;
 }
 
 static void endFrame(Oracle& O) {
@@ -881,7 +1008,12 @@ static void runStage(Oracle& O, int stage, int round) {
             break;
         case VRESTIR_STAGE_COPY_RESERVOIRS: timed(O, 4, [&] { stageCopy(O); }); break;
         case VRESTIR_STAGE_FINAL_SHADING: timed(O, 5, [&] { stageFinal(O); }); break;
+        case VRESTIR_STAGE_LEARNED_PROPOSAL: timed(O, 1, [&] { stageLearnedProposal(O); }); break;
+        case VRESTIR_STAGE_LEARNED_TRAIN: timed(O, 1, [&] { stageLearnedTrain(O); }); break;
     }
+}
+static bool learnedActive(const Oracle& O) {
+    return O.learned && !O.P.useReference && (O.P.enableSpatialReuse || O.P.enableTemporalReuse);
 }
 
 // VolumetricReSTIR::execute (VR/VolumetricReSTIR.cpp:303-772), dispatch order.
@@ -902,7 +1034,9 @@ static void executeFrame(Oracle& O, int W, int H) {
         return;
     }
     runStage(O, VRESTIR_STAGE_GENERATE_FEATURES, 0);
+    if (learnedActive(O)) runStage(O, VRESTIR_STAGE_LEARNED_PROPOSAL, 0);
     runStage(O, VRESTIR_STAGE_TRACE_RAYS, 0);
+    if (learnedActive(O)) runStage(O, VRESTIR_STAGE_LEARNED_TRAIN, 0);
     if (!p.useReference && p.enableTemporalReuse) runStage(O, VRESTIR_STAGE_TEMPORAL_REUSE, 0);
     if (!p.useReference && p.enableSpatialReuse)
         for (int r = 0; r < p.spatialReuseRounds; r++) runStage(O, VRESTIR_STAGE_SPATIAL_REUSE, r);
@@ -1265,6 +1399,8 @@ int orc_end_frame(void* h) { endFrame(*(Oracle*)h); return VRESTIR_OK; }
 
 static void* bufPtr(Oracle* O, int id, size_t& bytes) {
     switch (id) {
+        case VRESTIR_BUF_LEARNED_CDF: bytes = O->learnedCdf.size() * 2; return O->learnedCdf.data();
+        case VRESTIR_BUF_TRAIN_RECORDS: bytes = O->trainRecs.size() * 4; return O->trainRecs.data();
         case VRESTIR_BUF_FEATURES: bytes = O->features.size() * 8; return O->features.data();
         case VRESTIR_BUF_TEMPORAL_FEATURES: bytes = O->temporalFeatures.size() * 8; return O->temporalFeatures.data();
         case VRESTIR_BUF_RESERVOIRS0: bytes = O->res[0].size() * 36; return O->res[0].data();
@@ -1526,3 +1662,88 @@ float orc_voxel(void* h, int slot, int x, int y, int z) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- learned proposal
+int orc_set_learned(void* h, int enable, int train, int bx, int by, int steps, float lr, float alpha, int judge) {
+    Oracle& O = *(Oracle*)h;
+    if (bx < 1 || by < 1 || steps < 1 || !(alpha >= 1.f / 64.f && alpha <= 1.f)) return VRESTIR_EINVAL;
+    O.learned = enable != 0;
+    O.learnedTrain = train != 0;
+    O.trainBX = bx; O.trainBY = by;
+    O.learnedSteps = steps;
+    O.learnedLr = lr;
+    O.learnedAlpha = alpha;
+    O.learnedJudge = judge != 0;
+    O.learnedInUse = true;
+    O.adamStep = 0;
+    std::fill(O.mlpM.begin(), O.mlpM.end(), 0.f);
+    std::fill(O.mlpV.begin(), O.mlpV.end(), 0.f);
+    O.trainRecs.clear();
+    return VRESTIR_OK;
+}
+int orc_learned_set_weights(void* h, const float* w, int n) {
+    Oracle& O = *(Oracle*)h;
+    if (n != mlp::kParams) return VRESTIR_EINVAL;
+    std::memcpy(O.mlpParams.data(), w, sizeof(float) * n);
+    std::fill(O.mlpM.begin(), O.mlpM.end(), 0.f);
+    std::fill(O.mlpV.begin(), O.mlpV.end(), 0.f);
+    O.adamStep = 0;
+    return VRESTIR_OK;
+}
+int orc_learned_get_weights(void* h, float* w, int n) {
+    Oracle& O = *(Oracle*)h;
+    if (n != mlp::kParams) return VRESTIR_EINVAL;
+    std::memcpy(w, O.mlpParams.data(), sizeof(float) * n);
+    return VRESTIR_OK;
+}
+void orc_learned_stats(void* h, float out[5]) { std::memcpy(out, ((Oracle*)h)->learnedStats, sizeof(float) * 5); }
+int orc_learned_raw(void* h, int px, int py, int W, int H, float* raw) {
+    Oracle& O = *(Oracle*)h;
+    Ray ray{O.S.cam.pos, rayDir(O.S.cam, px, py, W, H), 0, kRayTMax};
+    learnedRaw(O.S.med, ray, raw);
+    return VRESTIR_OK;
+}
+void orc_mlp_encode(const float* raw, float* out) { mlp::encode(raw, out); }
+// emulate != 0: bf16 operands as the HIP kernels; 0: exact (double) network.
+void orc_mlp_forward(const float* prm, const float* raw, int n, float alpha, int emulate, float* z, uint16_t* cdf) {
+    for (int i = 0; i < n; i++) {
+        float p[mlp::kOut];
+        if (emulate) {
+            mlp::Acts<float> a;
+            mlp::forward<float>(prm, raw + (size_t)i * mlp::kRaw, alpha, true, a);
+            for (int j = 0; j < mlp::kOut; j++) { p[j] = a.p[j]; if (z) z[(size_t)i * mlp::kOut + j] = a.z[j]; }
+        } else {
+            mlp::Acts<double> a;
+            mlp::forward<double>(prm, raw + (size_t)i * mlp::kRaw, alpha, false, a);
+            for (int j = 0; j < mlp::kOut; j++) {
+                p[j] = (float)a.p[j];
+                if (z) z[(size_t)i * mlp::kOut + j] = (float)a.z[j];
+            }
+        }
+        if (cdf) mlp::quantize(p, cdf + (size_t)i * mlp::kOut);
+    }
+}
+double orc_mlp_loss(const float* prm, const float* recs, int n, float alpha, int emulate) {
+    double l = 0.0;
+    for (int i = 0; i < n; i++)
+        l += emulate ? (double)mlp::recordLoss<float>(prm, recs + (size_t)i * mlp::kRecFloats, alpha, true)
+                     : mlp::recordLoss<double>(prm, recs + (size_t)i * mlp::kRecFloats, alpha, false);
+    return l;
+}
+void orc_mlp_backward(const float* prm, const float* recs, int n, float alpha, float scale, int emulate, double* grad,
+                      double* stats) {
+    for (int i = 0; i < mlp::kParams; i++) grad[i] = 0.0;
+    stats[0] = stats[1] = stats[2] = 0.0;
+    for (int i = 0; i < n; i++) {
+        if (emulate) mlp::backward<float>(prm, recs + (size_t)i * mlp::kRecFloats, alpha, scale, true, grad, stats);
+        else mlp::backward<double>(prm, recs + (size_t)i * mlp::kRecFloats, alpha, scale, false, grad, stats);
+    }
+}
+void orc_mlp_adam(float* prm, float* m, float* v, const float* g, int n, int t, float lr) {
+    mlp::adam(prm, m, v, g, n, t, lr, mlp::kAdamB1, mlp::kAdamB2, mlp::kAdamEps);
+}
+float orc_mlp_sample(const uint16_t* cdf, float u, int* bin, float* g) { return mlp::sampleC(cdf, u, *bin, *g); }
+float orc_mlp_mass_above(const uint16_t* cdf, float x) { return mlp::massAbove(cdf, x); }
+int orc_mlp_train_slot(int x, int y, int W, int H, int bx, int by, int frame) {
+    return mlp::trainSlot(x, y, W, H, bx, by, frame);
+}

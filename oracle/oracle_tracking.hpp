@@ -320,8 +320,11 @@ struct TrRayMarch {
 };
 
 // SampleMediumAnalyticAdapterGVDB (VR/VolumeTrackingAdapterGVDB.slang:212-436).
+// targets: the point-sampled form also takes pre-drawn optical-depth targets in
+// outTr (as the trilinear form does) instead of a fresh uniform per cell; the
+// two draw the same distance distribution (the learned proposal's sampler).
 struct SampleAnalytic {
-    const Medium* m; bool linear; int numSamples;
+    const Medium* m; bool linear; int numSamples; bool targets = false;
     float hit[4], outTr[4], pdf[4];
     float tau = 0.f; bool init = false; float tNear = 0, tFar = 0; Ray ray;
     void setRayInfo(float a, float b, const Ray& r) { tNear = a; tFar = b; ray = r; }
@@ -352,6 +355,20 @@ struct SampleAnalytic {
                             hit[s] = t + tsol;
                             outTr[s] = std::exp(-outTr[s]);
                             pdf[s] = polySigma(tsol, c) * outTr[s];
+                            finished++;
+                        }
+                    } else finished++;
+                }
+            } else if (targets) {
+                float dens = densityInAtlas(*m, f3of(dl.p) + o + 0.5f, slot, false);
+                float st = dens * m->sigma_t;
+                dTau = maxDeltaT * st;
+                for (int s = 0; s < numSamples; s++) {
+                    if (hit[s] == -1) {
+                        if (dTau > 0.f && tau + dTau >= outTr[s]) {
+                            hit[s] = t + (outTr[s] - tau) / st;
+                            outTr[s] = std::exp(-outTr[s]);
+                            pdf[s] = st * outTr[s];
                             finished++;
                         }
                     } else finished++;
@@ -539,6 +556,15 @@ static inline void sampleAnalytic(const Medium& m, const Ray& r, SampleGen& sg, 
     SampleAnalytic a; a.m = &m; a.linear = linear; a.numSamples = numSamples;
     if (linear)
         for (int i = 0; i < numSamples; i++) a.outTr[i] = -std::log(1 - next1D(sg));
+    trackGVDB(m, r, slot, sg, a, linear);
+    for (int i = 0; i < 4; i++) { hit[i] = a.hit[i]; tr[i] = a.outTr[i]; pdf[i] = a.pdf[i]; }
+}
+// The same sampler with the optical-depth targets given (point or trilinear form).
+static inline void sampleAnalyticTargets(const Medium& m, const Ray& r, SampleGen& sg, bool linear, int slot,
+                                         const float targets[4], int numSamples, float hit[4], float pdf[4],
+                                         float tr[4]) {
+    SampleAnalytic a; a.m = &m; a.linear = linear; a.numSamples = numSamples; a.targets = !linear;
+    for (int i = 0; i < numSamples; i++) a.outTr[i] = targets[i];
     trackGVDB(m, r, slot, sg, a, linear);
     for (int i = 0; i < 4; i++) { hit[i] = a.hit[i]; tr[i] = a.outTr[i]; pdf[i] = a.pdf[i]; }
 }

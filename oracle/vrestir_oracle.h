@@ -70,6 +70,21 @@ int orc_wrs(int n, const float* w, const float* M, float cap, uint32_t px, uint3
 int orc_camera_ray(void* h, int px, int py, int W, int H, float out[3]);
 int orc_reproject(void* h, const float p[3], int W, int H, float out[2]);
 int orc_trace_bricks(void* h, int slot, const float ray[8], int vertexCenter, int maxOut, float* out);
+/* learned proposal */
+int orc_set_learned(void* h, int enable, int train, int bx, int by, int steps, float lr, float alpha, int judge);
+int orc_learned_set_weights(void* h, const float* w, int n);
+int orc_learned_get_weights(void* h, float* w, int n);
+void orc_learned_stats(void* h, float out[5]);
+int orc_learned_raw(void* h, int px, int py, int W, int H, float* raw);
+void orc_mlp_encode(const float* raw, float* out);
+void orc_mlp_forward(const float* prm, const float* raw, int n, float alpha, int emulate, float* z, uint16_t* cdf);
+double orc_mlp_loss(const float* prm, const float* recs, int n, float alpha, int emulate);
+void orc_mlp_backward(const float* prm, const float* recs, int n, float alpha, float scale, int emulate, double* grad,
+                      double* stats);
+void orc_mlp_adam(float* prm, float* m, float* v, const float* g, int n, int t, float lr);
+float orc_mlp_sample(const uint16_t* cdf, float u, int* bin, float* g);
+float orc_mlp_mass_above(const uint16_t* cdf, float x);
+int orc_mlp_train_slot(int x, int y, int W, int H, int bx, int by, int frame);
 #ifdef __cplusplus
 }
 #endif

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 W, H = 48, 36
 
 
-def gpu_frames(params, n, reference, scene_name="config0"):
+def gpu_frames(params, n, reference, scene_name="config0", setup=None):
     import torch
     spec = scene(scene_name, W, H)
     spec.params.update(finalVisibilityTrackingMethod=3, finalLightTrackingMethod=3)
@@ -25,6 +25,8 @@ def gpu_frames(params, n, reference, scene_name="config0"):
         spec.params.update(useReference=1, baselineSamplePerPixel=4)
     r = VolumetricReSTIR()
     r.setScene(spec)
+    if setup is not None and not reference:
+        setup(r)
     img = torch.zeros((H, W, 4), device="cuda")
     out = torch.empty((n, H, W, 3), dtype=torch.float64, device="cuda")
     for k in range(n):

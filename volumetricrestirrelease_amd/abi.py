@@ -32,8 +32,9 @@ TUNE_SPLIT_REUSE = 1
 
 STAGE_GENERATE_FEATURES, STAGE_TRACE_RAYS, STAGE_TEMPORAL_REUSE, STAGE_SPATIAL_REUSE, \
     STAGE_COPY_RESERVOIRS, STAGE_FINAL_SHADING = range(6)
+STAGE_LEARNED_PROPOSAL, STAGE_LEARNED_TRAIN = 6, 7
 BUF_FEATURES, BUF_TEMPORAL_FEATURES, BUF_RESERVOIRS0, BUF_RESERVOIRS1, BUF_TEMPORAL_RESERVOIRS, \
-    BUF_EXTRA0, BUF_EXTRA1, BUF_TEMPORAL_EXTRA, BUF_COLOR = range(9)
+    BUF_EXTRA0, BUF_EXTRA1, BUF_TEMPORAL_EXTRA, BUF_COLOR, BUF_LEARNED_CDF, BUF_TRAIN_RECORDS = range(11)
 
 TRACK_TR_ANALYTIC, TRACK_TR_RAY_MARCHING, TRACK_TR_RATIO, TRACK_TR_RESIDUAL_RATIO, \
     TRACK_TR_ANALOG_RESIDUAL_RATIO, TRACK_SAMPLE_ANALYTIC, TRACK_SAMPLE_DECOMPOSITION, \
@@ -125,6 +126,23 @@ class Camera(ctypes.Structure):
                 ("fovY", c_float), ("nearZ", c_float), ("farZ", c_float)]
 
 
+class LearnedDesc(ctypes.Structure):
+    """vrestir_learned_desc: the learned free-flight proposal's options."""
+    _fields_ = [("enable", ctypes.c_int32), ("train", ctypes.c_int32), ("trainBlockX", ctypes.c_int32),
+                ("trainBlockY", ctypes.c_int32), ("trainStepsPerFrame", ctypes.c_int32),
+                ("learningRate", ctypes.c_float), ("defensiveFraction", ctypes.c_float), ("judge", ctypes.c_int32),
+                ("seed", ctypes.c_uint64)]
+
+
+LEARNED_PARAMS, LEARNED_RECORD_FLOATS, LEARNED_RAW_FLOATS = 12768, 72, 8
+
+
+def default_learned() -> "LearnedDesc":
+    d = LearnedDesc()
+    load_library().vrestir_default_learned(ctypes.byref(d))
+    return d
+
+
 class LibraryMissing(RuntimeError):
     pass
 
@@ -187,6 +205,15 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "vrestir_accumulate": (i32, [vp, vp, vp, i32, i32, i32]),
         "vrestir_accumulated": (i32, [vp, vp, vp, POINTER(c_int64)]),
         "vrestir_relmse": (i32, [vp, vp, vp, vp, i32, i32, POINTER(ctypes.c_double)]),
+        "vrestir_default_learned": (None, [POINTER(LearnedDesc)]),
+        "vrestir_set_learned": (i32, [vp, POINTER(LearnedDesc)]),
+        "vrestir_get_learned": (i32, [vp, POINTER(LearnedDesc)]),
+        "vrestir_learned_stats": (i32, [vp, POINTER(f32)]),
+        "vrestir_learned_get_weights": (i32, [vp, POINTER(f32), i32]),
+        "vrestir_learned_set_weights": (i32, [vp, POINTER(f32), i32]),
+        "vrestir_learned_forward": (i32, [vp, vp, vp, i32, vp, vp]),
+        "vrestir_learned_gradient": (i32, [vp, vp, vp, i32, f32, vp]),
+        "vrestir_learned_train": (i32, [vp, vp, vp, i32]),
         "vrestir_synth_fbm": (i32, [c_uint64, i32, i32, i32, i32, f32, f32, f32, i32, f32, f32, POINTER(f32)]),
         "vrestir_orbit_camera": (i32, [POINTER(Camera), POINTER(f32), i32, f32, POINTER(Camera)]),
         "vrestir_count_bricks": (i32, [POINTER(f32), i32, i32, i32, POINTER(c_int64)]),
@@ -211,6 +238,9 @@ EXPORTED_SYMBOLS = [
     "vrestir_get_params", "vrestir_reset_history", "vrestir_execute", "vrestir_frame_count",
     "vrestir_prepare_frame", "vrestir_run_stage", "vrestir_end_frame", "vrestir_read_buffer",
     "vrestir_write_buffer", "vrestir_track_batch", "vrestir_set_tuning", "vrestir_accumulate", "vrestir_accumulated", "vrestir_relmse",
+    "vrestir_default_learned", "vrestir_set_learned", "vrestir_get_learned", "vrestir_learned_stats",
+    "vrestir_learned_get_weights", "vrestir_learned_set_weights", "vrestir_learned_forward",
+    "vrestir_learned_gradient", "vrestir_learned_train",
     "vrestir_synth_fbm", "vrestir_orbit_camera",
     "vrestir_count_bricks", "vrestir_write_vbx", "vrestir_read_vbx",
 ]

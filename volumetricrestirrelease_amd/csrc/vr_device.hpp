@@ -486,7 +486,11 @@ struct TrMarch {
 };
 
 // Up to 4 free-flight samples in one traversal (VR/VolumeTrackingAdapterGVDB.slang:212-436).
-struct SampleAnalytic {
+// TARGETS: the point-sampled form also takes pre-drawn optical-depth targets in
+// tr (as the trilinear form does) instead of a fresh uniform per cell; both draw
+// the same distance distribution. The learned proposal drives it this way.
+template <bool TARGETS>
+struct SampleAnalyticT {
     const Scene* S; const Grid* G; bool linear; int numSamples;
     float hit[4], tr[4], pdf[4];
     float tau = 0.f; bool init = false; float tNear = 0, tFar = 0; Ray ray;
@@ -524,6 +528,21 @@ struct SampleAnalytic {
                             hit[s] = t + tsol;
                             tr[s] = expf(-tr[s]);
                             pdf[s] = polySigma(tsol, c) * tr[s];
+                            finished++;
+                        }
+                    } else finished++;
+                }
+            } else if (TARGETS) {
+                float st = voxelDensity(*S, *G, bc, dl.p) * S->sigma_t;
+                dTau = maxDeltaT * st;
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    if (s >= numSamples) break;
+                    if (hit[s] == -1) {
+                        if (dTau > 0.f && tau + dTau >= tr[s]) {
+                            hit[s] = t + (tr[s] - tau) / st;
+                            tr[s] = expf(-tr[s]);
+                            pdf[s] = st * tr[s];
                             finished++;
                         }
                     } else finished++;
@@ -713,6 +732,18 @@ VR_DEV float trResidual(const Scene& S, const Ray& r, int slot, Rng& rng, bool a
     ResidualRatio a; a.S = &S; a.G = &S.grid[slot]; a.global = global; a.analog = analog || global;
     traverse(S, r, slot, rng, a, false);
     return a.Tr;
+}
+using SampleAnalytic = SampleAnalyticT<false>;
+// The same with the optical-depth targets given (point or trilinear form).
+VR_DEV void sampleAnalyticTargets(const Scene& S, const Ray& r, Rng& rng, bool linear, int slot, int ns,
+                                  const float targets[4], float hit[4], float pdf[4], float tr[4]) {
+    SampleAnalyticT<true> a; a.S = &S; a.G = &S.grid[slot]; a.linear = linear; a.numSamples = ns;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        if (i < ns) a.tr[i] = targets[i];
+    traverse(S, r, slot, rng, a, linear);
+#pragma unroll
+    for (int i = 0; i < 4; i++) { hit[i] = a.hit[i]; pdf[i] = a.pdf[i]; tr[i] = a.tr[i]; }
 }
 VR_DEV void sampleAnalytic(const Scene& S, const Ray& r, Rng& rng, bool linear, int slot, int ns, float hit[4],
                            float pdf[4], float tr[4]) {

@@ -84,6 +84,23 @@ struct vrestir_handle {
     int curFrame = 0;
     bool hasAnimation = false;
     bool freezeFrame = false;
+    // learned free-flight proposal: FP32 master weights and Adam moments, their
+    // bf16 MFMA fragments, gradient partials, statistics and the judge's flag,
+    // the per-pixel CDF and the training records
+    vrestir_learned_desc L{};
+    float* lPrm = nullptr;
+    float* lM = nullptr;
+    float* lV = nullptr;
+    uint4* lFrags = nullptr;
+    float* lPartial = nullptr;
+    int lPartialWaves = 0;
+    float* lStats = nullptr;
+    int* lInUse = nullptr;
+    uint16_t* lCdf = nullptr;
+    size_t lCdfPixels = 0;
+    float* lRecs = nullptr;
+    int lRecSlots = 0;
+    int lStep = 0;
 };
 
 static int fail(vrestir_handle* h, int code, const std::string& msg) {
@@ -102,6 +119,12 @@ template <class T>
 static void dfree(T*& p) {
     if (p) (void)hipFree((void*)p);
     p = nullptr;
+}
+
+static void freeLearned(vrestir_handle* h) {
+    dfree(h->lPrm); dfree(h->lM); dfree(h->lV); dfree(h->lFrags); dfree(h->lPartial); dfree(h->lStats);
+    dfree(h->lInUse); dfree(h->lCdf); dfree(h->lRecs);
+    h->lPartialWaves = 0; h->lCdfPixels = 0; h->lRecSlots = 0;
 }
 
 static void freeSplit(vrestir_handle* h) {
@@ -265,6 +288,7 @@ extern "C" int32_t vrestir_destroy(vrestir_handle* h) {
     dfree(h->relPartial);
     dfree(h->relOut);
     freeSplit(h);
+    freeLearned(h);
     delete h;
     return VRESTIR_OK;
 }
@@ -728,6 +752,218 @@ static int allocFrameBuffers(vrestir_handle* h, int W, int H, int B) {
     return VRESTIR_OK;
 }
 
+// ------------------------------------------------------------------ learned free-flight proposal
+// (an addition of this project: no counterpart in the reference)
+static bool learnedActive(const vrestir_handle* h) {
+    return h->L.enable && h->lPrm && !h->P.useReference && (h->P.enableSpatialReuse || h->P.enableTemporalReuse);
+}
+static int learnedSlots(int W, int H, const vrestir_learned_desc& L) {
+    return ((W + L.trainBlockX - 1) / L.trainBlockX) * ((H + L.trainBlockY - 1) / L.trainBlockY);
+}
+static int learnedFrameBuffers(vrestir_handle* h) {
+    const size_t n = (size_t)h->W * h->H;
+    if (h->lCdfPixels != n) {
+        dfree(h->lCdf);
+        HIPCHK(h, hipMalloc((void**)&h->lCdf, n * 64));
+        HIPCHK(h, hipMemset(h->lCdf, 0, n * 64));
+        h->lCdfPixels = n;
+    }
+    const int slots = learnedSlots(h->W, h->H, h->L);
+    if (h->lRecSlots != slots) {
+        dfree(h->lRecs);
+        HIPCHK(h, hipMalloc((void**)&h->lRecs, (size_t)slots * kLearnedRecFloats * 4));
+        HIPCHK(h, hipMemset(h->lRecs, 0, (size_t)slots * kLearnedRecFloats * 4));
+        h->lRecSlots = slots;
+    }
+    return VRESTIR_OK;
+}
+static int learnedPartials(vrestir_handle* h, int records) {
+    const int waves = (records + 31) / 32;
+    if (waves > h->lPartialWaves) {
+        dfree(h->lPartial);
+        HIPCHK(h, hipMalloc((void**)&h->lPartial, (size_t)waves * kLearnedPartialStride * 4));
+        h->lPartialWaves = waves;
+    }
+    return VRESTIR_OK;
+}
+// One Adam step on the mean cross-entropy of n records (device memory).
+static int learnedStep(vrestir_handle* h, hipStream_t st, const float* recs, int n) {
+    if (n <= 0) return VRESTIR_OK;
+    int rc = learnedPartials(h, n);
+    if (rc) return rc;
+    const vrestir_learned_desc& L = h->L;
+    HIPCHK(h, launchLearnedGrad(st, recs, n, L.defensiveFraction, 1.f / (float)n, h->lFrags, h->lPrm, h->lPartial));
+    h->lStep++;
+    const float c1 = (float)(1.0 - std::pow(0.9, (double)h->lStep));
+    const float c2 = (float)(1.0 - std::pow(0.999, (double)h->lStep));
+    HIPCHK(h, launchLearnedReduce(st, h->lPartial, (n + 31) / 32, nullptr, true, h->lPrm, h->lM, h->lV, L.learningRate,
+                                  c1, c2, h->lStats, h->lInUse, L.judge != 0, h->lStep));
+    HIPCHK(h, launchLearnedPack(st, h->lPrm, h->lFrags));
+    return VRESTIR_OK;
+}
+// Stage: the network's CDF for every camera ray of the frame.
+static int learnedProposal(vrestir_handle* h, hipStream_t st) {
+    if (!learnedActive(h)) return fail(h, VRESTIR_ESTATE, "the learned proposal is not enabled for these options");
+    int rc = learnedFrameBuffers(h);
+    if (rc) return rc;
+    const int n = h->W * h->H;
+    HIPCHK(h, launchLearnedInfer(st, h->sceneDev, h->frame, nullptr, n, h->lFrags, h->lPrm, h->lInUse,
+                                 h->L.defensiveFraction, h->lCdf, nullptr));
+    return VRESTIR_OK;
+}
+// Stage: Adam steps on this frame's training rays, split into trainStepsPerFrame minibatches.
+static int learnedTrainFrame(vrestir_handle* h, hipStream_t st) {
+    if (!learnedActive(h)) return fail(h, VRESTIR_ESTATE, "the learned proposal is not enabled for these options");
+    if (!h->L.train) return VRESTIR_OK;
+    int rc0 = learnedFrameBuffers(h);
+    if (rc0) return rc0;
+    const int n = h->lRecSlots, K = std::max(1, h->L.trainStepsPerFrame);
+    for (int k = 0; k < K; k++) {
+        const int b0 = (int)((int64_t)n * k / K), b1 = (int)((int64_t)n * (k + 1) / K);
+        int rc = learnedStep(h, st, h->lRecs + (size_t)b0 * kLearnedRecFloats, b1 - b0);
+        if (rc) return rc;
+    }
+    return VRESTIR_OK;
+}
+// He-uniform hidden layers from SplitMix64(seed), zero output layer and biases:
+// the untrained network is the uniform pdf, i.e. the analytic proposal.
+static void learnedInitWeights(uint64_t seed, std::vector<float>& w) {
+    w.assign(kLearnedParams, 0.f);
+    uint64_t st = seed;
+    auto next = [&]() {
+        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    const int sizes[3][3] = {{0, 36, 64}, {2368, 64, 64}, {6528, 64, 64}};
+    for (const auto& L : sizes) {
+        const double lim = std::sqrt(6.0 / (double)L[1]);
+        for (int i = 0; i < L[1] * L[2]; i++) w[L[0] + i] = (float)((2.0 * next() - 1.0) * lim);
+    }
+}
+static int learnedUpload(vrestir_handle* h, const float* w) {
+    HIPCHK(h, hipMemcpy(h->lPrm, w, kLearnedParams * 4, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemset(h->lM, 0, kLearnedParams * 4));
+    HIPCHK(h, hipMemset(h->lV, 0, kLearnedParams * 4));
+    const float st[5] = {0.f, 0.f, 0.f, 1.f, 0.f};
+    const int one = 1;
+    HIPCHK(h, hipMemcpy(h->lStats, st, sizeof(st), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->lInUse, &one, sizeof(int), hipMemcpyHostToDevice));
+    h->lStep = 0;
+    HIPCHK(h, launchLearnedPack(nullptr, h->lPrm, h->lFrags));
+    HIPCHK(h, hipDeviceSynchronize());
+    return VRESTIR_OK;
+}
+
+extern "C" void vrestir_default_learned(vrestir_learned_desc* d) {
+    if (!d) return;
+    std::memset(d, 0, sizeof(*d));
+    d->enable = 0;
+    d->train = 1;
+    d->trainBlockX = 16;
+    d->trainBlockY = 8;
+    d->trainStepsPerFrame = 1;
+    d->learningRate = 2e-3f;
+    d->defensiveFraction = 0.25f;
+    d->judge = 1;
+    d->seed = 1;
+}
+
+extern "C" int32_t vrestir_set_learned(vrestir_handle* h, const vrestir_learned_desc* d) {
+    if (!h || !d) return VRESTIR_EINVAL;
+    if (d->trainBlockX < 1 || d->trainBlockY < 1 || d->trainBlockX > 4096 || d->trainBlockY > 4096)
+        return fail(h, VRESTIR_EINVAL, "training block sizes must be in [1, 4096]");
+    if (d->trainStepsPerFrame < 1 || d->trainStepsPerFrame > 64)
+        return fail(h, VRESTIR_EINVAL, "trainStepsPerFrame must be in [1, 64]");
+    if (!(d->learningRate > 0.f && d->learningRate < 1.f)) return fail(h, VRESTIR_EINVAL, "learningRate out of range");
+    if (!(d->defensiveFraction >= 1.f / 64.f && d->defensiveFraction <= 1.f))
+        return fail(h, VRESTIR_EINVAL, "defensiveFraction must be in [1/64, 1]");
+    HIPCHK(h, hipSetDevice(h->device));
+    if (!h->lPrm) {
+        HIPCHK(h, hipMalloc((void**)&h->lPrm, kLearnedParams * 4));
+        HIPCHK(h, hipMalloc((void**)&h->lM, kLearnedParams * 4));
+        HIPCHK(h, hipMalloc((void**)&h->lV, kLearnedParams * 4));
+        HIPCHK(h, hipMalloc((void**)&h->lFrags, (size_t)kLearnedFrags * 64 * sizeof(uint4)));
+        HIPCHK(h, hipMalloc((void**)&h->lStats, 8 * sizeof(float)));
+        HIPCHK(h, hipMalloc((void**)&h->lInUse, sizeof(int)));
+    }
+    const bool sizeChanged = d->trainBlockX != h->L.trainBlockX || d->trainBlockY != h->L.trainBlockY;
+    h->L = *d;
+    if (sizeChanged) { dfree(h->lRecs); h->lRecSlots = 0; }
+    std::vector<float> w;
+    learnedInitWeights(d->seed, w);
+    return learnedUpload(h, w.data());
+}
+
+extern "C" int32_t vrestir_get_learned(const vrestir_handle* h, vrestir_learned_desc* out) {
+    if (!h || !out) return VRESTIR_EINVAL;
+    *out = h->L;
+    return VRESTIR_OK;
+}
+
+extern "C" int32_t vrestir_learned_stats(vrestir_handle* h, float stats[5]) {
+    if (!h || !stats) return VRESTIR_EINVAL;
+    if (!h->lPrm) return fail(h, VRESTIR_ESTATE, "the learned proposal is not set up (vrestir_set_learned)");
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipDeviceSynchronize());
+    HIPCHK(h, hipMemcpy(stats, h->lStats, 5 * sizeof(float), hipMemcpyDeviceToHost));
+    return VRESTIR_OK;
+}
+
+extern "C" int32_t vrestir_learned_get_weights(vrestir_handle* h, float* out, int32_t n) {
+    if (!h || !out) return VRESTIR_EINVAL;
+    if (n != kLearnedParams) return fail(h, VRESTIR_EINVAL, "the network has VRESTIR_LEARNED_PARAMS parameters");
+    if (!h->lPrm) return fail(h, VRESTIR_ESTATE, "the learned proposal is not set up (vrestir_set_learned)");
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipDeviceSynchronize());
+    HIPCHK(h, hipMemcpy(out, h->lPrm, kLearnedParams * 4, hipMemcpyDeviceToHost));
+    return VRESTIR_OK;
+}
+
+extern "C" int32_t vrestir_learned_set_weights(vrestir_handle* h, const float* in, int32_t n) {
+    if (!h || !in) return VRESTIR_EINVAL;
+    if (n != kLearnedParams) return fail(h, VRESTIR_EINVAL, "the network has VRESTIR_LEARNED_PARAMS parameters");
+    if (!h->lPrm) return fail(h, VRESTIR_ESTATE, "the learned proposal is not set up (vrestir_set_learned)");
+    for (int i = 0; i < n; i++)
+        if (!std::isfinite(in[i])) return fail(h, VRESTIR_EINVAL, "non-finite weight");
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipDeviceSynchronize());
+    return learnedUpload(h, in);
+}
+
+extern "C" int32_t vrestir_learned_forward(vrestir_handle* h, void* stream, const float* rawDev, int32_t n,
+                                          uint16_t* cdfDev, float* logitsDev) {
+    if (!h || n < 0 || (n > 0 && (!rawDev || !cdfDev))) return VRESTIR_EINVAL;
+    if (!h->lPrm) return fail(h, VRESTIR_ESTATE, "the learned proposal is not set up (vrestir_set_learned)");
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, launchLearnedInfer((hipStream_t)stream, h->sceneDev, h->frame, rawDev, n, h->lFrags, h->lPrm, nullptr,
+                                 h->L.defensiveFraction, cdfDev, logitsDev));
+    return VRESTIR_OK;
+}
+
+extern "C" int32_t vrestir_learned_gradient(vrestir_handle* h, void* stream, const float* recsDev, int32_t n,
+                                           float scale, float* outDev) {
+    if (!h || n <= 0 || !recsDev || !outDev) return VRESTIR_EINVAL;
+    if (!h->lPrm) return fail(h, VRESTIR_ESTATE, "the learned proposal is not set up (vrestir_set_learned)");
+    HIPCHK(h, hipSetDevice(h->device));
+    int rc = learnedPartials(h, n);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(h, launchLearnedGrad(st, recsDev, n, h->L.defensiveFraction, scale, h->lFrags, h->lPrm, h->lPartial));
+    HIPCHK(h, launchLearnedReduce(st, h->lPartial, (n + 31) / 32, outDev, false, nullptr, nullptr, nullptr, 0.f, 1.f,
+                                  1.f, nullptr, nullptr, false, 0));
+    return VRESTIR_OK;
+}
+
+extern "C" int32_t vrestir_learned_train(vrestir_handle* h, void* stream, const float* recsDev, int32_t n) {
+    if (!h || n <= 0 || !recsDev) return VRESTIR_EINVAL;
+    if (!h->lPrm) return fail(h, VRESTIR_ESTATE, "the learned proposal is not set up (vrestir_set_learned)");
+    HIPCHK(h, hipSetDevice(h->device));
+    return learnedStep(h, (hipStream_t)stream, recsDev, n);
+}
+
 // Frame set-up: reallocation, history reset on option change, round
 // bookkeeping and the three SamplingOptions (VR/VolumetricReSTIR.cpp:339-496).
 extern "C" int32_t vrestir_prepare_frame(vrestir_handle* h, int32_t W, int32_t H) {
@@ -813,6 +1049,10 @@ extern "C" int32_t vrestir_prepare_frame(vrestir_handle* h, int32_t W, int32_t H
         return fail(h, VRESTIR_EINVAL, "useAnalyticLights is set but no point light is bound");
     if (h->sceneHost.hasEmission && h->sceneHost.lutRows == 0)
         return fail(h, VRESTIR_ESTATE, "an emission grid is bound but no blackbody table (vrestir_set_blackbody_lut)");
+    if (learnedActive(h)) {
+        int rc = learnedFrameBuffers(h);
+        if (rc) return rc;
+    }
     h->totalRoundId = 0;
     h->frameOpen = true;
     return VRESTIR_OK;
@@ -956,7 +1196,15 @@ extern "C" int32_t vrestir_run_stage(vrestir_handle* h, void* stream, int32_t st
             a.res0 = h->res[t % 2]; a.extra0 = h->extra[t % 2]; a.color0 = h->color[t % 2];
             a.out = outRGBA ? (float4*)outRGBA : h->outInternal;
             break;
+        case VRESTIR_STAGE_LEARNED_PROPOSAL: return learnedProposal(h, st);
+        case VRESTIR_STAGE_LEARNED_TRAIN: return learnedTrainFrame(h, st);
         default: return fail(h, VRESTIR_EINVAL, "unknown stage");
+    }
+    if (stage == VRESTIR_STAGE_TRACE_RAYS && learnedActive(h)) {
+        int rc = learnedFrameBuffers(h);  // options may have changed since prepare_frame
+        if (rc) return rc;
+        a.learnedOn = 1;
+        a.learned = {h->lCdf, h->lRecs, h->L.trainBlockX, h->L.trainBlockY};
     }
     // the grids each stage traverses most: its options' visibility and lighting mips
     const Frame& F = h->frame;
@@ -1023,7 +1271,10 @@ extern "C" int32_t vrestir_execute(vrestir_handle* h, void* stream, int32_t W, i
         return vrestir_end_frame(h);
     }
     if ((rc = vrestir_run_stage(h, stream, VRESTIR_STAGE_GENERATE_FEATURES, 0, nullptr))) return rc;
+    const bool learned = learnedActive(h);
+    if (learned && (rc = vrestir_run_stage(h, stream, VRESTIR_STAGE_LEARNED_PROPOSAL, 0, nullptr))) return rc;
     if ((rc = vrestir_run_stage(h, stream, VRESTIR_STAGE_TRACE_RAYS, 0, nullptr))) return rc;
+    if (learned && (rc = vrestir_run_stage(h, stream, VRESTIR_STAGE_LEARNED_TRAIN, 0, nullptr))) return rc;
     if (!p.useReference && p.enableTemporalReuse)
         if ((rc = vrestir_run_stage(h, stream, VRESTIR_STAGE_TEMPORAL_REUSE, 0, outMotionVec))) return rc;
     if (!p.useReference && p.enableSpatialReuse)
@@ -1059,6 +1310,8 @@ static int bufInfo(vrestir_handle* h, int id, void** dev, size_t& elems, int& ki
         case VRESTIR_BUF_EXTRA1: *dev = h->extra[1]; elems = ne * 12; return 0;
         case VRESTIR_BUF_TEMPORAL_EXTRA: *dev = h->extraT; elems = ne * 12; return 0;
         case VRESTIR_BUF_COLOR: *dev = h->outInternal; elems = n * 16; return 0;
+        case VRESTIR_BUF_LEARNED_CDF: *dev = h->lCdf; elems = h->lCdfPixels * 64; return 0;
+        case VRESTIR_BUF_TRAIN_RECORDS: *dev = h->lRecs; elems = (size_t)h->lRecSlots * kLearnedRecFloats * 4; return 0;
     }
     return -1;
 }

@@ -12,10 +12,18 @@ hipError_t launchTraceRays(const LaunchArgs& a) {
     VR_DISPATCH_B(a, {
         if (a.frame.useReference)
             hipLaunchKernelGGL(kReference<B>, g, blk, a.occBytes, a.stream, a.scene, a.frame, a.color0);
+        else if (a.learnedOn && spMarch && inMarch)
+            hipLaunchKernelGGL((kTrace<B, 2, 2, 2, true>), g, blk, a.occBytes, a.stream, a.scene, a.frame, a.res0,
+                               a.extra0, a.learned);
+        else if (a.learnedOn)
+            hipLaunchKernelGGL((kTrace<B, -1, -1, -1, true>), g, blk, a.occBytes, a.stream, a.scene, a.frame, a.res0,
+                               a.extra0, a.learned);
         else if (spMarch && inMarch)
-            hipLaunchKernelGGL((kTrace<B, 2, 2, 2>), g, blk, a.occBytes, a.stream, a.scene, a.frame, a.res0, a.extra0);
+            hipLaunchKernelGGL((kTrace<B, 2, 2, 2>), g, blk, a.occBytes, a.stream, a.scene, a.frame, a.res0, a.extra0,
+                               a.learned);
         else
-            hipLaunchKernelGGL((kTrace<B, -1, -1, -1>), g, blk, a.occBytes, a.stream, a.scene, a.frame, a.res0, a.extra0);
+            hipLaunchKernelGGL((kTrace<B, -1, -1, -1>), g, blk, a.occBytes, a.stream, a.scene, a.frame, a.res0,
+                               a.extra0, a.learned);
     })
     return hipGetLastError();
 }

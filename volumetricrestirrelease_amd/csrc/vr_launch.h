@@ -31,6 +31,8 @@ struct LaunchArgs {
     float4* color0;
     float4* out;
     float2* mvec;
+    int learnedOn;           // TraceRays samples its initial distances from the learned proposal
+    LearnedTraceArgs learned;
 };
 
 hipError_t launchStage(const LaunchArgs& a);
@@ -52,6 +54,19 @@ hipError_t launchRelMse(hipStream_t st, const float4* img, const float4* ref, in
 hipError_t launchTrack(hipStream_t stream, const Scene* scene, const EnvXf& env, int mode, int mip, int linear,
                        float tStepScale, int numSamples, int n, const float* rays, const uint32_t* seeds, float* out,
                        int occSlot, int occBytes);
+// Learned proposal (vr_k_learned.hip): the network's CDF for n rays (rawIn NULL:
+// the frame's camera rays, ray = pixel), the per-wave gradient partials of n
+// training records, their fixed-order sum (with adam: the Adam step, statistics
+// and the judge's flag), and the bf16 operand fragments of the master weights.
+hipError_t launchLearnedInfer(hipStream_t st, const Scene* scene, const Frame& frame, const float* rawIn, int n,
+                              const uint4* frags, const float* prm, const int* inUse, float alpha, uint16_t* cdf,
+                              float* logits);
+hipError_t launchLearnedGrad(hipStream_t st, const float* recs, int n, float alpha, float scale, const uint4* frags,
+                             const float* prm, float* partial);
+hipError_t launchLearnedReduce(hipStream_t st, const float* partial, int waves, float* out, bool adam, float* prm,
+                               float* m, float* v, float lr, float c1, float c2, float* stats, int* inUse, bool judge,
+                               int step);
+hipError_t launchLearnedPack(hipStream_t st, const float* prm, uint4* frags);
 // Importance mip chain of a lat-long map: level k of dim >> k texels at imp + off[k].
 hipError_t launchEnvImportance(hipStream_t st, const float* tex, int W, int H, float* imp, const int* off, int dim,
                                int spp);

@@ -10,6 +10,7 @@
 #pragma once
 #include <type_traits>
 #include "vr_launch.h"
+#include "vr_learned.hpp"
 #include "vr_pass.hpp"
 
 namespace vr {
@@ -96,9 +97,13 @@ __global__ __launch_bounds__(256) void kReference(const Scene* __restrict__ sc, 
 
 // TraceRays (VR/TraceRays.cs.slang:64-201). IM: initial lighting tracking
 // method; VM / LM: the spatial options' methods used by the final p-hat.
-template <int B, int IM, int VM, int LM>
+// LEARNED: the initial distances come from the learned proposal (the pixel's
+// CDF over c drives the analytic tracker's optical-depth targets) and one pixel
+// per block writes its candidates' weights as a training record.
+template <int B, int IM, int VM, int LM, bool LEARNED = false>
 VR_TRACING_KERNEL void kTrace(const Scene* __restrict__ sc, const Frame fr,
-                                              ResStore* __restrict__ outRes, float* __restrict__ outExtra) {
+                                              ResStore* __restrict__ outRes, float* __restrict__ outExtra,
+                                              const LearnedTraceArgs la) {
     const Scene& S = *sc;
     const Frame& F = fr;
     occCacheLoad(S, F.occSlot[0], F.occSlot[1]);
@@ -113,10 +118,36 @@ VR_TRACING_KERNEL void kTrace(const Scene* __restrict__ sc, const Frame fr,
     Res fin = newRes();
     const int M = F.initialM;
     const int rounds = (M + 3) / 4;
+    float* rec = nullptr;
+    if constexpr (LEARNED) {
+        const int slot = learned::trainSlot(px.x, px.y, F.width, F.height, la.bx, la.by, F.frameCount);
+        if (slot >= 0) {
+            rec = la.recs + (size_t)slot * learned::kRecFloats;
+            float raw[learned::kRaw];
+            learned::rawInputs(S, ray, raw);
+            float4* r4 = reinterpret_cast<float4*>(rec);
+            r4[0] = make_float4(raw[0], raw[1], raw[2], raw[3]);
+            r4[1] = make_float4(raw[4], raw[5], raw[6], 0.f);
+#pragma unroll
+            for (int i = 2; i < learned::kRecFloats / 4; i++) r4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (!(raw[6] > 0.f)) rec = nullptr;  // a ray that misses the volume does not train
+        }
+    }
     for (int r = 0; r < rounds; r++) {
         float h[4] = {0, 0, 0, 0}, pd[4] = {0, 0, 0, 0}, tr[4] = {0, 0, 0, 0};
         int rs = r == rounds - 1 ? (M - 4 * (rounds - 1)) : 4;
-        if (!F.noReuse)
+        float gc[4] = {1.f, 1.f, 1.f, 1.f};
+        int bins[4] = {0, 0, 0, 0};
+        if constexpr (LEARNED) {
+            learned::Cdf cdf;
+            cdf.load(la.cdf + (size_t)px.id * learned::kOut);
+            float targets[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < 4; s++)
+                if (s < rs) targets[s] = -logf(1 - cdf.sample(u01(rng), bins[s], gc[s]));
+            sampleAnalyticTargets(S, ray, rng, F.initialOpt.visibilityUseLinearSampler != 0,
+                                  F.initialOpt.visibilityMipLevel, rs, targets, h, pd, tr);
+        } else if (!F.noReuse)
             sampleAnalytic(S, ray, rng, F.initialOpt.visibilityUseLinearSampler != 0, F.initialOpt.visibilityMipLevel,
                            rs, h, pd, tr);
         for (int s = 0; s < rs; s++) {
@@ -126,7 +157,31 @@ VR_TRACING_KERNEL void kTrace(const Scene* __restrict__ sc, const Frame fr,
             float hs = s == 0 ? h[0] : (s == 1 ? h[1] : (s == 2 ? h[2] : h[3]));
             float ps = s == 0 ? pd[0] : (s == 1 ? pd[1] : (s == 2 ? pd[2] : pd[3]));
             float ts = s == 0 ? tr[0] : (s == 1 ? tr[1] : (s == 2 ? tr[2] : tr[3]));
+            float ratio = 1.f, gs = 1.f;
+            int bs = 0;
+            if constexpr (LEARNED) {
+                // the learned source pdf over the analytic one: g(c) inside the
+                // volume, the CDF mass above 1 - Tr over Tr for an escape
+                gs = s == 0 ? gc[0] : (s == 1 ? gc[1] : (s == 2 ? gc[2] : gc[3]));
+                bs = s == 0 ? bins[0] : (s == 1 ? bins[1] : (s == 2 ? bins[2] : bins[3]));
+                ratio = gs;
+                if (hs == kTMax) {
+                    learned::Cdf cdf;
+                    cdf.load(la.cdf + (size_t)px.id * learned::kOut);
+                    ratio = ts > 0.f ? cdf.massAbove(1.f - ts) / ts : gs;
+                }
+                ps *= ratio;
+            }
             Res o = computeInitialSample<B, IM>(S, F, ray, hs, ps, ts, rng, F.initialOpt, ex);
+            if constexpr (LEARNED) {
+                if (rec) {
+                    // the sample's weight against the analytic proposal over g(c)
+                    const float wt = o.runningSum * (ratio / gs);
+                    rec[7] += wt;
+                    rec[learned::kRecHist + bs] += wt;
+                    rec[learned::kRecSq + bs] += wt * wt * gs;
+                }
+            }
             bool sel = resample(o, fin, rng);
             if (sel && B > 1) {
                 int mi = decodeMaxIndirect(fin.sampledPixel);

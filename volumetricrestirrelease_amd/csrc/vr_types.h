@@ -160,4 +160,19 @@ struct SplitBufs {
 // Neighbour offsets of spatial reuse: [16 frame seeds][kMaxSpatialSamples][2] ints.
 static const int kMaxSpatialSamples = 32;
 
+// Learned free-flight proposal: FP32 parameters, bf16 MFMA operand fragments
+// (64 lanes x 8 values each), the per-wave gradient partial, a training record.
+static const int kLearnedParams = 12768;
+static const int kLearnedFrags = 46;
+static const int kLearnedPartialStride = 12800;  // gradient, loss, two second moments, padding
+static const int kLearnedRecFloats = 72;
+static const int kLearnedRaw = 8;
+// What TraceRays needs of the learned proposal: each pixel's 16-bit CDF (32
+// values) and the training records, one ray per bx x by block.
+struct LearnedTraceArgs {
+    const uint16_t* cdf;
+    float* recs;
+    int32_t bx, by;
+};
+
 }  // namespace vr

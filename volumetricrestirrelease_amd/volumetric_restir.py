@@ -225,6 +225,52 @@ class VolumetricReSTIR:
                                             reference_rgba.data_ptr(), W, H, ctypes.byref(out)), "relmse")
         return out.value
 
+    # ---- learned free-flight proposal (an addition: no counterpart in the reference)
+    def setLearned(self, **fields):
+        """vrestir_set_learned with vrestir_default_learned's values overridden by `fields`;
+        (re)initialises the network."""
+        d = abi.default_learned()
+        for k, v in fields.items():
+            setattr(d, k, v)
+        self._check(self.lib.vrestir_set_learned(self.h, ctypes.byref(d)), "set_learned")
+        return d
+
+    def learnedStats(self) -> np.ndarray:
+        """[loss, analytic second moment, network second moment, network in use, Adam steps]."""
+        out = np.zeros(5, dtype=np.float32)
+        self._check(self.lib.vrestir_learned_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+                    "learned_stats")
+        return out
+
+    def learnedWeights(self) -> np.ndarray:
+        out = np.zeros(abi.LEARNED_PARAMS, dtype=np.float32)
+        self._check(self.lib.vrestir_learned_get_weights(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                                         out.size), "learned_get_weights")
+        return out
+
+    def setLearnedWeights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._check(self.lib.vrestir_learned_set_weights(self.h, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                                         w.size), "learned_set_weights")
+
+    def learnedForward(self, raw, cdf, logits=None, stream=None):
+        """The network on raw inputs [n, 8] (device float32) -> cdf [n, 32] (device int16/uint16), logits [n, 32]."""
+        n = int(raw.shape[0])
+        lg = ctypes.c_void_p(logits.data_ptr()) if logits is not None else None
+        self._check(self.lib.vrestir_learned_forward(self.h, _stream_ptr(stream), ctypes.c_void_p(raw.data_ptr()), n,
+                                                     ctypes.c_void_p(cdf.data_ptr()), lg), "learned_forward")
+
+    def learnedGradient(self, recs, out, scale=1.0, stream=None):
+        """Summed gradient of scale x loss over records [n, 72] (device) into out [12768 + 3] (device)."""
+        n = int(recs.shape[0])
+        self._check(self.lib.vrestir_learned_gradient(self.h, _stream_ptr(stream), ctypes.c_void_p(recs.data_ptr()),
+                                                      n, scale, ctypes.c_void_p(out.data_ptr())), "learned_gradient")
+
+    def learnedTrain(self, recs, stream=None):
+        n = int(recs.shape[0])
+        self._check(self.lib.vrestir_learned_train(self.h, _stream_ptr(stream), ctypes.c_void_p(recs.data_ptr()), n),
+                    "learned_train")
+
     def track(self, mode, mip, linear, tstep, nsamples, rays, seeds, out, stream=None):
         """Batched tracking primitive over device tensors rays [n,8] f32, seeds [n,3] u32/i32, out [n,8] f32."""
         n = int(rays.shape[0])
