@@ -1,14 +1,30 @@
 #!/usr/bin/env python3
-"""Frames/s of the VolumetricReSTIR pass on one MI355X.
+"""1080p frames/s of the VolumetricReSTIR pass at matched relMSE, learned vs plain
+proposal, on one MI355X.
 
-Workload (default): 512^3 sparse fBm cloud in 8^3 fp16 bricks (~15% occupied, 4 mips),
-1920x1080, path length 3, 32 point lights + constant environment, 8 initial candidates,
-temporal reuse (cap 2.5 x M), 2 spatial rounds of 5 neighbours in 10 px, camera
-orbiting 1 degree per frame. One step = one frame of the pass (all six kernels).
+Workload (config1): 512^3 sparse fBm cloud in 8^3 fp16 bricks (4 mips), 1920x1080,
+path length 3, 32 point lights + constant environment, temporal reuse (cap 2.5 x M),
+2 spatial rounds of 5 neighbours in 10 px, camera orbiting 1 degree per frame.
+One step = one frame of the pass (every kernel, the learned proposal's inference and
+training included when it is on).
 
-Prints one JSON line. `roofline` is for the kernel that takes the most time, with
-the compulsory per-pixel buffer bytes of that kernel as the algorithmic traffic;
-`cpu_baseline` times the CPU oracle on a band of rows of the same workload.
+How the headline number is made:
+1. Quality runs: the plain pass (M = 8 initial candidates, the config's count) renders
+   the 60-frame orbit; frames 56-59 are each compared with a 65,536-spp FP64
+   reference of the volume path tracer (the same camera) on the uniform lattice of
+   every 8th pixel in x and y (240 x 135 pixels: relMSE over a uniform pixel
+   subsample is an unbiased estimate of the full frame's), and relMSE is the mean. The learned-proposal pass, trained online from
+   frame 0, renders the same orbit at M = 8, 6, 4, 2 until its relMSE exceeds the
+   plain pass's; the smallest M that still matches it is the learned configuration.
+2. Timed run: W warmup and K timed frames of the learned configuration (the metric's
+   configuration; the plain M = 8 pass when no learned M matches), bracketed by
+   device synchronisation; `value` = K / elapsed. The plain M = 8 pass is timed the
+   same way and reported beside it.
+
+`roofline` is for the kernel family that takes the most time per frame, with its
+compulsory per-pixel buffer bytes plus the brick pools of the grids it traverses as
+algorithmic bytes; `cpu_baseline` times the CPU oracle on bands of rows of the
+same workload, single-threaded and on all cores of the host's share.
 """
 import argparse
 import json
@@ -19,54 +35,180 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-STAGE_NAMES = ["GenerateFeatures", "TraceRays", "TemporalReuse", "SpatialReuse", "CopyReservoirs", "FinalShading"]
+STAGE_NAMES = {0: "GenerateFeatures", 1: "TraceRays", 2: "TemporalReuse", 3: "SpatialReuse", 4: "CopyReservoirs",
+               5: "FinalShading", 6: "LearnedProposal", 7: "LearnedTrain"}
 HBM_PEAK_GBS = 8000.0
-# HBM traffic per launch of each stage at config1 (SpatialReuse, TemporalReuse and
-# FinalShading: their record, trace and replay kernels), from rocprofv3 --pmc FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00
-# (factors calibrated on a 256 MiB copy in the same process); commands and figures
-# in profiles/SpatialReuse_config1_1080p.md.
-MEASURED_TRAFFIC_BYTES_CONFIG1 = {"SpatialReuse": 32.743e9, "TraceRays": 12.905e9, "FinalShading": 4.322e9,
-                                  "TemporalReuse": 3.363e9, "GenerateFeatures": 0.183e9, "CopyReservoirs": 0.156e9}
+# HBM traffic per launch of each stage at config1 (record, trace and replay kernels
+# summed), from rocprofv3 --pmc FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (factors
+# calibrated on a 256 MiB copy in the same process). A profile constant, not
+# measured by this run: see the `traffic_source` field.
+PROFILE_TRAFFIC = {
+    "file": "profiles/SpatialReuse_config1_1080p.md",
+    "bytes": {"SpatialReuse": 32.743e9, "TraceRays": 12.905e9, "FinalShading": 4.322e9, "TemporalReuse": 3.363e9,
+              "GenerateFeatures": 0.183e9, "CopyReservoirs": 0.156e9},
+}
+LATTICE = 8
+QUALITY_FRAMES = 60
+EVAL_FRAMES = (56, 57, 58, 59)   # relMSE is the mean over these frames of the orbit
+LEARNED_M = (8, 6, 4, 2)
 
 
 def stage_bytes_per_pixel(stage, b, k):
-    """Compulsory buffer bytes per pixel of each kernel in the reference's layout:
+    """Compulsory buffer bytes per pixel of each stage in the reference's layout:
     reservoir R = 36 B, extra-bounce E = 12 (b - 1) B, features F = 8 B, colour C = 16 B."""
     R, E, F, C = 36, 12 * (b - 1), 8, 16
     return {0: F, 1: R + E, 2: 2 * R + 2 * F + R + 2 * E, 3: k * R + F + R + 2 * E, 4: 2 * (R + E),
-            5: R + E + F + C}[stage]
+            5: R + E + F + C, 6: 64, 7: 0}[stage]
 
 
-def cpu_baseline(spec, rows, threads):
+def stage_slots(stage, params):
+    """Grid slots (mip k: k, conservative mip k: 8 + k) a stage's rays traverse."""
+    P = params
+    ini_vis = P.initialBaseMipLevel if P.initialVisibilityUseLinearSampler else P.initialBaseMipLevel + 8
+    return {0: {0}, 1: {ini_vis, P.initialLightingMipLevel}, 2: {P.spatialVisibilityMipLevel, P.spatialLightingMipLevel},
+            3: {P.spatialVisibilityMipLevel, P.spatialLightingMipLevel}, 4: set(), 5: {0}, 6: set(), 7: set()}[stage]
+
+
+def set_candidates(g, M):
+    p = g.getParams()
+    p.initialM = M
+    g.setParams(p)
+
+
+def configure(g, M, learned):
+    """M initial candidates, the learned proposal on (a freshly initialised network)
+    or off, and an empty history: the state of a new pass on the same scene."""
+    set_candidates(g, M)
+    g.setLearned(enable=1 if learned else 0)
+    g.resetHistory()
+
+
+def lattice_of(img):
+    return img[LATTICE // 2::LATTICE, LATTICE // 2::LATTICE].contiguous()
+
+
+def quality_run(g, spec, M, learned, out, stream):
+    """The 60-frame orbit from an empty history; returns the evaluation frames on the
+    lattice and the learned statistics (None for the plain pass)."""
+    import torch
+    configure(g, M, learned)
+    imgs = {}
+    for f in range(QUALITY_FRAMES):
+        g.setCamera(spec, f)
+        g.execute(out, stream=stream)
+        if f in EVAL_FRAMES:
+            imgs[f] = lattice_of(out)
+    torch.cuda.synchronize()
+    return imgs, (g.learnedStats().tolist() if learned else None)
+
+
+def relmse_of(g, imgs, refs):
+    return sum(g.relmse(imgs[f], refs[f]) for f in EVAL_FRAMES) / len(EVAL_FRAMES)
+
+
+def lattice_reference(g, spec, spp_total, stream):
+    """spp_total path-traced samples per lattice pixel at the bound camera, summed in FP64."""
+    import torch
+    W, H = spec.width, spec.height
+    lh, lw = g.latticeShape(W, H, LATTICE)
+    chunks, spp = 16, 256
+    while chunks * spp > spp_total:
+        spp //= 2
+    buf = torch.zeros((chunks, lh, lw, 4), device="cuda")
+    ref = torch.zeros((lh, lw, 4), device="cuda")
+    done, sn, first = 0, 1 << 24, True
+    t0 = time.perf_counter()
+    while done < spp_total:
+        g.referenceLattice(W, H, LATTICE, spp, sn, buf, stream=stream)
+        for c in range(chunks):
+            g.accumulate(buf[c], reset=first, stream=stream)
+            first = False
+        sn += chunks
+        done += chunks * spp
+    g.accumulated(ref, stream=stream)
+    torch.cuda.synchronize()
+    return ref, done, time.perf_counter() - t0
+
+
+def timed_run(g, spec, M, learned, warmup, steps, out, stream):
+    """Warmup frames, then `steps` frames bracketed by device synchronisation, with HIP
+    events around each stage on the stream the stages run on."""
+    import torch
+    configure(g, M, learned)
+    W, H = spec.width, spec.height
+    rounds = spec.params["spatialReuseRounds"]
+    frame = 0
+    for _ in range(warmup):
+        g.setCamera(spec, frame)
+        g.execute(out, stream=stream)
+        frame += 1
+    seq = [(0, 0)] + ([(6, 0)] if learned else []) + [(1, 0)] + ([(7, 0)] if learned else []) + [(2, 0)] + \
+          [(3, r) for r in range(rounds)] + [(4, 0), (5, 0)]
+    ev = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.setCamera(spec, frame)
+        g.prepare_frame(W, H)
+        for st, r in seq:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            g.run_stage(st, r, out=out if st == 5 else None, stream=stream)
+            e1.record(stream)
+            ev.append((st, e0, e1))
+        g.end_frame()
+        frame += 1
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms, calls = {}, {}
+    for st, e0, e1 in ev:
+        ms[st] = ms.get(st, 0.0) + e0.elapsed_time(e1)
+        calls[st] = calls.get(st, 0) + 1
+    return elapsed, {s: ms[s] / calls[s] for s in ms}, {s: ms[s] / steps for s in ms}
+
+
+def cpu_baseline(spec, threads):
+    """The oracle on bands of rows: 5 frames after 1 warm-up, median frame time, once
+    single-threaded (8 rows) and once on `threads` threads (64 rows); per-pixel time
+    scaled to the full frame."""
     from oracle.binding import Oracle
     o = Oracle()
-    t0 = time.perf_counter()
     o.set_scene(spec)
-    setup = time.perf_counter() - t0
-    y0 = spec.height // 2 - rows // 2
-    y1 = y0 + rows
-    o.set_camera(spec, 0)
-    o.execute(spec.width, spec.height, y0, y1, threads)      # frame 0 (no temporal history yet)
-    o.set_camera(spec, 1)
-    t0 = time.perf_counter()
-    o.execute(spec.width, spec.height, y0, y1, threads)      # frame 1: all six kernels
-    dt = time.perf_counter() - t0
-    px = rows * spec.width
-    frame_s = dt / px * spec.width * spec.height
-    return {"value": 1.0 / frame_s, "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": f"oracle frame 1 on rows {y0}-{y1} of {spec.width}x{spec.height} ({px} px, every kernel "
-                      f"restricted to those rows), {dt:.2f} s on {threads} threads, per-pixel time scaled to the "
-                      f"full frame; volume set-up {setup:.1f} s not counted"}
+    res = {}
+    for nt, rows in ((1, 8), (threads, 64)):
+        y0 = spec.height // 2 - rows // 2
+        y1 = y0 + rows
+        ts = []
+        for f in range(6):
+            o.set_camera(spec, f)
+            t0 = time.perf_counter()
+            o.execute(spec.width, spec.height, y0, y1, nt)
+            if f > 0:
+                ts.append(time.perf_counter() - t0)
+        ts.sort()
+        med = ts[len(ts) // 2]
+        res[nt] = (med, rows, y0, y1, med / (rows * spec.width) * spec.width * spec.height)
+    m1, mt = res[1], res[threads]
+    nproc = os.cpu_count()
+    return {"value": 1.0 / mt[4], "unit": "frames/s", "cores": threads, "kind": "port",
+            "single_thread_value": 1.0 / m1[4],
+            "sample": f"oracle, frames 1-5 after frame 0 (median), every kernel restricted to a band of full-width "
+                      f"rows, per-pixel time scaled to {spec.width}x{spec.height}: {threads} threads on rows "
+                      f"{mt[2]}-{mt[3]} ({mt[0]:.2f} s per band frame), 1 thread on rows {m1[2]}-{m1[3]} "
+                      f"({m1[0]:.2f} s per band frame); host reports {nproc} CPUs, the GPU's share is 16"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="config1", choices=["config0", "config1"])
+    ap.add_argument("--reference-spp", type=int, default=65536)
+    ap.add_argument("--no-learned", action="store_true", help="time and report the plain pass only")
+    ap.add_argument("--no-quality", action="store_true", help="skip the relMSE runs (profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=64)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a known-size kernel that "
@@ -80,7 +222,6 @@ def main():
     import torch
 
     from volumetricrestirrelease_amd import abi, scenes
-    from volumetricrestirrelease_amd.volumetric_restir import VolumetricReSTIR
 
     torch.cuda.set_device(0)
     if args.calibration_copy_mb > 0:
@@ -94,51 +235,69 @@ def main():
     spec = scenes.config1() if args.config == "config1" else scenes.config0()
     if args.config == "config0":
         spec.orbit_degrees_per_frame = 1.0
+    W, H = spec.width, spec.height
+    P = spec.params
+    b, k, rounds, M0 = P["maxBounces"], P["spatialSampleCount"], P["spatialReuseRounds"], P["initialM"]
+    stream = torch.cuda.current_stream()
+    out = torch.zeros((H, W, 4), device="cuda")
+    say = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+
+    from volumetricrestirrelease_amd.volumetric_restir import VolumetricReSTIR
     g = VolumetricReSTIR()
     g.setScene(spec)
     setup_s = time.perf_counter() - t0
-    W, H = spec.width, spec.height
-    P = spec.params
-    b, k, rounds = P["maxBounces"], P["spatialSampleCount"], P["spatialReuseRounds"]
-    stream = torch.cuda.current_stream()
-    out = torch.zeros((H, W, 4), device="cuda")
+    quality = None
+    learned_M = None
+    if not args.no_quality:
+        imgs, _ = quality_run(g, spec, M0, False, out, stream)
+        refs, ref_s = {}, 0.0
+        for f in EVAL_FRAMES:
+            g.setCamera(spec, f)
+            refs[f], spp, dt = lattice_reference(g, spec, args.reference_spp, stream)
+            ref_s += dt
+        plain_rel = relmse_of(g, imgs, refs)
+        say(f"references: {spp} spp on the lattice, {len(EVAL_FRAMES)} frames in {ref_s:.1f} s; "
+            f"plain M={M0} relMSE {plain_rel:.5f}")
+        learned_runs = {}
+        if not args.no_learned:
+            for M in LEARNED_M:
+                imgl, st = quality_run(g, spec, M, True, out, stream)
+                r = relmse_of(g, imgl, refs)
+                learned_runs[M] = {"relmse": r, "loss": st[0], "second_moment_analytic": st[1],
+                                   "second_moment_network": st[2], "network_in_use": bool(st[3]),
+                                   "adam_steps": int(st[4])}
+                say(f"learned M={M}: relMSE {r:.5f}, stats {st}")
+                if r <= plain_rel:
+                    learned_M = M
+                else:
+                    break
+        quality = {"reference_spp": spp, "reference_s": round(ref_s, 2),
+                   "lattice": f"every {LATTICE}th pixel from ({LATTICE // 2}, {LATTICE // 2}): "
+                              f"{imgs[EVAL_FRAMES[0]].shape[1]}x{imgs[EVAL_FRAMES[0]].shape[0]} pixels",
+                   "frames": f"mean over frames {EVAL_FRAMES[0]}-{EVAL_FRAMES[-1]} of a {QUALITY_FRAMES}-frame orbit "
+                             f"from an empty history, one reference per frame",
+                   "plain": {"M": M0, "relmse": plain_rel},
+                   "learned": {str(m): v for m, v in learned_runs.items()},
+                   "learned_M_matched": learned_M}
 
-    frame = 0
-    for _ in range(args.warmup):
-        g.setCamera(spec, frame)
-        g.execute(out, stream=stream)
-        frame += 1
-    torch.cuda.synchronize()
-
-    stage_ms = {s: 0.0 for s in range(6)}
-    stage_calls = {s: 0 for s in range(6)}
-    ev_pairs = []
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        g.setCamera(spec, frame)
-        g.prepare_frame(W, H)
-        seq = [(0, 0), (1, 0), (2, 0)] + [(3, r) for r in range(rounds)] + [(4, 0), (5, 0)]
-        for st, r in seq:
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            g.run_stage(st, r, out=out if st == 5 else None, stream=stream)
-            e1.record(stream)
-            ev_pairs.append((st, e0, e1))
-        g.end_frame()
-        frame += 1
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-    for st, e0, e1 in ev_pairs:
-        stage_ms[st] += e0.elapsed_time(e1)
-        stage_calls[st] += 1
-
+    headline_learned = learned_M is not None
+    M_head = learned_M if headline_learned else M0
+    elapsed, avg_ms, frame_ms = timed_run(g, spec, M_head, headline_learned, args.warmup, args.steps, out, stream)
+    plain = None
+    if headline_learned:
+        e_p, avg_p, frame_p = timed_run(g, spec, M0, False, args.warmup, args.steps, out, stream)
+        plain = {"value": round(args.steps / e_p, 4), "ms_per_step": round(1000.0 * e_p / args.steps, 3),
+                 "M": M0, "stage_ms_avg": {STAGE_NAMES[s]: round(v, 3) for s, v in sorted(frame_p.items())}}
     fps = args.steps / elapsed
-    dom = max(stage_ms, key=lambda s: stage_ms[s])
-    avg_ms = stage_ms[dom] / stage_calls[dom]
-    alg_bytes = stage_bytes_per_pixel(dom, b, k) * W * H
-    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    dom = max(frame_ms, key=lambda s: frame_ms[s])
+    prm = g.getParams()
+    pools = {slot: g.brickPoolBytes(slot) for slot in stage_slots(dom, prm)}
+    occupancy = g.volumeOccupancy(0)
+    b_stream = stage_bytes_per_pixel(dom, b, k) * W * H
+    b_vol = sum(pools.values())
+    alg_bytes = b_stream + b_vol
+    achieved = alg_bytes / (avg_ms[dom] * 1e-3) / 1e9
+    traffic = PROFILE_TRAFFIC["bytes"].get(STAGE_NAMES[dom]) if args.config == "config1" else None
     line = {
         "metric": "1080p frames/s at matched relMSE vs 64k-spp ref; learned vs plain proposal",
         "value": round(fps, 4),
@@ -153,13 +312,18 @@ def main():
         "dtype": "f32 (fp16 bricks)" if spec.brick_format == abi.BRICK_FP16 else "f32",
         "data": "synthetic (seeded fBm density, seeded point lights)",
         "config": {
-            "workload": f"{spec.name}: {spec.density.shape[0]}^3 fBm, {W}x{H}, path length {b}, "
-                        f"M={P['initialM']}, {rounds} spatial rounds x {k - 1} neighbours, {len(spec.lights)} "
-                        f"point lights + env, orbiting camera",
-            "proposal": "plain (analytic / ray-marching trackers; learned proposal not built)",
-            "stage_ms_avg": {STAGE_NAMES[s]: round(stage_ms[s] / max(1, stage_calls[s]), 3) for s in range(6)},
+            "workload": f"{spec.name}: {spec.density.shape[0]}^3 fBm cloud, 8^3 "
+                        f"{'fp16' if spec.brick_format == abi.BRICK_FP16 else 'fp32'} bricks, {spec.num_mips} "
+                        f"average-reduction mips with conservative (dilated) twins, {W}x{H}, path length {b}, "
+                        f"{rounds} spatial rounds x {k - 1} neighbours, {len(spec.lights)} point lights + env, "
+                        f"orbiting camera; {occupancy:.1%} of mip-0 brick cells occupied",
+            "proposal": (f"learned (online-trained MLP free-flight proposal), M={M_head}" if headline_learned else
+                         f"plain (analytic trackers), M={M_head}"),
+            "stage_ms_avg": {STAGE_NAMES[s]: round(v, 3) for s, v in sorted(frame_ms.items())},
             "volume_setup_s": round(setup_s, 2),
         },
+        "quality": quality,
+        "plain": plain,
         "roofline": {
             "bound": "hbm",
             "kernel": STAGE_NAMES[dom],
@@ -167,14 +331,18 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": MEASURED_TRAFFIC_BYTES_CONFIG1.get(STAGE_NAMES[dom]) if args.config == "config1" else None,
+            "traffic": traffic,
+            "traffic_source": (f"profile constant ({PROFILE_TRAFFIC['file']}, plain M=8 pass), not measured in this "
+                               f"run" if traffic else None),
             "algorithmic_bytes_per_launch": alg_bytes,
-            "avg_launch_ms": round(avg_ms, 3),
+            "algorithmic_bytes": {"stream": b_stream, "volume_brick_pools": b_vol,
+                                  "volume_slots": sorted(pools)},
+            "avg_launch_ms": round(avg_ms[dom], 3),
         },
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(spec, args.cpu_rows, args.cpu_threads)
+        line["cpu_baseline"] = cpu_baseline(spec, args.cpu_threads)
     print(json.dumps(line), flush=True)
 
 

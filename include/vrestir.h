@@ -285,7 +285,17 @@ enum {
     VRESTIR_TRACK_ENV_EVAL = 11,           /* environment radiance towards dir: out rgb */
     VRESTIR_TRACK_ENV_SAMPLE = 12,         /* importance sample at u = origin.xy: out dir, pdf, Le, lightID */
     VRESTIR_TRACK_EMISSION = 13,           /* emission at the point origin: out rgb, temperature */
-    VRESTIR_TRACK_VELOCITY = 14            /* velocity grid at the point origin: out xyz */
+    VRESTIR_TRACK_VELOCITY = 14,           /* velocity grid at the point origin: out xyz */
+    /* Brick and voxel indexing at the point origin (GVS/gvdbNodes.slang:261-279,
+     * GVS/gvdb.slang:6-29): out = {point-sampled density, trilinear density, the
+     * leaf brick's index-space minimum xyz (-1 when no brick holds the point),
+     * the voxel floor(index-space point) xyz}. */
+    VRESTIR_TRACK_DENSITY_INDEX = 15,
+    /* The hierarchical DDA's visit sequence (VR/VolumeUtils.slang:171-282) over the
+     * grid slot mipLevel, vertex-centred when useLinearSampler: out = {bricks
+     * visited, FNV-1a hash of their index-space minima in visit order (uint32
+     * bits), sum of entry distances, first entry, last entry, tNear, tFar, 0}. */
+    VRESTIR_TRACK_VISITED_BRICKS = 16
 };
 int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t mode, int32_t mipLevel,
                             int32_t useLinearSampler, float tStepScale, int32_t numSamples, int32_t n,
@@ -367,6 +377,25 @@ int32_t vrestir_accumulated(vrestir_handle* h, void* stream, float* outRGBA, int
  * both images RGBA32F in device memory; the result is written to *out (host) after the stream drains. */
 int32_t vrestir_relmse(vrestir_handle* h, void* stream, const float* imageRGBA, const float* referenceRGBA,
                        int32_t width, int32_t height, double* out);
+
+/* The volume path tracer of useReference (VR/VolumePathTracingFunctions.slang:3-132,
+ * VR/TraceRays.cs.slang:85-103) on a uniform pixel lattice: lattice point (i, j) is
+ * pixel (stride i + stride/2, stride j + stride/2) of the width x height frame, seen
+ * through the bound camera, lw = ceil((width - stride/2) / stride) points per row,
+ * lh likewise. Image c of outRGBA (device float4[chunks*lh*lw]) averages spp samples
+ * from each pixel's stream of sample number sampleNumber + c. A frame of this size
+ * must have been prepared (the path tracer's options are that frame's). Summed over
+ * calls with distinct sample numbers (vrestir_accumulate), it gives a converged
+ * reference on the lattice, where relMSE is an unbiased estimate of the full
+ * frame's. No counterpart in the reference beyond its full-frame reference mode. */
+int32_t vrestir_reference_lattice(vrestir_handle* h, void* stream, int32_t width, int32_t height, int32_t stride,
+                                  int32_t spp, uint32_t sampleNumber, int32_t chunks, float* outRGBA);
+
+/* Size of bound grid slot `slot` (mip k: k, conservative mip k: 8 + k): voxel
+ * dimensions, occupied 8^3 bricks, brick cells of its table, and the device bytes
+ * of its brick pool (1000 values per brick with the apron), tables and bounds. */
+int32_t vrestir_grid_info(const vrestir_handle* h, int32_t slot, int32_t dims[3], int64_t* numBricks,
+                          int64_t* brickCells, int64_t* deviceBytes);
 
 /* ---- host-side helpers (no GPU needed) ---- */
 /* Seeded fBm density (4-octave Perlin by default) for the synthetic workloads. */

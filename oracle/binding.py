@@ -38,6 +38,8 @@ def lib():
         "orc_execute": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_int]),
         "orc_prepare_frame": (c_int, [c_void_p, c_int, c_int, c_int]),
         "orc_run_stage": (c_int, [c_void_p, c_int, c_int]),
+        "orc_set_rows": (c_int, [c_void_p, c_int, c_int]),
+        "orc_set_slot": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_void_p, c_int]),
         "orc_end_frame": (c_int, [c_void_p]),
         "orc_read_buffer": (c_int, [c_void_p, c_int, c_void_p, c_size_t]),
         "orc_write_buffer": (c_int, [c_void_p, c_int, c_void_p, c_size_t]),
@@ -240,6 +242,17 @@ class Oracle:
     def prepare(self, W, H, nthreads=1):
         assert self.L.orc_prepare_frame(self.h, W, H, nthreads) == 0
 
+    def set_slot(self, slot, density, index_to_world, fp16):
+        """Replace grid slot `slot` with a dense (nz, ny, nx) grid and its own index -> world."""
+        d = np.ascontiguousarray(density, dtype=np.float32)
+        m = np.ascontiguousarray(index_to_world, dtype=np.float32)
+        nz, ny, nx = d.shape
+        assert self.L.orc_set_slot(self.h, slot, d.ctypes.data, nx, ny, nz, m.ctypes.data, int(fp16)) == 0
+
+    def set_rows(self, row_begin, row_end):
+        """Restrict the following stages to rows [row_begin, row_end)."""
+        assert self.L.orc_set_rows(self.h, row_begin, row_end) == 0
+
     def run_stage(self, stage, rnd=0):
         assert self.L.orc_run_stage(self.h, stage, rnd) == 0
 
@@ -346,6 +359,7 @@ def ref_xoshiro128ss(seed, n: int):
 
 # ---- the learned proposal's network (oracle/oracle_mlp.hpp)
 MLP_IN, MLP_HID, MLP_OUT, MLP_RAW, MLP_REC = 36, 64, 32, 8, 72
+ORC_BUF_MVEC = 100  # the oracle's motion-vector buffer (float2 per pixel)
 MLP_PARAMS = 12768
 MLP_OFFSETS = {"W1": (0, (36, 64)), "b1": (2304, (64,)), "W2": (2368, (64, 64)), "b2": (6464, (64,)),
                "W3": (6528, (64, 64)), "b3": (10624, (64,)), "W4": (10688, (64, 32)), "b4": (12736, (32,))}

@@ -1136,7 +1136,10 @@ static int buildFrame(const vrestir_volume_desc* d, Oracle::VolFrame& f) {
     DenseGrid base;
     base.nx = d->nx; base.ny = d->ny; base.nz = d->nz;
     base.v.assign(d->density, d->density + (size_t)d->nx * d->ny * d->nz);
-    for (float& v : base.v) v = v > 0.f ? v : 0.f;
+    for (float& v : base.v) {
+        if (!std::isfinite(v)) return VRESTIR_EINVAL;
+        v = v > 0.f ? v : 0.f;
+    }
     bool fp16 = d->brickFormat == VRESTIR_BRICK_FP16;
     DenseGrid cons = dilateConservative(base);
     DenseGrid curN = base, curC = cons;
@@ -1271,6 +1274,23 @@ int orc_set_temperature(void* h, const float* t, int nx, int ny, int nz, const f
     return VRESTIR_OK;
 }
 
+// Replaces grid slot `slot` (mip k: k, conservative mip k: 8 + k) of the current
+// frame with a dense grid of its own and its own index -> world transform, as a
+// file set with precomputed mips binds them (SC/Scene.cpp:2794-2850).
+int orc_set_slot(void* h, int slot, const float* v, int nx, int ny, int nz, const float* indexToWorld, int fp16) {
+    Oracle* O = (Oracle*)h;
+    if (!v || slot < 0 || slot >= 16 || nx <= 0 || ny <= 0 || nz <= 0 || nx % 8 || ny % 8 || nz % 8 ||
+        O->frames.empty())
+        return VRESTIR_EINVAL;
+    DenseGrid g;
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    g.v.assign(v, v + (size_t)nx * ny * nz);
+    for (float& x : g.v) x = x > 0.f ? x : 0.f;
+    buildSlot(O->frames[O->curFrame].grids[slot], g, indexToWorld, 1, fp16 != 0);
+    bindFrame(*O, O->curFrame, O->hasAnimation);
+    return VRESTIR_OK;
+}
+
 int orc_set_blackbody_lut(void* h, const float* rgb, int rows) {
     Oracle* O = (Oracle*)h;
     if (rows < 1 || rows > 256 || !rgb) return VRESTIR_EINVAL;
@@ -1391,6 +1411,12 @@ int orc_prepare_frame(void* h, int W, int H, int nthreads) {
     prepareFrame(*O, W, H);
     return VRESTIR_OK;
 }
+int orc_set_rows(void* h, int rowBegin, int rowEnd) {
+    Oracle* O = (Oracle*)h;
+    if (rowBegin < 0 || rowEnd > O->H || rowBegin >= rowEnd) return VRESTIR_EINVAL;
+    O->rowBegin = rowBegin; O->rowEnd = rowEnd;
+    return VRESTIR_OK;
+}
 int orc_run_stage(void* h, int stage, int round) {
     runStage(*(Oracle*)h, stage, round);
     return VRESTIR_OK;
@@ -1410,6 +1436,7 @@ static void* bufPtr(Oracle* O, int id, size_t& bytes) {
         case VRESTIR_BUF_EXTRA1: bytes = O->extra[1].size() * 12; return O->extra[1].data();
         case VRESTIR_BUF_TEMPORAL_EXTRA: bytes = O->temporalExtra.size() * 12; return O->temporalExtra.data();
         case VRESTIR_BUF_COLOR: bytes = O->output.size() * 16; return O->output.data();
+        case ORC_BUF_MVEC: bytes = O->mvec.size() * 8; return O->mvec.data();
     }
     bytes = 0;
     return nullptr;
@@ -1451,6 +1478,43 @@ int orc_final_costs(void* h, long long* out) {
             out[2 * pid + 1] = g_cellSteps;
         }
     return VRESTIR_OK;
+}
+
+// Visit sequence of the hierarchical walk (VR/VolumeUtils.slang:171-282): every
+// leaf handed to the adapter, FNV-1a-hashed over its index-space minimum.
+struct VisitHashAdapter {
+    uint32_t count = 0, hash = 2166136261u;
+    float tSum = 0.f, tFirst = 0.f, tLast = 0.f, tNear = 0.f, tFar = 0.f;
+    void mix(uint32_t v) {
+        for (int k = 0; k < 4; k++) { hash ^= (v >> (8 * k)) & 0xFFu; hash *= 16777619u; }
+    }
+    void setRayInfo(float a, float b, const Ray&) { tNear = a; tFar = b; }
+    void start() {}
+    bool mainStep(const HDDA&, f3 vminLeaf, f3, f4, int, float& t, SampleGen&) {
+        mix((uint32_t)(int)vminLeaf.x); mix((uint32_t)(int)vminLeaf.y); mix((uint32_t)(int)vminLeaf.z);
+        if (count == 0) tFirst = t;
+        tLast = t;
+        tSum += t;
+        count++;
+        return false;
+    }
+    void end() {}
+};
+
+// getNodeAtPoint + getValueAtPoint (GVS/gvdbNodes.slang:261-279, GVS/gvdb.slang:6-29)
+// at a world point: point and trilinear density, the leaf's minimum, the voxel.
+static void indexAtPoint(const Medium& m, f3 pW, int slot, float* o) {
+    const GridSlot& g = m.grid[slot];
+    f3 p = worldToMediumP(m, pW, slot);
+    o[1] = densityIndex(m, p, slot);
+    o[2] = o[3] = o[4] = -1.f;
+    o[5] = std::floor(p.x); o[6] = std::floor(p.y); o[7] = std::floor(p.z);
+    if (p.x < g.bmin.x || p.y < g.bmin.y || p.z < g.bmin.z || p.x >= g.bmax.x || p.y >= g.bmax.y || p.z >= g.bmax.z)
+        return;
+    const LeafNode* leaf = leafAtPoint(g, p);
+    if (!leaf) return;
+    o[0] = valueAtPoint(g, p, false) * m.densityScale;
+    o[2] = (float)leaf->vmin.x; o[3] = (float)leaf->vmin.y; o[4] = (float)leaf->vmin.z;
 }
 
 // Batched tracking primitive: same contract as vrestir_track_batch, on host memory.
@@ -1505,6 +1569,15 @@ int orc_track_batch(void* h, int mode, int mip, int linear, float tStepScale, in
             case VRESTIR_TRACK_VELOCITY: {
                 f3 v = velocityWorld(m, ray.origin);
                 o[0] = v.x; o[1] = v.y; o[2] = v.z;
+                break;
+            }
+            case VRESTIR_TRACK_DENSITY_INDEX: indexAtPoint(m, ray.origin, mip, o); break;
+            case VRESTIR_TRACK_VISITED_BRICKS: {
+                VisitHashAdapter vh;
+                trackGVDB(m, ray, mip, sg, vh, linear != 0);
+                uint32_t hb = vh.hash;
+                o[0] = (float)vh.count; std::memcpy(&o[1], &hb, 4); o[2] = vh.tSum;
+                o[3] = vh.tFirst; o[4] = vh.tLast; o[5] = vh.tNear; o[6] = vh.tFar;
                 break;
             }
             default: return VRESTIR_EINVAL;

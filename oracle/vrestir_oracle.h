@@ -22,6 +22,7 @@ int orc_set_freeze_frame(void* h, int freeze);
 int orc_set_temperature(void* h, const float* t, int nx, int ny, int nz, const float* indexToWorld, int fp16,
                         float LeScale, float cutoff, float scale);
 int orc_set_blackbody_lut(void* h, const float* rgb, int rows);
+int orc_set_slot(void* h, int slot, const float* v, int nx, int ny, int nz, const float* indexToWorld, int fp16);
 int orc_set_env_map(void* h, int w, int hgt, const float* rgb, float intensity, const float tint[3],
                     const float rotationDegrees[3], int impDim, int impSpp);
 int orc_set_env_rotation(void* h, const float deg[3]);
@@ -37,6 +38,10 @@ void orc_emission(void* h, const float p[3], float out[4]);
 void orc_blackbody(void* h, float temperature, float out[3]);
 int orc_execute(void* h, int width, int height, float* outRGBA, int rowBegin, int rowEnd, int nthreads);
 int orc_prepare_frame(void* h, int width, int height, int nthreads);
+/* Restrict the following stages to image rows [rowBegin, rowEnd) (after prepare). */
+int orc_set_rows(void* h, int rowBegin, int rowEnd);
+/* Oracle-only buffer id: the motion vectors TemporalReuse writes (float2 per pixel). */
+#define ORC_BUF_MVEC 100
 int orc_run_stage(void* h, int stage, int round);
 int orc_end_frame(void* h);
 int orc_read_buffer(void* h, int id, void* dst, size_t bytes);

@@ -149,3 +149,74 @@ def test_fp16_storage_is_round_to_nearest_even(fp16):
     d = sparse_grid(16, seed=6)
     o, _ = make_oracle(d, num_mips=1, fp16=True)
     np.testing.assert_array_equal(voxels(o, 0, 16), d.astype(np.float16).astype(np.float32))
+
+
+def fnv1a_minima(minima):
+    h = 2166136261
+    for v in minima:
+        for b in int(v).to_bytes(4, "little", signed=True):
+            h ^= b
+            h = (h * 16777619) & 0xFFFFFFFF
+    return h
+
+
+def test_index_mode_is_brute_force_lookup():
+    """The indexing mode's nearest density, brick minimum and voxel equal the dense
+    array's value at floor(p), 8 floor(p / 8) and floor(p) for the index-space
+    point p (a power-of-two scaling and one rounded offset from the world point)."""
+    d = sparse_grid(32, seed=3)
+    o, spec = make_oracle(d, num_mips=1)
+    rng = np.random.default_rng(4)
+    n = 4000
+    p_idx = rng.uniform(-1.0, 33.0, size=(n, 3)).astype(np.float32)
+    p_idx[: n // 4] = np.floor(p_idx[: n // 4])  # voxel corners
+    s = np.float32(2.0 / 32)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3] = p_idx * s - np.float32(1.0)
+    seeds = np.zeros((n, 3), np.uint32)
+    out = o.track(abi.TRACK_DENSITY_INDEX, 0, 0, 1.0, 1, rays, seeds)
+    # world -> index: x 16 is exact, + 16 rounds once (as the transform does)
+    p_act = rays[:, :3] * np.float32(16.0) + np.float32(16.0)
+    assert np.abs(p_act - p_idx).max() < 1e-5
+    vox = np.floor(p_act).astype(np.int64)
+    np.testing.assert_array_equal(out[:, 5:8], vox.astype(np.float32))
+    inside = np.all((vox >= 0) & (vox < 32), axis=1)
+    bmin = (vox // 8) * 8
+    occ = d.reshape(4, 8, 4, 8, 4, 8).max(axis=(1, 3, 5)) > 0
+    for i in range(n):
+        if inside[i] and occ[tuple(vox[i, ::-1] // 8)]:
+            assert tuple(out[i, 2:5].astype(int)) == tuple(bmin[i])
+            assert out[i, 0] == d[vox[i, 2], vox[i, 1], vox[i, 0]] * spec.density_scale
+        else:
+            assert tuple(out[i, 2:5]) == (-1.0, -1.0, -1.0) and out[i, 0] == 0.0
+
+
+@pytest.mark.parametrize("vertex_center", [0, 1])
+def test_visit_mode_hashes_the_brick_log(vertex_center):
+    """The visit-sequence mode's count and hash are those of the brick log that
+    tests/test_oracle_pass.py pins against a brute-force segment walk; a hash that
+    skipped or reordered a brick would differ."""
+    d = sparse_grid(64, seed=5)
+    o, _ = make_oracle(d, num_mips=1)
+    rng = np.random.default_rng(6)
+    n = 200
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3] = rng.uniform(-1.3, 1.3, size=(n, 3))
+    t = rng.uniform(-0.5, 0.5, size=(n, 3)) - rays[:, :3]
+    rays[:, 3:6] = t / np.linalg.norm(t, axis=1, keepdims=True)
+    rays[:, 6] = 3.4e38
+    out = o.track(abi.TRACK_VISITED_BRICKS, 0, vertex_center, 1.0, 1, rays, np.zeros((n, 3), np.uint32))
+    log = np.zeros((512, 5), np.float32)
+    nonzero = 0
+    for i in range(n):
+        cnt = o.L.orc_trace_bricks(o.h, 0, rays[i].ctypes.data, vertex_center, 512, log.ctypes.data)
+        assert out[i, 0] == cnt
+        minima = log[:cnt, 2:5].astype(np.int64).reshape(-1)
+        assert out[i, 1:2].view(np.uint32)[0] == fnv1a_minima(minima)
+        if cnt:
+            nonzero += 1
+            assert out[i, 3] == log[0, 0] and out[i, 4] == log[cnt - 1, 0]
+            wrong = minima.copy().reshape(-1, 3)[::-1].reshape(-1)
+            if cnt > 1 and not np.array_equal(wrong, minima):
+                assert out[i, 1:2].view(np.uint32)[0] != fnv1a_minima(wrong)
+    assert nonzero > 50

@@ -39,7 +39,8 @@ BUF_FEATURES, BUF_TEMPORAL_FEATURES, BUF_RESERVOIRS0, BUF_RESERVOIRS1, BUF_TEMPO
 TRACK_TR_ANALYTIC, TRACK_TR_RAY_MARCHING, TRACK_TR_RATIO, TRACK_TR_RESIDUAL_RATIO, \
     TRACK_TR_ANALOG_RESIDUAL_RATIO, TRACK_SAMPLE_ANALYTIC, TRACK_SAMPLE_DECOMPOSITION, \
     TRACK_SAMPLE_CELL_BY_DENSITY, TRACK_FEATURE_MARCH, TRACK_DENSITY_AT_POINT, TRACK_RNG_STREAM, \
-    TRACK_ENV_EVAL, TRACK_ENV_SAMPLE, TRACK_EMISSION, TRACK_VELOCITY = range(15)
+    TRACK_ENV_EVAL, TRACK_ENV_SAMPLE, TRACK_EMISSION, TRACK_VELOCITY, TRACK_DENSITY_INDEX, \
+    TRACK_VISITED_BRICKS = range(17)
 
 # (name, ctype) in declaration order of vrestir_params
 PARAM_FIELDS = [
@@ -205,6 +206,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "vrestir_accumulate": (i32, [vp, vp, vp, i32, i32, i32]),
         "vrestir_accumulated": (i32, [vp, vp, vp, POINTER(c_int64)]),
         "vrestir_relmse": (i32, [vp, vp, vp, vp, i32, i32, POINTER(ctypes.c_double)]),
+        "vrestir_reference_lattice": (i32, [vp, vp, i32, i32, i32, i32, ctypes.c_uint32, i32, vp]),
+        "vrestir_grid_info": (i32, [vp, i32, POINTER(i32), POINTER(c_int64), POINTER(c_int64), POINTER(c_int64)]),
         "vrestir_default_learned": (None, [POINTER(LearnedDesc)]),
         "vrestir_set_learned": (i32, [vp, POINTER(LearnedDesc)]),
         "vrestir_get_learned": (i32, [vp, POINTER(LearnedDesc)]),
@@ -238,6 +241,7 @@ EXPORTED_SYMBOLS = [
     "vrestir_get_params", "vrestir_reset_history", "vrestir_execute", "vrestir_frame_count",
     "vrestir_prepare_frame", "vrestir_run_stage", "vrestir_end_frame", "vrestir_read_buffer",
     "vrestir_write_buffer", "vrestir_track_batch", "vrestir_set_tuning", "vrestir_accumulate", "vrestir_accumulated", "vrestir_relmse",
+    "vrestir_reference_lattice", "vrestir_grid_info",
     "vrestir_default_learned", "vrestir_set_learned", "vrestir_get_learned", "vrestir_learned_stats",
     "vrestir_learned_get_weights", "vrestir_learned_set_weights", "vrestir_learned_forward",
     "vrestir_learned_gradient", "vrestir_learned_train",

@@ -415,11 +415,9 @@ static int bindFrame(vrestir_handle* h, int cur, bool animating) {
     return uploadScene(h);
 }
 
-extern "C" int32_t vrestir_set_volume(vrestir_handle* h, const vrestir_volume_desc* d) {
-    if (!h || !d) return VRESTIR_EINVAL;
-    HostVolume hv;
-    int rc = buildVolume(*d, hv, h->err);
-    if (rc) return rc;
+// Uploads a built volume as frame 0 of a new sequence with the material of d.
+static int installVolume(vrestir_handle* h, HostVolume& hv, const vrestir_volume_desc* d) {
+    int rc;
     HIPCHK(h, hipSetDevice(h->device));
     HIPCHK(h, hipDeviceSynchronize());
     freeVolume(h);
@@ -439,6 +437,14 @@ extern "C" int32_t vrestir_set_volume(vrestir_handle* h, const vrestir_volume_de
     h->volumeSet = true;
     h->optionsChanged = true;
     return bindFrame(h, 0, false);
+}
+
+extern "C" int32_t vrestir_set_volume(vrestir_handle* h, const vrestir_volume_desc* d) {
+    if (!h || !d) return VRESTIR_EINVAL;
+    HostVolume hv;
+    int rc = buildVolume(*d, hv, h->err);
+    if (rc) return rc;
+    return installVolume(h, hv, d);
 }
 
 // addGVDBVolumeSequence (SC/Scene.cpp:3329-3355): one more frame of the sequence,
@@ -606,26 +612,30 @@ extern "C" int32_t vrestir_set_env_intensity(vrestir_handle* h, float intensity,
     return uploadScene(h);
 }
 
+// addGVDBVolume's file set (SC/Scene.cpp:2794-2850): <prefix>_mip{k}.vbx and
+// <prefix>_mip{k}c.vbx per mip, built straight from the files' bricks; levels
+// whose files are missing are derived from the level above (vr_volume.cpp:
+// buildVolumeFromFiles).
 extern "C" int32_t vrestir_load_vbx(vrestir_handle* h, const char* pathPrefix, int32_t numMips, int32_t brickFormat,
                                     const float sigmaA[3], const float sigmaS[3], float g, float densityScale) {
-    if (!h || !pathPrefix) return VRESTIR_EINVAL;
-    std::string path = std::string(pathPrefix) + "_mip0.vbx";
-    int nx = 0, ny = 0, nz = 0;
+    if (!h || !pathPrefix || !sigmaA || !sigmaS) return VRESTIR_EINVAL;
+    if (brickFormat != VRESTIR_BRICK_FP32 && brickFormat != VRESTIR_BRICK_FP16)
+        return fail(h, VRESTIR_EINVAL, "unknown brickFormat");
+    if (std::fabs((sigmaS[0] + sigmaA[0]) - (sigmaS[1] + sigmaA[1])) > 1e-6f * (sigmaS[0] + sigmaA[0]) ||
+        std::fabs((sigmaS[1] + sigmaA[1]) - (sigmaS[2] + sigmaA[2])) > 1e-6f * (sigmaS[0] + sigmaA[0]))
+        return fail(h, VRESTIR_EINVAL, "sigma_a + sigma_s must be grey");
+    HostVolume hv;
     vrestir_volume_desc d;
     std::memset(&d, 0, sizeof(d));
-    int rc = vrestir_read_vbx(path.c_str(), nullptr, &nx, &ny, &nz, d.indexToWorld);
-    if (rc) return fail(h, rc, "cannot read " + path);
-    std::vector<float> dens((size_t)nx * ny * nz);
-    rc = vrestir_read_vbx(path.c_str(), dens.data(), &nx, &ny, &nz, d.indexToWorld);
-    if (rc) return fail(h, rc, "cannot read " + path);
-    d.nx = nx; d.ny = ny; d.nz = nz;
-    d.density = dens.data();
-    d.numMips = numMips;
+    int rc = buildVolumeFromFiles(pathPrefix, numMips, brickFormat == VRESTIR_BRICK_FP16, hv, d.indexToWorld, h->err);
+    if (rc) return rc;
+    d.nx = hv.slots[0].nx; d.ny = hv.slots[0].ny; d.nz = hv.slots[0].nz;
+    d.numMips = hv.numMips;
     d.brickFormat = brickFormat;
     for (int i = 0; i < 3; i++) { d.sigmaA[i] = sigmaA[i]; d.sigmaS[i] = sigmaS[i]; }
     d.g = g;
     d.densityScale = densityScale;
-    return vrestir_set_volume(h, &d);
+    return installVolume(h, hv, &d);
 }
 
 extern "C" int32_t vrestir_set_lights(vrestir_handle* h, const vrestir_point_light* lights, int32_t count,
@@ -1128,12 +1138,25 @@ static int splitPlan(vrestir_handle* h, LaunchArgs& a, int stage) {
             return VRESTIR_OK;
         cap = p.maxBounces + 1;  // the primary ray, one per extra bounce, the light ray
         tk = 1;
+        a.traceGridBlocks = (int64_t)g0.nnx * g0.nny * g0.nnz;
     } else {
         return VRESTIR_OK;
     }
-    if (n * (size_t)cap * (sizeof(TraceRec) + 2 * sizeof(float)) + n * 8 > kSplitBudget) return VRESTIR_OK;
-    int rc = splitEnsure(h, cap, n);
-    if (rc) return rc;
+    // the record buffers must fit the budget and the device's free memory (keeping
+    // a quarter of it); a failed allocation falls back to the single kernel
+    const size_t need = n * (size_t)cap * (sizeof(TraceRec) + 2 * sizeof(float)) + n * 8;
+    const size_t held = h->splitPixels * (size_t)h->split.cap * (sizeof(TraceRec) + 2 * sizeof(float)) +
+                        h->splitPixels * 8;
+    size_t freeB = 0, totalB = 0;
+    if (need > kSplitBudget) return VRESTIR_OK;
+    if (need > held && hipMemGetInfo(&freeB, &totalB) == hipSuccess && need > (freeB + held) / 4 * 3)
+        return VRESTIR_OK;
+    if (splitEnsure(h, cap, n) != VRESTIR_OK) {
+        freeSplit(h);
+        (void)hipGetLastError();
+        h->err.clear();
+        return VRESTIR_OK;
+    }
     a.split = h->split;
     a.split.cap = cap;
     a.splitTK = tk;
@@ -1148,9 +1171,44 @@ extern "C" int32_t vrestir_set_tuning(vrestir_handle* h, int32_t key, int32_t va
         case VRESTIR_TUNE_SPLIT_REUSE:
             if (value < 0 || value > 2) return fail(h, VRESTIR_EINVAL, "split reuse mode must be 0, 1 or 2");
             h->splitReuse = value;
+            if (value == 0) {
+                HIPCHK(h, hipSetDevice(h->device));
+                HIPCHK(h, hipDeviceSynchronize());
+                freeSplit(h);
+            }
             return VRESTIR_OK;
         default: return fail(h, VRESTIR_EINVAL, "unknown tuning key");
     }
+}
+
+// The volume path tracer (useReference's kernel) on a uniform pixel lattice of
+// the width x height frame, for converged references on a pixel subsample. The
+// camera is the bound one; the other frame state is the last prepared frame's.
+extern "C" int32_t vrestir_reference_lattice(vrestir_handle* h, void* stream, int32_t width, int32_t height,
+                                             int32_t stride, int32_t spp, uint32_t sampleNumber, int32_t chunks,
+                                             float* outRGBA) {
+    if (!h) return VRESTIR_EINVAL;
+    if (!h->volumeSet || !h->camSet) return fail(h, VRESTIR_ESTATE, "no volume or camera bound");
+    if (width != h->W || height != h->H)
+        return fail(h, VRESTIR_ESTATE, "prepare or execute a frame of this size first");
+    if (stride < 1 || stride > width || stride > height || spp < 1 || spp > 65536 || chunks < 1 ||
+        chunks > 65535 || !outRGBA)
+        return fail(h, VRESTIR_EINVAL, "bad lattice arguments");
+    if (h->frameOpen) return fail(h, VRESTIR_ESTATE, "a frame is open (vrestir_end_frame first)");
+    HIPCHK(h, hipSetDevice(h->device));
+    LaunchArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.stream = (hipStream_t)stream;
+    a.maxBounces = h->P.maxBounces;
+    a.width = width;
+    a.height = height;
+    a.scene = h->sceneDev;
+    a.frame = h->frame;
+    float view[16], proj[16];
+    buildCamera(h->camDesc, (float)width / (float)height, a.frame.cam, view, proj);
+    a.occBytes = occPlan(h, 0, -1, a.frame.occSlot);
+    HIPCHK(h, launchReferenceLattice(a, stride, spp, sampleNumber, chunks, (float4*)outRGBA));
+    return VRESTIR_OK;
 }
 
 extern "C" int32_t vrestir_run_stage(vrestir_handle* h, void* stream, int32_t stage, int32_t round, float* outRGBA) {
@@ -1421,12 +1479,13 @@ extern "C" int32_t vrestir_relmse(vrestir_handle* h, void* stream, const float* 
     return VRESTIR_OK;
 }
 
+
 extern "C" int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t mode, int32_t mip,
                                        int32_t useLinearSampler, float tStepScale, int32_t numSamples, int32_t n,
                                        const float* rays, const uint32_t* seeds, float* out) {
     if (!h) return VRESTIR_EINVAL;
     if (!h->volumeSet) return fail(h, VRESTIR_ESTATE, "no volume bound");
-    if (mode < 0 || mode > VRESTIR_TRACK_VELOCITY) return fail(h, VRESTIR_EINVAL, "unknown tracking mode");
+    if (mode < 0 || mode > VRESTIR_TRACK_VISITED_BRICKS) return fail(h, VRESTIR_EINVAL, "unknown tracking mode");
     if (mode == VRESTIR_TRACK_EMISSION && h->sceneHost.hasEmission && h->sceneHost.lutRows == 0)
         return fail(h, VRESTIR_ESTATE, "no blackbody table bound");
     if (mip < 0 || mip >= kMaxSlots || !h->meta.slots[mip].valid) return fail(h, VRESTIR_EINVAL, "grid slot not bound");
@@ -1436,6 +1495,23 @@ extern "C" int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t 
     const int occBytes = occPlan(h, mip, -1, occSlots);
     HIPCHK(h, launchTrack((hipStream_t)stream, h->sceneDev, h->envCur, mode, mip, useLinearSampler, tStepScale, numSamples, n,
                           rays, seeds, out, occSlots[0], occBytes));
+    return VRESTIR_OK;
+}
+
+// Size of a bound grid slot: voxel dimensions, occupied bricks, brick cells, and
+// the device bytes of its brick pool (with aprons), tables and per-brick bounds.
+extern "C" int32_t vrestir_grid_info(const vrestir_handle* h, int32_t slot, int32_t dims[3], int64_t* numBricks,
+                                     int64_t* brickCells, int64_t* deviceBytes) {
+    if (!h || slot < 0 || slot >= kMaxSlots) return VRESTIR_EINVAL;
+    const HostGrid& g = h->meta.slots[slot];
+    if (!h->volumeSet || !g.valid) return VRESTIR_EINVAL;
+    if (dims) { dims[0] = g.nx; dims[1] = g.ny; dims[2] = g.nz; }
+    const int64_t cells = (int64_t)g.nbx * g.nby * g.nbz;
+    if (numBricks) *numBricks = g.numBricks;
+    if (brickCells) *brickCells = cells;
+    if (deviceBytes)
+        *deviceBytes = (int64_t)g.numBricks * kBrickVals * (g.fp16 ? 2 : 4) * g.channels + cells * 4 +
+                       (int64_t)g.nnx * g.nny * g.nnz * 4 + ((cells + 31) / 32) * 8 + (int64_t)g.numBricks * 16;
     return VRESTIR_OK;
 }
 

@@ -19,6 +19,45 @@ __global__ __launch_bounds__(256) void kFeatures(const Scene* __restrict__ sc, c
 }
 
 
+// The hierarchical walk's visit sequence: the occupied bricks handed to a
+// tracker, hashed in order (FNV-1a over their index-space minima).
+struct VisitHash {
+    uint32_t count = 0, hash = 2166136261u;
+    float tSum = 0.f, tFirst = 0.f, tLast = 0.f, tNear = 0.f, tFar = 0.f;
+    VR_DEV void mix(uint32_t v) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) { hash ^= (v >> (8 * k)) & 0xFFu; hash *= 16777619u; }
+    }
+    VR_DEV void setRayInfo(float a, float b, const Ray&) { tNear = a; tFar = b; }
+    VR_DEV void start() {}
+    VR_DEV bool brickStep(const Dda&, const BrickCtx& bc, int, float& t, Rng&) {
+        mix((uint32_t)(int)bc.vmin.x); mix((uint32_t)(int)bc.vmin.y); mix((uint32_t)(int)bc.vmin.z);
+        if (count == 0) tFirst = t;
+        tLast = t;
+        tSum += t;
+        count++;
+        return false;
+    }
+    VR_DEV void end() {}
+};
+
+// Brick and voxel indexing at an index-space point: point-sampled density, the
+// leaf brick's minimum (-1 when none) and the voxel.
+VR_DEV void indexAt(const Scene& S, int slot, v3 pW, float o[8]) {
+    const Grid& G = S.grid[slot];
+    const v3 p = xformP(G.w2i, pW);
+    o[1] = densityIndex(S, slot, p);
+    o[2] = o[3] = o[4] = -1.f;
+    o[5] = floorf(p.x); o[6] = floorf(p.y); o[7] = floorf(p.z);
+    if (p.x < 0.f || p.y < 0.f || p.z < 0.f || p.x >= G.bmax[0] || p.y >= G.bmax[1] || p.z >= G.bmax[2]) return;
+    const int bx = (int)p.x / 8, by = (int)p.y / 8, bz = (int)p.z / 8;
+    const uint32_t b = brickLookup(G, slot, bx, by, bz);
+    if (b == kEmpty) return;
+    const v3 q = p - V3((float)(bx * 8), (float)(by * 8), (float)(bz * 8));
+    o[0] = sampleBrick(G, b, q, false) * S.densityScale;
+    o[2] = (float)(bx * 8); o[3] = (float)(by * 8); o[4] = (float)(bz * 8);
+}
+
 // Batched tracking primitive: one lane per ray.
 __global__ __launch_bounds__(256) void kTrack(const Scene* __restrict__ sc, const EnvXf ex, int mode, int mip,
                                               int linear,
@@ -91,6 +130,14 @@ __global__ __launch_bounds__(256) void kTrack(const Scene* __restrict__ sc, cons
             v3 E = emissionWorld(S, ray.o);
             o[0] = E.x; o[1] = E.y; o[2] = E.z;
             o[3] = S.hasEmission ? temperatureWorld(S, ray.o) : 0.f;
+            break;
+        }
+        case 15: indexAt(S, mip, ray.o, o); break;
+        case 16: {
+            VisitHash vh;
+            traverse(S, ray, mip, rng, vh, linear != 0);
+            o[0] = (float)vh.count; o[1] = __uint_as_float(vh.hash); o[2] = vh.tSum;
+            o[3] = vh.tFirst; o[4] = vh.tLast; o[5] = vh.tNear; o[6] = vh.tFar;
             break;
         }
         default: break;

@@ -680,6 +680,11 @@ hipError_t launchTraceSegments(const LaunchArgs& a, const Options& opt, bool isF
 
 hipError_t launchTraceCells(const LaunchArgs& a, const Options& opt, bool isFinal) {
     const int n = a.width * a.height;
+    // the cell trace walks one grid whose blocks fit its LDS bit mask; it traces
+    // nothing otherwise, so refuse what it cannot trace instead of leaving trs unwritten
+    if (opt.visibilityMipLevel != opt.lightingMipLevel || a.traceGridBlocks <= 0 ||
+        a.traceGridBlocks > kNodeWords * 32)
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(kTraceSegments<1>, dim3((n + 255) / 256), dim3(256), a.occBytes, a.stream, a.scene, a.frame,
                        opt, isFinal ? 1 : 0, a.split.recs, a.split.counts, a.split.trs, a.split.cap, n);
     return hipGetLastError();

@@ -17,6 +17,7 @@ struct LaunchArgs {
     int splitTK;   // > 0: run the reuse stage as record / trace / replay with tracker kind splitTK
     int splitMode; // the trace of that split: 1 = walker queue (kTraceQueue), 2 = brick segments (kTraceSegments)
     SplitBufs split;
+    int64_t traceGridBlocks;  // 128^3 blocks of the grid the analytic cell trace walks (host-side count)
     const Scene* scene;
     Frame frame;
     const int* offsets;
@@ -38,6 +39,9 @@ struct LaunchArgs {
 hipError_t launchStage(const LaunchArgs& a);
 // Per-stage launchers, one translation unit each (vr_k_*.hip).
 hipError_t launchTraceRays(const LaunchArgs& a);
+void latticeSize(int width, int height, int stride, int& lw, int& lh);
+hipError_t launchReferenceLattice(const LaunchArgs& a, int stride, int spp, uint32_t sampleNumber, int chunks,
+                                  float4* out);
 hipError_t launchTemporalReuse(const LaunchArgs& a);
 hipError_t launchSpatialReuse(const LaunchArgs& a);
 hipError_t launchFinalShading(const LaunchArgs& a);

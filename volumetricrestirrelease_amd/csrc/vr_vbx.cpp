@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/vrestir.h"
+#include "vr_volume.h"
 
 namespace {
 
@@ -251,9 +252,19 @@ extern "C" int32_t vrestir_write_vbx(const char* path, const float* density, int
     return VRESTIR_OK;
 }
 
-extern "C" int32_t vrestir_read_vbx(const char* path, float* densityOrNull, int32_t* nxOut, int32_t* nyOut,
-                                    int32_t* nzOut, float indexToWorld[16]) {
-    if (!path || !nxOut || !nyOut || !nzOut) return VRESTIR_EINVAL;
+namespace {
+
+// A parsed file: the dense extent, index -> world, the bricks reachable from the
+// root and (when asked for) the atlas.
+struct Parsed {
+    int ex = 0, ey = 0, ez = 0, bd = 8;
+    float M[16];
+    int32_t axisRes[3] = {0, 0, 0};
+    std::vector<NodeRec> bricks;
+    std::vector<float> atlas;
+};
+
+int32_t parseVbx(const char* path, bool wantAtlas, Parsed& P) {
     File fh;
     fh.f = std::fopen(path, "rb");
     if (!fh.f) return VRESTIR_EIO;
@@ -387,25 +398,72 @@ extern "C" int32_t vrestir_read_vbx(const char* path, float* densityOrNull, int3
         ex = std::max(ex, b.pos[0] + bd); ey = std::max(ey, b.pos[1] + bd); ez = std::max(ez, b.pos[2] + bd);
     }
     if (haveExtent && vmax[0] > 0 && vmax[1] > 0 && vmax[2] > 0) { ex = vmax[0]; ey = vmax[1]; ez = vmax[2]; }
-    *nxOut = ex; *nyOut = ey; *nzOut = ez;
-    if (indexToWorld) std::memcpy(indexToWorld, M, sizeof(M));
-    if (!densityOrNull) return VRESTIR_OK;
+    P.ex = ex; P.ey = ey; P.ez = ez; P.bd = bd;
+    std::memcpy(P.M, M, sizeof(M));
+    std::memcpy(P.axisRes, axisRes, sizeof(P.axisRes));
+    for (const NodeRec& b : bricks)
+        for (int a = 0; a < 3; a++)
+            if (b.value[a] < 0 || b.value[a] + bd > axisRes[a]) return VRESTIR_EIO;
+    P.bricks = std::move(bricks);
+    if (!wantAtlas) return VRESTIR_OK;
+    P.atlas.resize(atlasVox);
+    if (!get(f, P.atlas.data(), P.atlas.size())) return VRESTIR_EIO;
+    return VRESTIR_OK;
+}
 
-    std::vector<float> atlas(atlasVox);
-    if (!get(f, atlas.data(), atlas.size())) return VRESTIR_EIO;
-    std::memset(densityOrNull, 0, sizeof(float) * (size_t)ex * ey * ez);
-    for (const NodeRec& b : bricks) {
-        for (int z = 0; z < bd; z++)
-            for (int y = 0; y < bd; y++)
-                for (int x = 0; x < bd; x++) {
+// Value of voxel (x, y, z) of brick b (0 outside the dense extent).
+float brickVoxel(const Parsed& P, const NodeRec& b, int x, int y, int z) {
+    const int gx = b.pos[0] + x, gy = b.pos[1] + y, gz = b.pos[2] + z;
+    if (gx < 0 || gy < 0 || gz < 0 || gx >= P.ex || gy >= P.ey || gz >= P.ez) return 0.f;
+    const int tx = b.value[0] + x, ty = b.value[1] + y, tz = b.value[2] + z;
+    return P.atlas[((size_t)tz * P.axisRes[1] + ty) * P.axisRes[0] + tx];
+}
+
+}  // namespace
+
+extern "C" int32_t vrestir_read_vbx(const char* path, float* densityOrNull, int32_t* nxOut, int32_t* nyOut,
+                                    int32_t* nzOut, float indexToWorld[16]) {
+    if (!path || !nxOut || !nyOut || !nzOut) return VRESTIR_EINVAL;
+    Parsed P;
+    int32_t rc = parseVbx(path, densityOrNull != nullptr, P);
+    if (rc) return rc;
+    *nxOut = P.ex; *nyOut = P.ey; *nzOut = P.ez;
+    if (indexToWorld) std::memcpy(indexToWorld, P.M, sizeof(P.M));
+    if (!densityOrNull) return VRESTIR_OK;
+    std::memset(densityOrNull, 0, sizeof(float) * (size_t)P.ex * P.ey * P.ez);
+    for (const NodeRec& b : P.bricks)
+        for (int z = 0; z < P.bd; z++)
+            for (int y = 0; y < P.bd; y++)
+                for (int x = 0; x < P.bd; x++) {
                     const int gx = b.pos[0] + x, gy = b.pos[1] + y, gz = b.pos[2] + z;
-                    if (gx < 0 || gy < 0 || gz < 0 || gx >= ex || gy >= ey || gz >= ez) continue;
-                    const int tx = b.value[0] + x, ty = b.value[1] + y, tz = b.value[2] + z;
-                    if (tx < 0 || ty < 0 || tz < 0 || tx >= axisRes[0] || ty >= axisRes[1] || tz >= axisRes[2])
-                        return VRESTIR_EIO;
-                    densityOrNull[((size_t)gz * ey + gy) * ex + gx] =
-                        atlas[((size_t)tz * axisRes[1] + ty) * axisRes[0] + tx];
+                    if (gx < 0 || gy < 0 || gz < 0 || gx >= P.ex || gy >= P.ey || gz >= P.ez) continue;
+                    densityOrNull[((size_t)gz * P.ey + gy) * P.ex + gx] = brickVoxel(P, b, x, y, z);
                 }
+    return VRESTIR_OK;
+}
+
+namespace vr {
+
+// The bricks of a file without a dense grid: each brick's minimum (index space)
+// and its bd^3 values (x fastest; 0 past the dense extent).
+int32_t readVbxBricks(const char* path, VbxBricks& out) {
+    Parsed P;
+    int32_t rc = parseVbx(path, true, P);
+    if (rc) return rc;
+    out.nx = P.ex; out.ny = P.ey; out.nz = P.ez; out.brickDim = P.bd;
+    std::memcpy(out.indexToWorld, P.M, sizeof(P.M));
+    const size_t nv = (size_t)P.bd * P.bd * P.bd;
+    out.pos.resize(P.bricks.size() * 3);
+    out.vals.resize(P.bricks.size() * nv);
+    for (size_t i = 0; i < P.bricks.size(); i++) {
+        const NodeRec& b = P.bricks[i];
+        for (int a = 0; a < 3; a++) out.pos[3 * i + a] = b.pos[a];
+        float* v = out.vals.data() + i * nv;
+        for (int z = 0; z < P.bd; z++)
+            for (int y = 0; y < P.bd; y++)
+                for (int x = 0; x < P.bd; x++) v[(z * P.bd + y) * P.bd + x] = brickVoxel(P, b, x, y, z);
     }
     return VRESTIR_OK;
 }
+
+}  // namespace vr

@@ -123,21 +123,21 @@ static DenseF dilate(const DenseF& s) {
 // One slot: stored values (tiny-value clamp of SC/Scene.cpp:3154-3156, then the
 // storage precision), brick table, block table, apron pool, bounds
 // (min / max / sum/512 over the 10^3 neighbourhood, SC/Scene.cpp:2985-3012).
-static void buildSlot(HostGrid& g, const DenseF& src, bool fp16) {
-    g.nx = src.nx; g.ny = src.ny; g.nz = src.nz;
+// Src gives the raw value of any voxel (0 outside the grid) and whether a brick
+// cell can hold a non-zero value; the result depends only on the values, so a
+// dense grid and the bricks that hold it build the same slot.
+template <class Src>
+static void buildSlotFrom(HostGrid& g, const Src& src, int nx, int ny, int nz, float gmax, bool fp16) {
+    g.nx = nx; g.ny = ny; g.nz = nz;
     g.fp16 = fp16;
-    float gmax = 0.f;
-    for (float x : src.v) gmax = std::max(gmax, x);
-    DenseF vals = src;
-    for (float& x : vals.v) {
+    // stored value of a raw value; monotone, so the largest stored value is q(gmax)
+    auto q = [&](float x) {
         x = (gmax > 0.f && x / gmax < 1e-9f) ? 0.f : x;
         if (gmax <= 0.f) x = 0.f;
-        if (fp16) x = halfBitsToFloat(floatToHalfBits(x));
-    }
-    float smax = 0.f;
-    for (float x : vals.v) smax = std::max(smax, x);
-    g.maxValue = smax;
-    g.nbx = src.nx / 8; g.nby = src.ny / 8; g.nbz = src.nz / 8;
+        return fp16 ? halfBitsToFloat(floatToHalfBits(x)) : x;
+    };
+    g.maxValue = q(gmax);
+    g.nbx = nx / 8; g.nby = ny / 8; g.nbz = nz / 8;
     g.nnx = (g.nbx + 15) / 16; g.nny = (g.nby + 15) / 16; g.nnz = (g.nbz + 15) / 16;
     size_t nbTot = (size_t)g.nbx * g.nby * g.nbz;
     g.brickTable.assign(nbTot, kEmpty);
@@ -148,10 +148,11 @@ static void buildSlot(HostGrid& g, const DenseF& src, bool fp16) {
         for (int by = 0; by < g.nby; by++)
             for (int bx = 0; bx < g.nbx; bx++) {
                 bool any = false;
-                for (int z = 0; z < 8 && !any; z++)
-                    for (int y = 0; y < 8 && !any; y++)
-                        for (int x = 0; x < 8 && !any; x++)
-                            if (vals.v[vals.idx(bx * 8 + x, by * 8 + y, bz * 8 + z)] > 0.f) any = true;
+                if (src.mayHold(bx, by, bz))
+                    for (int z = 0; z < 8 && !any; z++)
+                        for (int y = 0; y < 8 && !any; y++)
+                            for (int x = 0; x < 8 && !any; x++)
+                                if (q(src.at(bx * 8 + x, by * 8 + y, bz * 8 + z)) > 0.f) any = true;
                 occ[bx + (size_t)g.nbx * (by + (size_t)g.nby * bz)] = any ? 1 : 0;
             }
     });
@@ -190,7 +191,7 @@ static void buildSlot(HostGrid& g, const DenseF& src, bool fp16) {
         for (int i = -1; i <= 8; i++)
             for (int j = -1; j <= 8; j++)
                 for (int k = -1; k <= 8; k++) {
-                    float v = vals.at(x0 + i, y0 + j, z0 + k);
+                    float v = q(src.at(x0 + i, y0 + j, z0 + k));
                     size_t o = (size_t)id * kBrickVals + (size_t)(i + 1) + kApron * ((size_t)(j + 1) + kApron * (size_t)(k + 1));
                     if (fp16) g.pool16[o] = floatToHalfBits(v);
                     else g.pool32[o] = v;
@@ -204,6 +205,74 @@ static void buildSlot(HostGrid& g, const DenseF& src, bool fp16) {
         g.bounds[4 * (size_t)id + 3] = 0.f;
     });
 }
+
+struct DenseSrc {
+    const DenseF& d;
+    float at(int x, int y, int z) const { return d.at(x, y, z); }
+    bool mayHold(int, int, int) const { return true; }
+};
+
+static void buildSlot(HostGrid& g, const DenseF& src, bool fp16) {
+    float gmax = 0.f;
+    for (float x : src.v) gmax = std::max(gmax, x);
+    buildSlotFrom(g, DenseSrc{src}, src.nx, src.ny, src.nz, gmax, fp16);
+}
+
+// Voxels of a brick list: a table over brick cells names the brick holding each
+// cell (values <= 0 read 0, as a clamped dense grid).
+struct BrickSrc {
+    const VbxBricks& b;
+    int nbx, nby, nbz;
+    std::vector<int32_t> cell;
+    float at(int x, int y, int z) const {
+        if (x < 0 || y < 0 || z < 0) return 0.f;
+        const int cx = x >> 3, cy = y >> 3, cz = z >> 3;
+        if (cx >= nbx || cy >= nby || cz >= nbz) return 0.f;
+        const int32_t id = cell[cx + (size_t)nbx * (cy + (size_t)nby * cz)];
+        if (id < 0) return 0.f;
+        const float v = b.vals[(size_t)id * 512 + ((z & 7) * 8 + (y & 7)) * 8 + (x & 7)];
+        return v > 0.f ? v : 0.f;
+    }
+    bool mayHold(int bx, int by, int bz) const { return cell[bx + (size_t)nbx * (by + (size_t)nby * bz)] >= 0; }
+};
+
+static int brickSource(const VbxBricks& b, BrickSrc& s, int& nx, int& ny, int& nz, float& gmax, std::string& err) {
+    if (b.brickDim != 8) { err = "only 8^3 bricks are supported"; return VRESTIR_EUNSUPPORTED; }
+    nx = (b.nx + 7) / 8 * 8; ny = (b.ny + 7) / 8 * 8; nz = (b.nz + 7) / 8 * 8;
+    if (nx <= 0 || ny <= 0 || nz <= 0 || nx > 4096 || ny > 4096 || nz > 4096) {
+        err = "grid dimensions must be in [1, 4096]";
+        return VRESTIR_EINVAL;
+    }
+    s.nbx = nx / 8; s.nby = ny / 8; s.nbz = nz / 8;
+    s.cell.assign((size_t)s.nbx * s.nby * s.nbz, -1);
+    gmax = 0.f;
+    const size_t n = b.pos.size() / 3;
+    for (size_t i = 0; i < n; i++) {
+        const int x = b.pos[3 * i], y = b.pos[3 * i + 1], z = b.pos[3 * i + 2];
+        if ((x | y | z) & 7) { err = "brick positions must be multiples of 8"; return VRESTIR_EUNSUPPORTED; }
+        if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) continue;  // past the extent: all 0
+        s.cell[(x >> 3) + (size_t)s.nbx * ((y >> 3) + (size_t)s.nby * (z >> 3))] = (int32_t)i;
+        for (int k = 0; k < 512; k++) {
+            const float v = b.vals[i * 512 + k];
+            if (!std::isfinite(v)) { err = "the grid holds a non-finite value"; return VRESTIR_EINVAL; }
+            gmax = std::max(gmax, v);
+        }
+    }
+    return VRESTIR_OK;
+}
+
+int buildSlotFromBricks(HostGrid& g, const VbxBricks& b, bool fp16, std::string& err) {
+    BrickSrc s{b, 0, 0, 0, {}};
+    int nx, ny, nz;
+    float gmax;
+    int rc = brickSource(b, s, nx, ny, nz, gmax, err);
+    if (rc) return rc;
+    g = HostGrid();
+    buildSlotFrom(g, s, nx, ny, nz, gmax, fp16);
+    g.valid = true;
+    return VRESTIR_OK;
+}
+
 
 // world -> index transform of mip k from the mip-0 index -> world transform.
 static void mipTransforms(const float* i2w0, int scale, float* w2i) {
@@ -222,6 +291,95 @@ static void mipTransforms(const float* i2w0, int scale, float* w2i) {
     for (int cc = 0; cc < 3; cc++)
         w2i[12 + cc] = (float)(-(M[12] * I[0 * 3 + cc] + M[13] * I[1 * 3 + cc] + M[14] * I[2 * 3 + cc]));
     w2i[15] = 1.f;
+}
+
+void slotTransform(const float* indexToWorld, float* w2i) { mipTransforms(indexToWorld, 1, w2i); }
+
+// The dense grid a brick list holds (values <= 0 read 0).
+static DenseF denseOf(const VbxBricks& b) {
+    BrickSrc s{b, 0, 0, 0, {}};
+    int nx = 0, ny = 0, nz = 0;
+    float gmax;
+    std::string err;
+    DenseF d;
+    if (brickSource(b, s, nx, ny, nz, gmax, err)) return d;
+    d.nx = nx; d.ny = ny; d.nz = nz;
+    d.v.assign((size_t)nx * ny * nz, 0.f);
+    parallelFor(nz, [&](int z) {
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) d.v[d.idx(x, y, z)] = s.at(x, y, z);
+    });
+    return d;
+}
+
+static bool fileExists(const std::string& path) {
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (f) std::fclose(f);
+    return f != nullptr;
+}
+
+// addGVDBVolume's file set (SC/Scene.cpp:2794-2850): <prefix>_mip{k}.vbx and its
+// conservative twin <prefix>_mip{k}c.vbx per mip, each with its own index -> world
+// transform; slots are built straight from the files' bricks. Where a file is
+// missing the level is derived as buildVolume derives it: a conservative twin
+// from the dilation of its mip (mip 0) or the 2x reduction of the twin above it,
+// a mip from the 2x reduction of the mip above it (through dense grids of those
+// levels). Mip 0 must exist.
+int buildVolumeFromFiles(const std::string& prefix, int numMips, bool fp16, HostVolume& out, float* i2w0,
+                         std::string& err) {
+    if (numMips < 1 || numMips > 8) { err = "numMips must be in [1, 8]"; return VRESTIR_EINVAL; }
+    out = HostVolume();
+    DenseF prevN, prevC;   // the level above, when a later level must be derived from it
+    float prevI2W[16];
+    for (int k = 0; k < numMips; k++) {
+        const std::string base = prefix + "_mip" + std::to_string(k);
+        const bool haveN = fileExists(base + ".vbx"), haveC = fileExists(base + "c.vbx");
+        if (k == 0 && !haveN) { err = "cannot read " + base + ".vbx"; return VRESTIR_EIO; }
+        float i2w[16];
+        VbxBricks bn, bc;
+        int rc;
+        if (haveN) {
+            if ((rc = readVbxBricks((base + ".vbx").c_str(), bn))) { err = "cannot read " + base + ".vbx"; return rc; }
+            if ((rc = buildSlotFromBricks(out.slots[k], bn, fp16, err))) return rc;
+            std::memcpy(i2w, bn.indexToWorld, sizeof(i2w));
+        } else {
+            for (int r = 0; r < 16; r++) i2w[r] = (r < 12 ? 2.f : 1.f) * prevI2W[r];
+            prevN = reduce2(prevN);
+            buildSlot(out.slots[k], prevN, fp16);
+            out.slots[k].valid = true;
+        }
+        slotTransform(i2w, out.slots[k].w2i);
+        if (k == 0) std::memcpy(i2w0, i2w, sizeof(i2w));
+        DenseF cDerived;
+        if (haveC) {
+            if ((rc = readVbxBricks((base + "c.vbx").c_str(), bc))) { err = "cannot read " + base + "c.vbx"; return rc; }
+            if ((rc = buildSlotFromBricks(out.slots[8 + k], bc, fp16, err))) return rc;
+            slotTransform(bc.indexToWorld, out.slots[8 + k].w2i);
+        } else {
+            cDerived = k == 0 ? dilate(denseOf(bn)) : reduce2(prevC);
+            buildSlot(out.slots[8 + k], cDerived, fp16);
+            out.slots[8 + k].valid = true;
+            slotTransform(i2w, out.slots[8 + k].w2i);
+        }
+        // dense copies of this level, kept only when the next level is derived from them
+        const std::string next = prefix + "_mip" + std::to_string(k + 1);
+        if (k + 1 < numMips) {
+            if (!fileExists(next + ".vbx") && haveN) prevN = denseOf(bn);
+            if (!fileExists(next + "c.vbx")) prevC = haveC ? denseOf(bc) : std::move(cDerived);
+        }
+        std::memcpy(prevI2W, i2w, sizeof(i2w));
+        out.numMips = k + 1;
+    }
+    volumeSpacing(i2w0, out);
+    return VRESTIR_OK;
+}
+
+// Ray-marching step and supervoxel diagonal from mip 0's index -> world transform
+// (SC/Scene.cpp:3074-3080).
+void volumeSpacing(const float* X, HostVolume& v) {
+    auto rl2 = [&](int r) { return X[r * 4] * X[r * 4] + X[r * 4 + 1] * X[r * 4 + 1] + X[r * 4 + 2] * X[r * 4 + 2]; };
+    v.tStep = (std::sqrt(rl2(0)) + std::sqrt(rl2(1)) + std::sqrt(rl2(2))) / 3.f;
+    v.superVoxelDiag = 8.f * std::sqrt(rl2(0) + rl2(1) + rl2(2));
 }
 
 int buildVolume(const vrestir_volume_desc& d, HostVolume& out, std::string& err) {
@@ -246,7 +404,11 @@ int buildVolume(const vrestir_volume_desc& d, HostVolume& out, std::string& err)
     DenseF base;
     base.nx = d.nx; base.ny = d.ny; base.nz = d.nz;
     base.v.assign(d.density, d.density + (size_t)d.nx * d.ny * d.nz);
-    for (float& v : base.v) v = v > 0.f ? v : 0.f;
+    for (float& v : base.v) {
+        // the pass's majorants and optical depths assume finite densities
+        if (!std::isfinite(v)) { err = "the density grid holds a non-finite value"; return VRESTIR_EINVAL; }
+        v = v > 0.f ? v : 0.f;
+    }
     bool fp16 = d.brickFormat == VRESTIR_BRICK_FP16;
     DenseF cons = dilate(base);
     DenseF n = std::move(base), c = std::move(cons);
@@ -268,10 +430,7 @@ int buildVolume(const vrestir_volume_desc& d, HostVolume& out, std::string& err)
         int rc = buildVelocitySlot(d.velocity, d.nx, d.ny, d.nz, d.indexToWorld, out.slots[kVelocitySlot], err);
         if (rc) return rc;
     }
-    const float* X = d.indexToWorld;
-    auto rl2 = [&](int r) { return X[r * 4] * X[r * 4] + X[r * 4 + 1] * X[r * 4 + 1] + X[r * 4 + 2] * X[r * 4 + 2]; };
-    out.tStep = (std::sqrt(rl2(0)) + std::sqrt(rl2(1)) + std::sqrt(rl2(2))) / 3.f;
-    out.superVoxelDiag = 8.f * std::sqrt(rl2(0) + rl2(1) + rl2(2));
+    volumeSpacing(d.indexToWorld, out);
     return VRESTIR_OK;
 }
 

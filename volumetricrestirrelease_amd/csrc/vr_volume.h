@@ -43,6 +43,26 @@ struct HostVolume {
     float superVoxelDiag = 1.f;
 };
 
+// The bricks of a .vbx file (vr_vbx.cpp): dense extent, index -> world, and per
+// brick its index-space minimum and brickDim^3 values (x fastest).
+struct VbxBricks {
+    int nx = 0, ny = 0, nz = 0, brickDim = 8;
+    float indexToWorld[16];
+    std::vector<int32_t> pos;
+    std::vector<float> vals;
+};
+int32_t readVbxBricks(const char* path, VbxBricks& out);
+// One grid slot straight from bricks (no dense grid): the same slot buildVolume
+// makes from the dense grid those bricks hold.
+int buildSlotFromBricks(HostGrid& g, const VbxBricks& b, bool fp16, std::string& err);
+// A slot of a volume loaded from files: its own index -> world transform.
+void slotTransform(const float* indexToWorld, float* w2i);
+void volumeSpacing(const float* indexToWorld, HostVolume& v);
+// A volume from a .vbx file set <prefix>_mip{k}.vbx / _mip{k}c.vbx (vr_volume.cpp);
+// i2w0 receives mip 0's index -> world transform.
+int buildVolumeFromFiles(const std::string& prefix, int numMips, bool fp16, HostVolume& out, float* i2w0,
+                         std::string& err);
+
 uint16_t floatToHalfBits(float f);
 float halfBitsToFloat(uint16_t h);
 int buildVolume(const vrestir_volume_desc& d, HostVolume& out, std::string& err);

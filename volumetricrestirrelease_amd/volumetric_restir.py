@@ -178,6 +178,10 @@ class VolumetricReSTIR:
         mv = ctypes.c_void_p(out_mvec.data_ptr()) if out_mvec is not None else None
         self._check(self.lib.vrestir_execute(self.h, s, W, H, ctypes.c_void_p(out_rgba.data_ptr()), mv), "execute")
 
+    def resetHistory(self):
+        """Drop the temporal history and restart the frame count (as an option change does)."""
+        self._check(self.lib.vrestir_reset_history(self.h), "reset_history")
+
     def frameCount(self) -> int:
         v = ctypes.c_int32()
         self._check(self.lib.vrestir_frame_count(self.h, ctypes.byref(v)), "frame_count")
@@ -224,6 +228,36 @@ class VolumetricReSTIR:
         self._check(self.lib.vrestir_relmse(self.h, _stream_ptr(stream), image_rgba.data_ptr(),
                                             reference_rgba.data_ptr(), W, H, ctypes.byref(out)), "relmse")
         return out.value
+
+    def referenceLattice(self, width, height, stride, spp, sample_number, out, stream=None):
+        """The volume path tracer on the pixel lattice (stride i + stride/2, stride j + stride/2)
+        of a width x height frame; out: device float32 [chunks, lh, lw, 4], image c from the
+        streams of sample number sample_number + c."""
+        chunks = int(out.shape[0])
+        self._check(self.lib.vrestir_reference_lattice(self.h, _stream_ptr(stream), width, height, stride, spp,
+                                                       sample_number, chunks, ctypes.c_void_p(out.data_ptr())),
+                    "reference_lattice")
+
+    def gridInfo(self, slot):
+        """dims, occupied bricks, brick cells and device bytes of a bound grid slot."""
+        dims = (ctypes.c_int32 * 3)()
+        nb, cells, nbytes = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.vrestir_grid_info(self.h, slot, dims, ctypes.byref(nb), ctypes.byref(cells),
+                                               ctypes.byref(nbytes)), "grid_info")
+        return {"dims": tuple(dims), "bricks": nb.value, "brick_cells": cells.value, "bytes": nbytes.value}
+
+    def volumeOccupancy(self, slot=0):
+        """Fraction of a slot's 8^3 brick cells that hold a brick."""
+        gi = self.gridInfo(slot)
+        return gi["bricks"] / max(1, gi["brick_cells"])
+
+    def brickPoolBytes(self, slot):
+        return self.gridInfo(slot)["bytes"]
+
+    @staticmethod
+    def latticeShape(width, height, stride):
+        """(lh, lw) of vrestir_reference_lattice's lattice."""
+        return (height - stride // 2 + stride - 1) // stride, (width - stride // 2 + stride - 1) // stride
 
     # ---- learned free-flight proposal (an addition: no counterpart in the reference)
     def setLearned(self, **fields):
