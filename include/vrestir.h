@@ -377,6 +377,16 @@ int32_t vrestir_accumulated(vrestir_handle* h, void* stream, float* outRGBA, int
  * both images RGBA32F in device memory; the result is written to *out (host) after the stream drains. */
 int32_t vrestir_relmse(vrestir_handle* h, void* stream, const float* imageRGBA, const float* referenceRGBA,
                        int32_t width, int32_t height, double* out);
+/* vrestir_relmse that allocates nothing (no hipMalloc / hipFree): no counterpart in
+ * the reference. For callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid it). Its partial sums and result use
+ * a static 256 KiB device array per GPU, held until the process ends, so images are
+ * limited to VRESTIR_RELMSE_NOALLOC_MAX_PIXELS pixels (larger: VRESTIR_EINVAL).
+ * Calls that overlap on one GPU share that array and race. Same result as
+ * vrestir_relmse, bit for bit. */
+#define VRESTIR_RELMSE_NOALLOC_MAX_PIXELS 8388352
+int32_t vrestir_relmse_noalloc(vrestir_handle* h, void* stream, const float* imageRGBA, const float* referenceRGBA,
+                               int32_t width, int32_t height, double* out);
 
 /* The volume path tracer of useReference (VR/VolumePathTracingFunctions.slang:3-132,
  * VR/TraceRays.cs.slang:85-103) on a uniform pixel lattice: lattice point (i, j) is

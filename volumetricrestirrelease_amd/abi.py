@@ -20,6 +20,8 @@ VRESTIR_EHIP = 3
 VRESTIR_ENOMEM = 4
 VRESTIR_ESTATE = 5
 VRESTIR_EUNSUPPORTED = 6
+# largest image vrestir_relmse_noalloc accepts (its static scratch: 32767 partial sums of 256 pixels)
+VRESTIR_RELMSE_NOALLOC_MAX_PIXELS = 8388352
 
 # tracking methods / MIS / sampler / reprojection (VR/HostDeviceSharedConstants.slang:5-15, 30-32)
 RATIO_TRACKING, ANALYTIC_TRACKING, RAY_MARCHING, RESIDUAL_RATIO_TRACKING, ANALOG_RESIDUAL_RATIO_TRACKING = range(5)
@@ -206,6 +208,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "vrestir_accumulate": (i32, [vp, vp, vp, i32, i32, i32]),
         "vrestir_accumulated": (i32, [vp, vp, vp, POINTER(c_int64)]),
         "vrestir_relmse": (i32, [vp, vp, vp, vp, i32, i32, POINTER(ctypes.c_double)]),
+        "vrestir_relmse_noalloc": (i32, [vp, vp, vp, vp, i32, i32, POINTER(ctypes.c_double)]),
         "vrestir_reference_lattice": (i32, [vp, vp, i32, i32, i32, i32, ctypes.c_uint32, i32, vp]),
         "vrestir_grid_info": (i32, [vp, i32, POINTER(i32), POINTER(c_int64), POINTER(c_int64), POINTER(c_int64)]),
         "vrestir_default_learned": (None, [POINTER(LearnedDesc)]),
@@ -241,7 +244,7 @@ EXPORTED_SYMBOLS = [
     "vrestir_get_params", "vrestir_reset_history", "vrestir_execute", "vrestir_frame_count",
     "vrestir_prepare_frame", "vrestir_run_stage", "vrestir_end_frame", "vrestir_read_buffer",
     "vrestir_write_buffer", "vrestir_track_batch", "vrestir_set_tuning", "vrestir_accumulate", "vrestir_accumulated", "vrestir_relmse",
-    "vrestir_reference_lattice", "vrestir_grid_info",
+    "vrestir_relmse_noalloc", "vrestir_reference_lattice", "vrestir_grid_info",
     "vrestir_default_learned", "vrestir_set_learned", "vrestir_get_learned", "vrestir_learned_stats",
     "vrestir_learned_get_weights", "vrestir_learned_set_weights", "vrestir_learned_forward",
     "vrestir_learned_gradient", "vrestir_learned_train",

@@ -265,6 +265,10 @@ extern "C" int32_t vrestir_create(const vrestir_params* params, vrestir_handle**
     const float zero[3] = {0.f, 0.f, 0.f};
     envRotation(zero, h->envCur);
     h->envLast = h->envCur;
+    if ((e = relMseScratchInit(nullptr)) != hipSuccess) {
+        delete h;
+        return fail(nullptr, VRESTIR_EHIP, std::string("scratch address: ") + hipGetErrorString(e));
+    }
     if (hipMalloc((void**)&h->sceneDev, sizeof(Scene)) != hipSuccess ||
         hipMalloc((void**)&h->offsetsDev, sizeof(int) * 16 * kMaxSpatialSamples * 2) != hipSuccess) {
         dfree(h->sceneDev);
@@ -1476,6 +1480,24 @@ extern "C" int32_t vrestir_relmse(vrestir_handle* h, void* stream, const float* 
     HIPCHK(h, launchRelMse(st, (const float4*)imageRGBA, (const float4*)referenceRGBA, n, h->relPartial, h->relOut));
     HIPCHK(h, hipMemcpyAsync(out, h->relOut, sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(h, hipStreamSynchronize(st));
+    return VRESTIR_OK;
+}
+
+static_assert(VRESTIR_RELMSE_NOALLOC_MAX_PIXELS == kRelMseNoAllocMaxPixels, "no-alloc relMSE limit");
+
+// vrestir_relmse without allocating: the partial sums and the result live in the
+// library's static device scratch (vr_k_accum.hip), whose address was looked up
+// when the handle was created on this device.
+extern "C" int32_t vrestir_relmse_noalloc(vrestir_handle* h, void* stream, const float* imageRGBA,
+                                          const float* referenceRGBA, int32_t width, int32_t height, double* out) {
+    if (!h) return VRESTIR_EINVAL;
+    if (!imageRGBA || !referenceRGBA || !out || width <= 0 || height <= 0)
+        return fail(h, VRESTIR_EINVAL, "bad image arguments");
+    if ((int64_t)width * height > kRelMseNoAllocMaxPixels)
+        return fail(h, VRESTIR_EINVAL, "image larger than VRESTIR_RELMSE_NOALLOC_MAX_PIXELS");
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, relMseNoAlloc((hipStream_t)stream, (const float4*)imageRGBA, (const float4*)referenceRGBA,
+                            width * height, out));
     return VRESTIR_OK;
 }
 

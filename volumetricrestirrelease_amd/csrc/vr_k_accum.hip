@@ -14,9 +14,18 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "vr_launch.h"
 
 namespace vr {
+
+// Scratch of the allocation-free relMSE: kRelMseScratchDoubles doubles (256 KiB),
+// the per-workgroup partial sums from the start and the result in the last slot,
+// resident on every device the library runs on until the process ends. Host code never
+// passes `scratch` itself to a kernel: its name is a host-side stand-in, and the
+// device address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) double scratch[kRelMseScratchDoubles];
 
 __global__ __launch_bounds__(256) void kAccumulate(const float4* __restrict__ frame, double4* __restrict__ sum,
                                                    int n, int reset) {
@@ -91,6 +100,45 @@ hipError_t launchRelMse(hipStream_t st, const float4* img, const float4* ref, in
     hipLaunchKernelGGL(kRelMsePartial, dim3(m), dim3(256), 0, st, img, ref, n, partial);
     hipLaunchKernelGGL(kSumPartials, dim3(1), dim3(256), 0, st, partial, m, 1.0 / (double)n, out);
     return hipGetLastError();
+}
+
+// Device address of `scratch` on each device, looked up once per device with
+// that device current (each device holds its own copy at its own address).
+static std::mutex gScratchLock;
+static double* gScratchPtr[kMaxDevices] = {};
+
+hipError_t relMseScratchInit(double** ptrOut) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(gScratchLock);
+    if (!gScratchPtr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        gScratchPtr[dev] = (double*)p;
+    }
+    if (ptrOut) *ptrOut = gScratchPtr[dev];
+    return hipSuccess;
+}
+
+// relMSE through the same two kernels as launchRelMse, with the partial sums
+// and the result in the current device's `scratch`; copies the result to *hostOut
+// and waits for the stream. Nothing in `scratch` needs clearing: the first kernel
+// writes every partial the second one reads. Calls that overlap on one device
+// share the array and race.
+hipError_t relMseNoAlloc(hipStream_t st, const float4* img, const float4* ref, int n, double* hostOut) {
+    if (n <= 0 || relMsePartialCount(n) + 1 > kRelMseScratchDoubles) return hipErrorInvalidValue;
+    double* base = nullptr;
+    hipError_t e = relMseScratchInit(&base);
+    if (e != hipSuccess) return e;
+    double* out = base + (kRelMseScratchDoubles - 1);
+    e = launchRelMse(st, img, ref, n, base, out);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(hostOut, out, sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(st);
 }
 
 }  // namespace vr

@@ -55,6 +55,12 @@ hipError_t launchAccumulate(hipStream_t st, const float4* frame, double4* sum, i
 hipError_t launchResolve(hipStream_t st, const double4* sum, double inv, float4* out, int n);
 int relMsePartialCount(int n);
 hipError_t launchRelMse(hipStream_t st, const float4* img, const float4* ref, int n, double* partial, double* out);
+// Allocation-free relMSE: partials and result in a static device array of
+// kRelMseScratchDoubles doubles (images of up to 32767 * 256 = 8,388,352 pixels).
+constexpr int kRelMseScratchDoubles = 32768, kMaxDevices = 64;
+constexpr int kRelMseNoAllocMaxPixels = (kRelMseScratchDoubles - 1) * 256;
+hipError_t relMseScratchInit(double** ptrOut);
+hipError_t relMseNoAlloc(hipStream_t st, const float4* img, const float4* ref, int n, double* hostOut);
 hipError_t launchTrack(hipStream_t stream, const Scene* scene, const EnvXf& env, int mode, int mip, int linear,
                        float tStepScale, int numSamples, int n, const float* rays, const uint32_t* seeds, float* out,
                        int occSlot, int occBytes);
