@@ -199,15 +199,102 @@ def cpu_baseline(spec, threads):
                       f"({m1[0]:.2f} s per band frame); host reports {nproc} CPUs, the GPU's share is 16"}
 
 
+def bench_batched(args):
+    """config2: 16 seeded views and volumes of uneven size. One step = one frame of
+    every view. Timed twice over the same frames: the batch (one persistent launch per
+    step, the frame a replayed HIP graph, queue shape from the autotuner) and the
+    passes executed one after another on one stream."""
+    import torch
+
+    from volumetricrestirrelease_amd import scenes
+    from volumetricrestirrelease_amd.volumetric_restir import VolumetricReSTIR, VolumetricReSTIRBatch
+    t0 = time.perf_counter()
+    specs = scenes.config2()
+    passes = []
+    for spec in specs:
+        g = VolumetricReSTIR()
+        g.setScene(spec)
+        passes.append(g)
+    setup_s = time.perf_counter() - t0
+    outs = [torch.zeros((s.height, s.width, 4), device="cuda") for s in specs]
+    stream = torch.cuda.current_stream()
+    px = sum(s.width * s.height for s in specs)
+    b = VolumetricReSTIRBatch(passes)
+    for g, spec in zip(passes, specs):
+        g.setCamera(spec, 0)
+    tune_ms = b.autotune(outs, frames=2, stream=stream)
+    grid, per = b.tuning()
+
+    def run(batched, first):
+        frame = first
+        for _ in range(args.warmup):
+            for g, spec in zip(passes, specs):
+                g.setCamera(spec, frame)
+            if batched:
+                b.execute(outs, stream=stream)
+            else:
+                for g, o in zip(passes, outs):
+                    g.execute(o, stream=stream)
+            frame += 1
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            for g, spec in zip(passes, specs):
+                g.setCamera(spec, frame)
+            if batched:
+                b.execute(outs, stream=stream)
+            else:
+                for g, o in zip(passes, outs):
+                    g.execute(o, stream=stream)
+            frame += 1
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    el_b = run(True, 0)
+    for g in passes:
+        g.resetHistory()
+    el_s = run(False, 0)
+    line = {
+        "metric": "config2 batched frames/s (every view, one frame each)",
+        "value": round(args.steps / el_b, 4),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * el_b / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (fp16 bricks above 128^3)",
+        "data": "synthetic (16 seeded fBm volumes, seeded point lights, seeded views)",
+        "config": {
+            "workload": "config2: " + ", ".join(f"{s.width}x{s.height}/{s.density.shape[0]}^3/{len(s.lights)}L"
+                                                for s in specs),
+            "pixels_per_step": px,
+            "autotuned": {"persistent_workgroups": grid, "tiles_per_item": per, "ms_per_frame": round(tune_ms, 3)},
+            "one_after_another": {"value": round(args.steps / el_s, 4),
+                                  "ms_per_step": round(1000.0 * el_s / args.steps, 3)},
+            "speedup": round(el_s / el_b, 3),
+            "volume_setup_s": round(setup_s, 2),
+        },
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="config1", choices=["config0", "config1"])
+    ap.add_argument("--config", default="config1", choices=["config0", "config1", "config2"],
+                    help="config2: the batched workload (16 seeded views / volumes in one persistent launch per "
+                         "step, replayed as a HIP graph) against the same passes rendered one after another")
     ap.add_argument("--reference-spp", type=int, default=65536)
     ap.add_argument("--no-learned", action="store_true", help="time and report the plain pass only")
     ap.add_argument("--no-quality", action="store_true", help="skip the relMSE runs (profiling)")
+    ap.add_argument("--time-learned", action="store_true",
+                    help="time the learned-proposal pass at the config's M whatever the quality runs find "
+                         "(profiling its network kernels)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
@@ -224,6 +311,9 @@ def main():
     from volumetricrestirrelease_amd import abi, scenes
 
     torch.cuda.set_device(0)
+    if args.config == "config2":
+        bench_batched(args)
+        return
     if args.calibration_copy_mb > 0:
         n = args.calibration_copy_mb * (1 << 20) // 4
         src = torch.ones(n, device="cuda")
@@ -280,6 +370,8 @@ def main():
                    "learned": {str(m): v for m, v in learned_runs.items()},
                    "learned_M_matched": learned_M}
 
+    if args.time_learned:
+        learned_M = M0
     headline_learned = learned_M is not None
     M_head = learned_M if headline_learned else M0
     elapsed, avg_ms, frame_ms = timed_run(g, spec, M_head, headline_learned, args.warmup, args.steps, out, stream)

@@ -407,6 +407,38 @@ int32_t vrestir_reference_lattice(vrestir_handle* h, void* stream, int32_t width
 int32_t vrestir_grid_info(const vrestir_handle* h, int32_t slot, int32_t dims[3], int64_t* numBricks,
                           int64_t* brickCells, int64_t* deviceBytes);
 
+/* ---- batches: many passes in one persistent launch per step (no counterpart in the
+ * reference, which renders one view per pass). Each step of a frame -- a stage, or
+ * one record / trace / replay pass of a split stage -- runs as ONE persistent
+ * launch whose workgroups take 16x16-pixel tiles (256-pixel ranges for the
+ * traces) of every pass from a work queue in device memory; the frame's launches
+ * are captured into a HIP graph and replayed. The passes keep their own volumes,
+ * lights, cameras, resolutions and histories, and give the same images as when
+ * rendered one by one; they must share path length and spatial rounds and run
+ * with temporal and spatial reuse, the split stage forms (VRESTIR_TUNE_SPLIT_REUSE
+ * 2), ray-marching spatial and initial-lighting trackers and analytic final
+ * shading (the defaults), without the learned proposal or reference mode
+ * (otherwise VRESTIR_EUNSUPPORTED). At most 64 passes, all on one GPU. ---- */
+typedef struct vrestir_batch vrestir_batch;
+int32_t vrestir_batch_create(vrestir_handle* const* passes, int32_t count, vrestir_batch** out);
+int32_t vrestir_batch_destroy(vrestir_batch* b);
+const char* vrestir_batch_last_error(const vrestir_batch* b);
+/* One frame of every pass: widths/heights/outRGBA per pass (outRGBA[i]: device
+ * float4[widths[i]*heights[i]]). Work is ordered after what `stream` holds and
+ * `stream` waits for the frame; the batch's own stream runs the graph. */
+int32_t vrestir_batch_execute(vrestir_batch* b, void* stream, const int32_t* widths, const int32_t* heights,
+                              float* const* outRGBA);
+/* Queue shape: persistent workgroups per launch and 16x16 tiles per queue item;
+ * useGraph 0 issues the launches directly. */
+int32_t vrestir_batch_set_tuning(vrestir_batch* b, int32_t grid, int32_t tilesPerItem, int32_t useGraph);
+int32_t vrestir_batch_get_tuning(const vrestir_batch* b, int32_t* grid, int32_t* tilesPerItem);
+/* The autotuner: renders `frames` frames per candidate queue shape (2, 4 and 8
+ * workgroups per CU x 1, 2 and 4 tiles per item), keeps the fastest, then resets
+ * every pass's history (vrestir_reset_history); *bestMs (optional) receives its
+ * time per frame. */
+int32_t vrestir_batch_autotune(vrestir_batch* b, void* stream, const int32_t* widths, const int32_t* heights,
+                               float* const* outRGBA, int32_t frames, float* bestMs);
+
 /* ---- host-side helpers (no GPU needed) ---- */
 /* Seeded fBm density (4-octave Perlin by default) for the synthetic workloads. */
 int32_t vrestir_synth_fbm(uint64_t seed, int32_t nx, int32_t ny, int32_t nz, int32_t octaves,

@@ -211,6 +211,13 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "vrestir_relmse_noalloc": (i32, [vp, vp, vp, vp, i32, i32, POINTER(ctypes.c_double)]),
         "vrestir_reference_lattice": (i32, [vp, vp, i32, i32, i32, i32, ctypes.c_uint32, i32, vp]),
         "vrestir_grid_info": (i32, [vp, i32, POINTER(i32), POINTER(c_int64), POINTER(c_int64), POINTER(c_int64)]),
+        "vrestir_batch_create": (i32, [vp, i32, POINTER(vp)]),
+        "vrestir_batch_destroy": (i32, [vp]),
+        "vrestir_batch_last_error": (c_char_p, [vp]),
+        "vrestir_batch_execute": (i32, [vp, vp, POINTER(i32), POINTER(i32), vp]),
+        "vrestir_batch_set_tuning": (i32, [vp, i32, i32, i32]),
+        "vrestir_batch_get_tuning": (i32, [vp, POINTER(i32), POINTER(i32)]),
+        "vrestir_batch_autotune": (i32, [vp, vp, POINTER(i32), POINTER(i32), vp, i32, POINTER(f32)]),
         "vrestir_default_learned": (None, [POINTER(LearnedDesc)]),
         "vrestir_set_learned": (i32, [vp, POINTER(LearnedDesc)]),
         "vrestir_get_learned": (i32, [vp, POINTER(LearnedDesc)]),
@@ -245,6 +252,8 @@ EXPORTED_SYMBOLS = [
     "vrestir_prepare_frame", "vrestir_run_stage", "vrestir_end_frame", "vrestir_read_buffer",
     "vrestir_write_buffer", "vrestir_track_batch", "vrestir_set_tuning", "vrestir_accumulate", "vrestir_accumulated", "vrestir_relmse",
     "vrestir_relmse_noalloc", "vrestir_reference_lattice", "vrestir_grid_info",
+    "vrestir_batch_create", "vrestir_batch_destroy", "vrestir_batch_last_error", "vrestir_batch_execute",
+    "vrestir_batch_set_tuning", "vrestir_batch_get_tuning", "vrestir_batch_autotune",
     "vrestir_default_learned", "vrestir_set_learned", "vrestir_get_learned", "vrestir_learned_stats",
     "vrestir_learned_get_weights", "vrestir_learned_set_weights", "vrestir_learned_forward",
     "vrestir_learned_gradient", "vrestir_learned_train",

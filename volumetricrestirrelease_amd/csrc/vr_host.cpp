@@ -1215,12 +1215,10 @@ extern "C" int32_t vrestir_reference_lattice(vrestir_handle* h, void* stream, in
     return VRESTIR_OK;
 }
 
-extern "C" int32_t vrestir_run_stage(vrestir_handle* h, void* stream, int32_t stage, int32_t round, float* outRGBA) {
-    if (!h) return VRESTIR_EINVAL;
-    if (!h->frameOpen) return fail(h, VRESTIR_ESTATE, "vrestir_prepare_frame must precede stages");
+// The launch arguments of one stage of the open frame: its buffers, the
+// occupancy tables it keeps in LDS and its split plan. Launches nothing.
+static int stageArgs(vrestir_handle* h, hipStream_t st, int stage, int round, float* outRGBA, LaunchArgs& a) {
     const vrestir_params& p = h->P;
-    hipStream_t st = (hipStream_t)stream;
-    LaunchArgs a;
     std::memset(&a, 0, sizeof(a));
     a.stream = st;
     a.stage = stage;
@@ -1235,8 +1233,7 @@ extern "C" int32_t vrestir_run_stage(vrestir_handle* h, void* stream, int32_t st
     a.featuresT = h->featuresT;
     a.resT = h->resT;
     a.extraT = h->extraT;
-    size_t n = (size_t)h->W * h->H, ne = n * (size_t)std::max(p.maxBounces - 1, 1);
-    int t = h->totalRoundId;
+    const int t = h->totalRoundId;
     switch (stage) {
         case VRESTIR_STAGE_GENERATE_FEATURES: break;
         case VRESTIR_STAGE_TRACE_RAYS:
@@ -1258,8 +1255,6 @@ extern "C" int32_t vrestir_run_stage(vrestir_handle* h, void* stream, int32_t st
             a.res0 = h->res[t % 2]; a.extra0 = h->extra[t % 2]; a.color0 = h->color[t % 2];
             a.out = outRGBA ? (float4*)outRGBA : h->outInternal;
             break;
-        case VRESTIR_STAGE_LEARNED_PROPOSAL: return learnedProposal(h, st);
-        case VRESTIR_STAGE_LEARNED_TRAIN: return learnedTrainFrame(h, st);
         default: return fail(h, VRESTIR_EINVAL, "unknown stage");
     }
     if (stage == VRESTIR_STAGE_TRACE_RAYS && learnedActive(h)) {
@@ -1290,6 +1285,21 @@ extern "C" int32_t vrestir_run_stage(vrestir_handle* h, void* stream, int32_t st
         int rc = splitPlan(h, a, stage);
         if (rc) return rc;
     }
+    return VRESTIR_OK;
+}
+
+extern "C" int32_t vrestir_run_stage(vrestir_handle* h, void* stream, int32_t stage, int32_t round, float* outRGBA) {
+    if (!h) return VRESTIR_EINVAL;
+    if (!h->frameOpen) return fail(h, VRESTIR_ESTATE, "vrestir_prepare_frame must precede stages");
+    const vrestir_params& p = h->P;
+    hipStream_t st = (hipStream_t)stream;
+    if (stage == VRESTIR_STAGE_LEARNED_PROPOSAL) return learnedProposal(h, st);
+    if (stage == VRESTIR_STAGE_LEARNED_TRAIN) return learnedTrainFrame(h, st);
+    LaunchArgs a;
+    int rc = stageArgs(h, st, stage, round, outRGBA, a);
+    if (rc) return rc;
+    const size_t n = (size_t)h->W * h->H, ne = n * (size_t)std::max(p.maxBounces - 1, 1);
+    const int t = h->totalRoundId;
     HIPCHK(h, launchStage(a));
     if (stage == VRESTIR_STAGE_TEMPORAL_REUSE) {
         if (!p.enableSpatialReuse) {
@@ -1501,7 +1511,6 @@ extern "C" int32_t vrestir_relmse_noalloc(vrestir_handle* h, void* stream, const
     return VRESTIR_OK;
 }
 
-
 extern "C" int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t mode, int32_t mip,
                                        int32_t useLinearSampler, float tStepScale, int32_t numSamples, int32_t n,
                                        const float* rays, const uint32_t* seeds, float* out) {
@@ -1534,6 +1543,331 @@ extern "C" int32_t vrestir_grid_info(const vrestir_handle* h, int32_t slot, int3
     if (deviceBytes)
         *deviceBytes = (int64_t)g.numBricks * kBrickVals * (g.fp16 ? 2 : 4) * g.channels + cells * 4 +
                        (int64_t)g.nnx * g.nny * g.nnz * 4 + ((cells + 31) / 32) * 8 + (int64_t)g.numBricks * 16;
+    return VRESTIR_OK;
+}
+
+// ------------------------------------------------------------------ batches
+// Many passes (views and volumes of uneven size) rendered together: each step of
+// the frame -- a stage, or one record / trace / replay pass of a split stage --
+// is one persistent launch over every pass's tiles (vr_k_batch.hip), and the
+// frame's launches are captured into one HIP graph and replayed. The passes keep
+// their own scenes, buffers and frame state; they must share the options that
+// select kernels: path length, spatial rounds, temporal and spatial reuse on, the
+// ray-marching spatial and initial-lighting trackers, analytic final shading, the
+// split stage forms, no learned proposal, no reference mode.
+struct vrestir_batch {
+    std::vector<vrestir_handle*> passes;
+    int device = 0;
+    int grid = 1024, per = 1;  // persistent workgroups per launch, tiles per queue item
+    int useGraph = 1;
+    hipStream_t stream = nullptr;  // the batch's own stream (graph capture needs a non-null one)
+    hipEvent_t evIn = nullptr, evOut = nullptr;
+    int stepCap = 0;
+    LaunchArgs* argsDev = nullptr;
+    BatchList* listsDev = nullptr;
+    int* countersDev = nullptr;
+    LaunchArgs* argsHost = nullptr;  // pinned staging
+    BatchList* listsHost = nullptr;
+    std::vector<int> ops, occ;
+    hipGraphExec_t exec = nullptr;
+    std::vector<int> capturedKey;
+    std::string err;
+};
+
+static int bfail(vrestir_batch* b, int code, const std::string& msg) {
+    if (b) b->err = msg;
+    return code;
+}
+#define BHIP(b, call)                                                                             \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) return bfail((b), VRESTIR_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static void batchFreeGraph(vrestir_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+    b->capturedKey.clear();
+}
+
+extern "C" int32_t vrestir_batch_create(vrestir_handle* const* passes, int32_t count, vrestir_batch** out) {
+    if (!out) return VRESTIR_EINVAL;
+    *out = nullptr;
+    if (!passes || count < 1 || count > kMaxBatch) return fail(nullptr, VRESTIR_EINVAL, "batch of 1..64 passes");
+    for (int i = 0; i < count; i++) {
+        if (!passes[i]) return fail(nullptr, VRESTIR_EINVAL, "a pass handle is NULL");
+        if (passes[i]->device != passes[0]->device) return fail(nullptr, VRESTIR_EINVAL, "passes on different GPUs");
+        for (int j = 0; j < i; j++)
+            if (passes[j] == passes[i]) return fail(nullptr, VRESTIR_EINVAL, "a pass appears twice");
+    }
+    vrestir_batch* b = new vrestir_batch();
+    b->passes.assign(passes, passes + count);
+    b->device = passes[0]->device;
+    if (hipSetDevice(b->device) != hipSuccess || hipStreamCreate(&b->stream) != hipSuccess ||
+        hipEventCreateWithFlags(&b->evIn, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->evOut, hipEventDisableTiming) != hipSuccess) {
+        delete b;
+        return fail(nullptr, VRESTIR_EHIP, "batch stream / events");
+    }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) == hipSuccess && cus > 0)
+        b->grid = 4 * cus;  // four 256-lane workgroups per CU: 16 waves, the tracing kernels' occupancy
+    *out = b;
+    return VRESTIR_OK;
+}
+
+extern "C" int32_t vrestir_batch_destroy(vrestir_batch* b) {
+    if (!b) return VRESTIR_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batchFreeGraph(b);
+    dfree(b->argsDev); dfree(b->listsDev); dfree(b->countersDev);
+    if (b->argsHost) (void)hipHostFree(b->argsHost);
+    if (b->listsHost) (void)hipHostFree(b->listsHost);
+    if (b->evIn) (void)hipEventDestroy(b->evIn);
+    if (b->evOut) (void)hipEventDestroy(b->evOut);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return VRESTIR_OK;
+}
+
+extern "C" const char* vrestir_batch_last_error(const vrestir_batch* b) { return b ? b->err.c_str() : ""; }
+
+extern "C" int32_t vrestir_batch_set_tuning(vrestir_batch* b, int32_t grid, int32_t tilesPerItem, int32_t useGraph) {
+    if (!b) return VRESTIR_EINVAL;
+    if (grid < 1 || grid > 65536 || tilesPerItem < 1 || tilesPerItem > 64)
+        return bfail(b, VRESTIR_EINVAL, "grid in [1, 65536], tiles per item in [1, 64]");
+    if (grid != b->grid || tilesPerItem != b->per || (useGraph != 0) != (b->useGraph != 0)) batchFreeGraph(b);
+    b->grid = grid; b->per = tilesPerItem; b->useGraph = useGraph != 0;
+    return VRESTIR_OK;
+}
+
+extern "C" int32_t vrestir_batch_get_tuning(const vrestir_batch* b, int32_t* grid, int32_t* tilesPerItem) {
+    if (!b || !grid || !tilesPerItem) return VRESTIR_EINVAL;
+    *grid = b->grid; *tilesPerItem = b->per;
+    return VRESTIR_OK;
+}
+
+static bool sameKernelOptions(const vrestir_params& a, const vrestir_params& c) {
+    return a.maxBounces == c.maxBounces && a.spatialReuseRounds == c.spatialReuseRounds &&
+           a.enableTemporalReuse == c.enableTemporalReuse && a.enableSpatialReuse == c.enableSpatialReuse &&
+           a.useReference == c.useReference;
+}
+
+// One batched frame: every pass's frame prepared, its steps' launch arguments
+// staged, the launches (or their graph) issued on the batch stream, the frame
+// closed. The caller's stream waits for the batch and the batch for the caller.
+static int batchFrame(vrestir_batch* b, hipStream_t user, const int32_t* widths, const int32_t* heights,
+                      float* const* outs, bool allowGraph) {
+    const int n = (int)b->passes.size();
+    const vrestir_params& p0 = b->passes[0]->P;
+    for (int i = 0; i < n; i++) {
+        vrestir_handle* h = b->passes[i];
+        if (!widths || !heights || !outs || !outs[i]) return bfail(b, VRESTIR_EINVAL, "sizes and outputs for every pass");
+        if (!sameKernelOptions(h->P, p0)) return bfail(b, VRESTIR_EUNSUPPORTED, "passes differ in kernel options");
+        if (!h->P.enableTemporalReuse || !h->P.enableSpatialReuse || h->P.useReference || learnedActive(h) ||
+            h->freezeFrame || h->splitReuse != 2)
+            return bfail(b, VRESTIR_EUNSUPPORTED,
+                         "batches need temporal and spatial reuse, split stages, no learned proposal or reference");
+    }
+    BHIP(b, hipSetDevice(b->device));
+    // the steps of the frame
+    struct Step { int op, stage, round; };
+    std::vector<Step> steps = {{kOpFeatures, VRESTIR_STAGE_GENERATE_FEATURES, 0},
+                               {kOpTrace, VRESTIR_STAGE_TRACE_RAYS, 0},
+                               {kOpTemporalRecord, VRESTIR_STAGE_TEMPORAL_REUSE, 0},
+                               {kOpSegmentsMarch, VRESTIR_STAGE_TEMPORAL_REUSE, 0},
+                               {kOpTemporalReplay, VRESTIR_STAGE_TEMPORAL_REUSE, 0},
+                               {kOpCopyFeatures, VRESTIR_STAGE_TEMPORAL_REUSE, 0}};
+    for (int r = 0; r < p0.spatialReuseRounds; r++) {
+        steps.push_back({kOpSpatialRecord, VRESTIR_STAGE_SPATIAL_REUSE, r});
+        steps.push_back({kOpSegmentsMarch, VRESTIR_STAGE_SPATIAL_REUSE, r});
+        steps.push_back({kOpSpatialReplay, VRESTIR_STAGE_SPATIAL_REUSE, r});
+    }
+    steps.push_back({kOpCopyReservoirs, VRESTIR_STAGE_COPY_RESERVOIRS, 0});
+    steps.push_back({kOpFinalRecord, VRESTIR_STAGE_FINAL_SHADING, 0});
+    steps.push_back({kOpSegmentsCells, VRESTIR_STAGE_FINAL_SHADING, 0});
+    steps.push_back({kOpFinalReplay, VRESTIR_STAGE_FINAL_SHADING, 0});
+    const int S = (int)steps.size();
+    if (S > b->stepCap) {
+        batchFreeGraph(b);
+        dfree(b->argsDev); dfree(b->listsDev); dfree(b->countersDev);
+        if (b->argsHost) (void)hipHostFree(b->argsHost);
+        if (b->listsHost) (void)hipHostFree(b->listsHost);
+        b->argsHost = nullptr; b->listsHost = nullptr; b->stepCap = 0;
+        BHIP(b, hipMalloc((void**)&b->argsDev, sizeof(LaunchArgs) * (size_t)S * kMaxBatch));
+        BHIP(b, hipMalloc((void**)&b->listsDev, sizeof(BatchList) * (size_t)S));
+        BHIP(b, hipMalloc((void**)&b->countersDev, sizeof(int) * (size_t)S));
+        BHIP(b, hipHostMalloc((void**)&b->argsHost, sizeof(LaunchArgs) * (size_t)S * kMaxBatch));
+        BHIP(b, hipHostMalloc((void**)&b->listsHost, sizeof(BatchList) * (size_t)S));
+        b->stepCap = S;
+    }
+    // the previous frame's staging may still be in flight
+    BHIP(b, hipStreamSynchronize(b->stream));
+    for (int i = 0; i < n; i++) {
+        int rc = vrestir_prepare_frame(b->passes[i], widths[i], heights[i]);
+        if (rc) return bfail(b, rc, std::string("pass ") + std::to_string(i) + ": " + b->passes[i]->err);
+    }
+    // Reservation pass: a stage's split plan (re)allocates its pass's record buffers
+    // when it needs more rays per pixel than they hold, which would leave the
+    // arguments of an earlier step of this frame pointing at freed buffers. Run
+    // every step's plan once, so the buffers hold the frame's largest capacity,
+    // then build the arguments that are launched.
+    {
+        std::vector<int> rounds(n);
+        for (int i = 0; i < n; i++) rounds[i] = b->passes[i]->totalRoundId;
+        for (const Step& sp : steps)
+            for (int i = 0; i < n; i++) {
+                LaunchArgs tmp;
+                int rc = stageArgs(b->passes[i], b->stream, sp.stage, sp.round, nullptr, tmp);
+                if (rc) return bfail(b, rc, std::string("pass ") + std::to_string(i) + ": " + b->passes[i]->err);
+                if (sp.op == kOpSpatialReplay) b->passes[i]->totalRoundId++;
+            }
+        for (int i = 0; i < n; i++) b->passes[i]->totalRoundId = rounds[i];
+    }
+    b->ops.assign(S, 0);
+    b->occ.assign(S, 0);
+    std::vector<int> key = {n, S, b->grid, b->per};
+    for (int k = 0; k < S; k++) {
+        const Step& sp = steps[k];
+        BatchList& L = b->listsHost[k];
+        std::memset(&L, 0, sizeof(L));
+        L.n = n;
+        L.per = b->per;
+        for (int i = 0; i < n; i++) {
+            vrestir_handle* h = b->passes[i];
+            LaunchArgs& a = b->argsHost[(size_t)k * kMaxBatch + i];
+            int rc = stageArgs(h, b->stream, sp.stage, sp.round,
+                               sp.stage == VRESTIR_STAGE_FINAL_SHADING ? outs[i] : nullptr, a);
+            if (rc) return bfail(b, rc, std::string("pass ") + std::to_string(i) + ": " + h->err);
+            const bool needSplit = sp.stage == VRESTIR_STAGE_TEMPORAL_REUSE || sp.stage == VRESTIR_STAGE_SPATIAL_REUSE ||
+                                   sp.stage == VRESTIR_STAGE_FINAL_SHADING;
+            const int wantTK = sp.stage == VRESTIR_STAGE_FINAL_SHADING ? 1 : 2;
+            const Options& so = a.frame.spatialOpt;
+            if (sp.op == kOpTrace && (a.frame.initialOpt.lightingTrackingMethod != 2u ||
+                                      so.visibilityTrackingMethod != 2u || so.lightingTrackingMethod != 2u))
+                return bfail(b, VRESTIR_EUNSUPPORTED, "batches need the ray-marching spatial and lighting trackers");
+            if (needSplit && (a.splitTK != wantTK || a.splitMode != 2))
+                return bfail(b, VRESTIR_EUNSUPPORTED, std::string("pass ") + std::to_string(i) +
+                                                         ": a stage of this pass does not run in its split form");
+            const bool seg = sp.op == kOpSegmentsMarch || sp.op == kOpSegmentsCells;
+            const int W = a.width, H = a.height;
+            const int gx = seg ? (W * H + 255) / 256 : (W + 15) / 16;
+            const int tiles = seg ? gx : gx * ((H + 15) / 16);
+            L.gx[i] = gx;
+            L.tiles[i] = tiles;
+            L.start[i + 1] = L.start[i] + (tiles + L.per - 1) / L.per;
+            b->occ[k] = std::max(b->occ[k], a.occBytes);
+            key.push_back(a.occBytes);
+            key.push_back(tiles);
+        }
+        b->ops[k] = sp.op;
+        key.push_back(sp.op);
+        if (sp.stage == VRESTIR_STAGE_SPATIAL_REUSE && sp.op == kOpSpatialReplay)
+            for (int i = 0; i < n; i++) b->passes[i]->totalRoundId++;
+    }
+    // every staged record buffer must be the one its pass holds now
+    for (int k = 0; k < S; k++)
+        for (int i = 0; i < n; i++) {
+            const LaunchArgs& a = b->argsHost[(size_t)k * kMaxBatch + i];
+            const vrestir_handle* h = b->passes[i];
+            if (a.splitTK && (a.split.recs != h->split.recs || a.split.trs != h->split.trs ||
+                              a.split.dens != h->split.dens || a.split.counts != h->split.counts))
+                return bfail(b, VRESTIR_ESTATE, "record buffers changed while the frame was built");
+        }
+    // stage the arguments (the graph's kernels read them from device memory)
+    BHIP(b, hipEventRecord(b->evIn, user));
+    BHIP(b, hipStreamWaitEvent(b->stream, b->evIn, 0));
+    for (int k = 0; k < S; k++)
+        BHIP(b, hipMemcpyAsync(b->argsDev + (size_t)k * kMaxBatch, b->argsHost + (size_t)k * kMaxBatch,
+                               sizeof(LaunchArgs) * n, hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipMemcpyAsync(b->listsDev, b->listsHost, sizeof(BatchList) * S, hipMemcpyHostToDevice, b->stream));
+    auto issue = [&]() -> int {
+        BHIP(b, launchZeroCounters(b->stream, b->countersDev, S));
+        for (int k = 0; k < S; k++) {
+            const LaunchArgs* args = b->argsDev + (size_t)k * kMaxBatch;
+            const int op = b->ops[k];
+            if (op == kOpSegmentsMarch || op == kOpSegmentsCells)
+                BHIP(b, launchBatchedSegments(b->stream, op, args, b->listsDev + k, b->countersDev + k, b->grid,
+                                              b->occ[k]));
+            else
+                BHIP(b, launchBatched(b->stream, op, p0.maxBounces, args, b->listsDev + k, b->countersDev + k,
+                                      b->grid, b->occ[k]));
+        }
+        return VRESTIR_OK;
+    };
+    int rc = VRESTIR_OK;
+    if (allowGraph && b->useGraph) {
+        if (!b->exec || key != b->capturedKey) {
+            batchFreeGraph(b);
+            hipGraph_t g = nullptr;
+            BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            rc = issue();
+            hipError_t ce = hipStreamEndCapture(b->stream, &g);
+            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+            BHIP(b, ce);
+            hipError_t ie = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            BHIP(b, ie);
+            b->capturedKey = key;
+        }
+        BHIP(b, hipGraphLaunch(b->exec, b->stream));
+    } else {
+        rc = issue();
+        if (rc) return rc;
+    }
+    BHIP(b, hipEventRecord(b->evOut, b->stream));
+    BHIP(b, hipStreamWaitEvent(user, b->evOut, 0));
+    for (int i = 0; i < n; i++) vrestir_end_frame(b->passes[i]);
+    return VRESTIR_OK;
+}
+
+extern "C" int32_t vrestir_batch_execute(vrestir_batch* b, void* stream, const int32_t* widths, const int32_t* heights,
+                                         float* const* outRGBA) {
+    if (!b) return VRESTIR_EINVAL;
+    return batchFrame(b, (hipStream_t)stream, widths, heights, outRGBA, true);
+}
+
+// The project's autotuner: renders `frames` batched frames with each candidate
+// (persistent workgroups x tiles per queue item), keeps the fastest by HIP-event
+// time, then resets every pass's history so the caller's sequence starts afresh.
+extern "C" int32_t vrestir_batch_autotune(vrestir_batch* b, void* stream, const int32_t* widths,
+                                          const int32_t* heights, float* const* outRGBA, int32_t frames,
+                                          float* bestMs) {
+    if (!b || frames < 1 || frames > 16) return VRESTIR_EINVAL;
+    hipStream_t user = (hipStream_t)stream;
+    BHIP(b, hipSetDevice(b->device));
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
+    const int grids[3] = {2 * cus, 4 * cus, 8 * cus};
+    const int pers[3] = {1, 2, 4};
+    hipEvent_t e0, e1;
+    BHIP(b, hipEventCreate(&e0));
+    BHIP(b, hipEventCreate(&e1));
+    float best = 1e30f;
+    int bg = b->grid, bp = b->per;
+    int rc = VRESTIR_OK;
+    // one untimed frame allocates every pass's buffers
+    b->grid = grids[1]; b->per = 1;
+    rc = batchFrame(b, user, widths, heights, outRGBA, false);
+    for (int gi = 0; gi < 3 && rc == VRESTIR_OK; gi++)
+        for (int pi = 0; pi < 3 && rc == VRESTIR_OK; pi++) {
+            b->grid = grids[gi]; b->per = pers[pi];
+            if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = VRESTIR_EHIP; break; }
+            for (int f = 0; f < frames && rc == VRESTIR_OK; f++) rc = batchFrame(b, user, widths, heights, outRGBA, false);
+            if (rc) break;
+            float ms = 0.f;
+            if (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = VRESTIR_EHIP; break; }
+            if (ms < best) { best = ms; bg = grids[gi]; bp = pers[pi]; }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    batchFreeGraph(b);
+    b->grid = bg; b->per = bp;
+    for (vrestir_handle* h : b->passes) vrestir_reset_history(h);
+    if (bestMs) *bestMs = best / (float)frames;
     return VRESTIR_OK;
 }
 

@@ -4,18 +4,10 @@
 
 namespace vr {
 
-// GenerateFeatures (VR/GenerateFeatures.cs.slang:57-102).
+// GenerateFeatures (VR/GenerateFeatures.cs.slang:57-102); the body is in vr_stages.hpp.
 __global__ __launch_bounds__(256) void kFeatures(const Scene* __restrict__ sc, const Frame fr,
                                                  float2* __restrict__ features) {
-    const Scene& S = *sc;
-    const Frame& F = fr;
-    occCacheLoad(S, F.occSlot[0], F.occSlot[1]);
-    PixelIdx px = pixelOf(F);
-    if (!px.ok || F.useReference) return;
-    Rng rng{0, 0, 0, 0};
-    Ray r{ld3(F.cam.pos), rayDir(F.cam, px.x, px.y, F.width, F.height), 0.f, kTMax};
-    float accT = featureMarch(S, r, 0, true, F.initialOpt.visibilityTStepScale, rng);
-    features[px.id] = make_float2(__int_as_float(1), accT);
+    featuresBody(sc, fr, features, launchBlock());
 }
 
 

@@ -105,6 +105,7 @@ static constexpr int kNodeWords = 64;           // block-occupancy bits kept in 
 static constexpr int kRayRefill = 16;           // lanes that must want a ray before a refill round
 static constexpr int kCarryLanes = 32;          // march lanes left when the rest of the march is carried over
 static constexpr int kBlockBatch = 16;          // lanes at a block border before their block entries run
+static constexpr int kCellsPerStep = 1;         // empty cells a walking lane may cross per walk iteration
 
 struct SegQueue {
     int rec[kRaySlots];  // the ray's transmittance slot
@@ -486,10 +487,9 @@ VR_DEV void segEnterBlock(FlatWalk& w, const WalkCtx& c) { w.enterBlock(c.g, c.n
 // (CellQueue, cell integration; launched only on a grid whose block bits fit LDS,
 // one grid and one filter for both ray kinds).
 template <int TK>
-__global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ sc, const Frame fr, Options opt,
-                                                      int isFinal, const TraceRec* __restrict__ recs,
-                                                      const int* __restrict__ counts, float* __restrict__ trs,
-                                                      int cap, int numPixels) {
+VR_DEV void traceSegmentsBody(const Scene* __restrict__ sc, const Frame& fr, const Options& opt, int isFinal,
+                              const TraceRec* __restrict__ recs, const int* __restrict__ counts,
+                              float* __restrict__ trs, int cap, int numPixels, int blockX) {
     const Scene& S = *sc;
     occCacheLoad(S, fr.occSlot[0], fr.occSlot[1]);
     __shared__ int pre[4][65];
@@ -498,7 +498,7 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
     __shared__ uint32_t nodeBits[kNodeWords];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint64_t below = (1ull << lane) - 1ull;
-    const int p0 = (blockIdx.x * 4 + wv) * 64;
+    const int p0 = (blockX * 4 + wv) * 64;
     const int cnt = p0 + lane < numPixels ? counts[p0 + lane] : 0;
     int incl = cnt;
 #pragma unroll
@@ -605,8 +605,13 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
                 i3v cell = {0, 0, 0};
                 const int rs = 2 * lane + bank;
                 if (w.active && !segBlockEntry(w)) {
-                    if constexpr (TK == 2) push = segStep(w, wc, bk, t0, q0);
-                    else push = w.stepCells(wc.g, bk, t0, cell);
+                    // up to kCellsPerStep cells while they hold no brick: the iteration's
+                    // ballots and queue bookkeeping are paid once for a run of empty cells
+                    for (int k = 0; k < kCellsPerStep; k++) {
+                        if constexpr (TK == 2) push = segStep(w, wc, bk, t0, q0);
+                        else push = w.stepCells(wc.g, bk, t0, cell);
+                        if (push || !w.active || segBlockEntry(w)) break;
+                    }
                 }
                 if (wasActive && !w.active) {
                     waitMask |= 1 << bank;
@@ -670,11 +675,57 @@ __global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ 
         trace(w);
     }
 }
+template <int TK>
+__global__ __launch_bounds__(256) void kTraceSegments(const Scene* __restrict__ sc, const Frame fr, Options opt,
+                                                      int isFinal, const TraceRec* __restrict__ recs,
+                                                      const int* __restrict__ counts, float* __restrict__ trs,
+                                                      int cap, int numPixels) {
+    traceSegmentsBody<TK>(sc, fr, opt, isFinal, recs, counts, trs, cap, numPixels, (int)blockIdx.x);
+}
 
 hipError_t launchTraceSegments(const LaunchArgs& a, const Options& opt, bool isFinal) {
     const int n = a.width * a.height;
     hipLaunchKernelGGL(kTraceSegments<2>, dim3((n + 255) / 256), dim3(256), a.occBytes, a.stream, a.scene, a.frame,
                        opt, isFinal ? 1 : 0, a.split.recs, a.split.counts, a.split.trs, a.split.cap, n);
+    return hipGetLastError();
+}
+
+// The segment traces of a batched step: work items are 256-pixel ranges of one
+// instance (its recorded rays), the same ranges a plain launch gives a workgroup.
+template <int TK>
+__global__ __launch_bounds__(256) void kBatchedSegments(const LaunchArgs* __restrict__ args,
+                                                        const BatchList* __restrict__ list,
+                                                        int* __restrict__ counter) {
+    __shared__ int sItem;
+    const int n = list->n, total = list->start[n], per = list->per;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) sItem = atomicAdd(counter, 1);
+        __syncthreads();
+        const int item = sItem;
+        if (item >= total) break;
+        int i = 0;
+        while (i + 1 < n && item >= list->start[i + 1]) i++;
+        const LaunchArgs& a = args[i];
+        const int t0 = (item - list->start[i]) * per, t1 = min(t0 + per, list->tiles[i]);
+        const bool fin = TK == 1;
+        for (int t = t0; t < t1; t++) {
+            if (t > t0) __syncthreads();
+            traceSegmentsBody<TK>(a.scene, a.frame, fin ? a.frame.finalOpt : a.frame.spatialOpt, fin ? 1 : 0,
+                                  a.split.recs, a.split.counts, a.split.trs, a.split.cap, a.width * a.height, t);
+        }
+    }
+}
+
+hipError_t launchBatchedSegments(hipStream_t st, int op, const LaunchArgs* args, const BatchList* list, int* counter,
+                                 int grid, int occBytes) {
+    if (grid <= 0) return hipErrorInvalidValue;
+    if (op == kOpSegmentsMarch)
+        hipLaunchKernelGGL(kBatchedSegments<2>, dim3(grid), dim3(256), occBytes, st, args, list, counter);
+    else if (op == kOpSegmentsCells)
+        hipLaunchKernelGGL(kBatchedSegments<1>, dim3(grid), dim3(256), occBytes, st, args, list, counter);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

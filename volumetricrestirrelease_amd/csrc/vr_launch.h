@@ -50,6 +50,34 @@ hipError_t launchTraceQueue(const LaunchArgs& a, const Options& opt, bool isFina
 hipError_t launchTraceSegments(const LaunchArgs& a, const Options& opt, bool isFinal);
 // The same for analytic tracking (kTraceSegments<1>): one grid whose block bits fit LDS, one filter.
 hipError_t launchTraceCells(const LaunchArgs& a, const Options& opt, bool isFinal);
+// ---- batched launches: many passes' tiles in one persistent launch
+// The work list of one batched step: instance i owns work items [start[i],
+// start[i+1]); an item is `per` consecutive tiles of that instance, tiles
+// row-major over its tile grid of gx columns (16x16-pixel tiles for the stage
+// bodies, 256-pixel ranges for the segment traces).
+constexpr int kMaxBatch = 64;
+struct BatchList {
+    int n, per;
+    int start[kMaxBatch + 1];
+    int tiles[kMaxBatch];
+    int gx[kMaxBatch];
+};
+// The bodies a batched step runs; every instance of a batch shares the path
+// length and the tracker kinds that select them.
+enum BatchOp {
+    kOpFeatures = 0, kOpTrace, kOpTemporalRecord, kOpTemporalReplay, kOpCopyFeatures, kOpSpatialRecord,
+    kOpSpatialReplay, kOpCopyReservoirs, kOpFinalRecord, kOpFinalReplay, kOpSegmentsMarch, kOpSegmentsCells,
+    kOpCount
+};
+// One persistent launch of `grid` workgroups draining the step's work list
+// (counter: the step's queue head, zero before the launch). args and list are
+// device memory; occBytes is the largest dynamic LDS any instance needs.
+hipError_t launchBatched(hipStream_t st, int op, int maxBounces, const LaunchArgs* args, const BatchList* list,
+                         int* counter, int grid, int occBytes);
+hipError_t launchBatchedSegments(hipStream_t st, int op, const LaunchArgs* args, const BatchList* list, int* counter,
+                                 int grid, int occBytes);
+hipError_t launchZeroCounters(hipStream_t st, int* counters, int n);
+
 // Ground-truth accumulation and relMSE (vr_k_accum.hip).
 hipError_t launchAccumulate(hipStream_t st, const float4* frame, double4* sum, int n, bool reset);
 hipError_t launchResolve(hipStream_t st, const double4* sum, double inv, float4* out, int n);

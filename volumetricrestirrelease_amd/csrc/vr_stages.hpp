@@ -15,11 +15,17 @@
 
 namespace vr {
 
+// The tile a workgroup works on: its own block in a plain launch, or a work item
+// a batched launch took from its queue. The stage bodies below take it as an
+// argument; each kernel is a thin wrapper that passes blockIdx.
+struct VBlock { int x, y; };
+VR_DEV VBlock launchBlock() { return {(int)blockIdx.x, (int)blockIdx.y}; }
+
 struct PixelIdx { int x, y, id; bool ok; };
-VR_DEV PixelIdx pixelOf(const Frame& F) {
+VR_DEV PixelIdx pixelOf(const Frame& F, VBlock b) {
     PixelIdx p;
-    p.x = blockIdx.x * 16 + (threadIdx.x & 15);
-    p.y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    p.x = b.x * 16 + (threadIdx.x & 15);
+    p.y = b.y * 16 + (threadIdx.x >> 4);
     p.ok = p.x < F.width && p.y < F.height;
     p.id = p.y * F.width + p.x;
     return p;
@@ -30,6 +36,12 @@ VR_DEV bool inRange(int x, int y, int W, int H) { return x >= 0 && x < W && y >=
 // loads); four resident waves per SIMD (<= 128 VGPRs) hide more of that
 // latency than three with fewer spills.
 #define VR_TRACING_KERNEL __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+// TraceRays' own occupancy target (resident waves per SIMD): its candidate loop
+// keeps more state live than the reuse kernels.
+#ifndef VR_TRACE_WAVES
+#define VR_TRACE_WAVES 4
+#endif
+#define VR_TRACE_KERNEL __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VR_TRACE_WAVES, 8)))
 
 static inline dim3 tiles(int W, int H) { return dim3((W + 15) / 16, (H + 15) / 16, 1); }
 
@@ -42,6 +54,20 @@ static inline dim3 tiles(int W, int H) { return dim3((W + 15) / 16, (H + 15) / 1
         case 4: { constexpr int B = 4; body; } break;            \
         default: return hipErrorInvalidValue;                    \
     }
+
+// GenerateFeatures (VR/GenerateFeatures.cs.slang:57-102): the camera ray's
+// transmittance (feature march) and the "no reflective surface" flag.
+VR_DEV void featuresBody(const Scene* __restrict__ sc, const Frame& fr, float2* __restrict__ features, VBlock vb) {
+    const Scene& S = *sc;
+    const Frame& F = fr;
+    occCacheLoad(S, F.occSlot[0], F.occSlot[1]);
+    PixelIdx px = pixelOf(F, vb);
+    if (!px.ok || F.useReference) return;
+    Rng rng{0, 0, 0, 0};
+    Ray r{ld3(F.cam.pos), rayDir(F.cam, px.x, px.y, F.width, F.height), 0.f, kTMax};
+    float accT = featureMarch(S, r, 0, true, F.initialOpt.visibilityTStepScale, rng);
+    features[px.id] = make_float2(__int_as_float(1), accT);
+}
 
 // IntegrateByVolumePathTracing (VR/VolumePathTracingFunctions.slang:3-132), NEE on.
 VR_DEV v3 pathTrace(const Scene& S, const Frame& Fr, Ray ray, Rng& rng, int lightSamples, int maxBounces) {
@@ -79,12 +105,11 @@ VR_DEV v3 pathTrace(const Scene& S, const Frame& Fr, Ray ray, Rng& rng, int ligh
 
 // TraceRays in reference mode (VR/TraceRays.cs.slang:85-103): the volume path tracer.
 template <int B>
-__global__ __launch_bounds__(256) void kReference(const Scene* __restrict__ sc, const Frame fr,
-                                                  float4* __restrict__ outColor) {
+VR_DEV void referenceBody(const Scene* __restrict__ sc, const Frame& fr, float4* __restrict__ outColor, VBlock vb) {
     const Scene& S = *sc;
     const Frame& F = fr;
     occCacheLoad(S, F.occSlot[0], F.occSlot[1]);
-    PixelIdx px = pixelOf(F);
+    PixelIdx px = pixelOf(F, vb);
     if (!px.ok) return;
     Rng rng = rngCreate(px.x, px.y, (uint32_t)(F.numTotalRounds * F.frameCount));
     Ray ray{ld3(F.cam.pos), rayDir(F.cam, px.x, px.y, F.width, F.height), 0.f, kTMax};
@@ -94,6 +119,11 @@ __global__ __launch_bounds__(256) void kReference(const Scene* __restrict__ sc, 
     avg = avg / (float)F.baselineSpp;
     outColor[px.id] = make_float4(avg.x, avg.y, avg.z, 1.f);
 }
+template <int B>
+__global__ __launch_bounds__(256) void kReference(const Scene* __restrict__ sc, const Frame fr,
+                                                  float4* __restrict__ outColor) {
+    referenceBody<B>(sc, fr, outColor, launchBlock());
+}
 
 // TraceRays (VR/TraceRays.cs.slang:64-201). IM: initial lighting tracking
 // method; VM / LM: the spatial options' methods used by the final p-hat.
@@ -101,13 +131,12 @@ __global__ __launch_bounds__(256) void kReference(const Scene* __restrict__ sc, 
 // CDF over c drives the analytic tracker's optical-depth targets) and one pixel
 // per block writes its candidates' weights as a training record.
 template <int B, int IM, int VM, int LM, bool LEARNED = false>
-VR_TRACING_KERNEL void kTrace(const Scene* __restrict__ sc, const Frame fr,
-                                              ResStore* __restrict__ outRes, float* __restrict__ outExtra,
-                                              const LearnedTraceArgs la) {
+VR_DEV void traceBody(const Scene* __restrict__ sc, const Frame& fr, ResStore* __restrict__ outRes,
+                      float* __restrict__ outExtra, const LearnedTraceArgs& la, VBlock vb) {
     const Scene& S = *sc;
     const Frame& F = fr;
     occCacheLoad(S, F.occSlot[0], F.occSlot[1]);
-    PixelIdx px = pixelOf(F);
+    PixelIdx px = pixelOf(F, vb);
     if (!px.ok) return;
     Rng rng = rngCreate(px.x, px.y, (uint32_t)(F.numTotalRounds * F.frameCount));
     Ray ray{ld3(F.cam.pos), rayDir(F.cam, px.x, px.y, F.width, F.height), 0.f, kTMax};
@@ -211,6 +240,11 @@ VR_TRACING_KERNEL void kTrace(const Scene* __restrict__ sc, const Frame fr,
     }
     storeRes(outRes, px.id, fin);
 }
+template <int B, int IM, int VM, int LM, bool LEARNED = false>
+VR_TRACE_KERNEL void kTrace(const Scene* __restrict__ sc, const Frame fr, ResStore* __restrict__ outRes,
+                              float* __restrict__ outExtra, const LearnedTraceArgs la) {
+    traceBody<B, IM, VM, LM, LEARNED>(sc, fr, outRes, outExtra, la, launchBlock());
+}
 
 VR_DEV float2 featuresAt(const float2* f, int id, int n) {
     if (id < 0 || id >= n) return make_float2(0.f, 0.f);  // out-of-range loads read 0
@@ -231,17 +265,15 @@ using TracePolicy = std::conditional_t<MODE == kDirect, DirectTrace<VM, LM>,
 // trackers draw no random numbers, so resampling can be left out); kReplay reads
 // their transmittances back in the same order.
 template <int B, int VM, int LM, int MODE = kDirect>
-VR_TRACING_KERNEL void kTemporal(const Scene* __restrict__ sc, const Frame fr,
-                                                 ResStore* __restrict__ cur, float* __restrict__ curExtra,
-                                                 const ResStore* __restrict__ temporal,
-                                                 const float* __restrict__ temporalExtra,
-                                                 const float2* __restrict__ features,
-                                                 const float2* __restrict__ temporalFeatures,
-                                                 float2* __restrict__ mvec, SplitBufs sb) {
+VR_DEV void temporalBody(const Scene* __restrict__ sc, const Frame& fr, ResStore* __restrict__ cur,
+                         float* __restrict__ curExtra, const ResStore* __restrict__ temporal,
+                         const float* __restrict__ temporalExtra, const float2* __restrict__ features,
+                         const float2* __restrict__ temporalFeatures, float2* __restrict__ mvec, const SplitBufs& sb,
+                         VBlock vb) {
     const Scene& S = *sc;
     const Frame& F = fr;
     occCacheLoad(S, F.occSlot[0], F.occSlot[1]);
-    PixelIdx px = pixelOf(F);
+    PixelIdx px = pixelOf(F, vb);
     if (!px.ok) return;
     TracePolicy<MODE, VM, LM> tp;
     if constexpr (MODE == kRecord) { tp.out = sb.recs + (size_t)px.id * sb.cap; tp.n = 0; tp.dens = sb.dens + px.id; tp.dstride = (size_t)F.width * F.height; tp.nd = 0; sb.counts[px.id] = 0; }
@@ -395,6 +427,15 @@ VR_TRACING_KERNEL void kTemporal(const Scene* __restrict__ sc, const Frame fr,
     }
     storeRes(cur, px.id, output);
 }
+template <int B, int VM, int LM, int MODE = kDirect>
+VR_TRACING_KERNEL void kTemporal(const Scene* __restrict__ sc, const Frame fr, ResStore* __restrict__ cur,
+                                 float* __restrict__ curExtra, const ResStore* __restrict__ temporal,
+                                 const float* __restrict__ temporalExtra, const float2* __restrict__ features,
+                                 const float2* __restrict__ temporalFeatures, float2* __restrict__ mvec,
+                                 SplitBufs sb) {
+    temporalBody<B, VM, LM, MODE>(sc, fr, cur, curExtra, temporal, temporalExtra, features, temporalFeatures, mvec, sb,
+                                  launchBlock());
+}
 
 // SpatialReuse (VR/SpatialReuse.cs.slang:94-265) for one round. MODE: the
 // direct form, or the record / replay passes around a balanced trace of the
@@ -403,14 +444,14 @@ VR_TRACING_KERNEL void kTemporal(const Scene* __restrict__ sc, const Frame fr,
 // where the direct form would have made them (the record pass decides with
 // every visibility 1, a superset).
 template <int B, int VM, int LM, int MODE>
-VR_TRACING_KERNEL void kSpatial(const Scene* __restrict__ sc, const Frame fr, int roundId,
-                                const int* __restrict__ offsets, const ResStore* __restrict__ in,
-                                ResStore* __restrict__ out, const float* __restrict__ inExtra,
-                                float* __restrict__ outExtra, const float2* __restrict__ features, SplitBufs sb) {
+VR_DEV void spatialBody(const Scene* __restrict__ sc, const Frame& fr, int roundId, const int* __restrict__ offsets,
+                        const ResStore* __restrict__ in, ResStore* __restrict__ out, const float* __restrict__ inExtra,
+                        float* __restrict__ outExtra, const float2* __restrict__ features, const SplitBufs& sb,
+                        VBlock vb) {
     const Scene& S = *sc;
     const Frame& F = fr;
     occCacheLoad(S, F.occSlot[0], F.occSlot[1]);
-    PixelIdx px = pixelOf(F);
+    PixelIdx px = pixelOf(F, vb);
     if (!px.ok) return;
     const int W = F.width, H = F.height;
     const int roundOffset = (F.enableTemporal ? 1 : 0) + 1;
@@ -513,14 +554,20 @@ VR_TRACING_KERNEL void kSpatial(const Scene* __restrict__ sc, const Frame fr, in
     if (B > 1) output.exStart = centerEx;
     storeRes(out, px.id, output);
 }
+template <int B, int VM, int LM, int MODE>
+VR_TRACING_KERNEL void kSpatial(const Scene* __restrict__ sc, const Frame fr, int roundId,
+                                const int* __restrict__ offsets, const ResStore* __restrict__ in,
+                                ResStore* __restrict__ out, const float* __restrict__ inExtra,
+                                float* __restrict__ outExtra, const float2* __restrict__ features, SplitBufs sb) {
+    spatialBody<B, VM, LM, MODE>(sc, fr, roundId, offsets, in, out, inExtra, outExtra, features, sb, launchBlock());
+}
 
 // CopyReservoirs (VR/CopyReservoirs.cs.slang:84-95).
 template <int B>
-__global__ __launch_bounds__(256) void kCopy(const Frame fr, const ResStore* __restrict__ cur,
-                                             const float* __restrict__ curExtra, ResStore* __restrict__ temporal,
-                                             float* __restrict__ temporalExtra) {
+VR_DEV void copyBody(const Frame& fr, const ResStore* __restrict__ cur, const float* __restrict__ curExtra,
+                     ResStore* __restrict__ temporal, float* __restrict__ temporalExtra, VBlock vb) {
     const Frame& F = fr;
-    PixelIdx px = pixelOf(F);
+    PixelIdx px = pixelOf(F, vb);
     if (!px.ok) return;
     ResStore r = cur[px.id];
     temporal[px.id] = r;
@@ -535,19 +582,24 @@ __global__ __launch_bounds__(256) void kCopy(const Frame fr, const ResStore* __r
         }
     }
 }
+template <int B>
+__global__ __launch_bounds__(256) void kCopy(const Frame fr, const ResStore* __restrict__ cur,
+                                             const float* __restrict__ curExtra, ResStore* __restrict__ temporal,
+                                             float* __restrict__ temporalExtra) {
+    copyBody<B>(fr, cur, curExtra, temporal, temporalExtra, launchBlock());
+}
 
 // FinalShading (VR/FinalShading.cs.slang:71-141). MODE: kDirect traces each
 // transmittance ray on the spot; kRecord writes them to sb (and nothing else);
 // kReplay reads their transmittances back in the same order.
 template <int B, int VM, int LM, int MODE = kDirect>
-__global__ __launch_bounds__(256) void kFinal(const Scene* __restrict__ sc, const Frame fr,
-                                              const ResStore* __restrict__ cur, const float* __restrict__ curExtra,
-                                              const float2* __restrict__ features, const float4* __restrict__ colorIn,
-                                              float4* __restrict__ out, SplitBufs sb) {
+VR_DEV void finalBody(const Scene* __restrict__ sc, const Frame& fr, const ResStore* __restrict__ cur,
+                      const float* __restrict__ curExtra, const float2* __restrict__ features,
+                      const float4* __restrict__ colorIn, float4* __restrict__ out, const SplitBufs& sb, VBlock vb) {
     const Scene& S = *sc;
     const Frame& F = fr;
     occCacheLoad(S, F.occSlot[0], F.occSlot[1]);
-    PixelIdx px = pixelOf(F);
+    PixelIdx px = pixelOf(F, vb);
     if (!px.ok) return;
     Rng rng = rngCreate(px.x, px.y, (uint32_t)(F.numTotalRounds * F.frameCount + F.numTotalRounds - 1));
     TracePolicy<MODE, VM, LM> tp;
@@ -576,6 +628,13 @@ __global__ __launch_bounds__(256) void kFinal(const Scene* __restrict__ sc, cons
     if (isnan(c.x) || isnan(c.y) || isnan(c.z) || isinf(c.x) || isinf(c.y) || isinf(c.z))
         col = make_float4(0.f, 0.f, 0.f, 0.f);
     out[px.id] = col;
+}
+template <int B, int VM, int LM, int MODE = kDirect>
+__global__ __launch_bounds__(256) void kFinal(const Scene* __restrict__ sc, const Frame fr,
+                                              const ResStore* __restrict__ cur, const float* __restrict__ curExtra,
+                                              const float2* __restrict__ features, const float4* __restrict__ colorIn,
+                                              float4* __restrict__ out, SplitBufs sb) {
+    finalBody<B, VM, LM, MODE>(sc, fr, cur, curExtra, features, colorIn, out, sb, launchBlock());
 }
 
 

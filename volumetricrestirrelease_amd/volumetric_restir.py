@@ -319,3 +319,64 @@ RESERVOIR_DTYPE = np.dtype([("runningSum", "<f4"), ("M", "<f4"), ("depth", "<f4"
                             ("lightUV", "<f4", 2), ("lightID", "<i4"), ("sampledPixel", "<i4"),
                             ("extraBounceStartId", "<i4")])
 FEATURES_DTYPE = np.dtype([("noReflectiveSurface", "<i4"), ("transmittance", "<f4")])
+
+
+class VolumetricReSTIRBatch:
+    """Many passes rendered together: each step of a frame is one persistent launch
+    over every pass's tiles, the frame's launches one replayed HIP graph
+    (vrestir_batch_*; no counterpart in the reference)."""
+
+    def __init__(self, passes):
+        self.lib = abi.load_library()
+        self.passes = list(passes)
+        arr = (ctypes.c_void_p * len(self.passes))(*[p.h.value for p in self.passes])
+        b = ctypes.c_void_p()
+        rc = self.lib.vrestir_batch_create(arr, len(self.passes), ctypes.byref(b))
+        if rc != 0:
+            raise VolumetricReSTIRError(f"vrestir_batch_create: {self.lib.vrestir_last_error(None).decode()} ({rc})")
+        self.b = b
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise VolumetricReSTIRError(f"{what}: {self.lib.vrestir_batch_last_error(self.b).decode()} (status {rc})")
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.lib.vrestir_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _views(self, outs):
+        n = len(self.passes)
+        W = (ctypes.c_int32 * n)(*[int(o.shape[1]) for o in outs])
+        H = (ctypes.c_int32 * n)(*[int(o.shape[0]) for o in outs])
+        O = (ctypes.c_void_p * n)(*[o.data_ptr() for o in outs])
+        return W, H, O
+
+    def execute(self, outs, stream=None):
+        """One frame of every pass into outs[i] (float32 [H_i, W_i, 4] device tensors)."""
+        W, H, O = self._views(outs)
+        self._check(self.lib.vrestir_batch_execute(self.b, _stream_ptr(stream), W, H, O), "batch_execute")
+
+    def setTuning(self, grid, tiles_per_item, use_graph=True):
+        self._check(self.lib.vrestir_batch_set_tuning(self.b, grid, tiles_per_item, 1 if use_graph else 0),
+                    "batch_set_tuning")
+
+    def tuning(self):
+        g, p = ctypes.c_int32(), ctypes.c_int32()
+        self._check(self.lib.vrestir_batch_get_tuning(self.b, ctypes.byref(g), ctypes.byref(p)), "batch_get_tuning")
+        return g.value, p.value
+
+    def autotune(self, outs, frames=2, stream=None):
+        """Times the candidate queue shapes, keeps the fastest and resets every pass's
+        history; returns its milliseconds per frame."""
+        W, H, O = self._views(outs)
+        ms = ctypes.c_float()
+        self._check(self.lib.vrestir_batch_autotune(self.b, _stream_ptr(stream), W, H, O, frames, ctypes.byref(ms)),
+                    "batch_autotune")
+        return ms.value
