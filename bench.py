@@ -21,6 +21,12 @@ How the headline number is made:
    device synchronisation; `value` = K / elapsed. The plain M = 8 pass is timed the
    same way and reported beside it.
 
+3. Batched: after the config1 timing, `batched` reports config2 (16 seeded views
+   and volumes of uneven size, one persistent launch per step over all of them, the
+   frame a replayed HIP graph) against the same passes rendered one after another
+   (`--config config2` prints that measurement as its own line; `--no-batched`
+   leaves it out).
+
 `roofline` is for the kernel family that takes the most time per frame, with its
 compulsory per-pixel buffer bytes plus the brick pools of the grids it traverses as
 algorithmic bytes; `cpu_baseline` times the CPU oracle on bands of rows of the
@@ -199,11 +205,14 @@ def cpu_baseline(spec, threads):
                       f"({m1[0]:.2f} s per band frame); host reports {nproc} CPUs, the GPU's share is 16"}
 
 
-def bench_batched(args):
+def bench_batched(args, steps=None, warmup=None, emit=True):
     """config2: 16 seeded views and volumes of uneven size. One step = one frame of
     every view. Timed twice over the same frames: the batch (one persistent launch per
     step, the frame a replayed HIP graph, queue shape from the autotuner) and the
-    passes executed one after another on one stream."""
+    passes executed one after another on one stream. Prints its own line (emit) or
+    returns the measurement."""
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     import torch
 
     from volumetricrestirrelease_amd import scenes
@@ -227,7 +236,7 @@ def bench_batched(args):
 
     def run(batched, first):
         frame = first
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             for g, spec in zip(passes, specs):
                 g.setCamera(spec, frame)
             if batched:
@@ -238,7 +247,7 @@ def bench_batched(args):
             frame += 1
         torch.cuda.synchronize()
         t = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             for g, spec in zip(passes, specs):
                 g.setCamera(spec, frame)
             if batched:
@@ -254,14 +263,16 @@ def bench_batched(args):
     for g in passes:
         g.resetHistory()
     el_s = run(False, 0)
+    b.close()
+    del passes
     line = {
         "metric": "config2 batched frames/s (every view, one frame each)",
-        "value": round(args.steps / el_b, 4),
+        "value": round(steps / el_b, 4),
         "unit": "frames/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(1000.0 * el_b / args.steps, 3),
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(1000.0 * el_b / steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -272,13 +283,15 @@ def bench_batched(args):
                                                 for s in specs),
             "pixels_per_step": px,
             "autotuned": {"persistent_workgroups": grid, "tiles_per_item": per, "ms_per_frame": round(tune_ms, 3)},
-            "one_after_another": {"value": round(args.steps / el_s, 4),
-                                  "ms_per_step": round(1000.0 * el_s / args.steps, 3)},
+            "one_after_another": {"value": round(steps / el_s, 4),
+                                  "ms_per_step": round(1000.0 * el_s / steps, 3)},
             "speedup": round(el_s / el_b, 3),
             "volume_setup_s": round(setup_s, 2),
         },
     }
-    print(json.dumps(line), flush=True)
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
 
 
 def main():
@@ -296,6 +309,8 @@ def main():
                     help="time the learned-proposal pass at the config's M whatever the quality runs find "
                          "(profiling its network kernels)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="leave out the config2 batched measurement that follows the config1 line's timing")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a known-size kernel that "
@@ -433,6 +448,16 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not args.no_batched and args.config == "config1":
+        del g
+        torch.cuda.synchronize()
+        bl = bench_batched(args, steps=5, warmup=2, emit=False)
+        line["batched"] = {"workload": bl["config"]["workload"], "frames_per_s": bl["value"],
+                           "ms_per_step": bl["ms_per_step"], "steps": bl["steps"],
+                           "pixels_per_step": bl["config"]["pixels_per_step"],
+                           "autotuned": bl["config"]["autotuned"],
+                           "one_after_another": bl["config"]["one_after_another"],
+                           "speedup": bl["config"]["speedup"]}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(spec, args.cpu_threads)
     print(json.dumps(line), flush=True)

@@ -22,16 +22,25 @@ def batch_specs():
     return scenes.config2(seed=5, count=5, min_res=24, max_res=112, min_grid=64, max_grid=128, max_lights=8)
 
 
-def singles(specs):
+def make_pass(spec, setup=None):
+    g = VolumetricReSTIR()
+    g.setScene(spec)
+    if setup:
+        setup(g)
+    return g
+
+
+def singles(specs, setup=None, before_frame=None):
     import torch
     out = []
-    for spec in specs:
-        g = VolumetricReSTIR()
-        g.setScene(spec)
+    for i, spec in enumerate(specs):
+        g = make_pass(spec, setup(i) if setup else None)
         img = torch.zeros((spec.height, spec.width, 4), device="cuda")
         frames = []
         for f in range(FRAMES):
             g.setCamera(spec, f)
+            if before_frame:
+                before_frame(g, f)
             g.execute(img)
             torch.cuda.synchronize()
             frames.append(img.cpu().numpy().copy())
@@ -39,13 +48,9 @@ def singles(specs):
     return out
 
 
-def batched(specs, tuning=None, autotune=False):
+def batched(specs, tuning=None, autotune=False, setup=None, before_frame=None):
     import torch
-    passes = []
-    for spec in specs:
-        g = VolumetricReSTIR()
-        g.setScene(spec)
-        passes.append(g)
+    passes = [make_pass(spec, setup(i) if setup else None) for i, spec in enumerate(specs)]
     b = VolumetricReSTIRBatch(passes)
     outs = [torch.zeros((s.height, s.width, 4), device="cuda") for s in specs]
     if tuning:
@@ -59,6 +64,8 @@ def batched(specs, tuning=None, autotune=False):
     for f in range(FRAMES):
         for g, spec in zip(passes, specs):
             g.setCamera(spec, f)
+            if before_frame:
+                before_frame(g, f)
         b.execute(outs)
         torch.cuda.synchronize()
         for i, o in enumerate(outs):
@@ -114,4 +121,48 @@ def test_batch_refuses_passes_with_other_kernels():
     b = VolumetricReSTIRBatch(passes)
     outs = [torch.zeros((s.height, s.width, 4), device="cuda") for s in specs]
     with pytest.raises(VolumetricReSTIRError, match="kernel options"):
+        b.execute(outs)
+
+
+def test_batch_of_learned_passes_matches_passes_rendered_alone():
+    """Each pass proposes with and trains its own network (per-pass launches of the
+    network stages between the batched steps, no graph)."""
+    specs = batch_specs()
+    say("case: batch of 5 passes with the learned proposal on (online training), 64 x 2, 3 frames")
+
+    def setup(i):
+        return lambda g: g.setLearned(enable=1, judge=0, trainBlockX=8, trainBlockY=4, seed=11 + i)
+
+    compare(singles(specs, setup=setup), batched(specs, tuning=(64, 2, True), setup=setup), specs)
+
+
+def test_batch_of_animated_passes_matches_passes_rendered_alone():
+    """Moving volumes with previous-frame grids in temporal reuse: the batch runs the
+    one-kernel temporal form for every pass."""
+    seqs = [scenes.moving_sequence(res=32, width=40, height=32, seed=9),
+            scenes.moving_sequence(res=32, shift=1, width=56, height=24, seed=4)]
+    specs = [s0 for s0, _ in seqs]
+    say("case: batch of 2 moving 32^3 sequences (40x32, 56x24) with usePrevVolumeForReproj, 3 frames")
+
+    def setup(i):
+        def add(g):
+            for s in seqs[i][1][1:]:
+                g.addVolumeFrame(s)
+        return add
+
+    def advance(g, f):
+        g.setVolumeFrame(f, animating=f > 0)
+
+    compare(singles(specs, setup=setup, before_frame=advance),
+            batched(specs, tuning=(64, 1, True), setup=setup, before_frame=advance), specs)
+
+
+def test_batch_refuses_mixed_learned_passes():
+    import torch
+    say("case: a learned and a plain pass in one batch")
+    specs = batch_specs()[:2]
+    passes = [make_pass(specs[0], lambda g: g.setLearned(enable=1)), make_pass(specs[1])]
+    b = VolumetricReSTIRBatch(passes)
+    outs = [torch.zeros((s.height, s.width, 4), device="cuda") for s in specs]
+    with pytest.raises(VolumetricReSTIRError, match="learned proposal"):
         b.execute(outs)

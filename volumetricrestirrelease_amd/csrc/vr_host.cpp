@@ -1661,24 +1661,32 @@ static int batchFrame(vrestir_batch* b, hipStream_t user, const int32_t* widths,
                       float* const* outs, bool allowGraph) {
     const int n = (int)b->passes.size();
     const vrestir_params& p0 = b->passes[0]->P;
+    const bool learned = learnedActive(b->passes[0]);
     for (int i = 0; i < n; i++) {
         vrestir_handle* h = b->passes[i];
         if (!widths || !heights || !outs || !outs[i]) return bfail(b, VRESTIR_EINVAL, "sizes and outputs for every pass");
         if (!sameKernelOptions(h->P, p0)) return bfail(b, VRESTIR_EUNSUPPORTED, "passes differ in kernel options");
-        if (!h->P.enableTemporalReuse || !h->P.enableSpatialReuse || h->P.useReference || learnedActive(h) ||
-            h->freezeFrame || h->splitReuse != 2)
-            return bfail(b, VRESTIR_EUNSUPPORTED,
-                         "batches need temporal and spatial reuse, split stages, no learned proposal or reference");
+        if (learnedActive(h) != learned)
+            return bfail(b, VRESTIR_EUNSUPPORTED, "passes differ in whether the learned proposal is on");
+        if (!h->P.enableTemporalReuse || !h->P.enableSpatialReuse || h->P.useReference || h->freezeFrame ||
+            h->splitReuse != 2)
+            return bfail(b, VRESTIR_EUNSUPPORTED, "batches need temporal and spatial reuse, split stages, no reference");
     }
+    // the learned proposal's Adam step takes host-computed bias corrections, so
+    // frames with it run their launches directly instead of replaying a graph
+    if (learned) allowGraph = false;
     BHIP(b, hipSetDevice(b->device));
     // the steps of the frame
+    // steps with op < 0 are per-pass launches of the learned stages (-1 proposal, -2 training)
     struct Step { int op, stage, round; };
-    std::vector<Step> steps = {{kOpFeatures, VRESTIR_STAGE_GENERATE_FEATURES, 0},
-                               {kOpTrace, VRESTIR_STAGE_TRACE_RAYS, 0},
-                               {kOpTemporalRecord, VRESTIR_STAGE_TEMPORAL_REUSE, 0},
-                               {kOpSegmentsMarch, VRESTIR_STAGE_TEMPORAL_REUSE, 0},
-                               {kOpTemporalReplay, VRESTIR_STAGE_TEMPORAL_REUSE, 0},
-                               {kOpCopyFeatures, VRESTIR_STAGE_TEMPORAL_REUSE, 0}};
+    std::vector<Step> steps = {{kOpFeatures, VRESTIR_STAGE_GENERATE_FEATURES, 0}};
+    if (learned) steps.push_back({-1, VRESTIR_STAGE_LEARNED_PROPOSAL, 0});
+    steps.push_back({learned ? kOpTraceLearned : kOpTrace, VRESTIR_STAGE_TRACE_RAYS, 0});
+    if (learned) steps.push_back({-2, VRESTIR_STAGE_LEARNED_TRAIN, 0});
+    steps.push_back({kOpTemporalRecord, VRESTIR_STAGE_TEMPORAL_REUSE, 0});
+    steps.push_back({kOpSegmentsMarch, VRESTIR_STAGE_TEMPORAL_REUSE, 0});
+    steps.push_back({kOpTemporalReplay, VRESTIR_STAGE_TEMPORAL_REUSE, 0});
+    steps.push_back({kOpCopyFeatures, VRESTIR_STAGE_TEMPORAL_REUSE, 0});
     for (int r = 0; r < p0.spatialReuseRounds; r++) {
         steps.push_back({kOpSpatialRecord, VRESTIR_STAGE_SPATIAL_REUSE, r});
         steps.push_back({kOpSegmentsMarch, VRESTIR_STAGE_SPATIAL_REUSE, r});
@@ -1688,19 +1696,20 @@ static int batchFrame(vrestir_batch* b, hipStream_t user, const int32_t* widths,
     steps.push_back({kOpFinalRecord, VRESTIR_STAGE_FINAL_SHADING, 0});
     steps.push_back({kOpSegmentsCells, VRESTIR_STAGE_FINAL_SHADING, 0});
     steps.push_back({kOpFinalReplay, VRESTIR_STAGE_FINAL_SHADING, 0});
-    const int S = (int)steps.size();
-    if (S > b->stepCap) {
+    int S = (int)steps.size();
+    const int stepCap = S;  // the most steps a frame has (the temporal one-kernel form has fewer)
+    if (stepCap > b->stepCap) {
         batchFreeGraph(b);
         dfree(b->argsDev); dfree(b->listsDev); dfree(b->countersDev);
         if (b->argsHost) (void)hipHostFree(b->argsHost);
         if (b->listsHost) (void)hipHostFree(b->listsHost);
         b->argsHost = nullptr; b->listsHost = nullptr; b->stepCap = 0;
-        BHIP(b, hipMalloc((void**)&b->argsDev, sizeof(LaunchArgs) * (size_t)S * kMaxBatch));
-        BHIP(b, hipMalloc((void**)&b->listsDev, sizeof(BatchList) * (size_t)S));
-        BHIP(b, hipMalloc((void**)&b->countersDev, sizeof(int) * (size_t)S));
-        BHIP(b, hipHostMalloc((void**)&b->argsHost, sizeof(LaunchArgs) * (size_t)S * kMaxBatch));
-        BHIP(b, hipHostMalloc((void**)&b->listsHost, sizeof(BatchList) * (size_t)S));
-        b->stepCap = S;
+        BHIP(b, hipMalloc((void**)&b->argsDev, sizeof(LaunchArgs) * (size_t)stepCap * kMaxBatch));
+        BHIP(b, hipMalloc((void**)&b->listsDev, sizeof(BatchList) * (size_t)stepCap));
+        BHIP(b, hipMalloc((void**)&b->countersDev, sizeof(int) * (size_t)stepCap));
+        BHIP(b, hipHostMalloc((void**)&b->argsHost, sizeof(LaunchArgs) * (size_t)stepCap * kMaxBatch));
+        BHIP(b, hipHostMalloc((void**)&b->listsHost, sizeof(BatchList) * (size_t)stepCap));
+        b->stepCap = stepCap;
     }
     // the previous frame's staging may still be in flight
     BHIP(b, hipStreamSynchronize(b->stream));
@@ -1713,17 +1722,37 @@ static int batchFrame(vrestir_batch* b, hipStream_t user, const int32_t* widths,
     // arguments of an earlier step of this frame pointing at freed buffers. Run
     // every step's plan once, so the buffers hold the frame's largest capacity,
     // then build the arguments that are launched.
+    int temporalTK = -1;
     {
         std::vector<int> rounds(n);
         for (int i = 0; i < n; i++) rounds[i] = b->passes[i]->totalRoundId;
-        for (const Step& sp : steps)
+        for (const Step& sp : steps) {
+            if (sp.op < 0) continue;
             for (int i = 0; i < n; i++) {
                 LaunchArgs tmp;
                 int rc = stageArgs(b->passes[i], b->stream, sp.stage, sp.round, nullptr, tmp);
                 if (rc) return bfail(b, rc, std::string("pass ") + std::to_string(i) + ": " + b->passes[i]->err);
                 if (sp.op == kOpSpatialReplay) b->passes[i]->totalRoundId++;
+                if (sp.op == kOpTemporalRecord) {
+                    if (temporalTK >= 0 && temporalTK != tmp.splitTK)
+                        return bfail(b, VRESTIR_EUNSUPPORTED, "passes differ in the form of their temporal reuse");
+                    temporalTK = tmp.splitTK;
+                }
             }
+        }
         for (int i = 0; i < n; i++) b->passes[i]->totalRoundId = rounds[i];
+    }
+    // temporal reuse on the previous frame's grids (an animated volume) runs as one kernel
+    if (temporalTK == 0) {
+        std::vector<Step> s2;
+        for (const Step& sp : steps) {
+            if (sp.op == kOpTemporalRecord) s2.push_back({kOpTemporalDirect, VRESTIR_STAGE_TEMPORAL_REUSE, 0});
+            else if (!(sp.stage == VRESTIR_STAGE_TEMPORAL_REUSE &&
+                       (sp.op == kOpSegmentsMarch || sp.op == kOpTemporalReplay)))
+                s2.push_back(sp);
+        }
+        steps.swap(s2);
+        S = (int)steps.size();
     }
     b->ops.assign(S, 0);
     b->occ.assign(S, 0);
@@ -1734,17 +1763,20 @@ static int batchFrame(vrestir_batch* b, hipStream_t user, const int32_t* widths,
         std::memset(&L, 0, sizeof(L));
         L.n = n;
         L.per = b->per;
+        b->ops[k] = sp.op;
+        key.push_back(sp.op);
+        if (sp.op < 0) continue;
         for (int i = 0; i < n; i++) {
             vrestir_handle* h = b->passes[i];
             LaunchArgs& a = b->argsHost[(size_t)k * kMaxBatch + i];
             int rc = stageArgs(h, b->stream, sp.stage, sp.round,
                                sp.stage == VRESTIR_STAGE_FINAL_SHADING ? outs[i] : nullptr, a);
             if (rc) return bfail(b, rc, std::string("pass ") + std::to_string(i) + ": " + h->err);
-            const bool needSplit = sp.stage == VRESTIR_STAGE_TEMPORAL_REUSE || sp.stage == VRESTIR_STAGE_SPATIAL_REUSE ||
-                                   sp.stage == VRESTIR_STAGE_FINAL_SHADING;
+            const bool needSplit = (sp.stage == VRESTIR_STAGE_TEMPORAL_REUSE && temporalTK != 0) ||
+                                   sp.stage == VRESTIR_STAGE_SPATIAL_REUSE || sp.stage == VRESTIR_STAGE_FINAL_SHADING;
             const int wantTK = sp.stage == VRESTIR_STAGE_FINAL_SHADING ? 1 : 2;
             const Options& so = a.frame.spatialOpt;
-            if (sp.op == kOpTrace && (a.frame.initialOpt.lightingTrackingMethod != 2u ||
+            if ((sp.op == kOpTrace || sp.op == kOpTraceLearned) && (a.frame.initialOpt.lightingTrackingMethod != 2u ||
                                       so.visibilityTrackingMethod != 2u || so.lightingTrackingMethod != 2u))
                 return bfail(b, VRESTIR_EUNSUPPORTED, "batches need the ray-marching spatial and lighting trackers");
             if (needSplit && (a.splitTK != wantTK || a.splitMode != 2))
@@ -1761,14 +1793,12 @@ static int batchFrame(vrestir_batch* b, hipStream_t user, const int32_t* widths,
             key.push_back(a.occBytes);
             key.push_back(tiles);
         }
-        b->ops[k] = sp.op;
-        key.push_back(sp.op);
         if (sp.stage == VRESTIR_STAGE_SPATIAL_REUSE && sp.op == kOpSpatialReplay)
             for (int i = 0; i < n; i++) b->passes[i]->totalRoundId++;
     }
     // every staged record buffer must be the one its pass holds now
     for (int k = 0; k < S; k++)
-        for (int i = 0; i < n; i++) {
+        for (int i = 0; i < n && b->ops[k] >= 0; i++) {
             const LaunchArgs& a = b->argsHost[(size_t)k * kMaxBatch + i];
             const vrestir_handle* h = b->passes[i];
             if (a.splitTK && (a.split.recs != h->split.recs || a.split.trs != h->split.trs ||
@@ -1787,6 +1817,14 @@ static int batchFrame(vrestir_batch* b, hipStream_t user, const int32_t* widths,
         for (int k = 0; k < S; k++) {
             const LaunchArgs* args = b->argsDev + (size_t)k * kMaxBatch;
             const int op = b->ops[k];
+            if (op < 0) {
+                for (int i = 0; i < n; i++) {
+                    vrestir_handle* h = b->passes[i];
+                    const int rc = op == -1 ? learnedProposal(h, b->stream) : learnedTrainFrame(h, b->stream);
+                    if (rc) return bfail(b, rc, std::string("pass ") + std::to_string(i) + ": " + h->err);
+                }
+                continue;
+            }
             if (op == kOpSegmentsMarch || op == kOpSegmentsCells)
                 BHIP(b, launchBatchedSegments(b->stream, op, args, b->listsDev + k, b->countersDev + k, b->grid,
                                               b->occ[k]));

@@ -21,6 +21,11 @@ template <int OP, int B>
 VR_DEV void runOp(const LaunchArgs& a, VBlock vb) {
     if constexpr (OP == kOpFeatures) featuresBody(a.scene, a.frame, a.features, vb);
     else if constexpr (OP == kOpTrace) traceBody<B, 2, 2, 2, false>(a.scene, a.frame, a.res0, a.extra0, a.learned, vb);
+    else if constexpr (OP == kOpTraceLearned)
+        traceBody<B, 2, 2, 2, true>(a.scene, a.frame, a.res0, a.extra0, a.learned, vb);
+    else if constexpr (OP == kOpTemporalDirect)
+        temporalBody<B, 2, 2, kDirect>(a.scene, a.frame, a.res0, a.extra0, a.resT, a.extraT, a.features, a.featuresT,
+                                       a.mvec, a.split, vb);
     else if constexpr (OP == kOpTemporalRecord)
         temporalBody<B, 2, 2, kRecord>(a.scene, a.frame, a.res0, a.extra0, a.resT, a.extraT, a.features, a.featuresT,
                                        a.mvec, a.split, vb);
@@ -84,6 +89,8 @@ static hipError_t launchOp(hipStream_t st, int op, const LaunchArgs* args, const
         VR_BATCH_CASE(kOpCopyReservoirs)
         VR_BATCH_CASE(kOpFinalRecord)
         VR_BATCH_CASE(kOpFinalReplay)
+        VR_BATCH_CASE(kOpTraceLearned)
+        VR_BATCH_CASE(kOpTemporalDirect)
         default: return hipErrorInvalidValue;
     }
 #undef VR_BATCH_CASE

@@ -67,7 +67,7 @@ struct BatchList {
 enum BatchOp {
     kOpFeatures = 0, kOpTrace, kOpTemporalRecord, kOpTemporalReplay, kOpCopyFeatures, kOpSpatialRecord,
     kOpSpatialReplay, kOpCopyReservoirs, kOpFinalRecord, kOpFinalReplay, kOpSegmentsMarch, kOpSegmentsCells,
-    kOpCount
+    kOpTraceLearned, kOpTemporalDirect, kOpCount
 };
 // One persistent launch of `grid` workgroups draining the step's work list
 // (counter: the step's queue head, zero before the launch). args and list are
