@@ -59,7 +59,9 @@ VR_TRACING_KERNEL void kBatched(const LaunchArgs* __restrict__ args, const Batch
         __syncthreads();
         if (threadIdx.x == 0) sItem = atomicAdd(counter, 1);
         __syncthreads();
-        const int item = sItem;
+        // the same value in every lane: made wave-uniform so that the pass's
+        // arguments below are read with scalar loads into scalar registers
+        const int item = __builtin_amdgcn_readfirstlane(sItem);
         if (item >= total) break;
         int i = 0;
         while (i + 1 < n && item >= list->start[i + 1]) i++;

@@ -692,8 +692,11 @@ hipError_t launchTraceSegments(const LaunchArgs& a, const Options& opt, bool isF
 
 // The segment traces of a batched step: work items are 256-pixel ranges of one
 // instance (its recorded rays), the same ranges a plain launch gives a workgroup.
+// Held to the plain kernel's four waves per SIMD: left to itself the compiler
+// gives the persistent loop 175 VGPRs (two waves), and the trace, latency bound,
+// runs about a quarter slower.
 template <int TK>
-__global__ __launch_bounds__(256) void kBatchedSegments(const LaunchArgs* __restrict__ args,
+VR_TRACING_KERNEL void kBatchedSegments(const LaunchArgs* __restrict__ args,
                                                         const BatchList* __restrict__ list,
                                                         int* __restrict__ counter) {
     __shared__ int sItem;
@@ -702,7 +705,9 @@ __global__ __launch_bounds__(256) void kBatchedSegments(const LaunchArgs* __rest
         __syncthreads();
         if (threadIdx.x == 0) sItem = atomicAdd(counter, 1);
         __syncthreads();
-        const int item = sItem;
+        // the same value in every lane: made wave-uniform so that the pass's
+        // arguments below are read with scalar loads into scalar registers
+        const int item = __builtin_amdgcn_readfirstlane(sItem);
         if (item >= total) break;
         int i = 0;
         while (i + 1 < n && item >= list->start[i + 1]) i++;
