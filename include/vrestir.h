@@ -258,7 +258,10 @@ enum {
     /* learned proposal: float[72] per training ray (VRESTIR_LEARNED_RECORD_FLOATS): raw inputs
      * (entry point in mip 0's normalised box, direction, segment length / box diagonal),
      * the summed weight W, then per bin the summed weights and summed weight^2 * g_used */
-    VRESTIR_BUF_TRAIN_RECORDS = 10
+    VRESTIR_BUF_TRAIN_RECORDS = 10,
+    /* split TraceRays (VRESTIR_TUNE_SPLIT_TRACE): int32 count of the last frame's pixels
+     * that were traced again in one kernel (resampling draws that differed from the record's) */
+    VRESTIR_BUF_TRACE_REDONE = 11
 };
 int32_t vrestir_prepare_frame(vrestir_handle* h, int32_t width, int32_t height);
 int32_t vrestir_run_stage(vrestir_handle* h, void* stream, int32_t stage, int32_t round, float* outRGBA);
@@ -361,8 +364,15 @@ int32_t vrestir_learned_train(vrestir_handle* h, void* stream, const float* recs
  * 2048 128^3 blocks (the segments are integrated cell by cell); otherwise they
  * run as one kernel.
  * 0 runs each stage as one kernel. Results agree to float rounding (2 sums each
- * ray's optical depth per brick). */
-enum { VRESTIR_TUNE_SPLIT_REUSE = 1 };
+ * ray's optical depth per brick).
+ * VRESTIR_TUNE_SPLIT_TRACE (default 1; with VRESTIR_TUNE_SPLIT_REUSE 2): TraceRays
+ * records its candidates' shadow rays (keeping their free-flight samples), traces
+ * them as brick segments and replays the candidates with their transmittances,
+ * when the initial lighting rays use ray marching and the learned proposal is off.
+ * A pixel whose resampling draws differ from the record's (a candidate weight that
+ * its real shadow transmittance makes zero) is traced again in one kernel, so the
+ * results agree with one kernel's to float rounding. 0 runs TraceRays as one kernel. */
+enum { VRESTIR_TUNE_SPLIT_REUSE = 1, VRESTIR_TUNE_SPLIT_TRACE = 2 };
 int32_t vrestir_set_tuning(vrestir_handle* h, int32_t key, int32_t value);
 
 /* ---- ground truth and error (AccumulatePass double mode, AccumulatePass.cpp:42,93;

@@ -23,8 +23,12 @@ def batch_specs():
 
 
 def make_pass(spec, setup=None):
+    """A pass on the spec. A batch runs TraceRays as one kernel, so the passes
+    rendered alone do too (the split TraceRays agrees with it to float rounding
+    only, which would hide whether frame 0 is bit for bit)."""
     g = VolumetricReSTIR()
     g.setScene(spec)
+    assert abi.load_library().vrestir_set_tuning(g.h, abi.TUNE_SPLIT_TRACE, 0) == 0
     if setup:
         setup(g)
     return g

@@ -31,12 +31,14 @@ REPROJECTION_LINEAR, REPROJECTION_NONE, REPROJECTION_NO_BACKGROUND = 0, 1, 2
 BRICK_FP32, BRICK_FP16 = 0, 1
 
 TUNE_SPLIT_REUSE = 1
+TUNE_SPLIT_TRACE = 2
 
 STAGE_GENERATE_FEATURES, STAGE_TRACE_RAYS, STAGE_TEMPORAL_REUSE, STAGE_SPATIAL_REUSE, \
     STAGE_COPY_RESERVOIRS, STAGE_FINAL_SHADING = range(6)
 STAGE_LEARNED_PROPOSAL, STAGE_LEARNED_TRAIN = 6, 7
 BUF_FEATURES, BUF_TEMPORAL_FEATURES, BUF_RESERVOIRS0, BUF_RESERVOIRS1, BUF_TEMPORAL_RESERVOIRS, \
-    BUF_EXTRA0, BUF_EXTRA1, BUF_TEMPORAL_EXTRA, BUF_COLOR, BUF_LEARNED_CDF, BUF_TRAIN_RECORDS = range(11)
+    BUF_EXTRA0, BUF_EXTRA1, BUF_TEMPORAL_EXTRA, BUF_COLOR, BUF_LEARNED_CDF, BUF_TRAIN_RECORDS, \
+    BUF_TRACE_REDONE = range(12)
 
 TRACK_TR_ANALYTIC, TRACK_TR_RAY_MARCHING, TRACK_TR_RATIO, TRACK_TR_RESIDUAL_RATIO, \
     TRACK_TR_ANALOG_RESIDUAL_RATIO, TRACK_SAMPLE_ANALYTIC, TRACK_SAMPLE_DECOMPOSITION, \

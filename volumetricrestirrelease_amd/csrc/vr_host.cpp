@@ -72,6 +72,8 @@ struct vrestir_handle {
     double* relOut = nullptr;
     // record / trace / replay buffers of the reuse stages
     int splitReuse = 2;  // 0: one kernel; 1: balanced walker queue; 2: brick-segment trace
+    int splitTrace = 1;  // TraceRays' shadow rays as brick segments (with splitReuse == 2)
+    bool inBatch = false;  // staged by a batch, whose TraceRays step is one kernel
     SplitBufs split{};
     size_t splitPixels = 0;
     // textured environment map: texels, importance mip chain, orientation now and at the last frame
@@ -129,6 +131,7 @@ static void freeLearned(vrestir_handle* h) {
 
 static void freeSplit(vrestir_handle* h) {
     dfree(h->split.recs); dfree(h->split.trs); dfree(h->split.dens); dfree(h->split.counts); dfree(h->split.plan);
+    dfree(h->split.fix);
     h->split.cap = 0;
     h->splitPixels = 0;
 }
@@ -1103,6 +1106,7 @@ static int splitEnsure(vrestir_handle* h, int cap, size_t n) {
     HIPCHK(h, hipMalloc((void**)&h->split.dens, n * (size_t)cap * sizeof(float)));
     HIPCHK(h, hipMalloc((void**)&h->split.counts, n * sizeof(int32_t)));
     HIPCHK(h, hipMalloc((void**)&h->split.plan, n * sizeof(uint32_t)));
+    HIPCHK(h, hipMalloc((void**)&h->split.fix, sizeof(int32_t)));
     h->split.cap = cap;
     h->splitPixels = n;
     return VRESTIR_OK;
@@ -1114,7 +1118,17 @@ static int splitPlan(vrestir_handle* h, LaunchArgs& a, int stage) {
     const vrestir_params& p = h->P;
     const size_t n = (size_t)h->W * h->H;
     int cap = 0, tk = 0;
-    if (stage == VRESTIR_STAGE_SPATIAL_REUSE) {
+    if (stage == VRESTIR_STAGE_TRACE_RAYS) {
+        // the shadow rays of the candidates (ray marching only; not with the learned
+        // proposal, the reference path tracer, or without reuse)
+        const Frame& F = h->frame;
+        if (h->splitReuse != 2 || !h->splitTrace || h->inBatch || learnedActive(h) || p.useReference || F.noReuse ||
+            F.initialOpt.lightingTrackingMethod != 2u)
+            return VRESTIR_OK;
+        const int M = std::max(1, p.initialM), B = std::max(1, p.maxBounces);
+        cap = std::max(M * B, candStashSlots(M, B));  // one shadow ray per vertex; the free-flight samples
+        tk = 2;
+    } else if (stage == VRESTIR_STAGE_SPATIAL_REUSE) {
         const Options& so = h->frame.spatialOpt;
         if (so.visibilityTrackingMethod != 2u || so.lightingTrackingMethod != 2u) return VRESTIR_OK;
         const int K = p.spatialSampleCount;
@@ -1180,6 +1194,10 @@ extern "C" int32_t vrestir_set_tuning(vrestir_handle* h, int32_t key, int32_t va
                 HIPCHK(h, hipDeviceSynchronize());
                 freeSplit(h);
             }
+            return VRESTIR_OK;
+        case VRESTIR_TUNE_SPLIT_TRACE:
+            if (value < 0 || value > 1) return fail(h, VRESTIR_EINVAL, "split TraceRays must be 0 or 1");
+            h->splitTrace = value;
             return VRESTIR_OK;
         default: return fail(h, VRESTIR_EINVAL, "unknown tuning key");
     }
@@ -1280,8 +1298,8 @@ static int stageArgs(vrestir_handle* h, hipStream_t st, int stage, int round, fl
         default: break;
     }
     a.occBytes = occPlan(h, oa, ob, a.frame.occSlot);
-    if (stage == VRESTIR_STAGE_TEMPORAL_REUSE || stage == VRESTIR_STAGE_SPATIAL_REUSE ||
-        stage == VRESTIR_STAGE_FINAL_SHADING) {
+    if (stage == VRESTIR_STAGE_TRACE_RAYS || stage == VRESTIR_STAGE_TEMPORAL_REUSE ||
+        stage == VRESTIR_STAGE_SPATIAL_REUSE || stage == VRESTIR_STAGE_FINAL_SHADING) {
         int rc = splitPlan(h, a, stage);
         if (rc) return rc;
     }
@@ -1384,6 +1402,7 @@ static int bufInfo(vrestir_handle* h, int id, void** dev, size_t& elems, int& ki
         case VRESTIR_BUF_COLOR: *dev = h->outInternal; elems = n * 16; return 0;
         case VRESTIR_BUF_LEARNED_CDF: *dev = h->lCdf; elems = h->lCdfPixels * 64; return 0;
         case VRESTIR_BUF_TRAIN_RECORDS: *dev = h->lRecs; elems = (size_t)h->lRecSlots * kLearnedRecFloats * 4; return 0;
+        case VRESTIR_BUF_TRACE_REDONE: *dev = h->split.fix; elems = 4; return 0;
     }
     return -1;
 }
@@ -1675,6 +1694,16 @@ static int batchFrame(vrestir_batch* b, hipStream_t user, const int32_t* widths,
     // the learned proposal's Adam step takes host-computed bias corrections, so
     // frames with it run their launches directly instead of replaying a graph
     if (learned) allowGraph = false;
+    // a batch runs TraceRays as one kernel: its passes plan no TraceRays split while staged
+    struct InBatch {
+        const std::vector<vrestir_handle*>& ps;
+        explicit InBatch(const std::vector<vrestir_handle*>& v) : ps(v) {
+            for (vrestir_handle* h : ps) h->inBatch = true;
+        }
+        ~InBatch() {
+            for (vrestir_handle* h : ps) h->inBatch = false;
+        }
+    } inBatch(b->passes);
     BHIP(b, hipSetDevice(b->device));
     // the steps of the frame
     // steps with op < 0 are per-pass launches of the learned stages (-1 proposal, -2 training)

@@ -43,11 +43,53 @@ hipError_t launchReferenceLattice(const LaunchArgs& a, int stride, int spp, uint
     return hipGetLastError();
 }
 
+// Split TraceRays: record the candidates' shadow rays (and keep their free-flight
+// samples), trace the rays as brick segments with the initial lighting options,
+// replay the candidates with their transmittances, then run the pixels the replay
+// could not finish (a transmittance of exactly 0) in one kernel.
+static hipError_t launchTraceRaysSplit(const LaunchArgs& a, bool spMarch) {
+    const dim3 g = tiles(a.width, a.height), blk(256, 1, 1);
+    hipError_t e = launchZeroCounters(a.stream, a.split.fix, 1);
+    if (e != hipSuccess) return e;
+    VR_DISPATCH_B(a, hipLaunchKernelGGL((kTraceSplit<B, 2, 2, kRecord>), g, blk, a.occBytes, a.stream, a.scene,
+                                        a.frame, a.res0, a.extra0, a.split))
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // every record is a shadow ray (kind 1): one grid, filter and step for all segments
+    Options lo = a.frame.initialOpt;
+    lo.visibilityMipLevel = lo.lightingMipLevel;
+    lo.visibilityUseLinearSampler = lo.lightingUseLinearSampler;
+    lo.visibilityTStepScale = lo.lightingTStepScale;
+    lo.visibilityTrackingMethod = lo.lightingTrackingMethod;
+    lo.visibilitySamples = lo.lightSamples;
+    if ((e = launchTraceSegments(a, lo, false)) != hipSuccess) return e;
+    VR_DISPATCH_B(a, {
+        if (spMarch)
+            hipLaunchKernelGGL((kTraceSplit<B, 2, 2, kReplay>), g, blk, a.occBytes, a.stream, a.scene, a.frame, a.res0,
+                               a.extra0, a.split);
+        else
+            hipLaunchKernelGGL((kTraceSplit<B, -1, -1, kReplay>), g, blk, a.occBytes, a.stream, a.scene, a.frame,
+                               a.res0, a.extra0, a.split);
+    })
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const dim3 gf(64), bf(256);
+    VR_DISPATCH_B(a, {
+        if (spMarch)
+            hipLaunchKernelGGL((kTraceFix<B, 2, 2>), gf, bf, a.occBytes, a.stream, a.scene, a.frame, a.res0, a.extra0,
+                               a.split);
+        else
+            hipLaunchKernelGGL((kTraceFix<B, -1, -1>), gf, bf, a.occBytes, a.stream, a.scene, a.frame, a.res0,
+                               a.extra0, a.split);
+    })
+    return hipGetLastError();
+}
+
 hipError_t launchTraceRays(const LaunchArgs& a) {
     const dim3 g = tiles(a.width, a.height), blk(256, 1, 1);
     const Options& so = a.frame.spatialOpt;
     const bool spMarch = so.visibilityTrackingMethod == 2u && so.lightingTrackingMethod == 2u;
     const bool inMarch = a.frame.initialOpt.lightingTrackingMethod == 2u;
+    if (a.splitTK == 2 && inMarch && !a.learnedOn && !a.frame.useReference && !a.frame.noReuse)
+        return launchTraceRaysSplit(a, spMarch);
     VR_DISPATCH_B(a, {
         if (a.frame.useReference)
             hipLaunchKernelGGL(kReference<B>, g, blk, a.occBytes, a.stream, a.scene, a.frame, a.color0);

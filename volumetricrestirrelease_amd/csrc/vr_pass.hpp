@@ -303,26 +303,6 @@ VR_DEV v3 emissionWorld(const Scene& S, v3 pW, bool last = false) {
     return blackbody(S, temperatureWorld(S, pW, last ? kTemperatureSlot + kPrevExtraOffset : kTemperatureSlot));
 }
 
-// SampleDirectLighting (VR/VolumeUtils.slang:454-492).
-template <int LM>
-VR_DEV v3 sampleDirect(const Scene& S, const EnvXf& X, Rng& rng, float& pdf, const MediumInteraction& mi,
-                       const Options& o, bool enableShadow, int& lightIndex, float& uvx, float& uvy) {
-    pdf = 0.f;
-    LightSample ls;
-    ls.pdfArea = 0.f;
-    bool valid = sampleLights(S, X, mi.p, o.useEnv, o.useAnalytic, o.useEmissive, rng, ls, lightIndex, uvx, uvy);
-    pdf = ls.pdfArea;
-    if (!valid) { pdf = 0.f; return V3(0.f); }
-    Ray sh{mi.p, ls.rayDir, 0.f, ls.rayDistance};
-    if (enableShadow) {
-        float vis = visibilityT<LM>(S, sh, rng, o.lightSamples, o.lightingMipLevel, o.lightingUseLinearSampler != 0,
-                                    o.lightingTrackingMethod, o.lightingTStepScale);
-        ls.Li = ls.Li * vis;
-    }
-    float p = phaseHG(dot3(mi.wo, ls.dir), mi.g);
-    return V3(p) * ls.Li / 1.f;
-}
-
 // directLighting (VR/VolumeUtils.slang:416-450): NEE of the reference path tracer.
 VR_DEV v3 directLightingPT(const Scene& S, const EnvXf& X, Rng& rng, const MediumInteraction& mi, int nSamples,
                            const Options& o, int mip, uint32_t method) {
@@ -417,6 +397,105 @@ struct ReplayTrace {
     size_t dstride;
     int nd;
     VR_DEV float density(const Scene&, int, v3) { return dens[(size_t)nd++ * dstride]; }
+};
+
+// SampleDirectLighting (VR/VolumeUtils.slang:454-492); the shadow ray's
+// transmittance comes from the trace policy tp (lighting options, kind 1).
+template <class TP>
+VR_DEV v3 sampleDirectT(const Scene& S, const EnvXf& X, Rng& rng, float& pdf, const MediumInteraction& mi,
+                        const Options& o, bool enableShadow, int& lightIndex, float& uvx, float& uvy, TP& tp) {
+    pdf = 0.f;
+    LightSample ls;
+    ls.pdfArea = 0.f;
+    bool valid = sampleLights(S, X, mi.p, o.useEnv, o.useAnalytic, o.useEmissive, rng, ls, lightIndex, uvx, uvy);
+    pdf = ls.pdfArea;
+    if (!valid) { pdf = 0.f; return V3(0.f); }
+    Ray sh{mi.p, ls.rayDir, 0.f, ls.rayDistance};
+    if (enableShadow) {
+        float vis = tp.vis(S, sh, rng, o, 1, false, 0);
+        ls.Li = ls.Li * vis;
+    }
+    float p = phaseHG(dot3(mi.wo, ls.dir), mi.g);
+    return V3(p) * ls.Li / 1.f;
+}
+
+// How TraceRays' candidate generation obtains its free-flight samples, its
+// shadow rays' transmittances and its resampling draws. CandDirect traverses the
+// grid for both kinds of ray. The split form runs the candidate loop twice around
+// a balanced trace of the shadow rays:
+//   * CandRecord draws every free-flight sample (its traversal), keeps the
+//     results and the generator's state after the draw in the pixel's density
+//     slots (from slot 0 up), writes each shadow ray as a record with its
+//     visibility taken as 1, and keeps one bit per resampling step: whether that
+//     step drew a random number (a positive weight; from the last slot down);
+//   * CandReplay reads the samples and transmittances back in the same order
+//     instead of tracing.
+// This needs the shadow rays to be ray-marched (that tracker draws no random
+// numbers) and the random draws to depend on no shadow transmittance. They do in
+// one way only: a resampling step draws for a positive weight and not for a zero
+// one, and a weight that is positive with visibility 1 can be zero with the real
+// transmittance (exactly 0, or a product that underflows). The replay compares
+// each step with the record's bit and flags a pixel where one differs (zero);
+// such pixels are run again in one kernel.
+template <int IM>
+struct CandDirect : DirectTrace<IM, IM> {
+    VR_DEV void analytic(const Scene& S, const Ray& r, Rng& rng, bool linear, int slot, int ns, float h[4],
+                         float pd[4], float tr[4]) {
+        sampleAnalytic(S, r, rng, linear, slot, ns, h, pd, tr);
+    }
+    VR_DEV bool resample(const Res& r, Res& st, Rng& rng) { return vr::resample(r, st, rng); }
+};
+struct CandRecord : RecordTrace {
+    int cap;          // the pixel's density slots
+    uint32_t mask;    // the current word of draw bits
+    int bits, words;  // bits in it, words stored
+    VR_DEV void put(float v) { dens[(size_t)nd++ * dstride] = v; }
+    VR_DEV void analytic(const Scene& S, const Ray& r, Rng& rng, bool linear, int slot, int ns, float h[4],
+                         float pd[4], float tr[4]) {
+        sampleAnalytic(S, r, rng, linear, slot, ns, h, pd, tr);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (i < ns) { put(h[i]); put(pd[i]); put(tr[i]); }
+        put(__uint_as_float(rng.s0)); put(__uint_as_float(rng.s1));
+        put(__uint_as_float(rng.s2)); put(__uint_as_float(rng.s3));
+    }
+    VR_DEV void flushMask() {
+        dens[(size_t)(cap - 1 - words++) * dstride] = __uint_as_float(mask);
+        mask = 0u;
+        bits = 0;
+    }
+    VR_DEV bool resample(const Res& r, Res& st, Rng& rng) {
+        mask |= (r.runningSum > 0.f ? 1u : 0u) << bits;
+        if (++bits == 32) flushMask();
+        return vr::resample(r, st, rng);
+    }
+    VR_DEV void finish() {
+        if (bits) flushMask();
+    }
+};
+struct CandReplay : ReplayTrace {
+    int cap;
+    uint32_t mask;
+    int bits, words;  // bits of mask used (32: load the next word), words loaded
+    bool zero;        // a resampling step drew where the record's did not, or the reverse
+    VR_DEV float get() { return dens[(size_t)nd++ * dstride]; }
+    VR_DEV void analytic(const Scene&, const Ray&, Rng& rng, bool, int, int ns, float h[4], float pd[4],
+                         float tr[4]) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (i < ns) { h[i] = get(); pd[i] = get(); tr[i] = get(); }
+        rng.s0 = __float_as_uint(get()); rng.s1 = __float_as_uint(get());
+        rng.s2 = __float_as_uint(get()); rng.s3 = __float_as_uint(get());
+    }
+    VR_DEV bool resample(const Res& r, Res& st, Rng& rng) {
+        if (bits == 32) {
+            mask = __float_as_uint(dens[(size_t)(cap - 1 - words++) * dstride]);
+            bits = 0;
+        }
+        const bool drew = ((mask >> bits++) & 1u) != 0u;
+        zero = zero || drew != (r.runningSum > 0.f);
+        return vr::resample(r, st, rng);
+    }
 };
 
 // evaluate_F_ (VR/ReSTIRHelper.slang:91-423) + evaluate_L_in_volume (:443-496).
@@ -541,9 +620,10 @@ VR_DEV v3 evaluateF(const Scene& S, const EnvXf& X, const Res& tap, const float*
 
 // ComputeInitialSample (VR/ComputeInitialSample.slang:4-395). ex[] receives the
 // candidate's extra-bounce records; IM is the initial lighting tracking method.
-template <int B, int IM>
+// cp: the candidate policy (CandDirect, CandRecord or CandReplay).
+template <int B, int IM, class CP>
 VR_DEV Res computeInitialSample(const Scene& S, const Frame& Fr, Ray ray, float preHit, float prePdf, float preTr,
-                                Rng& rng, const Options& opt, v3 ex[B > 1 ? B - 1 : 1]) {
+                                Rng& rng, const Options& opt, v3 ex[B > 1 ? B - 1 : 1], CP& cp) {
     float pathPdf = 1.f, pathPHat = 1.f;
     Res combined = newRes();
     Res out = newRes();
@@ -564,7 +644,7 @@ VR_DEV Res computeInitialSample(const Scene& S, const Frame& Fr, Ray ray, float 
                 int vm = opt.visibilityMipLevel;
                 if (Fr.coarserIndirect) vm = min((opt.visibilityMipLevel >= 8 ? 8 : 0) + S.numMips - 1, vm + 1);
                 float h[4], pd[4], tr[4];
-                sampleAnalytic(S, ray, rng, opt.visibilityUseLinearSampler != 0, vm, 1, h, pd, tr);
+                cp.analytic(S, ray, rng, opt.visibilityUseLinearSampler != 0, vm, 1, h, pd, tr);
                 curHit = h[0]; pdfDist = pd[0]; Tr = tr[0];
                 mi = {at(ray, curHit), -ray.d, S.g, curHit != kTMax};
             }
@@ -607,8 +687,8 @@ VR_DEV Res computeInitialSample(const Scene& S, const Frame& Fr, Ray ray, float 
                 v3 Le = actualDensity > 0.f ? emissionWorld(S, mi.p) : V3(0.f);
                 v3 oma = V3(1.f) - albedo;
                 bool shadow = opt.lightSamples == 0 ? false : true;
-                v3 Ld = sampleDirect<IM>(S, Fr.env[0], rng, outLightPdf, mi, opt, shadow, out.lightID, out.uvx,
-                                         out.uvy);
+                v3 Ld = sampleDirectT(S, Fr.env[0], rng, outLightPdf, mi, opt, shadow, out.lightID, out.uvx, out.uvy,
+                                      cp);
                 v3 wi = V3(0.f);
                 if (B > 1) {
                     float u0 = u01(rng);
@@ -661,7 +741,7 @@ VR_DEV Res computeInitialSample(const Scene& S, const Frame& Fr, Ray ray, float 
                 hitEmpty = true;
             }
         }
-        if (B > 1) resample(out, combined, rng);
+        if (B > 1) cp.resample(out, combined, rng);
         if (hitEmpty) break;
     }
     if (B > 1) { combined.M = 1; return combined; }

@@ -125,19 +125,46 @@ __global__ __launch_bounds__(256) void kReference(const Scene* __restrict__ sc, 
     referenceBody<B>(sc, fr, outColor, launchBlock());
 }
 
-// TraceRays (VR/TraceRays.cs.slang:64-201). IM: initial lighting tracking
-// method; VM / LM: the spatial options' methods used by the final p-hat.
+// How a stage kernel obtains transmittances: traced on the spot (kDirect), or
+// split around a balanced trace of the recorded rays (kRecord, kReplay).
+enum { kDirect = 0, kRecord = 1, kReplay = 2 };
+
+// TraceRays (VR/TraceRays.cs.slang:64-201) for one pixel. IM: initial lighting
+// tracking method; VM / LM: the spatial options' methods used by the final p-hat.
 // LEARNED: the initial distances come from the learned proposal (the pixel's
 // CDF over c drives the analytic tracker's optical-depth targets) and one pixel
 // per block writes its candidates' weights as a training record.
-template <int B, int IM, int VM, int LM, bool LEARNED = false>
-VR_DEV void traceBody(const Scene* __restrict__ sc, const Frame& fr, ResStore* __restrict__ outRes,
-                      float* __restrict__ outExtra, const LearnedTraceArgs& la, VBlock vb) {
-    const Scene& S = *sc;
-    const Frame& F = fr;
-    occCacheLoad(S, F.occSlot[0], F.occSlot[1]);
-    PixelIdx px = pixelOf(F, vb);
-    if (!px.ok) return;
+// MODE (IM = 2, not LEARNED): kRecord writes the pixel's shadow rays, its
+// free-flight samples, its resampling draws and its count of rays (sb) and
+// nothing else; kReplay reads them back (CandReplay) and finishes the pixel, or,
+// when a resampling step disagrees with the record's, appends the pixel to
+// sb.plan / sb.fix instead.
+template <int B, int IM, int VM, int LM, bool LEARNED, int MODE>
+VR_DEV void tracePixel(const Scene& S, const Frame& F, PixelIdx px, ResStore* __restrict__ outRes,
+                       float* __restrict__ outExtra, const LearnedTraceArgs& la, const SplitBufs& sb) {
+    static_assert(MODE == kDirect || (IM == 2 && !LEARNED), "split TraceRays needs ray-marched shadow rays");
+    using CP = std::conditional_t<MODE == kDirect, CandDirect<IM>,
+                                  std::conditional_t<MODE == kRecord, CandRecord, CandReplay>>;
+    CP cp{};
+    if constexpr (MODE != kDirect) {
+        const size_t np = (size_t)F.width * F.height;
+        cp.n = 0;
+        cp.dens = sb.dens + px.id;
+        cp.dstride = np;
+        cp.nd = 0;
+        cp.cap = sb.cap;
+        cp.mask = 0u;
+        cp.words = 0;
+        if constexpr (MODE == kRecord) {
+            cp.out = sb.recs + (size_t)px.id * sb.cap;
+            cp.bits = 0;
+        } else {
+            cp.tr = sb.trs + px.id;
+            cp.trStride = np;
+            cp.bits = 32;
+            cp.zero = false;
+        }
+    }
     Rng rng = rngCreate(px.x, px.y, (uint32_t)(F.numTotalRounds * F.frameCount));
     Ray ray{ld3(F.cam.pos), rayDir(F.cam, px.x, px.y, F.width, F.height), 0.f, kTMax};
     constexpr int NE = B > 1 ? B - 1 : 1;
@@ -177,8 +204,8 @@ VR_DEV void traceBody(const Scene* __restrict__ sc, const Frame& fr, ResStore* _
             sampleAnalyticTargets(S, ray, rng, F.initialOpt.visibilityUseLinearSampler != 0,
                                   F.initialOpt.visibilityMipLevel, rs, targets, h, pd, tr);
         } else if (!F.noReuse)
-            sampleAnalytic(S, ray, rng, F.initialOpt.visibilityUseLinearSampler != 0, F.initialOpt.visibilityMipLevel,
-                           rs, h, pd, tr);
+            cp.analytic(S, ray, rng, F.initialOpt.visibilityUseLinearSampler != 0, F.initialOpt.visibilityMipLevel, rs,
+                        h, pd, tr);
         for (int s = 0; s < rs; s++) {
             v3 ex[NE];
 #pragma unroll
@@ -201,7 +228,7 @@ VR_DEV void traceBody(const Scene* __restrict__ sc, const Frame& fr, ResStore* _
                 }
                 ps *= ratio;
             }
-            Res o = computeInitialSample<B, IM>(S, F, ray, hs, ps, ts, rng, F.initialOpt, ex);
+            Res o = computeInitialSample<B, IM>(S, F, ray, hs, ps, ts, rng, F.initialOpt, ex, cp);
             if constexpr (LEARNED) {
                 if (rec) {
                     // the sample's weight against the analytic proposal over g(c)
@@ -211,13 +238,23 @@ VR_DEV void traceBody(const Scene* __restrict__ sc, const Frame& fr, ResStore* _
                     rec[learned::kRecSq + bs] += wt * wt * gs;
                 }
             }
-            bool sel = resample(o, fin, rng);
+            bool sel = cp.resample(o, fin, rng);
             if (sel && B > 1) {
                 int mi = decodeMaxIndirect(fin.sampledPixel);
 #pragma unroll
                 for (int k = 0; k < NE; k++)
                     if (k < mi) finalEx[k] = ex[k];
             }
+        }
+    }
+    if constexpr (MODE == kRecord) {
+        cp.finish();
+        sb.counts[px.id] = cp.n;
+        return;
+    } else if constexpr (MODE == kReplay) {
+        if (cp.zero) {
+            sb.plan[atomicAdd(sb.fix, 1)] = (uint32_t)px.id;
+            return;
         }
     }
     // Write the selected extra-bounce records first: the p-hat re-evaluation
@@ -241,9 +278,43 @@ VR_DEV void traceBody(const Scene* __restrict__ sc, const Frame& fr, ResStore* _
     storeRes(outRes, px.id, fin);
 }
 template <int B, int IM, int VM, int LM, bool LEARNED = false>
+VR_DEV void traceBody(const Scene* __restrict__ sc, const Frame& fr, ResStore* __restrict__ outRes,
+                      float* __restrict__ outExtra, const LearnedTraceArgs& la, VBlock vb) {
+    const Scene& S = *sc;
+    occCacheLoad(S, fr.occSlot[0], fr.occSlot[1]);
+    PixelIdx px = pixelOf(fr, vb);
+    if (!px.ok) return;
+    tracePixel<B, IM, VM, LM, LEARNED, kDirect>(S, fr, px, outRes, outExtra, la, SplitBufs{});
+}
+template <int B, int IM, int VM, int LM, bool LEARNED = false>
 VR_TRACE_KERNEL void kTrace(const Scene* __restrict__ sc, const Frame fr, ResStore* __restrict__ outRes,
                               float* __restrict__ outExtra, const LearnedTraceArgs la) {
     traceBody<B, IM, VM, LM, LEARNED>(sc, fr, outRes, outExtra, la, launchBlock());
+}
+// Split TraceRays: the record (MODE = kRecord) and replay (kReplay) launches
+// around the brick-segment trace of the recorded shadow rays.
+template <int B, int VM, int LM, int MODE>
+VR_TRACE_KERNEL void kTraceSplit(const Scene* __restrict__ sc, const Frame fr, ResStore* __restrict__ outRes,
+                                   float* __restrict__ outExtra, const SplitBufs sb) {
+    const Scene& S = *sc;
+    occCacheLoad(S, fr.occSlot[0], fr.occSlot[1]);
+    PixelIdx px = pixelOf(fr, launchBlock());
+    if (!px.ok) return;
+    tracePixel<B, 2, VM, LM, false, MODE>(S, fr, px, outRes, outExtra, LearnedTraceArgs{}, sb);
+}
+// The pixels the replay left (sb.plan[0, *sb.fix)), each traced again in one
+// kernel from its first random number; a fixed grid strides over the list.
+template <int B, int VM, int LM>
+VR_TRACE_KERNEL void kTraceFix(const Scene* __restrict__ sc, const Frame fr, ResStore* __restrict__ outRes,
+                                 float* __restrict__ outExtra, const SplitBufs sb) {
+    const Scene& S = *sc;
+    occCacheLoad(S, fr.occSlot[0], fr.occSlot[1]);
+    const int n = *sb.fix;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int id = (int)sb.plan[i];
+        const PixelIdx px{id % fr.width, id / fr.width, id, true};
+        tracePixel<B, 2, VM, LM, false, kDirect>(S, fr, px, outRes, outExtra, LearnedTraceArgs{}, sb);
+    }
 }
 
 VR_DEV float2 featuresAt(const float2* f, int id, int n) {
@@ -252,9 +323,6 @@ VR_DEV float2 featuresAt(const float2* f, int id, int n) {
 }
 VR_DEV bool isBackground(float2 f) { return f.y == 1.f && __float_as_int(f.x) != 0; }
 
-// How a stage kernel obtains transmittances: traced on the spot (kDirect), or
-// split around a balanced trace of the recorded rays (kRecord, kReplay).
-enum { kDirect = 0, kRecord = 1, kReplay = 2 };
 template <int MODE, int VM, int LM>
 using TracePolicy = std::conditional_t<MODE == kDirect, DirectTrace<VM, LM>,
                                        std::conditional_t<MODE == kRecord, RecordTrace, ReplayTrace>>;

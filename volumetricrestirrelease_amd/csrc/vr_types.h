@@ -154,9 +154,18 @@ struct SplitBufs {
     float* dens;      // the density lookups of the same evaluations: slot i of pixel p at i * pixels + p
     int32_t* counts;  // records per pixel
     uint32_t* plan;   // per pixel: bit s = candidate s's Talbot evaluations were recorded
+                      // (TraceRays: the list of pixels its replay leaves to the one-kernel form)
+    int32_t* fix;     // TraceRays: the length of that list
     int32_t cap;
     int32_t pad_;
 };
+// Density slots one pixel of split TraceRays fills (M candidates, path length B):
+// 3 per free-flight sample and 4 (the generator's state) per traversal, with the
+// camera ray's samples drawn in rounds of up to 4 and one sample per extra bounce;
+// and one 32-bit word per 32 resampling steps (at most B + 1 per candidate).
+constexpr int candStashSlots(int M, int B) {
+    return 3 * M + 4 * ((M + 3) / 4) + 7 * M * (B - 1) + (M * (B + 1) + 31) / 32;
+}
 // Neighbour offsets of spatial reuse: [16 frame seeds][kMaxSpatialSamples][2] ints.
 static const int kMaxSpatialSamples = 32;
 
