@@ -50,7 +50,7 @@ HBM_PEAK_GBS = 8000.0
 # measured by this run: see the `traffic_source` field.
 PROFILE_TRAFFIC = {
     "file": "profiles/SpatialReuse_config1_1080p.md",
-    "bytes": {"SpatialReuse": 32.743e9, "TraceRays": 12.905e9, "FinalShading": 4.322e9, "TemporalReuse": 3.363e9,
+    "bytes": {"SpatialReuse": 33.395e9, "TraceRays": 15.489e9, "FinalShading": 4.360e9, "TemporalReuse": 3.376e9,
               "GenerateFeatures": 0.183e9, "CopyReservoirs": 0.156e9},
 }
 LATTICE = 8
@@ -294,14 +294,128 @@ def bench_batched(args, steps=None, warmup=None, emit=True):
     return line
 
 
+def bench_config4(args):
+    """config4: learned + batched at full scale. A res^3 animated emissive plume
+    (density and temperature simulated on the GPU, 40 steps of warm-up, then one
+    step per frame), seen by 4 views at 1080p that render as one batch; every frame
+    each view rebuilds its volume frame on the GPU from the simulation's grids. The
+    60-frame animation runs twice from the same start: plain proposal, then learned
+    proposal (trained online from frame 0). relMSE: the mean over the views and the
+    last two frames against 65,536-spp FP64 lattice references of the same frames.
+    `value` is the learned batched form's frames/s over rendering (the volume
+    update -- simulation step and the 4 GPU volume builds -- is timed apart)."""
+    import torch
+
+    from volumetricrestirrelease_amd import scenes
+    from volumetricrestirrelease_amd.volumetric_restir import Plume, VolumetricReSTIR, VolumetricReSTIRBatch
+    say = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    res, frames, warm = args.c4_res, args.c4_frames, 40
+    specs = scenes.config4_views(res=res, frames=frames)
+    stream = torch.cuda.current_stream()
+    outs = [torch.zeros((s.height, s.width, 4), device="cuda") for s in specs]
+    evals = (frames - 2, frames - 1)
+    refs = {}
+
+    def run(learned):
+        t0 = time.perf_counter()
+        plume = Plume(res, 4)
+        plume.advance(warm)
+        d, t, _ = plume.grids()
+        passes = []
+        for s in specs:
+            g = VolumetricReSTIR()
+            g.setSceneStreamed(s, d, t)
+            g.setLearned(enable=1 if learned else 0)
+            passes.append(g)
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        b = VolumetricReSTIRBatch(passes)
+        render, update, ref_s, imgs = 0.0, 0.0, 0.0, {}
+        for f in range(frames):
+            if f > 0:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                plume.advance(1)
+                d, t, _ = plume.grids()
+                for g, s in zip(passes, specs):
+                    g.streamVolumeFrame(s, d, t)
+                torch.cuda.synchronize()
+                update += time.perf_counter() - t0
+            for g, s in zip(passes, specs):
+                g.setCamera(s, f)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            b.execute(outs, stream=stream)
+            torch.cuda.synchronize()
+            render += time.perf_counter() - t0
+            if f in evals:
+                for v, o in enumerate(outs):
+                    imgs[(v, f)] = lattice_of(o)
+                if (0, f) not in refs:  # the plain run makes the references, the learned run reuses them
+                    for v, (g, s) in enumerate(zip(passes, specs)):
+                        refs[(v, f)], spp, dt = lattice_reference(g, s, args.reference_spp, stream)
+                        ref_s += dt
+            if f % 10 == 0 or f == frames - 1:
+                say(f"config4 {'learned' if learned else 'plain'} frame {f}: render {1000 * render / (f + 1):.1f} "
+                    f"ms/frame, volume update {1000 * update / max(1, f):.1f} ms/frame")
+        gi = passes[0].gridInfo(0)
+        occ = gi["bricks"] / gi["brick_cells"]
+        rel = sum(passes[v].relmse(imgs[(v, f)], refs[(v, f)]) for v in range(len(specs)) for f in evals)
+        rel /= len(specs) * len(evals)
+        stats = passes[0].learnedStats().tolist() if learned else None
+        b.close()
+        for g in passes:
+            g.close()
+        plume.close()
+        torch.cuda.synchronize()
+        return {"frames_per_s": round(frames / render, 4), "ms_per_frame": round(1000 * render / frames, 3),
+                "volume_update_ms_per_frame": round(1000 * update / max(1, frames - 1), 3),
+                "relmse": rel, "setup_s": round(setup, 2), "reference_s": round(ref_s, 2),
+                "mip0_brick_occupancy": round(occ, 4), "learned_stats": stats}
+
+    plain = run(False)
+    say(f"config4 plain: {plain}")
+    learned = run(True)
+    say(f"config4 learned: {learned}")
+    line = {
+        "metric": "config4 learned + batched frames/s (4 views of a 1024^3 animated plume, 1080p each)",
+        "value": learned["frames_per_s"],
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": frames,
+        "warmup": 0,
+        "ms_per_step": learned["ms_per_frame"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (fp16 bricks)",
+        "data": "synthetic (seeded GPU plume simulation: buoyant updraft + curl noise, density + temperature)",
+        "config": {
+            "workload": f"config4: {res}^3 emissive plume, fp16 bricks, 4 mips with conservative twins, "
+                        f"{frames} frames after {warm} warm-up steps, 4 views x 1920x1080 in one batch, "
+                        "path length 3, M=8, 2 spatial rounds x 5 neighbours, 32 point lights + env, "
+                        "volume rebuilt on the GPU every frame",
+            "learned": learned,
+            "plain": plain,
+            "relmse_frames": list(evals),
+            "reference_spp": args.reference_spp,
+            "learned_matches_plain_relmse": learned["relmse"] <= plain["relmse"],
+        },
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="config1", choices=["config0", "config1", "config2"],
+    ap.add_argument("--config", default="config1", choices=["config0", "config1", "config2", "config4"],
                     help="config2: the batched workload (16 seeded views / volumes in one persistent launch per "
-                         "step, replayed as a HIP graph) against the same passes rendered one after another")
+                         "step, replayed as a HIP graph) against the same passes rendered one after another; "
+                         "config4: the learned + batched animated plume (4 views)")
+    ap.add_argument("--c4-res", type=int, default=1024, help="config4 plume grid side")
+    ap.add_argument("--c4-frames", type=int, default=60, help="config4 animation frames")
     ap.add_argument("--reference-spp", type=int, default=65536)
     ap.add_argument("--no-learned", action="store_true", help="time and report the plain pass only")
     ap.add_argument("--no-quality", action="store_true", help="skip the relMSE runs (profiling)")
@@ -328,6 +442,9 @@ def main():
     torch.cuda.set_device(0)
     if args.config == "config2":
         bench_batched(args)
+        return
+    if args.config == "config4":
+        bench_config4(args)
         return
     if args.calibration_copy_mb > 0:
         n = args.calibration_copy_mb * (1 << 20) // 4

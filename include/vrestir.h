@@ -173,6 +173,17 @@ const char* vrestir_last_error(const vrestir_handle* h);               /* NULL h
 
 /* ---- scene binding: setScene (VR/VolumetricReSTIR.cpp:1218-1256) ---- */
 int32_t vrestir_set_volume(vrestir_handle* h, const vrestir_volume_desc* desc);
+/* A volume built on the GPU (no counterpart in the reference, which loads prebuilt
+ * frames): desc->density and desc->temperature are DEVICE pointers (same layout as
+ * above; velocity must be NULL), read on `stream`. The grids equal what
+ * vrestir_set_volume builds from the same values, byte for byte. The first call
+ * (or the first after vrestir_set_volume / vrestir_load_vbx) starts a new volume
+ * with desc's material; each later call builds the next frame of an animation into
+ * a ring of two frames and binds it with the frame before it as the previous frame
+ * (as vrestir_set_volume_frame(h, i, 1) does for a resident sequence). Later frames
+ * keep the first frame's material and need its dimensions, mips, format and grids.
+ * Synchronises the device; temporary device memory is about three dense grids. */
+int32_t vrestir_stream_volume_frame(vrestir_handle* h, void* stream, const vrestir_volume_desc* desc);
 int32_t vrestir_load_vbx(vrestir_handle* h, const char* pathPrefix, int32_t numMips, int32_t brickFormat,
                          const float sigmaA[3], const float sigmaS[3], float g, float densityScale);
 int32_t vrestir_set_lights(vrestir_handle* h, const vrestir_point_light* lights, int32_t count,
@@ -448,6 +459,23 @@ int32_t vrestir_batch_get_tuning(const vrestir_batch* b, int32_t* grid, int32_t*
  * time per frame. */
 int32_t vrestir_batch_autotune(vrestir_batch* b, void* stream, const int32_t* widths, const int32_t* heights,
                                float* const* outRGBA, int32_t frames, float* bestMs);
+
+/* ---- the synthetic animated plume (GPU; no counterpart in the reference, whose
+ * animated volumes are files) ----
+ * Smoke density and temperature on an n^3 grid over the unit cube, rising with a
+ * buoyant updraft and stirred by seeded curl noise: one semi-Lagrangian step per
+ * vrestir_plume_advance step, a noisy hot source at the base. Deterministic for
+ * a seed. The grids (device memory, x fastest) feed vrestir_stream_volume_frame
+ * and stay valid until the next advance. n: a multiple of 8 in [16, 2048]; holds
+ * four n^3 float grids. */
+typedef struct vrestir_plume vrestir_plume;
+int32_t vrestir_plume_create(int32_t n, uint64_t seed, vrestir_plume** out);
+int32_t vrestir_plume_destroy(vrestir_plume* p);
+int32_t vrestir_plume_advance(vrestir_plume* p, void* stream, int32_t steps);
+int32_t vrestir_plume_grids(const vrestir_plume* p, const float** density, const float** temperature,
+                            int32_t* steps);
+/* Copies the current grids to host memory (n^3 floats each; either may be NULL). */
+int32_t vrestir_plume_read(const vrestir_plume* p, float* density, float* temperature);
 
 /* ---- host-side helpers (no GPU needed) ---- */
 /* Seeded fBm density (4-octave Perlin by default) for the synthetic workloads. */

@@ -181,6 +181,12 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "vrestir_destroy": (i32, [vp]),
         "vrestir_last_error": (c_char_p, [vp]),
         "vrestir_set_volume": (i32, [vp, POINTER(VolumeDesc)]),
+        "vrestir_stream_volume_frame": (i32, [vp, vp, POINTER(VolumeDesc)]),
+        "vrestir_plume_create": (i32, [i32, c_uint64, POINTER(vp)]),
+        "vrestir_plume_destroy": (i32, [vp]),
+        "vrestir_plume_advance": (i32, [vp, vp, i32]),
+        "vrestir_plume_grids": (i32, [vp, POINTER(vp), POINTER(vp), POINTER(i32)]),
+        "vrestir_plume_read": (i32, [vp, vp, vp]),
         "vrestir_load_vbx": (i32, [vp, c_char_p, i32, i32, POINTER(f32), POINTER(f32), f32, f32]),
         "vrestir_set_lights": (i32, [vp, POINTER(PointLight), i32, POINTER(f32)]),
         "vrestir_set_camera": (i32, [vp, POINTER(Camera)]),
@@ -245,7 +251,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
 
 EXPORTED_SYMBOLS = [
     "vrestir_version", "vrestir_default_params", "vrestir_create", "vrestir_destroy", "vrestir_last_error",
-    "vrestir_set_volume", "vrestir_load_vbx", "vrestir_set_lights", "vrestir_set_camera", "vrestir_set_env_map",
+    "vrestir_set_volume", "vrestir_stream_volume_frame", "vrestir_load_vbx", "vrestir_set_lights", "vrestir_set_camera", "vrestir_set_env_map",
     "vrestir_set_env_rotation", "vrestir_set_env_intensity", "vrestir_set_blackbody_lut", "vrestir_load_vbx_emission",
     "vrestir_add_volume_frame", "vrestir_set_volume_frame", "vrestir_volume_frame_count", "vrestir_set_freeze_frame",
     "vrestir_camera_animation",
@@ -259,6 +265,8 @@ EXPORTED_SYMBOLS = [
     "vrestir_default_learned", "vrestir_set_learned", "vrestir_get_learned", "vrestir_learned_stats",
     "vrestir_learned_get_weights", "vrestir_learned_set_weights", "vrestir_learned_forward",
     "vrestir_learned_gradient", "vrestir_learned_train",
+    "vrestir_plume_create", "vrestir_plume_destroy", "vrestir_plume_advance", "vrestir_plume_grids",
+    "vrestir_plume_read",
     "vrestir_synth_fbm", "vrestir_orbit_camera",
     "vrestir_count_bricks", "vrestir_write_vbx", "vrestir_read_vbx",
 ]

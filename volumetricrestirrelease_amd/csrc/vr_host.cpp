@@ -73,6 +73,7 @@ struct vrestir_handle {
     // record / trace / replay buffers of the reuse stages
     int splitReuse = 2;  // 0: one kernel; 1: balanced walker queue; 2: brick-segment trace
     int splitTrace = 1;  // TraceRays' shadow rays as brick segments (with splitReuse == 2)
+    bool streamed = false;  // the volume is a ring of two frames built on the GPU
     bool inBatch = false;  // staged by a batch, whose TraceRays step is one kernel
     SplitBufs split{};
     size_t splitPixels = 0;
@@ -377,6 +378,7 @@ static int uploadGrid(vrestir_handle* h, Grid& G, HostGrid& m, HostGrid& hg, std
     m.valid = true; m.nx = hg.nx; m.ny = hg.ny; m.nz = hg.nz; m.numBricks = hg.numBricks;
     m.maxValue = hg.maxValue; m.fp16 = hg.fp16; m.channels = hg.channels;
     m.nbx = hg.nbx; m.nby = hg.nby; m.nbz = hg.nbz;
+    m.nnx = hg.nnx; m.nny = hg.nny; m.nnz = hg.nnz;
     return VRESTIR_OK;
 }
 
@@ -422,14 +424,8 @@ static int bindFrame(vrestir_handle* h, int cur, bool animating) {
     return uploadScene(h);
 }
 
-// Uploads a built volume as frame 0 of a new sequence with the material of d.
-static int installVolume(vrestir_handle* h, HostVolume& hv, const vrestir_volume_desc* d) {
-    int rc;
-    HIPCHK(h, hipSetDevice(h->device));
-    HIPCHK(h, hipDeviceSynchronize());
-    freeVolume(h);
-    h->frames.emplace_back();
-    if ((rc = uploadFrame(h, hv, h->frames.back()))) return rc;
+// The material of a volume (a sequence keeps its first frame's).
+static void setMaterial(vrestir_handle* h, const vrestir_volume_desc* d) {
     Scene& S = h->sceneHost;
     S.sigma_t = d->sigmaS[0] + d->sigmaA[0];
     for (int i = 0; i < 3; i++) { S.sigma_s[i] = d->sigmaS[i]; S.sigma_a[i] = d->sigmaA[i]; }
@@ -441,11 +437,22 @@ static int installVolume(vrestir_handle* h, HostVolume& hv, const vrestir_volume
     S.tempCutoff = d->temperatureCutOff;
     S.tempScale = d->temperatureScale;
     S.velocityScale = d->velocityScale;
+}
+
+// Uploads a built volume as frame 0 of a new sequence with the material of d.
+static int installVolume(vrestir_handle* h, HostVolume& hv, const vrestir_volume_desc* d) {
+    int rc;
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipDeviceSynchronize());
+    freeVolume(h);
+    h->streamed = false;
+    h->frames.emplace_back();
+    if ((rc = uploadFrame(h, hv, h->frames.back()))) return rc;
+    setMaterial(h, d);
     h->volumeSet = true;
     h->optionsChanged = true;
     return bindFrame(h, 0, false);
 }
-
 extern "C" int32_t vrestir_set_volume(vrestir_handle* h, const vrestir_volume_desc* d) {
     if (!h || !d) return VRESTIR_EINVAL;
     HostVolume hv;
@@ -454,12 +461,146 @@ extern "C" int32_t vrestir_set_volume(vrestir_handle* h, const vrestir_volume_de
     return installVolume(h, hv, d);
 }
 
+// ------------------------------------------------------------------ volumes built on the GPU
+// (no counterpart in the reference, which loads every frame from prebuilt files)
+// Frame f from dense device grids: the slots buildVolume makes (mips, conservative
+// grids, temperature), built by the kernels of vr_k_build.hip value for value.
+static void freeFrame(VolumeFrame& f) {
+    for (void* p : f.allocs) (void)hipFree(p);
+    f = VolumeFrame();
+}
+static int buildFrameDevice(vrestir_handle* h, hipStream_t st, const vrestir_volume_desc& d, VolumeFrame& f) {
+    int rc = checkVolumeDesc(d, h->err);
+    if (rc) return rc;
+    if (d.velocity) return fail(h, VRESTIR_EUNSUPPORTED, "a volume built on the GPU takes no velocity grid");
+    freeFrame(f);
+    const bool fp16 = d.brickFormat == VRESTIR_BRICK_FP16;
+    const size_t n = (size_t)d.nx * d.ny * d.nz;
+    std::vector<void*> temps;
+    auto tmp = [&](size_t bytes, void** p) -> int {
+        HIPCHK(h, hipMalloc(p, std::max<size_t>(bytes, 16)));
+        temps.push_back(*p);
+        return VRESTIR_OK;
+    };
+    auto done = [&](int r) {
+        (void)hipStreamSynchronize(st);
+        for (void* p : temps) (void)hipFree(p);
+        if (r) freeFrame(f);
+        return r;
+    };
+    float *cur = nullptr, *cons = nullptr;
+    int* bad = nullptr;
+    void* scratch = nullptr;
+    if ((rc = tmp(n * 4, (void**)&cur)) || (rc = tmp(n * 4, (void**)&cons)) || (rc = tmp(16, (void**)&bad)) ||
+        (rc = tmp(buildScratchBytes(d.nx, d.ny, d.nz), &scratch)))
+        return done(rc);
+    // values are clamped to >= 0; a non-finite one is refused, as the host build refuses it
+    auto clampChecked = [&](const float* src, float* dst, const char* what) -> int {
+        int nbad = 0;
+        HIPCHK(h, hipMemsetAsync(bad, 0, 4, st));
+        HIPCHK(h, buildClamp(st, src, dst, n, bad));
+        HIPCHK(h, hipMemcpyAsync(&nbad, bad, 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(h, hipStreamSynchronize(st));
+        if (nbad) return fail(h, VRESTIR_EINVAL, std::string("the ") + what + " grid holds a non-finite value");
+        return VRESTIR_OK;
+    };
+    auto slot = [&](int s, const float* g, int nx, int ny, int nz, int scale) -> int {
+        uint32_t nb = 0;
+        float mv = 0.f;
+        HIPCHK(h, buildSlotDevice(st, g, nx, ny, nz, fp16, scratch, f.grids[s], nb, mv, f.allocs));
+        mipTransform(d.indexToWorld, scale, f.grids[s].w2i);
+        HostGrid& m = f.meta[s];
+        m = HostGrid();
+        m.valid = true; m.nx = nx; m.ny = ny; m.nz = nz; m.numBricks = nb; m.maxValue = mv; m.fp16 = fp16;
+        m.channels = 1; m.nbx = nx / 8; m.nby = ny / 8; m.nbz = nz / 8;
+        m.nnx = f.grids[s].nnx; m.nny = f.grids[s].nny; m.nnz = f.grids[s].nnz;
+        return VRESTIR_OK;
+    };
+    // a failed launch leaves through done(), which releases the temporaries
+    auto ck = [&](hipError_t e, const char* what) -> int {
+        return e == hipSuccess ? VRESTIR_OK : fail(h, VRESTIR_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    if ((rc = clampChecked(d.density, cur, "density"))) return done(rc);
+    if ((rc = ck(buildDilate(st, cur, cons, d.nx, d.ny, d.nz), "buildDilate"))) return done(rc);
+    int nx = d.nx, ny = d.ny, nz = d.nz;
+    for (int k = 0; k < d.numMips; k++) {
+        if (k > 0) {
+            float *n2 = nullptr, *c2 = nullptr;
+            const size_t m = (size_t)(nx / 2) * (ny / 2) * (nz / 2);
+            if ((rc = tmp(m * 4, (void**)&n2)) || (rc = tmp(m * 4, (void**)&c2))) return done(rc);
+            if ((rc = ck(buildReduce2(st, cur, n2, nx, ny, nz), "buildReduce2")) ||
+                (rc = ck(buildReduce2(st, cons, c2, nx, ny, nz), "buildReduce2")))
+                return done(rc);
+            cur = n2; cons = c2;
+            nx /= 2; ny /= 2; nz /= 2;
+        }
+        if ((rc = slot(k, cur, nx, ny, nz, 1 << k)) || (rc = slot(8 + k, cons, nx, ny, nz, 1 << k))) return done(rc);
+    }
+    if (d.temperature) {
+        // one full-resolution slot with mip 0's transform
+        float* t = nullptr;
+        if ((rc = tmp(n * 4, (void**)&t))) return done(rc);
+        if ((rc = clampChecked(d.temperature, t, "temperature"))) return done(rc);
+        if ((rc = slot(kTemperatureSlot, t, d.nx, d.ny, d.nz, 1))) return done(rc);
+    }
+    HostVolume hv;
+    volumeSpacing(d.indexToWorld, hv);
+    f.numMips = d.numMips;
+    f.tStep = hv.tStep;
+    f.superVoxelDiag = hv.superVoxelDiag;
+    return done(VRESTIR_OK);
+}
+
+extern "C" int32_t vrestir_stream_volume_frame(vrestir_handle* h, void* stream, const vrestir_volume_desc* d) {
+    if (!h || !d) return VRESTIR_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(h, hipSetDevice(h->device));
+    // the frame rebuilt below may still be read by work queued on any stream
+    HIPCHK(h, hipDeviceSynchronize());
+    int rc;
+    if (!h->volumeSet || !h->streamed) {
+        freeVolume(h);
+        h->frames.emplace_back();
+        if ((rc = buildFrameDevice(h, st, *d, h->frames[0]))) {
+            freeVolume(h);
+            h->volumeSet = false;
+            return rc;
+        }
+        setMaterial(h, d);
+        h->streamed = true;
+        h->volumeSet = true;
+        h->optionsChanged = true;
+        return bindFrame(h, 0, false);
+    }
+    const HostGrid& m0 = h->frames[h->curFrame].meta[0];
+    if (d->nx != m0.nx || d->ny != m0.ny || d->nz != m0.nz || d->numMips != h->frames[h->curFrame].numMips ||
+        (d->brickFormat == VRESTIR_BRICK_FP16) != m0.fp16 ||
+        (d->temperature != nullptr) != (h->frames[h->curFrame].grids[kTemperatureSlot].valid != 0))
+        return fail(h, VRESTIR_EINVAL, "every streamed frame needs the first frame's dimensions, mips, format and grids");
+    int target = 1 - h->curFrame;
+    if (h->frames.size() == 1) {
+        h->frames.emplace_back();
+        target = 1;
+    }
+    if ((rc = buildFrameDevice(h, st, *d, h->frames[target]))) {
+        // the bound scene's previous-frame slots named the frame just released:
+        // keep the current frame alone, as a one-frame sequence
+        const std::string err = h->err;
+        h->frames.erase(h->frames.begin() + target);
+        (void)bindFrame(h, 0, false);
+        h->err = err;
+        return rc;
+    }
+    return bindFrame(h, target, true);
+}
+
 // addGVDBVolumeSequence (SC/Scene.cpp:3329-3355): one more frame of the sequence,
 // kept resident in HBM. Its material (sigma, g, density and emission scales)
 // is the first frame's, as the reference keeps them across frames (SC/Scene.cpp:829-855).
 extern "C" int32_t vrestir_add_volume_frame(vrestir_handle* h, const vrestir_volume_desc* d, int32_t* index) {
     if (!h || !d) return VRESTIR_EINVAL;
     if (!h->volumeSet) return fail(h, VRESTIR_ESTATE, "bind the first frame with vrestir_set_volume");
+    if (h->streamed) return fail(h, VRESTIR_ESTATE, "a streamed volume takes frames from vrestir_stream_volume_frame");
     if (d->numMips != h->frames[0].numMips) return fail(h, VRESTIR_EINVAL, "every frame needs the first frame's numMips");
     HostVolume hv;
     int rc = buildVolume(*d, hv, h->err);

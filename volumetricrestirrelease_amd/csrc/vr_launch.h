@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "vr_types.h"
 
 namespace vr {
@@ -105,6 +107,16 @@ hipError_t launchLearnedReduce(hipStream_t st, const float* partial, int waves, 
                                float* m, float* v, float lr, float c1, float c2, float* stats, int* inUse, bool judge,
                                int step);
 hipError_t launchLearnedPack(hipStream_t st, const float* prm, uint4* frags);
+// Volume grids built on the GPU from dense device grids (vr_k_build.hip): the
+// host build's clamp, conservative dilation and 2x reduction, and one slot
+// (tables, occupancy words, pool, bounds) from a dense grid. buildSlotDevice
+// appends its allocations to allocs and synchronises the stream.
+hipError_t buildClamp(hipStream_t st, const float* in, float* out, size_t n, int* bad);
+hipError_t buildDilate(hipStream_t st, const float* s, float* d, int nx, int ny, int nz);
+hipError_t buildReduce2(hipStream_t st, const float* s, float* d, int nx, int ny, int nz);
+size_t buildScratchBytes(int nx, int ny, int nz);
+hipError_t buildSlotDevice(hipStream_t st, const float* dense, int nx, int ny, int nz, bool fp16, void* scratch,
+                           Grid& G, uint32_t& numBricks, float& maxValue, std::vector<void*>& allocs);
 // Importance mip chain of a lat-long map: level k of dim >> k texels at imp + off[k].
 hipError_t launchEnvImportance(hipStream_t st, const float* tex, int W, int H, float* imp, const int* off, int dim,
                                int spp);

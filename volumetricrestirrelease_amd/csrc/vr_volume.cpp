@@ -294,6 +294,7 @@ static void mipTransforms(const float* i2w0, int scale, float* w2i) {
 }
 
 void slotTransform(const float* indexToWorld, float* w2i) { mipTransforms(indexToWorld, 1, w2i); }
+void mipTransform(const float* indexToWorld, int scale, float* w2i) { mipTransforms(indexToWorld, scale, w2i); }
 
 // The dense grid a brick list holds (values <= 0 read 0).
 static DenseF denseOf(const VbxBricks& b) {
@@ -382,7 +383,7 @@ void volumeSpacing(const float* X, HostVolume& v) {
     v.superVoxelDiag = 8.f * std::sqrt(rl2(0) + rl2(1) + rl2(2));
 }
 
-int buildVolume(const vrestir_volume_desc& d, HostVolume& out, std::string& err) {
+int checkVolumeDesc(const vrestir_volume_desc& d, std::string& err) {
     if (d.numMips < 1 || d.numMips > 8) { err = "numMips must be in [1, 8]"; return VRESTIR_EINVAL; }
     if (!d.density) { err = "density is NULL"; return VRESTIR_EINVAL; }
     int align = 8 << (d.numMips - 1);
@@ -400,6 +401,12 @@ int buildVolume(const vrestir_volume_desc& d, HostVolume& out, std::string& err)
         err = "unknown brickFormat";
         return VRESTIR_EINVAL;
     }
+    return VRESTIR_OK;
+}
+
+int buildVolume(const vrestir_volume_desc& d, HostVolume& out, std::string& err) {
+    int rc0 = checkVolumeDesc(d, err);
+    if (rc0) return rc0;
     out = HostVolume();
     DenseF base;
     base.nx = d.nx; base.ny = d.ny; base.nz = d.nz;

@@ -65,7 +65,11 @@ int buildVolumeFromFiles(const std::string& prefix, int numMips, bool fp16, Host
 
 uint16_t floatToHalfBits(float f);
 float halfBitsToFloat(uint16_t h);
+// The checks buildVolume makes of a descriptor (mips, dimensions, grey sigma_t, format).
+int checkVolumeDesc(const vrestir_volume_desc& d, std::string& err);
 int buildVolume(const vrestir_volume_desc& d, HostVolume& out, std::string& err);
+// World -> index of mip level `scale` (1, 2, 4, ...) from mip 0's index -> world.
+void mipTransform(const float* indexToWorld, int scale, float* w2i);
 int buildTemperatureSlot(const float* t, int nx, int ny, int nz, const float* indexToWorld, bool fp16,
                          HostGrid& out, std::string& err);
 int buildVelocitySlot(const float* v, int nx, int ny, int nz, const float* indexToWorld, HostGrid& out,

@@ -50,6 +50,11 @@ class VolumetricReSTIRParams:
         return self._p
 
 
+def _dev_ptr(x):
+    """A device address: a torch tensor's data pointer, or an int address."""
+    return int(x) if isinstance(x, int) else x.data_ptr()
+
+
 def _stream_ptr(stream):
     return ctypes.c_void_p(stream.cuda_stream if stream is not None else 0)
 
@@ -90,7 +95,27 @@ class VolumetricReSTIR:
     def setScene(self, spec, frame: int = 0):
         d = spec.volume_desc()
         self._check(self.lib.vrestir_set_volume(self.h, ctypes.byref(d)), "set_volume")
-        if spec.temperature is not None:
+        self._setSceneRest(spec, frame, spec.temperature is not None)
+
+    def streamVolumeFrame(self, spec, density, temperature=None, stream=None):
+        """vrestir_stream_volume_frame: a volume frame built on the GPU from float32
+        device tensors shaped (nz, ny, nx); spec gives the dimensions, mips, format,
+        transform and material (its host grids are not read). The first call starts a
+        new volume, later ones advance its animation by one frame."""
+        d = spec.volume_desc(grids=False)
+        d.density = ctypes.cast(ctypes.c_void_p(_dev_ptr(density)), ctypes.POINTER(ctypes.c_float))
+        d.temperature = (ctypes.cast(ctypes.c_void_p(_dev_ptr(temperature)), ctypes.POINTER(ctypes.c_float))
+                         if temperature is not None else None)
+        self._check(self.lib.vrestir_stream_volume_frame(self.h, _stream_ptr(stream), ctypes.byref(d)),
+                    "stream_volume_frame")
+
+    def setSceneStreamed(self, spec, density, temperature=None, frame: int = 0):
+        """setScene with the volume built on the GPU from device tensors."""
+        self.streamVolumeFrame(spec, density, temperature)
+        self._setSceneRest(spec, frame, temperature is not None)
+
+    def _setSceneRest(self, spec, frame, emissive):
+        if emissive:
             lut, rows = spec.blackbody_array()
             self._check(self.lib.vrestir_set_blackbody_lut(self.h, lut.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                                                            rows), "set_blackbody_lut")
@@ -380,3 +405,49 @@ class VolumetricReSTIRBatch:
         self._check(self.lib.vrestir_batch_autotune(self.b, _stream_ptr(stream), W, H, O, frames, ctypes.byref(ms)),
                     "batch_autotune")
         return ms.value
+
+
+class Plume:
+    """The seeded synthetic animated plume (vrestir_plume_*), simulated on the GPU:
+    density and temperature on an n^3 grid over the unit cube."""
+
+    def __init__(self, n: int, seed: int):
+        self.lib = abi.load_library()
+        p = ctypes.c_void_p()
+        rc = self.lib.vrestir_plume_create(n, seed, ctypes.byref(p))
+        if rc != 0:
+            raise VolumetricReSTIRError(f"vrestir_plume_create: status {rc}")
+        self.p, self.n = p, n
+
+    def advance(self, steps: int = 1, stream=None):
+        rc = self.lib.vrestir_plume_advance(self.p, _stream_ptr(stream), steps)
+        if rc != 0:
+            raise VolumetricReSTIRError(f"vrestir_plume_advance: status {rc}")
+
+    def grids(self):
+        """(density address, temperature address, steps simulated) of the current grids."""
+        d, t, s = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_int32()
+        rc = self.lib.vrestir_plume_grids(self.p, ctypes.byref(d), ctypes.byref(t), ctypes.byref(s))
+        if rc != 0:
+            raise VolumetricReSTIRError(f"vrestir_plume_grids: status {rc}")
+        return d.value, t.value, s.value
+
+    def read(self):
+        """Host copies (nz, ny, nx) of the current density and temperature."""
+        d = np.zeros((self.n, self.n, self.n), np.float32)
+        t = np.zeros_like(d)
+        rc = self.lib.vrestir_plume_read(self.p, ctypes.c_void_p(d.ctypes.data), ctypes.c_void_p(t.ctypes.data))
+        if rc != 0:
+            raise VolumetricReSTIRError(f"vrestir_plume_read: status {rc}")
+        return d, t
+
+    def close(self):
+        if getattr(self, "p", None):
+            self.lib.vrestir_plume_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
