@@ -184,6 +184,13 @@ int32_t vrestir_set_volume(vrestir_handle* h, const vrestir_volume_desc* desc);
  * keep the first frame's material and need its dimensions, mips, format and grids.
  * Synchronises the device; temporary device memory is about three dense grids. */
 int32_t vrestir_stream_volume_frame(vrestir_handle* h, void* stream, const vrestir_volume_desc* desc);
+/* Binds owner's volume into h read-only (no counterpart in the reference, where the
+ * passes of a frame share the Scene): the current and previous frame's grids, the
+ * mips, the material and the animation state; h keeps its own lights, environment,
+ * camera and options. A copy of the binding at the call: call it again after the
+ * owner's volume changes (each vrestir_stream_volume_frame / vrestir_set_volume_frame).
+ * The owner must outlive h's use of the volume. */
+int32_t vrestir_share_volume(vrestir_handle* h, const vrestir_handle* owner);
 int32_t vrestir_load_vbx(vrestir_handle* h, const char* pathPrefix, int32_t numMips, int32_t brickFormat,
                          const float sigmaA[3], const float sigmaS[3], float g, float densityScale);
 int32_t vrestir_set_lights(vrestir_handle* h, const vrestir_point_light* lights, int32_t count,

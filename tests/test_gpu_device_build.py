@@ -162,3 +162,36 @@ def test_stream_refuses_bad_frames_and_keeps_rendering():
     d.velocity = ctypes.cast(ctypes.c_void_p(v.data_ptr()), ctypes.POINTER(ctypes.c_float))
     rc = abi.load_library().vrestir_stream_volume_frame(g.h, None, ctypes.byref(d))
     assert rc == abi.VRESTIR_EUNSUPPORTED
+
+
+def test_shared_volume_renders_like_an_own_copy():
+    """A view bound to another handle's streamed volume (vrestir_share_volume, again
+    after each owner frame) renders like a handle that streams its own copy, over a
+    3-frame animation with previous-frame grids in temporal reuse."""
+    say("case: 3-frame moving 32^3 sequence, a shared view vs its own streamed copy")
+    _, specs = scenes.moving_sequence(frames=3)
+    for s in specs:
+        s.velocity = None
+    owner = VolumetricReSTIR()
+    owner.setSceneStreamed(specs[0], dev(specs[0].density))
+    shared = VolumetricReSTIR()
+    shared.setSceneShared(specs[0], owner)
+    own = VolumetricReSTIR()
+    own.setSceneStreamed(specs[0], dev(specs[0].density))
+
+    def advance_shared(g, f):
+        if f > 0:
+            owner.streamVolumeFrame(specs[f], dev(specs[f].density))
+            g.shareVolume(owner)
+
+    def advance_own(g, f):
+        if f > 0:
+            g.streamVolumeFrame(specs[f], dev(specs[f].density))
+
+    ia = render(shared, specs[0], 3, advance_shared)
+    ib = render(own, specs[0], 3, advance_own)
+    for f in range(3):
+        assert ia[f][..., :3].mean() > 0
+        assert np.array_equal(ia[f], ib[f]), f
+    with pytest.raises(VolumetricReSTIRError, match="another handle"):
+        shared.addVolumeFrame(specs[1])

@@ -182,6 +182,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "vrestir_last_error": (c_char_p, [vp]),
         "vrestir_set_volume": (i32, [vp, POINTER(VolumeDesc)]),
         "vrestir_stream_volume_frame": (i32, [vp, vp, POINTER(VolumeDesc)]),
+        "vrestir_share_volume": (i32, [vp, vp]),
         "vrestir_plume_create": (i32, [i32, c_uint64, POINTER(vp)]),
         "vrestir_plume_destroy": (i32, [vp]),
         "vrestir_plume_advance": (i32, [vp, vp, i32]),
@@ -251,7 +252,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
 
 EXPORTED_SYMBOLS = [
     "vrestir_version", "vrestir_default_params", "vrestir_create", "vrestir_destroy", "vrestir_last_error",
-    "vrestir_set_volume", "vrestir_stream_volume_frame", "vrestir_load_vbx", "vrestir_set_lights", "vrestir_set_camera", "vrestir_set_env_map",
+    "vrestir_set_volume", "vrestir_stream_volume_frame", "vrestir_share_volume", "vrestir_load_vbx", "vrestir_set_lights", "vrestir_set_camera", "vrestir_set_env_map",
     "vrestir_set_env_rotation", "vrestir_set_env_intensity", "vrestir_set_blackbody_lut", "vrestir_load_vbx_emission",
     "vrestir_add_volume_frame", "vrestir_set_volume_frame", "vrestir_volume_frame_count", "vrestir_set_freeze_frame",
     "vrestir_camera_animation",

@@ -23,10 +23,33 @@ using namespace vr;
 static thread_local std::string g_createError;
 
 // One frame of a volume sequence: the grid descriptors of slots 0..17 and their allocations.
+// A device buffer kept across builds: reallocated only when a build needs more.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+static hipError_t growBuf(DevBuf& b, size_t bytes, void** out) {
+    bytes = std::max<size_t>(bytes, 16);
+    if (b.cap < bytes) {
+        if (b.p) (void)hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+        hipError_t e = hipMalloc(&b.p, bytes);
+        if (e != hipSuccess) return e;
+        b.cap = bytes;
+    }
+    *out = b.p;
+    return hipSuccess;
+}
+static void freeBuf(DevBuf& b) {
+    if (b.p) (void)hipFree(b.p);
+    b = DevBuf();
+}
 struct VolumeFrame {
     Grid grids[kFrameSlots];
     HostGrid meta[kFrameSlots];  // metadata only (pools are released after upload)
     std::vector<void*> allocs;
+    DevBuf built[kFrameSlots][kBufKinds];  // the slots of a frame built on the GPU
     int numMips = 0;
     float tStep = 1.f, superVoxelDiag = 1.f;
 };
@@ -74,6 +97,8 @@ struct vrestir_handle {
     int splitReuse = 2;  // 0: one kernel; 1: balanced walker queue; 2: brick-segment trace
     int splitTrace = 1;  // TraceRays' shadow rays as brick segments (with splitReuse == 2)
     bool streamed = false;  // the volume is a ring of two frames built on the GPU
+    const vrestir_handle* sharedFrom = nullptr;  // the owner of the bound volume (vrestir_share_volume)
+    DevBuf buildTmp[20];    // the GPU build's temporaries (grids per mip level, temperature, flags, scratch)
     bool inBatch = false;  // staged by a batch, whose TraceRays step is one kernel
     SplitBufs split{};
     size_t splitPixels = 0;
@@ -143,9 +168,13 @@ static void freeFrameBuffers(vrestir_handle* h) {
     h->W = h->H = 0;
 }
 static void freeVolume(vrestir_handle* h) {
-    for (VolumeFrame& f : h->frames)
+    for (VolumeFrame& f : h->frames) {
         for (void* p : f.allocs) (void)hipFree(p);
+        for (auto& slot : f.built)
+            for (DevBuf& b : slot) freeBuf(b);
+    }
     h->frames.clear();
+    for (DevBuf& b : h->buildTmp) freeBuf(b);
     h->curFrame = 0;
     h->hasAnimation = false;
 }
@@ -446,6 +475,7 @@ static int installVolume(vrestir_handle* h, HostVolume& hv, const vrestir_volume
     HIPCHK(h, hipDeviceSynchronize());
     freeVolume(h);
     h->streamed = false;
+    h->sharedFrom = nullptr;
     h->frames.emplace_back();
     if ((rc = uploadFrame(h, hv, h->frames.back()))) return rc;
     setMaterial(h, d);
@@ -467,47 +497,59 @@ extern "C" int32_t vrestir_set_volume(vrestir_handle* h, const vrestir_volume_de
 // grids, temperature), built by the kernels of vr_k_build.hip value for value.
 static void freeFrame(VolumeFrame& f) {
     for (void* p : f.allocs) (void)hipFree(p);
+    for (auto& slot : f.built)
+        for (DevBuf& b : slot) freeBuf(b);
     f = VolumeFrame();
 }
 static int buildFrameDevice(vrestir_handle* h, hipStream_t st, const vrestir_volume_desc& d, VolumeFrame& f) {
     int rc = checkVolumeDesc(d, h->err);
     if (rc) return rc;
     if (d.velocity) return fail(h, VRESTIR_EUNSUPPORTED, "a volume built on the GPU takes no velocity grid");
-    freeFrame(f);
+    // the frame keeps its slots' buffers and the handle its temporaries from one
+    // build to the next: a streamed frame of the same volume reallocates nothing
+    // unless a slot needs more bricks than before
+    for (void* p : f.allocs) (void)hipFree(p);
+    f.allocs.clear();
+    for (int s = 0; s < kFrameSlots; s++) {
+        std::memset(&f.grids[s], 0, sizeof(Grid));
+        f.meta[s] = HostGrid();
+    }
     const bool fp16 = d.brickFormat == VRESTIR_BRICK_FP16;
     const size_t n = (size_t)d.nx * d.ny * d.nz;
-    std::vector<void*> temps;
-    auto tmp = [&](size_t bytes, void** p) -> int {
-        HIPCHK(h, hipMalloc(p, std::max<size_t>(bytes, 16)));
-        temps.push_back(*p);
+    auto tmp = [&](int key, size_t bytes, void** p) -> int {
+        HIPCHK(h, growBuf(h->buildTmp[key], bytes, p));
         return VRESTIR_OK;
     };
     auto done = [&](int r) {
         (void)hipStreamSynchronize(st);
-        for (void* p : temps) (void)hipFree(p);
         if (r) freeFrame(f);
         return r;
     };
     float *cur = nullptr, *cons = nullptr;
-    int* bad = nullptr;
+    // per-build counters: [0] and [19] non-finite density / temperature values; the
+    // largest value of each grid as float bits: mip k at [1 + k], conservative mip k
+    // at [9 + k], temperature at [17] (each left by the kernel that writes the grid)
+    unsigned* stats = nullptr;
     void* scratch = nullptr;
-    if ((rc = tmp(n * 4, (void**)&cur)) || (rc = tmp(n * 4, (void**)&cons)) || (rc = tmp(16, (void**)&bad)) ||
-        (rc = tmp(buildScratchBytes(d.nx, d.ny, d.nz), &scratch)))
+    if ((rc = tmp(0, n * 4, (void**)&cur)) || (rc = tmp(1, n * 4, (void**)&cons)) ||
+        (rc = tmp(17, 20 * sizeof(unsigned), (void**)&stats)) ||
+        (rc = tmp(18, buildScratchBytes(d.nx, d.ny, d.nz), &scratch)))
         return done(rc);
+    HIPCHK(h, hipMemsetAsync(stats, 0, 20 * sizeof(unsigned), st));
     // values are clamped to >= 0; a non-finite one is refused, as the host build refuses it
-    auto clampChecked = [&](const float* src, float* dst, const char* what) -> int {
+    auto clampChecked = [&](const float* src, float* dst, int badAt, int maxAt, const char* what) -> int {
         int nbad = 0;
-        HIPCHK(h, hipMemsetAsync(bad, 0, 4, st));
-        HIPCHK(h, buildClamp(st, src, dst, n, bad));
-        HIPCHK(h, hipMemcpyAsync(&nbad, bad, 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(h, buildClamp(st, src, dst, n, (int*)stats + badAt, stats + maxAt));
+        HIPCHK(h, hipMemcpyAsync(&nbad, stats + badAt, 4, hipMemcpyDeviceToHost, st));
         HIPCHK(h, hipStreamSynchronize(st));
         if (nbad) return fail(h, VRESTIR_EINVAL, std::string("the ") + what + " grid holds a non-finite value");
         return VRESTIR_OK;
     };
-    auto slot = [&](int s, const float* g, int nx, int ny, int nz, int scale) -> int {
+    auto slot = [&](int s, const float* g, int nx, int ny, int nz, int scale, int maxAt) -> int {
         uint32_t nb = 0;
         float mv = 0.f;
-        HIPCHK(h, buildSlotDevice(st, g, nx, ny, nz, fp16, scratch, f.grids[s], nb, mv, f.allocs));
+        const BuildAlloc alloc = [&f, s](int kind, size_t bytes, void** p) { return growBuf(f.built[s][kind], bytes, p); };
+        HIPCHK(h, buildSlotDevice(st, g, nx, ny, nz, stats + maxAt, fp16, scratch, f.grids[s], nb, mv, alloc));
         mipTransform(d.indexToWorld, scale, f.grids[s].w2i);
         HostGrid& m = f.meta[s];
         m = HostGrid();
@@ -520,28 +562,29 @@ static int buildFrameDevice(vrestir_handle* h, hipStream_t st, const vrestir_vol
     auto ck = [&](hipError_t e, const char* what) -> int {
         return e == hipSuccess ? VRESTIR_OK : fail(h, VRESTIR_EHIP, std::string(what) + ": " + hipGetErrorString(e));
     };
-    if ((rc = clampChecked(d.density, cur, "density"))) return done(rc);
-    if ((rc = ck(buildDilate(st, cur, cons, d.nx, d.ny, d.nz), "buildDilate"))) return done(rc);
+    if ((rc = clampChecked(d.density, cur, 0, 1, "density"))) return done(rc);
+    if ((rc = ck(buildDilate(st, cur, cons, d.nx, d.ny, d.nz, stats + 9), "buildDilate"))) return done(rc);
     int nx = d.nx, ny = d.ny, nz = d.nz;
     for (int k = 0; k < d.numMips; k++) {
         if (k > 0) {
             float *n2 = nullptr, *c2 = nullptr;
             const size_t m = (size_t)(nx / 2) * (ny / 2) * (nz / 2);
-            if ((rc = tmp(m * 4, (void**)&n2)) || (rc = tmp(m * 4, (void**)&c2))) return done(rc);
-            if ((rc = ck(buildReduce2(st, cur, n2, nx, ny, nz), "buildReduce2")) ||
-                (rc = ck(buildReduce2(st, cons, c2, nx, ny, nz), "buildReduce2")))
+            if ((rc = tmp(2 * k, m * 4, (void**)&n2)) || (rc = tmp(2 * k + 1, m * 4, (void**)&c2))) return done(rc);
+            if ((rc = ck(buildReduce2(st, cur, n2, nx, ny, nz, stats + 1 + k), "buildReduce2")) ||
+                (rc = ck(buildReduce2(st, cons, c2, nx, ny, nz, stats + 9 + k), "buildReduce2")))
                 return done(rc);
             cur = n2; cons = c2;
             nx /= 2; ny /= 2; nz /= 2;
         }
-        if ((rc = slot(k, cur, nx, ny, nz, 1 << k)) || (rc = slot(8 + k, cons, nx, ny, nz, 1 << k))) return done(rc);
+        if ((rc = slot(k, cur, nx, ny, nz, 1 << k, 1 + k)) || (rc = slot(8 + k, cons, nx, ny, nz, 1 << k, 9 + k)))
+            return done(rc);
     }
     if (d.temperature) {
         // one full-resolution slot with mip 0's transform
         float* t = nullptr;
-        if ((rc = tmp(n * 4, (void**)&t))) return done(rc);
-        if ((rc = clampChecked(d.temperature, t, "temperature"))) return done(rc);
-        if ((rc = slot(kTemperatureSlot, t, d.nx, d.ny, d.nz, 1))) return done(rc);
+        if ((rc = tmp(16, n * 4, (void**)&t))) return done(rc);
+        if ((rc = clampChecked(d.temperature, t, 19, 17, "temperature"))) return done(rc);
+        if ((rc = slot(kTemperatureSlot, t, d.nx, d.ny, d.nz, 1, 17))) return done(rc);
     }
     HostVolume hv;
     volumeSpacing(d.indexToWorld, hv);
@@ -549,6 +592,49 @@ static int buildFrameDevice(vrestir_handle* h, hipStream_t st, const vrestir_vol
     f.tStep = hv.tStep;
     f.superVoxelDiag = hv.superVoxelDiag;
     return done(VRESTIR_OK);
+}
+
+// Binds another handle's volume: its current and previous frame's grids, mips,
+// material and animation state, read-only (the owner keeps the memory). Lights,
+// environment, camera and options stay h's own. Copied at the call, so a view
+// of a streamed volume calls it again after each vrestir_stream_volume_frame of
+// the owner.
+extern "C" int32_t vrestir_share_volume(vrestir_handle* h, const vrestir_handle* owner) {
+    if (!h || !owner || h == owner) return VRESTIR_EINVAL;
+    if (!owner->volumeSet) return fail(h, VRESTIR_ESTATE, "the owner has no volume");
+    if (owner->device != h->device) return fail(h, VRESTIR_EINVAL, "the owner is on another device");
+    if (!h->frames.empty()) {
+        HIPCHK(h, hipSetDevice(h->device));
+        HIPCHK(h, hipDeviceSynchronize());
+        freeVolume(h);
+    }
+    h->streamed = false;
+    Scene& S = h->sceneHost;
+    const Scene& O = owner->sceneHost;
+    std::memcpy(S.grid, O.grid, sizeof(S.grid));
+    S.sigma_t = O.sigma_t;
+    std::memcpy(S.sigma_s, O.sigma_s, sizeof(S.sigma_s));
+    std::memcpy(S.sigma_a, O.sigma_a, sizeof(S.sigma_a));
+    S.g = O.g;
+    S.densityScale = O.densityScale;
+    S.tStep = O.tStep;
+    S.worldScaling = O.worldScaling;
+    S.superVoxelDiag = O.superVoxelDiag;
+    S.numMips = O.numMips;
+    S.hasEmission = O.hasEmission;
+    S.LeScale = O.LeScale;
+    S.tempCutoff = O.tempCutoff;
+    S.tempScale = O.tempScale;
+    S.prevHasEmission = O.prevHasEmission;
+    S.hasVelocity = O.hasVelocity;
+    S.velocityScale = O.velocityScale;
+    h->meta = owner->meta;
+    h->hasAnimation = owner->hasAnimation;
+    h->volumeSet = true;
+    // a new volume resets the history; the owner's next frame is an animation step
+    if (h->sharedFrom != owner) h->optionsChanged = true;
+    h->sharedFrom = owner;
+    return uploadScene(h);
 }
 
 extern "C" int32_t vrestir_stream_volume_frame(vrestir_handle* h, void* stream, const vrestir_volume_desc* d) {
@@ -568,6 +654,7 @@ extern "C" int32_t vrestir_stream_volume_frame(vrestir_handle* h, void* stream, 
         }
         setMaterial(h, d);
         h->streamed = true;
+        h->sharedFrom = nullptr;
         h->volumeSet = true;
         h->optionsChanged = true;
         return bindFrame(h, 0, false);
@@ -601,6 +688,7 @@ extern "C" int32_t vrestir_add_volume_frame(vrestir_handle* h, const vrestir_vol
     if (!h || !d) return VRESTIR_EINVAL;
     if (!h->volumeSet) return fail(h, VRESTIR_ESTATE, "bind the first frame with vrestir_set_volume");
     if (h->streamed) return fail(h, VRESTIR_ESTATE, "a streamed volume takes frames from vrestir_stream_volume_frame");
+    if (h->frames.empty()) return fail(h, VRESTIR_ESTATE, "the bound volume is another handle's (vrestir_share_volume)");
     if (d->numMips != h->frames[0].numMips) return fail(h, VRESTIR_EINVAL, "every frame needs the first frame's numMips");
     HostVolume hv;
     int rc = buildVolume(*d, hv, h->err);
@@ -636,6 +724,7 @@ extern "C" int32_t vrestir_load_vbx_emission(vrestir_handle* h, const char* path
                                              float temperatureCutOff, float temperatureScale) {
     if (!h || !pathPrefix) return VRESTIR_EINVAL;
     if (!h->volumeSet) return fail(h, VRESTIR_ESTATE, "bind the density volume first");
+    if (h->frames.empty()) return fail(h, VRESTIR_ESTATE, "the bound volume is another handle's (vrestir_share_volume)");
     std::string path = std::string(pathPrefix) + "_temperature.vbx";
     int nx = 0, ny = 0, nz = 0;
     float i2w[16];

@@ -75,40 +75,57 @@ __device__ __forceinline__ float stored(float x, float gmax, int fp16) {
     return fp16 ? halfValue(halfBits(x)) : x;
 }
 
-// out = max(in, 0); bad counts non-finite inputs
+// The largest of each wave's values into *maxBits (values are >= 0, so their
+// bits order as the values do).
+__device__ __forceinline__ void waveMax(unsigned m, unsigned* __restrict__ maxBits) {
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o));
+    if ((threadIdx.x & 63) == 0 && m) atomicMax(maxBits, m);
+}
+// out = max(in, 0); bad counts non-finite inputs; maxBits: the largest output
 __global__ __launch_bounds__(kBuildThreads) void kClamp(const float* __restrict__ in, float* __restrict__ out,
-                                                        size_t n, int* __restrict__ bad) {
+                                                        size_t n, int* __restrict__ bad, unsigned* __restrict__ maxBits) {
+    unsigned m = 0u;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const float v = in[i];
         if (!isfinite(v)) atomicAdd(bad, 1);
-        out[i] = v > 0.f ? v : 0.f;
+        const float o = v > 0.f ? v : 0.f;
+        out[i] = o;
+        if (isfinite(v)) m = max(m, __float_as_uint(o));
     }
+    waveMax(m, maxBits);
 }
 // Conservative grid: an empty voxel takes the mean of the positive values of its
 // 27-neighbourhood (summed x outermost, as the host build does).
 __global__ __launch_bounds__(kBuildThreads) void kDilate(const float* __restrict__ s, float* __restrict__ d, int nx,
-                                                         int ny, int nz) {
+                                                         int ny, int nz, unsigned* __restrict__ maxBits) {
     const size_t n = (size_t)nx * ny * nz;
+    unsigned m = 0u;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((size_t)nx * ny));
         const float v = s[i];
-        if (v != 0.f) { d[i] = v; continue; }
-        float avg = 0.f;
-        for (int ii = -1; ii <= 1; ii++)
-            for (int jj = -1; jj <= 1; jj++)
-                for (int kk = -1; kk <= 1; kk++) {
-                    const float tv = at(s, x + ii, y + jj, z + kk, nx, ny, nz);
-                    avg += tv > 0.f ? tv : 0.f;
-                }
-        avg /= 27.f;
-        d[i] = avg > 0.f ? avg : v;
+        float o = v;
+        if (v == 0.f) {
+            float avg = 0.f;
+            for (int ii = -1; ii <= 1; ii++)
+                for (int jj = -1; jj <= 1; jj++)
+                    for (int kk = -1; kk <= 1; kk++) {
+                        const float tv = at(s, x + ii, y + jj, z + kk, nx, ny, nz);
+                        avg += tv > 0.f ? tv : 0.f;
+                    }
+            avg /= 27.f;
+            if (avg > 0.f) o = avg;
+        }
+        d[i] = o;
+        m = max(m, __float_as_uint(o));
     }
+    waveMax(m, maxBits);
 }
 // 2x box reduction, weight 1/8 per tap summed x-major, negative results to 0; d is nx/2 x ny/2 x nz/2
 __global__ __launch_bounds__(kBuildThreads) void kReduce2(const float* __restrict__ s, float* __restrict__ d, int nx,
-                                                          int ny, int nz) {
+                                                          int ny, int nz, unsigned* __restrict__ maxBits) {
     const int dx = nx / 2, dy = ny / 2, dz = nz / 2;
     const size_t n = (size_t)dx * dy * dz;
+    unsigned m = 0u;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const int x = (int)(i % dx), y = (int)((i / dx) % dy), z = (int)(i / ((size_t)dx * dy));
         float res = 0.f;
@@ -116,22 +133,17 @@ __global__ __launch_bounds__(kBuildThreads) void kReduce2(const float* __restric
             for (int b = 0; b < 2; b++)
                 for (int c = 0; c < 2; c++)
                     res = __fadd_rn(res, __fmul_rn(0.125f, s[lin(2 * x + a, 2 * y + b, 2 * z + c, nx, ny)]));
-        d[i] = res > 0.f ? res : 0.f;
+        const float o = res > 0.f ? res : 0.f;
+        d[i] = o;
+        m = max(m, __float_as_uint(o));
     }
-}
-// the largest value (all values are >= 0, so their bits order as the values do)
-__global__ __launch_bounds__(kBuildThreads) void kMax(const float* __restrict__ s, size_t n,
-                                                      unsigned* __restrict__ out) {
-    unsigned m = 0u;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        m = max(m, __float_as_uint(s[i]));
-    for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+    waveMax(m, maxBits);
 }
 // per brick cell: does any stored value of its 8^3 voxels exceed 0
 __global__ __launch_bounds__(kBuildThreads) void kBrickOcc(const float* __restrict__ s, int nx, int ny, int nz,
-                                                           int nbx, int nby, int nbz, float gmax, int fp16,
-                                                           uint8_t* __restrict__ occ) {
+                                                           int nbx, int nby, int nbz, const unsigned* __restrict__ gmaxBits,
+                                                           int fp16, uint8_t* __restrict__ occ) {
+    const float gmax = __uint_as_float(*gmaxBits);
     const size_t nb = (size_t)nbx * nby * nbz;
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < nb; c += (size_t)gridDim.x * blockDim.x) {
         const int bx = (int)(c % nbx), by = (int)((c / nbx) % nby), bz = (int)(c / ((size_t)nbx * nby));
@@ -202,8 +214,11 @@ __global__ __launch_bounds__(kBuildThreads) void kBrickIds(const uint32_t* __res
 template <int FP16>
 __global__ __launch_bounds__(kBuildThreads) void kBrickFill(const float* __restrict__ s, int nx, int ny, int nz,
                                                             int nbx, int nby, const uint32_t* __restrict__ cellOf,
-                                                            uint32_t nbricks, float gmax, void* __restrict__ pool,
-                                                            float* __restrict__ bounds) {
+                                                            const uint32_t* __restrict__ count,
+                                                            const unsigned* __restrict__ gmaxBits,
+                                                            void* __restrict__ pool, float* __restrict__ bounds) {
+    const float gmax = __uint_as_float(*gmaxBits);
+    const uint32_t nbricks = *count;
     for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < nbricks;
          id += (size_t)gridDim.x * blockDim.x) {
         const uint32_t c = cellOf[id];
@@ -231,77 +246,75 @@ dim3 gridFor(size_t n) { return dim3((unsigned)std::min<size_t>((n + kBuildThrea
 
 }  // namespace
 
-hipError_t buildClamp(hipStream_t st, const float* in, float* out, size_t n, int* bad) {
-    hipLaunchKernelGGL(kClamp, gridFor(n), dim3(kBuildThreads), 0, st, in, out, n, bad);
+hipError_t buildClamp(hipStream_t st, const float* in, float* out, size_t n, int* bad, unsigned* maxBits) {
+    hipLaunchKernelGGL(kClamp, gridFor(n), dim3(kBuildThreads), 0, st, in, out, n, bad, maxBits);
     return hipGetLastError();
 }
-hipError_t buildDilate(hipStream_t st, const float* s, float* d, int nx, int ny, int nz) {
-    hipLaunchKernelGGL(kDilate, gridFor((size_t)nx * ny * nz), dim3(kBuildThreads), 0, st, s, d, nx, ny, nz);
+hipError_t buildDilate(hipStream_t st, const float* s, float* d, int nx, int ny, int nz, unsigned* maxBits) {
+    hipLaunchKernelGGL(kDilate, gridFor((size_t)nx * ny * nz), dim3(kBuildThreads), 0, st, s, d, nx, ny, nz,
+                       maxBits);
     return hipGetLastError();
 }
-hipError_t buildReduce2(hipStream_t st, const float* s, float* d, int nx, int ny, int nz) {
+hipError_t buildReduce2(hipStream_t st, const float* s, float* d, int nx, int ny, int nz, unsigned* maxBits) {
     hipLaunchKernelGGL(kReduce2, gridFor((size_t)(nx / 2) * (ny / 2) * (nz / 2)), dim3(kBuildThreads), 0, st, s, d,
-                       nx, ny, nz);
+                       nx, ny, nz, maxBits);
     return hipGetLastError();
 }
 
-// One slot from a dense device grid (nx, ny, nz multiples of 8, values >= 0).
-// scratch: a device buffer of at least buildScratchBytes(nx, ny, nz) bytes.
-// Allocations of the slot are appended to allocs; the host-side metadata goes to
-// numBricks / maxValue. Synchronises the stream (the brick count sizes the pool).
+// One slot from a dense device grid (nx, ny, nz multiples of 8, values >= 0)
+// whose largest value's bits are at gmaxBits (device memory, left there by the
+// kernel that made the grid). scratch: a device buffer of at least
+// buildScratchBytes(nx, ny, nz) bytes.
+// The slot's buffers come from allocBuf (which may hand back a buffer of an
+// earlier build); the host-side metadata goes to numBricks / maxValue.
+// Synchronises the stream (the brick count sizes the pool).
 size_t buildScratchBytes(int nx, int ny, int nz) {
     const size_t nb = (size_t)(nx / 8) * (ny / 8) * (nz / 8), nw = (nb + 31) / 32;
     return 256 + nb + 4 * nw + 4 * nb + 256;
 }
-hipError_t buildSlotDevice(hipStream_t st, const float* dense, int nx, int ny, int nz, bool fp16, void* scratch,
-                           Grid& G, uint32_t& numBricks, float& maxValue, std::vector<void*>& allocs) {
+hipError_t buildSlotDevice(hipStream_t st, const float* dense, int nx, int ny, int nz, const unsigned* gmaxBits,
+                           bool fp16, void* scratch, Grid& G, uint32_t& numBricks, float& maxValue,
+                           const BuildAlloc& allocBuf) {
     hipError_t e;
     const int nbx = nx / 8, nby = ny / 8, nbz = nz / 8;
     const int nnx = (nbx + 15) / 16, nny = (nby + 15) / 16, nnz = (nbz + 15) / 16;
     const size_t n = (size_t)nx * ny * nz, nb = (size_t)nbx * nby * nbz, nw = (nb + 31) / 32;
     char* sp = (char*)scratch;
-    unsigned* maxBits = (unsigned*)sp;                    // [0]: max bits, [1]: brick count
+    uint32_t* count = (uint32_t*)sp;                     // the brick count
     uint8_t* occ = (uint8_t*)(sp + 256);
     uint32_t* cnt = (uint32_t*)(sp + 256 + ((nb + 255) / 256) * 256);
     uint32_t* cellOf = cnt + nw;
-    auto alloc = [&](size_t bytes, void** p) -> hipError_t {
-        hipError_t r = hipMalloc(p, std::max<size_t>(bytes, 16));
-        if (r == hipSuccess) allocs.push_back(*p);
-        return r;
-    };
     uint32_t *table, *nodes, *words;
-    if ((e = alloc(nb * 4, (void**)&table)) != hipSuccess) return e;
-    if ((e = alloc((size_t)nnx * nny * nnz * 4, (void**)&nodes)) != hipSuccess) return e;
-    if ((e = alloc(nw * 8, (void**)&words)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(maxBits, 0, 8, st)) != hipSuccess) return e;
+    if ((e = allocBuf(kBufTable, nb * 4, (void**)&table)) != hipSuccess) return e;
+    if ((e = allocBuf(kBufNodes, (size_t)nnx * nny * nnz * 4, (void**)&nodes)) != hipSuccess) return e;
+    if ((e = allocBuf(kBufWords, nw * 8, (void**)&words)) != hipSuccess) return e;
     if ((e = hipMemsetAsync(nodes, 0, (size_t)nnx * nny * nnz * 4, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(kMax, gridFor(n), dim3(kBuildThreads), 0, st, dense, n, maxBits);
-    unsigned hostMax = 0;
-    if ((e = hipMemcpyAsync(&hostMax, maxBits, 4, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
-    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
-    float gmax;
-    std::memcpy(&gmax, &hostMax, 4);
-    hipLaunchKernelGGL(kBrickOcc, gridFor(nb), dim3(kBuildThreads), 0, st, dense, nx, ny, nz, nbx, nby, nbz, gmax,
+    hipLaunchKernelGGL(kBrickOcc, gridFor(nb), dim3(kBuildThreads), 0, st, dense, nx, ny, nz, nbx, nby, nbz, gmaxBits,
                        fp16 ? 1 : 0, occ);
     hipLaunchKernelGGL(kOccWords, gridFor(nw), dim3(kBuildThreads), 0, st, occ, nb, words, cnt, nw);
-    hipLaunchKernelGGL(kScanWords, dim3(1), dim3(1024), 0, st, words, cnt, nw, (uint32_t*)maxBits + 1);
+    hipLaunchKernelGGL(kScanWords, dim3(1), dim3(1024), 0, st, words, cnt, nw, count);
     hipLaunchKernelGGL(kBrickIds, gridFor(nb), dim3(kBuildThreads), 0, st, words, nbx, nby, nbz, nnx, nny, table,
                        cellOf, nodes);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    // one wait per slot: the brick count sizes the pool (the maximum comes along)
     uint32_t nbr = 0;
-    if ((e = hipMemcpyAsync(&nbr, (uint32_t*)maxBits + 1, 4, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+    unsigned hostMax = 0;
+    if ((e = hipMemcpyAsync(&nbr, count, 4, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(&hostMax, gmaxBits, 4, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    float gmax;
+    std::memcpy(&gmax, &hostMax, 4);
     float* bounds;
     void* pool;
-    if ((e = alloc((size_t)nbr * 16, (void**)&bounds)) != hipSuccess) return e;
-    if ((e = alloc((size_t)nbr * kBrickVals * (fp16 ? 2 : 4), &pool)) != hipSuccess) return e;
+    if ((e = allocBuf(kBufBounds, (size_t)nbr * 16, (void**)&bounds)) != hipSuccess) return e;
+    if ((e = allocBuf(kBufPool, (size_t)nbr * kBrickVals * (fp16 ? 2 : 4), &pool)) != hipSuccess) return e;
     if (nbr > 0) {
         if (fp16)
             hipLaunchKernelGGL(kBrickFill<1>, gridFor(nbr), dim3(kBuildThreads), 0, st, dense, nx, ny, nz, nbx, nby,
-                               cellOf, nbr, gmax, pool, bounds);
+                               cellOf, count, gmaxBits, pool, bounds);
         else
             hipLaunchKernelGGL(kBrickFill<0>, gridFor(nbr), dim3(kBuildThreads), 0, st, dense, nx, ny, nz, nbx, nby,
-                               cellOf, nbr, gmax, pool, bounds);
+                               cellOf, count, gmaxBits, pool, bounds);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     // the stored form of the maximum (monotone, so the largest stored value)

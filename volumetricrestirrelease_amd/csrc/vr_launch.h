@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <vector>
 
 #include "vr_types.h"
@@ -111,12 +112,18 @@ hipError_t launchLearnedPack(hipStream_t st, const float* prm, uint4* frags);
 // host build's clamp, conservative dilation and 2x reduction, and one slot
 // (tables, occupancy words, pool, bounds) from a dense grid. buildSlotDevice
 // appends its allocations to allocs and synchronises the stream.
-hipError_t buildClamp(hipStream_t st, const float* in, float* out, size_t n, int* bad);
-hipError_t buildDilate(hipStream_t st, const float* s, float* d, int nx, int ny, int nz);
-hipError_t buildReduce2(hipStream_t st, const float* s, float* d, int nx, int ny, int nz);
+// Each also leaves the largest value it writes (as float bits, via atomicMax) at maxBits.
+hipError_t buildClamp(hipStream_t st, const float* in, float* out, size_t n, int* bad, unsigned* maxBits);
+hipError_t buildDilate(hipStream_t st, const float* s, float* d, int nx, int ny, int nz, unsigned* maxBits);
+hipError_t buildReduce2(hipStream_t st, const float* s, float* d, int nx, int ny, int nz, unsigned* maxBits);
 size_t buildScratchBytes(int nx, int ny, int nz);
-hipError_t buildSlotDevice(hipStream_t st, const float* dense, int nx, int ny, int nz, bool fp16, void* scratch,
-                           Grid& G, uint32_t& numBricks, float& maxValue, std::vector<void*>& allocs);
+// A slot's device buffer of at least `bytes` (kind: one of kBuf*), possibly one
+// kept from an earlier build of the same slot.
+enum { kBufTable = 0, kBufNodes, kBufWords, kBufBounds, kBufPool, kBufKinds };
+using BuildAlloc = std::function<hipError_t(int kind, size_t bytes, void** p)>;
+hipError_t buildSlotDevice(hipStream_t st, const float* dense, int nx, int ny, int nz, const unsigned* gmaxBits,
+                           bool fp16, void* scratch, Grid& G, uint32_t& numBricks, float& maxValue,
+                           const BuildAlloc& allocBuf);
 // Importance mip chain of a lat-long map: level k of dim >> k texels at imp + off[k].
 hipError_t launchEnvImportance(hipStream_t st, const float* tex, int W, int H, float* imp, const int* off, int dim,
                                int spp);

@@ -109,6 +109,15 @@ class VolumetricReSTIR:
         self._check(self.lib.vrestir_stream_volume_frame(self.h, _stream_ptr(stream), ctypes.byref(d)),
                     "stream_volume_frame")
 
+    def shareVolume(self, owner):
+        """vrestir_share_volume: bind owner's current volume frame (read-only)."""
+        self._check(self.lib.vrestir_share_volume(self.h, owner.h), "share_volume")
+
+    def setSceneShared(self, spec, owner, frame: int = 0, emissive: bool = False):
+        """setScene with owner's volume instead of one of its own."""
+        self.shareVolume(owner)
+        self._setSceneRest(spec, frame, emissive)
+
     def setSceneStreamed(self, spec, density, temperature=None, frame: int = 0):
         """setScene with the volume built on the GPU from device tensors."""
         self.streamVolumeFrame(spec, density, temperature)
