@@ -59,7 +59,7 @@ typedef struct vrestir_params {
     int32_t useReference;                /* volume path tracer instead of ReSTIR */
     int32_t useEnvironmentLights;
     int32_t useAnalyticLights;
-    int32_t useEmissiveLights;           /* emissive triangles: surface scenes only, must be 0 */
+    int32_t useEmissiveLights;           /* emissive triangles of vrestir_set_surfaces as lights */
     int32_t baselineSamplePerPixel;
     int32_t visualizeTotalTransmittance;
     int32_t useSurfaceScene;             /* must be 0 (VRESTIR_EUNSUPPORTED) */
@@ -153,6 +153,34 @@ typedef struct vrestir_point_light {
     float intensity[3];
 } vrestir_point_light;
 
+/* Untextured standard material, metal-rough shading model (SC/Material/Material.h
+ * and the MetalRough branch of _prepareShadingData, SC/Shading.slang:155-230). */
+typedef struct vrestir_material {
+    float baseColor[3];
+    float roughness;             /* linear roughness (specular.g) */
+    float metallic;              /* specular.b */
+    float IoR;                   /* dielectric F0 = ((IoR - 1) / (IoR + 1))^2; 1.5 gives 0.04 */
+    float emissive[3];           /* emissive colour x emissive factor; triangles emit on their front side */
+    int32_t doubleSided;
+} vrestir_material;
+
+/* Triangle geometry in world space (the scene's meshes after their instance
+ * transforms). Triangles whose material emits form the light collection
+ * (LT/LightCollection.cpp: face normal and area per triangle, flux =
+ * luminance(emissive) * area * pi, active when the flux is positive), sampled by
+ * the power sampler (LT/EmissivePowerSampler.slang:56-92: an alias table over the
+ * flux, then a uniform point on the triangle). */
+typedef struct vrestir_surfaces_desc {
+    const float* positions;      /* 3 floats per vertex */
+    int32_t numVertices;
+    const int32_t* indices;      /* 3 vertex indices per triangle */
+    const int32_t* materialIds;  /* one material index per triangle */
+    int32_t numTriangles;
+    const vrestir_material* materials;
+    int32_t numMaterials;
+    float emissiveIntensityMultiplier;  /* Scene::emissiveIntensityMultiplier */
+} vrestir_surfaces_desc;
+
 /* Pinhole camera (SC/Camera/Camera.cpp:113-189), no jitter. */
 typedef struct vrestir_camera {
     float posW[3];
@@ -196,6 +224,11 @@ int32_t vrestir_load_vbx(vrestir_handle* h, const char* pathPrefix, int32_t numM
 int32_t vrestir_set_lights(vrestir_handle* h, const vrestir_point_light* lights, int32_t count,
                            const float envRadiance[3]);
 int32_t vrestir_set_camera(vrestir_handle* h, const vrestir_camera* cam);
+/* The scene's triangles and materials (Scene::getLightCollection and the emissive
+ * sampler's update, VR/VolumetricReSTIR.cpp:157-207); NULL removes them. With
+ * useEmissiveLights their emissive triangles are lights of every stage, as in a
+ * volume scene of the reference (no surface intersections). */
+int32_t vrestir_set_surfaces(vrestir_handle* h, const vrestir_surfaces_desc* desc);
 /* Environment map in place of the constant envRadiance of vrestir_set_lights; NULL
  * returns to the constant. The importance map is built on the device. */
 int32_t vrestir_set_env_map(vrestir_handle* h, const vrestir_env_map_desc* desc);
@@ -316,7 +349,16 @@ enum {
      * grid slot mipLevel, vertex-centred when useLinearSampler: out = {bricks
      * visited, FNV-1a hash of their index-space minima in visit order (uint32
      * bits), sum of entry distances, first entry, last entry, tNear, tFar, 0}. */
-    VRESTIR_TRACK_VISITED_BRICKS = 16
+    VRESTIR_TRACK_VISITED_BRICKS = 16,
+    /* sampleSceneLights (VR/VolumeUtils.slang:12-149) at the point origin, the light
+     * kinds from numSamples' bits (1 environment, 2 point lights, 4 emissive
+     * triangles): out = {Li rgb, pdfArea, light index, light uv, ray distance}; all
+     * zero with index -9 when no light was sampled. */
+    VRESTIR_TRACK_LIGHT_SAMPLE = 17,
+    /* The emissive-triangle branch of evaluate_L_in_volume (VR/ReSTIRHelper.slang:
+     * 467-476) at the point origin, collection triangle dir.x, light uv dir.yz:
+     * out = {Le * cos / dist^2 rgb (no phase, no multiplier), dist, dir xyz, valid}. */
+    VRESTIR_TRACK_EMISSIVE_EVAL = 18
 };
 int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t mode, int32_t mipLevel,
                             int32_t useLinearSampler, float tStepScale, int32_t numSamples, int32_t n,

@@ -34,6 +34,9 @@ def lib():
         "orc_set_params": (c_int, [c_void_p, POINTER(abi.Params)]),
         "orc_set_volume": (c_int, [c_void_p, POINTER(abi.VolumeDesc)]),
         "orc_set_lights": (c_int, [c_void_p, POINTER(abi.PointLight), c_int, POINTER(c_float)]),
+        "orc_set_surfaces": (c_int, [c_void_p, POINTER(abi.SurfacesDesc)]),
+        "orc_sample_lights": (None, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+        "orc_emissive_eval": (None, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
         "orc_set_camera": (c_int, [c_void_p, POINTER(abi.Camera), c_int, c_int]),
         "orc_execute": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_int]),
         "orc_prepare_frame": (c_int, [c_void_p, c_int, c_int, c_int]),
@@ -142,6 +145,8 @@ class Oracle:
             assert self.L.orc_set_blackbody_lut(self.h, lut.ctypes.data, rows) == 0
         arr, n, env = spec.light_array()
         assert self.L.orc_set_lights(self.h, arr, n, env) == 0
+        sd, _keep = spec.surfaces_desc()
+        assert self.L.orc_set_surfaces(self.h, ctypes.byref(sd) if sd is not None else None) == 0
         if spec.env_map is not None:
             self.set_env_map(spec.env_map, spec.env_intensity, spec.env_tint, spec.env_rotation_at(0),
                              spec.env_importance_dim, spec.env_importance_spp)
@@ -149,6 +154,23 @@ class Oracle:
             self.set_env_map(None)
         self.set_camera(spec, frame)
         self.set_params(spec.make_params())
+
+    def sample_lights(self, kinds, points: np.ndarray, seeds: np.ndarray) -> np.ndarray:
+        """sampleSceneLights at points (kinds: 1 env, 2 point, 4 emissive): (n, 8)."""
+        p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        sd = np.ascontiguousarray(seeds, dtype=np.uint32).reshape(-1, 3)
+        out = np.zeros((p.shape[0], 8), np.float32)
+        self.L.orc_sample_lights(self.h, kinds, p.shape[0], p.ctypes.data, sd.ctypes.data, out.ctypes.data)
+        return out
+
+    def emissive_eval(self, points: np.ndarray, tris: np.ndarray, uvs: np.ndarray) -> np.ndarray:
+        """The emissive-triangle light evaluation at points: (n, 8)."""
+        p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1)
+        u = np.ascontiguousarray(uvs, dtype=np.float32).reshape(-1, 2)
+        out = np.zeros((p.shape[0], 8), np.float32)
+        self.L.orc_emissive_eval(self.h, p.shape[0], p.ctypes.data, t.ctypes.data, u.ctypes.data, out.ctypes.data)
+        return out
 
     def set_camera(self, spec, frame=0):
         c = spec.camera_struct(frame)

@@ -210,7 +210,12 @@ static f3 evaluateF(const Oracle& O, const EnvXf& X, Reservoir tap, const f3* ex
                     sh = {mi.p, ls.rayDir, 0, ls.rayDistance};
                     Ld = ls.Li * phaseHG(dot(mi.wo, ls.rayDir), mi.g);
                 } else {
-                    return mk3(0.f);  // emissive-triangle lights: surface scenes only
+                    TriangleLightSample ls;
+                    if (!powerSamplerGetSample(O.S, mi.p, tap.lightID - (int)O.S.lights.size(), tap.lightUV, ls))
+                        return mk3(0.f);
+                    sh = {mi.p, ls.dir, 0, ls.distance};
+                    Ld = ls.Le * O.S.emMult * phaseHG(dot(mi.wo, ls.dir), mi.g) * ls.cosTheta /
+                         (ls.distance * ls.distance);
                 }
                 float Tr = computeVisibility(m, sh, sg, opt.lightSamples, cull ? opt.lightingMipLevel + mipOff : 0,
                                              opt.lightingUseLinearSampler, opt.lightingTrackingMethod,
@@ -1118,7 +1123,7 @@ void vrestir_default_params_oracle(vrestir_params* p) {
 
 int orc_set_params(void* h, const vrestir_params* p) {
     Oracle* O = (Oracle*)h;
-    if (p->vertexReuse || p->useSurfaceScene || p->useEmissiveLights) return VRESTIR_EUNSUPPORTED;
+    if (p->vertexReuse || p->useSurfaceScene) return VRESTIR_EUNSUPPORTED;
     if (p->maxBounces < 1 || p->maxBounces > 8) return VRESTIR_EINVAL;
     bool bChanged = p->maxBounces != O->P.maxBounces;
     O->P = *p;
@@ -1385,6 +1390,133 @@ int orc_set_lights(void* h, const vrestir_point_light* L, int n, const float env
                                {L[i].intensity[0], L[i].intensity[1], L[i].intensity[2]}});
     O->S.env = {env[0], env[1], env[2]};
     return VRESTIR_OK;
+}
+
+// The light collection (LT/LightCollection.cpp, LT/BuildTriangleList.cs.slang,
+// LT/FinalizeIntegration.cs.slang:60-76) and the power sampler's alias table
+// (LT/EmissivePowerSampler.cpp:62-79, UT/Sampling/AliasTable.cpp:46-125, without
+// the final shuffle of the slots).
+int orc_set_surfaces(void* h, const vrestir_surfaces_desc* d) {
+    Oracle* O = (Oracle*)h;
+    SceneState& S = O->S;
+    S.emTris.clear();
+    S.emWeight.clear();
+    S.emAlias.clear();
+    S.emActive = 0;
+    S.emWeightSum = 0.f;
+    S.emMult = 1.f;
+    if (!d) return VRESTIR_OK;
+    S.emMult = d->emissiveIntensityMultiplier;
+    auto P = [&](int v) { return f3{d->positions[3 * v], d->positions[3 * v + 1], d->positions[3 * v + 2]}; };
+    for (int t = 0; t < d->numTriangles; t++) {
+        const vrestir_material& m = d->materials[d->materialIds[t]];
+        f3 Le{m.emissive[0], m.emissive[1], m.emissive[2]};
+        if (!(Le.x > 0.f || Le.y > 0.f || Le.z > 0.f)) continue;
+        EmissiveTriangle T;
+        for (int k = 0; k < 3; k++) T.p[k] = P(d->indices[3 * t + k]);
+        // computeFaceNormalAndAreaW (SC/Scene.slang:343-358)
+        f3 N = cross(T.p[1] - T.p[0], T.p[2] - T.p[0]);
+        T.area = 0.5f * length(N);
+        T.normal = mk3(0.f);
+        if (T.area > 0.f) {
+            f3 n = normalize(N);
+            // encodeNormal2x16 / decodeNormal2x16 (UT/Math/PackedFormats.slang:34-46)
+            auto wrap = [](float x, float y, float& ox, float& oy) {
+                ox = (1.f - std::fabs(y)) * (x >= 0.f ? 1.f : -1.f);
+                oy = (1.f - std::fabs(x)) * (y >= 0.f ? 1.f : -1.f);
+            };
+            float inv = 1.f / (std::fabs(n.x) + std::fabs(n.y) + std::fabs(n.z));
+            float ox = n.x * inv, oy = n.y * inv;
+            if (n.z < 0.f) wrap(n.x * inv, n.y * inv, ox, oy);
+            auto toSnorm = [](float v) {
+                v = std::isnan(v) ? 0.f : std::min(std::max(v, -1.f), 1.f);
+                return (int)std::trunc(v * 32767.f + (v >= 0.f ? 0.5f : -0.5f));
+            };
+            auto fromSnorm = [](int b) { return std::max((float)(int16_t)b / 32767.f, -1.f); };
+            f3 q{fromSnorm(toSnorm(ox) & 0xffff), fromSnorm(toSnorm(oy) & 0xffff), 0.f};
+            q.z = 1.f - std::fabs(q.x) - std::fabs(q.y);
+            if (q.z < 0.f) wrap(q.x, q.y, q.x, q.y);
+            T.normal = normalize(q);
+        }
+        T.Le = Le;
+        S.emTris.push_back(T);
+        float flux = luminance(Le) * T.area * (float)M_PI;
+        S.emWeight.push_back(flux);
+        if (flux > 0.f) S.emActive++;
+    }
+    const uint32_t count = (uint32_t)S.emWeight.size();
+    if (count == 0) return VRESTIR_OK;
+    double sum = 0.0;
+    for (float f : S.emWeight) sum += f;
+    S.emWeightSum = (float)sum;
+    S.emAlias.assign(count, AliasItem{1.f, 0u, 0u});
+    for (uint32_t i = 0; i < count; i++) S.emAlias[i] = {1.f, i, i};
+    if (!(sum > 0.0)) {
+        for (uint32_t i = 0; i < count; i++) S.emAlias[i] = {0.f, 0u, 0u};
+        return VRESTIR_OK;
+    }
+    std::vector<float> w(S.emWeight);
+    const double factor = count / sum;
+    for (float& f : w) f = (float)(f * factor);
+    std::vector<uint32_t> order(count);
+    for (uint32_t i = 0; i < count; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return w[a] < w[b]; });
+    uint32_t lo = 0, hi = count - 1;
+    while (lo != hi) {
+        uint32_t i = order[lo], j = order[hi];
+        S.emAlias[i].threshold = w[i];
+        S.emAlias[i].indexA = j;
+        w[j] -= 1.f - w[i];
+        if (lo == hi - 1) {
+            S.emAlias[j].threshold = 1.f;
+            S.emAlias[j].indexA = j;
+            break;
+        } else if (w[j] < 1.f) {
+            std::swap(order[lo], order[hi]);
+            hi--;
+        } else {
+            lo++;
+        }
+    }
+    return VRESTIR_OK;
+}
+
+// sampleSceneLights at points (light kinds in `kinds` bits: 1 env, 2 point, 4 emissive):
+// out[8] = {Li rgb, pdfArea, light index, uv, ray distance}; index -9 when none.
+void orc_sample_lights(void* h, int kinds, int n, const float* points, const uint32_t* seeds, float* out) {
+    Oracle* O = (Oracle*)h;
+    for (int i = 0; i < n; i++) {
+        SampleGen sg = makeSampleGen(seeds[3 * i], seeds[3 * i + 1], seeds[3 * i + 2]);
+        SceneLightSample ls{};
+        int idx = -9;
+        f2 uv{0.f, 0.f};
+        float* o = out + 8 * i;
+        for (int k = 0; k < 8; k++) o[k] = 0.f;
+        f3 p{points[3 * i], points[3 * i + 1], points[3 * i + 2]};
+        if (sampleSceneLights(O->S, O->frameEnv[0], p, kinds & 1, (kinds & 2) != 0, (kinds & 4) != 0, sg, ls, idx,
+                              uv)) {
+            o[0] = ls.Li.x; o[1] = ls.Li.y; o[2] = ls.Li.z; o[3] = ls.pdfArea;
+            o[4] = (float)idx; o[5] = uv.x; o[6] = uv.y; o[7] = ls.rayDistance;
+        } else {
+            o[4] = -9.f;
+        }
+    }
+}
+
+// The emissive-triangle branch of evaluate_L_in_volume at points: out[8] =
+// {Le cos / dist^2 rgb, dist, dir xyz, valid}.
+void orc_emissive_eval(void* h, int n, const float* points, const int* tris, const float* uvs, float* out) {
+    Oracle* O = (Oracle*)h;
+    for (int i = 0; i < n; i++) {
+        float* o = out + 8 * i;
+        for (int k = 0; k < 8; k++) o[k] = 0.f;
+        TriangleLightSample ls;
+        f3 p{points[3 * i], points[3 * i + 1], points[3 * i + 2]};
+        if (!powerSamplerGetSample(O->S, p, tris[i], {uvs[2 * i], uvs[2 * i + 1]}, ls)) continue;
+        f3 L = ls.Le * ls.cosTheta / (ls.distance * ls.distance);
+        o[0] = L.x; o[1] = L.y; o[2] = L.z; o[3] = ls.distance;
+        o[4] = ls.dir.x; o[5] = ls.dir.y; o[6] = ls.dir.z; o[7] = 1.f;
+    }
 }
 
 int orc_set_camera(void* h, const vrestir_camera* c, int W, int H) {

@@ -57,9 +57,21 @@ struct EnvMapState {
     std::vector<std::vector<float>> imp;   // level k: (impDim >> k)^2 texels, row-major
 };
 
+// Emissive triangle of the light collection (LT/LightCollectionShared.slang:41-60),
+// with its face normal as the collection stores it, and an alias-table entry
+// (UT/Sampling/AliasTable.slang:34-45).
+struct EmissiveTriangle { f3 p[3]; f3 normal; float area; f3 Le; };
+struct AliasItem { float threshold; uint32_t indexA, indexB; };
+
 struct SceneState {
     Medium med;
     std::vector<PointLight> lights;
+    std::vector<EmissiveTriangle> emTris;  // the light collection, in scene order
+    std::vector<float> emWeight;           // flux per collection triangle
+    std::vector<AliasItem> emAlias;        // EmissivePowerSampler's table over emWeight
+    int emActive = 0;                      // triangles with positive flux
+    float emWeightSum = 0.f;
+    float emMult = 1.f;                    // Scene::emissiveIntensityMultiplier
     f3 env{0, 0, 0};                       // constant environment (envMap.textured == false)
     EnvMapState envMap;
     Camera cam, prevCam;
@@ -338,8 +350,77 @@ static inline f3 velocityWorld(const Medium& m, f3 pW) {
 
 struct SceneLightSample { f3 dir; float distance; f3 Li; float pdf, pdfArea; f3 rayDir; float rayDistance; };
 
+// computeRayOrigin (UT/Helpers.slang:92-105).
+static inline f3 computeRayOrigin(f3 pos, f3 normal) {
+    const float origin = 1.f / 32.f, fScale = 1.f / 65536.f, iScale = 256.f;
+    float P[3] = {pos.x, pos.y, pos.z}, N[3] = {normal.x, normal.y, normal.z}, R[3];
+    for (int k = 0; k < 3; k++) {
+        int iOff = (int)(N[k] * iScale);
+        float iPos = asfloat(asint(P[k]) + (P[k] < 0.f ? -iOff : iOff));
+        float fOff = N[k] * fScale;
+        R[k] = std::fabs(P[k]) < origin ? P[k] + fOff : iPos;
+    }
+    return {R[0], R[1], R[2]};
+}
+
+struct TriangleLightSample { f3 posW, normalW, dir, Le; float distance, pdf, pdfArea, cosTheta; f2 uv; uint32_t tri; };
+
+// sampleTriangle (LT/EmissiveLightSamplerHelpers.slang:56-94) with sample_triangle
+// (UT/Math/MathHelpers.slang:356-361).
+static inline bool sampleTriangle(const SceneState& S, f3 posW, uint32_t tri, f2 u, TriangleLightSample& ls) {
+    ls = TriangleLightSample{};
+    ls.tri = tri;
+    const EmissiveTriangle& T = S.emTris[tri];
+    float su = std::sqrt(u.x);
+    float bx = 1.f - su, by = u.y * su;
+    f3 bary{1.f - bx - by, bx, by};
+    ls.uv = u;
+    ls.posW = T.p[0] * bary.x + T.p[1] * bary.y + T.p[2] * bary.z;
+    ls.posW = computeRayOrigin(ls.posW, T.normal);
+    f3 toLight = ls.posW - posW;
+    const float distSqr = std::max(FLT_MIN, dot(toLight, toLight));
+    ls.distance = std::sqrt(distSqr);
+    ls.dir = toLight / ls.distance;
+    ls.normalW = T.normal;
+    float cosTheta = dot(ls.normalW, -ls.dir);
+    if (!(cosTheta > 0.f)) return false;
+    ls.Le = T.Le;
+    float denom = std::max(FLT_MIN, cosTheta * T.area);
+    ls.pdf = distSqr / denom;
+    ls.cosTheta = cosTheta;
+    ls.pdfArea = 1.f / T.area;
+    return true;
+}
+
+// EmissivePowerSampler::sampleLight (LT/EmissivePowerSampler.slang:56-79).
+static inline bool powerSamplerSampleLight(const SceneState& S, f3 posW, SampleGen& sg, TriangleLightSample& ls) {
+    if (S.emActive == 0) return false;
+    f2 r;
+    r.x = next1D(sg);
+    r.y = next1D(sg);
+    const uint32_t count = (uint32_t)S.emAlias.size();
+    uint32_t index = std::min(count - 1, (uint32_t)(r.x * (float)count));
+    const AliasItem& it = S.emAlias[index];
+    uint32_t tri = r.y >= it.threshold ? it.indexA : it.indexB;
+    float sel = S.emWeight[tri] / S.emWeightSum;
+    f2 u;
+    u.x = next1D(sg);
+    u.y = next1D(sg);
+    if (!sampleTriangle(S, posW, tri, u, ls)) return false;
+    ls.pdf *= sel;
+    ls.pdfArea *= sel;
+    return true;
+}
+
+// EmissivePowerSampler::getEmissiveLightSample (LT/EmissivePowerSampler.slang:81-92).
+static inline bool powerSamplerGetSample(const SceneState& S, f3 posW, int lightID, f2 uv, TriangleLightSample& ls) {
+    if (S.emActive == 0) return false;
+    if (lightID < 0 || lightID >= (int)S.emTris.size()) return false;  // a light id of another scene
+    return sampleTriangle(S, posW, (uint32_t)lightID, uv, ls);
+}
+
 // sampleSceneLights (VR/VolumeUtils.slang:12-149): environment (constant or
-// textured) and point lights.
+// textured), point lights and emissive triangles (the power sampler).
 static inline bool sampleSceneLights(const SceneState& S, const EnvXf& X, f3 origin, bool useEnv, bool useAnalytic,
                                      bool useEmissive, SampleGen& sg, SceneLightSample& ls, int& lightIndex,
                                      f2& lightUV) {
@@ -394,6 +475,29 @@ static inline bool sampleSceneLights(const SceneState& S, const EnvXf& X, f3 ori
             return true;
         }
         u -= p[1];
+    }
+    if (useEmissive) {
+        if (u < p[2]) {
+            float sel = p[2];
+            TriangleLightSample t;
+            bool valid = powerSamplerSampleLight(S, origin, sg, t);
+            if (S.emActive == 0) return false;
+            float pdf = sel * t.pdf;
+            float pdfArea = sel * t.pdfArea;
+            f3 offsetPos = computeRayOrigin(t.posW, t.normalW);
+            f3 toLight = offsetPos - origin;
+            ls.rayDistance = length(toLight);
+            ls.rayDir = normalize(toLight);
+            ls.dir = t.dir;
+            ls.distance = t.distance;
+            ls.pdf = pdf;
+            ls.pdfArea = pdfArea;
+            ls.Li = pdf > 0.f ? t.Le * S.emMult / pdf : mk3(0.f);
+            lightIndex = (int)t.tri + (int)S.lights.size();
+            lightUV = t.uv;
+            return valid;
+        }
+        u -= p[2];
     }
     return false;
 }

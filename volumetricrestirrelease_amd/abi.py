@@ -44,7 +44,7 @@ TRACK_TR_ANALYTIC, TRACK_TR_RAY_MARCHING, TRACK_TR_RATIO, TRACK_TR_RESIDUAL_RATI
     TRACK_TR_ANALOG_RESIDUAL_RATIO, TRACK_SAMPLE_ANALYTIC, TRACK_SAMPLE_DECOMPOSITION, \
     TRACK_SAMPLE_CELL_BY_DENSITY, TRACK_FEATURE_MARCH, TRACK_DENSITY_AT_POINT, TRACK_RNG_STREAM, \
     TRACK_ENV_EVAL, TRACK_ENV_SAMPLE, TRACK_EMISSION, TRACK_VELOCITY, TRACK_DENSITY_INDEX, \
-    TRACK_VISITED_BRICKS = range(17)
+    TRACK_VISITED_BRICKS, TRACK_LIGHT_SAMPLE, TRACK_EMISSIVE_EVAL = range(19)
 
 # (name, ctype) in declaration order of vrestir_params
 PARAM_FIELDS = [
@@ -126,6 +126,20 @@ class PointLight(ctypes.Structure):
     _fields_ = [("posW", c_float * 3), ("intensity", c_float * 3)]
 
 
+class Material(ctypes.Structure):
+    """vrestir_material: untextured standard material (metal-rough)."""
+    _fields_ = [("baseColor", c_float * 3), ("roughness", c_float), ("metallic", c_float), ("IoR", c_float),
+                ("emissive", c_float * 3), ("doubleSided", ctypes.c_int32)]
+
+
+class SurfacesDesc(ctypes.Structure):
+    """vrestir_surfaces_desc: world-space triangles and their materials."""
+    _fields_ = [("positions", POINTER(c_float)), ("numVertices", ctypes.c_int32),
+                ("indices", POINTER(ctypes.c_int32)), ("materialIds", POINTER(ctypes.c_int32)),
+                ("numTriangles", ctypes.c_int32), ("materials", POINTER(Material)),
+                ("numMaterials", ctypes.c_int32), ("emissiveIntensityMultiplier", c_float)]
+
+
 class Camera(ctypes.Structure):
     _fields_ = [("posW", c_float * 3), ("target", c_float * 3), ("up", c_float * 3),
                 ("fovY", c_float), ("nearZ", c_float), ("farZ", c_float)]
@@ -190,6 +204,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "vrestir_plume_read": (i32, [vp, vp, vp]),
         "vrestir_load_vbx": (i32, [vp, c_char_p, i32, i32, POINTER(f32), POINTER(f32), f32, f32]),
         "vrestir_set_lights": (i32, [vp, POINTER(PointLight), i32, POINTER(f32)]),
+        "vrestir_set_surfaces": (i32, [vp, POINTER(SurfacesDesc)]),
         "vrestir_set_camera": (i32, [vp, POINTER(Camera)]),
         "vrestir_set_env_map": (i32, [vp, POINTER(EnvMapDesc)]),
         "vrestir_set_env_rotation": (i32, [vp, POINTER(f32)]),
@@ -252,7 +267,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
 
 EXPORTED_SYMBOLS = [
     "vrestir_version", "vrestir_default_params", "vrestir_create", "vrestir_destroy", "vrestir_last_error",
-    "vrestir_set_volume", "vrestir_stream_volume_frame", "vrestir_share_volume", "vrestir_load_vbx", "vrestir_set_lights", "vrestir_set_camera", "vrestir_set_env_map",
+    "vrestir_set_volume", "vrestir_stream_volume_frame", "vrestir_share_volume", "vrestir_load_vbx", "vrestir_set_lights", "vrestir_set_surfaces", "vrestir_set_camera", "vrestir_set_env_map",
     "vrestir_set_env_rotation", "vrestir_set_env_intensity", "vrestir_set_blackbody_lut", "vrestir_load_vbx_emission",
     "vrestir_add_volume_frame", "vrestir_set_volume_frame", "vrestir_volume_frame_count", "vrestir_set_freeze_frame",
     "vrestir_camera_animation",

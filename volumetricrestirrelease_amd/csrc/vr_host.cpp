@@ -12,6 +12,7 @@
 
 #include "../../include/vrestir.h"
 #include "vr_launch.h"
+#include "vr_surface.h"
 #include "vr_volume.h"
 
 namespace vr {
@@ -105,6 +106,10 @@ struct vrestir_handle {
     // textured environment map: texels, importance mip chain, orientation now and at the last frame
     float* envTexDev = nullptr;
     float* envImpDev = nullptr;
+    // the emissive triangles of vrestir_set_surfaces: collection, alias table, flux
+    float* emTriDev = nullptr;
+    uint32_t* emAliasDev = nullptr;
+    float* emWeightDev = nullptr;
     EnvXf envCur{}, envLast{};
     bool haveEnvLast = false;
     // the volume: frames of a sequence (one for a static volume), each resident in HBM
@@ -181,6 +186,18 @@ static void freeVolume(vrestir_handle* h) {
 static void freeEnv(vrestir_handle* h) {
     dfree(h->envTexDev);
     dfree(h->envImpDev);
+}
+static void freeSurfaces(vrestir_handle* h) {
+    dfree(h->emTriDev);
+    dfree(h->emAliasDev);
+    dfree(h->emWeightDev);
+    Scene& S = h->sceneHost;
+    S.emTri = nullptr;
+    S.emAlias = nullptr;
+    S.emWeight = nullptr;
+    S.numEmTris = S.numEmActive = 0;
+    S.emWeightSum = 0.f;
+    S.emMult = 1.f;
 }
 // EnvMap::setRotation (LT/EnvMap.cpp:57-72): local -> world = Rz * Ry * Rx of
 // glm::eulerAngleX/Y/Z (right-handed rotations by the XYZ angles in degrees),
@@ -260,7 +277,6 @@ static int checkParams(vrestir_handle* h, const vrestir_params* p) {
     if (!p) return fail(h, VRESTIR_EINVAL, "params is NULL");
     if (p->useSurfaceScene) return fail(h, VRESTIR_EUNSUPPORTED, "surface scenes are not part of this pass build");
     if (p->vertexReuse) return fail(h, VRESTIR_EUNSUPPORTED, "vertex reuse is not implemented");
-    if (p->useEmissiveLights) return fail(h, VRESTIR_EUNSUPPORTED, "emissive-triangle lights need a surface scene");
     if (p->maxBounces < 1 || p->maxBounces > 4) return fail(h, VRESTIR_EINVAL, "maxBounces must be in [1, 4]");
     if (p->initialM < 1) return fail(h, VRESTIR_EINVAL, "initialM must be >= 1");
     if (p->spatialSampleCount < 1 || p->spatialSampleCount > kMaxSpatialSamples)
@@ -295,6 +311,7 @@ extern "C" int32_t vrestir_create(const vrestir_params* params, vrestir_handle**
     h->device = dev;
     h->P = p;
     std::memset(&h->sceneHost, 0, sizeof(Scene));
+    h->sceneHost.emMult = 1.f;
     const float zero[3] = {0.f, 0.f, 0.f};
     envRotation(zero, h->envCur);
     h->envLast = h->envCur;
@@ -319,6 +336,7 @@ extern "C" int32_t vrestir_destroy(vrestir_handle* h) {
     freeFrameBuffers(h);
     freeVolume(h);
     freeEnv(h);
+    freeSurfaces(h);
     dfree(h->sceneDev);
     dfree(h->offsetsDev);
     dfree(h->accum);
@@ -888,6 +906,41 @@ extern "C" int32_t vrestir_set_lights(vrestir_handle* h, const vrestir_point_lig
             S.lights[6 * i + 3 + k] = lights[i].intensity[k];
         }
     for (int k = 0; k < 3; k++) S.env[k] = env ? env[k] : 0.f;
+    return uploadScene(h);
+}
+
+extern "C" int32_t vrestir_set_surfaces(vrestir_handle* h, const vrestir_surfaces_desc* d) {
+    if (!h) return VRESTIR_EINVAL;
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipDeviceSynchronize());
+    if (d) {
+        const std::string err = checkSurfacesDesc(*d);
+        if (!err.empty()) return fail(h, VRESTIR_EINVAL, err);
+    }
+    freeSurfaces(h);
+    if (!d) return uploadScene(h);
+    LightCollectionHost lc;
+    buildLightCollection(*d, lc);
+    Scene& S = h->sceneHost;
+    S.emMult = d->emissiveIntensityMultiplier;
+    const int n = (int)lc.weight.size();
+    if (n > 0) {
+        auto up = [&](const void* src, size_t bytes, void** dst) -> hipError_t {
+            hipError_t e = hipMalloc(dst, bytes);
+            if (e != hipSuccess) return e;
+            return hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+        };
+        HIPCHK(h, up(lc.tri.data(), lc.tri.size() * sizeof(float), (void**)&h->emTriDev));
+        HIPCHK(h, up(lc.alias.data(), lc.alias.size() * sizeof(uint32_t), (void**)&h->emAliasDev));
+        HIPCHK(h, up(lc.weight.data(), lc.weight.size() * sizeof(float), (void**)&h->emWeightDev));
+        S.emTri = h->emTriDev;
+        S.emAlias = h->emAliasDev;
+        S.emWeight = h->emWeightDev;
+        S.numEmTris = n;
+        S.numEmActive = lc.numActive;
+        S.emWeightSum = (float)lc.weightSum;
+    }
+    h->optionsChanged = true;
     return uploadScene(h);
 }
 
@@ -1765,7 +1818,7 @@ extern "C" int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t 
                                        const float* rays, const uint32_t* seeds, float* out) {
     if (!h) return VRESTIR_EINVAL;
     if (!h->volumeSet) return fail(h, VRESTIR_ESTATE, "no volume bound");
-    if (mode < 0 || mode > VRESTIR_TRACK_VISITED_BRICKS) return fail(h, VRESTIR_EINVAL, "unknown tracking mode");
+    if (mode < 0 || mode > VRESTIR_TRACK_EMISSIVE_EVAL) return fail(h, VRESTIR_EINVAL, "unknown tracking mode");
     if (mode == VRESTIR_TRACK_EMISSION && h->sceneHost.hasEmission && h->sceneHost.lutRows == 0)
         return fail(h, VRESTIR_ESTATE, "no blackbody table bound");
     if (mip < 0 || mip >= kMaxSlots || !h->meta.slots[mip].valid) return fail(h, VRESTIR_EINVAL, "grid slot not bound");

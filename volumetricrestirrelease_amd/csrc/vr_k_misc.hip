@@ -132,6 +132,31 @@ __global__ __launch_bounds__(256) void kTrack(const Scene* __restrict__ sc, cons
             o[3] = vh.tFirst; o[4] = vh.tLast; o[5] = vh.tNear; o[6] = vh.tFar;
             break;
         }
+        case 17: {  // scene light sample at the point origin
+            LightSample ls;
+            ls.Li = V3(0.f);
+            ls.pdfArea = ls.rayDistance = 0.f;
+            int idx = -9;
+            float ux = 0.f, uy = 0.f;
+            if (sampleLights(S, ex, ray.o, (numSamples & 1) != 0, (numSamples & 2) != 0, (numSamples & 4) != 0, rng,
+                             ls, idx, ux, uy)) {
+                o[0] = ls.Li.x; o[1] = ls.Li.y; o[2] = ls.Li.z; o[3] = ls.pdfArea;
+                o[4] = (float)idx; o[5] = ux; o[6] = uy; o[7] = ls.rayDistance;
+            } else {
+                o[4] = -9.f;
+            }
+            break;
+        }
+        case 18: {  // emissive-triangle light evaluation at the point origin
+            const int tri = (int)r[3];
+            TriLightSample ts;
+            if (S.numEmActive > 0 && tri >= 0 && tri < S.numEmTris && sampleTriangle(S, ray.o, tri, r[4], r[5], ts)) {
+                const v3 L = ts.Le * ts.cosTheta / (ts.distance * ts.distance);
+                o[0] = L.x; o[1] = L.y; o[2] = L.z; o[3] = ts.distance;
+                o[4] = ts.dir.x; o[5] = ts.dir.y; o[6] = ts.dir.z; o[7] = 1.f;
+            }
+            break;
+        }
         default: break;
     }
     float* d = out + 8 * (size_t)i;

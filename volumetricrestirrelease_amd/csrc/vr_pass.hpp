@@ -197,8 +197,64 @@ VR_DEV v3 envSample(const Scene& S, const EnvXf& X, float r0, float r1, float& p
 
 struct LightSample { v3 dir, Li, rayDir; float pdf, pdfArea, rayDistance; };
 
+// ------------------------------------------------------------------ emissive triangles
+// computeRayOrigin (UT/Helpers.slang:92-105): a point moved off its surface along
+// the normal, by a few ulps of each coordinate (by a fixed 2^-16 near the origin).
+VR_DEV v3 rayOriginOffset(v3 p, v3 n) {
+    const float origin = 1.f / 32.f, fScale = 1.f / 65536.f, iScale = 256.f;
+    const int ix = (int)(n.x * iScale), iy = (int)(n.y * iScale), iz = (int)(n.z * iScale);
+    const float px = __int_as_float(__float_as_int(p.x) + (p.x < 0.f ? -ix : ix));
+    const float py = __int_as_float(__float_as_int(p.y) + (p.y < 0.f ? -iy : iy));
+    const float pz = __int_as_float(__float_as_int(p.z) + (p.z < 0.f ? -iz : iz));
+    return V3(fabsf(p.x) < origin ? p.x + n.x * fScale : px, fabsf(p.y) < origin ? p.y + n.y * fScale : py,
+              fabsf(p.z) < origin ? p.z + n.z * fScale : pz);
+}
+struct TriLightSample { v3 posW, normal, dir, Le; float distance, pdf, pdfArea, cosTheta; };
+// sampleTriangle (LT/EmissiveLightSamplerHelpers.slang:56-94): the point of
+// sample_triangle(u) (UT/Math/MathHelpers.slang:356-361) on collection triangle
+// tri, offset off the triangle; false when the triangle faces away.
+VR_DEV bool sampleTriangle(const Scene& S, v3 posW, int tri, float ux, float uy, TriLightSample& ls) {
+    const float* T = S.emTri + (size_t)kEmTriFloats * tri;
+    const float su = sqrtf(ux);
+    const float b1 = 1.f - su, b2 = uy * su, b0 = 1.f - b1 - b2;
+    ls.normal = ld3(T + 9);
+    ls.posW = ld3(T) * b0 + ld3(T + 3) * b1 + ld3(T + 6) * b2;
+    ls.posW = rayOriginOffset(ls.posW, ls.normal);
+    const v3 toLight = ls.posW - posW;
+    const float distSqr = fmaxf(1.17549435e-38f, dot3(toLight, toLight));
+    ls.distance = sqrtf(distSqr);
+    ls.dir = toLight / ls.distance;
+    const float cosTheta = dot3(ls.normal, -ls.dir);
+    if (!(cosTheta > 0.f)) return false;
+    ls.Le = ld3(T + 13);
+    const float area = T[12];
+    ls.pdf = distSqr / fmaxf(1.17549435e-38f, cosTheta * area);
+    ls.cosTheta = cosTheta;
+    ls.pdfArea = 1.f / area;
+    return true;
+}
+// EmissivePowerSampler::sampleLight (LT/EmissivePowerSampler.slang:56-79): a
+// collection triangle from the alias table (UT/Sampling/AliasTable.slang:55-73),
+// then a uniform point on it.
+VR_DEV bool samplePowerLight(const Scene& S, v3 posW, Rng& rng, int& tri, float& ux, float& uy, TriLightSample& ls) {
+    if (S.numEmActive == 0) return false;
+    const float a0 = u01(rng);
+    const float a1 = u01(rng);
+    const uint32_t count = (uint32_t)S.numEmTris;
+    const uint32_t slot = min(count - 1u, (uint32_t)(a0 * (float)count));
+    const uint32_t* item = S.emAlias + 4 * (size_t)slot;
+    tri = (int)(a1 >= __uint_as_float(item[0]) ? item[1] : item[2]);
+    const float sel = S.emWeight[tri] / S.emWeightSum;
+    ux = u01(rng);
+    uy = u01(rng);
+    if (!sampleTriangle(S, posW, tri, ux, uy, ls)) return false;
+    ls.pdf *= sel;
+    ls.pdfArea *= sel;
+    return true;
+}
+
 // sampleSceneLights (VR/VolumeUtils.slang:12-149) for the environment (constant
-// or textured) and point lights.
+// or textured), point lights and emissive triangles.
 VR_DEV bool sampleLights(const Scene& S, const EnvXf& X, v3 origin, bool useEnv, bool useAnalytic, bool useEmissive,
                          Rng& rng, LightSample& ls, int& lightIndex, float& uvx, float& uvy) {
     if (!useEnv && !useAnalytic && !useEmissive) return false;
@@ -246,6 +302,31 @@ VR_DEV bool sampleLights(const Scene& S, const EnvXf& X, v3 origin, bool useEnv,
             ls.Li = Li / ls.pdf;
             uvx = 0.f; uvy = 0.f;
             return true;
+        }
+        u -= p1;
+    }
+    if (useEmissive) {
+        if (u < p2) {
+            // the emissive branch: the ray ends at the sampled point offset once more
+            int tri = 0;
+            float ux = 0.f, uy = 0.f;
+            TriLightSample ts;
+            ts.pdf = ts.pdfArea = 0.f;
+            ts.posW = ts.normal = ts.dir = ts.Le = V3(0.f);
+            ts.distance = 0.f;
+            if (S.numEmActive == 0) return false;
+            const bool valid = samplePowerLight(S, origin, rng, tri, ux, uy, ts);
+            const float pdf = p2 * ts.pdf, pdfArea = p2 * ts.pdfArea;
+            const v3 toLight = rayOriginOffset(ts.posW, ts.normal) - origin;
+            ls.rayDistance = len3(toLight);
+            ls.rayDir = norm3(toLight);
+            ls.dir = ts.dir;
+            ls.pdf = pdf;
+            ls.pdfArea = pdfArea;
+            ls.Li = pdf > 0.f ? ts.Le * S.emMult / pdf : V3(0.f);
+            lightIndex = tri + S.numLights;
+            uvx = ux; uvy = uy;
+            return valid;
         }
     }
     return false;
@@ -601,7 +682,14 @@ VR_DEV v3 evaluateF(const Scene& S, const EnvXf& X, Res tap, const float* extra,
                     sh = {mi.p, dir, 0.f, dist};
                     Ld = Li * phaseHG(dot3(mi.wo, dir), mi.g);
                 } else {
-                    return V3(0.f);
+                    // EmissivePowerSampler::getEmissiveLightSample (LT/EmissivePowerSampler.slang:81-92)
+                    const int tri = tap.lightID - S.numLights;
+                    if (S.numEmActive == 0 || tri >= S.numEmTris) return V3(0.f);
+                    TriLightSample ts;
+                    if (!sampleTriangle(S, mi.p, tri, tap.uvx, tap.uvy, ts)) return V3(0.f);
+                    sh = {mi.p, ts.dir, 0.f, ts.distance};
+                    Ld = ts.Le * S.emMult * phaseHG(dot3(mi.wo, ts.dir), mi.g) * ts.cosTheta /
+                         (ts.distance * ts.distance);
                 }
                 float Tr = tp.vis(S, sh, rng, opt, 1, isFinal, gridOff);
                 F = F * (Ld * Tr);

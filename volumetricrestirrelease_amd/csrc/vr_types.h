@@ -83,7 +83,18 @@ struct Scene {
     int32_t prevHasEmission, hasVelocity;
     float velocityScale, pad3_;
     float lut[kMaxLutRows * 3];
+    // Emissive triangles (LT/LightCollectionShared.slang:41-60, LT/EmissivePowerSampler.slang):
+    // emTri holds kEmTriFloats floats per collection triangle (positions p0 p1 p2, face
+    // normal as stored, area, emitted radiance); emAlias the power sampler's alias table
+    // (4 words per entry: threshold bits, index taken at or above it, index below it);
+    // emWeight the flux per triangle and emWeightSum their sum.
+    const float* emTri;
+    const uint32_t* emAlias;
+    const float* emWeight;
+    int32_t numEmTris, numEmActive;
+    float emWeightSum, emMult;
 };
+constexpr int kEmTriFloats = 16;
 
 // Orientation of the environment map for one frame (EnvMapData::transform and
 // invTransform, LT/EnvMapData.slang:35-41): world = toWorld * local, row-major.

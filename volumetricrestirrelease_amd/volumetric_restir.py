@@ -130,11 +130,19 @@ class VolumetricReSTIR:
                                                            rows), "set_blackbody_lut")
         arr, n, env = spec.light_array()
         self._check(self.lib.vrestir_set_lights(self.h, arr, n, env), "set_lights")
+        self.setSurfaces(spec)
         em = spec.env_map_desc()
         self._check(self.lib.vrestir_set_env_map(self.h, ctypes.byref(em) if em is not None else None),
                     "set_env_map")
         self.setCamera(spec, frame)
         self.setParams(spec.make_params())
+
+    def setSurfaces(self, spec):
+        """The scene's triangles (their emissive ones are the emissive lights); none
+        when spec.surfaces is None."""
+        d, _keep = spec.surfaces_desc()
+        self._check(self.lib.vrestir_set_surfaces(self.h, ctypes.byref(d) if d is not None else None),
+                    "set_surfaces")
 
     def setCamera(self, spec, frame: int = 0):
         """Camera (and env rotation, for animated environments) of a frame."""
