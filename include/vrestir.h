@@ -62,7 +62,8 @@ typedef struct vrestir_params {
     int32_t useEmissiveLights;           /* emissive triangles of vrestir_set_surfaces as lights */
     int32_t baselineSamplePerPixel;
     int32_t visualizeTotalTransmittance;
-    int32_t useSurfaceScene;             /* must be 0 (VRESTIR_EUNSUPPORTED) */
+    int32_t useSurfaceScene;             /* GenerateFeatures only: V-buffer and surface-limited features;
+                                            the other stages refuse (VRESTIR_EUNSUPPORTED) */
     int32_t usePrevVolumeForReproj;
     /* initial sampling */
     int32_t initialBaseMipLevel;
@@ -172,6 +173,7 @@ typedef struct vrestir_material {
  * flux, then a uniform point on the triangle). */
 typedef struct vrestir_surfaces_desc {
     const float* positions;      /* 3 floats per vertex */
+    const float* normals;        /* 3 floats per vertex (shading normals), or NULL for face normals */
     int32_t numVertices;
     const int32_t* indices;      /* 3 vertex indices per triangle */
     const int32_t* materialIds;  /* one material index per triangle */
@@ -312,7 +314,10 @@ enum {
     VRESTIR_BUF_TRAIN_RECORDS = 10,
     /* split TraceRays (VRESTIR_TUNE_SPLIT_TRACE): int32 count of the last frame's pixels
      * that were traced again in one kernel (resampling draws that differed from the record's) */
-    VRESTIR_BUF_TRACE_REDONE = 11
+    VRESTIR_BUF_TRACE_REDONE = 11,
+    /* surface scenes: the V-buffer of GenerateFeatures, per pixel {int32 triangle (-1: no
+     * hit), float barycentric weights of vertices 1 and 2, float ray distance (FLT_MAX)} */
+    VRESTIR_BUF_VBUFFER = 12
 };
 int32_t vrestir_prepare_frame(vrestir_handle* h, int32_t width, int32_t height);
 int32_t vrestir_run_stage(vrestir_handle* h, void* stream, int32_t stage, int32_t round, float* outRGBA);
@@ -358,7 +363,11 @@ enum {
     /* The emissive-triangle branch of evaluate_L_in_volume (VR/ReSTIRHelper.slang:
      * 467-476) at the point origin, collection triangle dir.x, light uv dir.yz:
      * out = {Le * cos / dist^2 rgb (no phase, no multiplier), dist, dir xyz, valid}. */
-    VRESTIR_TRACK_EMISSIVE_EVAL = 18
+    VRESTIR_TRACK_EMISSIVE_EVAL = 18,
+    /* FindSurfaceHit and FindIfOccluded (VR/InlineRayTracingHelpers.slang:42-155) over
+     * [0, tMax]: out = {closest t (FLT_MAX: none), its triangle, barycentric weights of
+     * vertices 1 and 2, occluded (any hit), 0, 0, 0}. */
+    VRESTIR_TRACK_SURFACE_HIT = 19
 };
 int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t mode, int32_t mipLevel,
                             int32_t useLinearSampler, float tStepScale, int32_t numSamples, int32_t n,

@@ -37,6 +37,7 @@ def lib():
         "orc_set_surfaces": (c_int, [c_void_p, POINTER(abi.SurfacesDesc)]),
         "orc_sample_lights": (None, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
         "orc_emissive_eval": (None, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+        "orc_surface_hit": (None, [c_void_p, c_int, c_void_p, c_void_p]),
         "orc_set_camera": (c_int, [c_void_p, POINTER(abi.Camera), c_int, c_int]),
         "orc_execute": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_int]),
         "orc_prepare_frame": (c_int, [c_void_p, c_int, c_int, c_int]),
@@ -161,6 +162,13 @@ class Oracle:
         sd = np.ascontiguousarray(seeds, dtype=np.uint32).reshape(-1, 3)
         out = np.zeros((p.shape[0], 8), np.float32)
         self.L.orc_sample_lights(self.h, kinds, p.shape[0], p.ctypes.data, sd.ctypes.data, out.ctypes.data)
+        return out
+
+    def surface_hit(self, rays: np.ndarray) -> np.ndarray:
+        """Closest and any hit over [0, tMax] per ray (n, 8): (n, 8) {t, tri, bary, occluded, 0...}."""
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        out = np.zeros((r.shape[0], 8), np.float32)
+        self.L.orc_surface_hit(self.h, r.shape[0], r.ctypes.data, out.ctypes.data)
         return out
 
     def emissive_eval(self, points: np.ndarray, tris: np.ndarray, uvs: np.ndarray) -> np.ndarray:

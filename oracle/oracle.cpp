@@ -50,6 +50,7 @@ struct Oracle {
     bool optionsChanged = true;
     // buffers (VR/VolumetricReSTIR.cpp:419-450)
     std::vector<Features> features, temporalFeatures;
+    std::vector<VItem4> vbuffer;  // surface scenes: {triangle, bary, t} per pixel
     std::vector<Reservoir> res[2], temporalRes;
     std::vector<f3> extra[2], temporalExtra;
     std::vector<f4> color[2], output;
@@ -417,14 +418,22 @@ static inline bool inRange(int x, int y, int W, int H) { return x >= 0 && x < W 
 // Kernels.
 
 // GenerateFeatures (VR/GenerateFeatures.cs.slang:57-102).
+// With a surface scene: the V-buffer, and the march and flag of VR/GenerateFeatures.cs.slang:78-99.
 static void stageFeatures(Oracle& O) {
-    if (O.P.useReference) return;
     forRows(O, [&](int x, int y) {
         SampleGen sg{};  // unused by the adapter
         f3 d = rayDir(O.S.cam, x, y, O.W, O.H);
         Ray r{O.S.cam.pos, d, 0, kRayTMax};
+        int noSurface = 1;
+        if (O.P.useSurfaceScene) {
+            HitInfo hi = findSurfaceHit(O.S, r.origin, r.dir, 0.f, kRayTMax);
+            O.vbuffer[y * O.W + x] = VItem4{hi.tri, hi.bary.x, hi.bary.y, hi.t};
+            r.tMax = hi.t;
+            if (hi.tri >= 0) noSurface = emitsTowards(O.S, hi.tri, -r.dir) ? 1 : 0;
+        }
+        if (O.P.useReference) return;
         float accT = featureMarch(O.S.med, r, 0, true, O.initialOpt.visibilityTStepScale, sg);
-        O.features[y * O.W + x] = {1, accT};
+        O.features[y * O.W + x] = {noSurface, accT};
     });
 }
 
@@ -881,6 +890,7 @@ static void prepareFrame(Oracle& O, int W, int H) {
         O.W = W; O.H = H;
         size_t n = (size_t)W * H;
         O.features.assign(n, {0, 0.f});
+        O.vbuffer.assign(n, VItem4{-1, 0.f, 0.f, kRayTMax});
         O.temporalFeatures.assign(n, {0, 0.f});
         for (int i = 0; i < 2; i++) {
             O.res[i].assign(n, Reservoir{0, 0, 0, 0, {0, 0}, 0, 0, 0});
@@ -1123,7 +1133,7 @@ void vrestir_default_params_oracle(vrestir_params* p) {
 
 int orc_set_params(void* h, const vrestir_params* p) {
     Oracle* O = (Oracle*)h;
-    if (p->vertexReuse || p->useSurfaceScene) return VRESTIR_EUNSUPPORTED;
+    if (p->vertexReuse) return VRESTIR_EUNSUPPORTED;
     if (p->maxBounces < 1 || p->maxBounces > 8) return VRESTIR_EINVAL;
     bool bChanged = p->maxBounces != O->P.maxBounces;
     O->P = *p;
@@ -1405,8 +1415,30 @@ int orc_set_surfaces(void* h, const vrestir_surfaces_desc* d) {
     S.emActive = 0;
     S.emWeightSum = 0.f;
     S.emMult = 1.f;
+    S.surfTris.clear();
+    S.surfMats.clear();
     if (!d) return VRESTIR_OK;
     S.emMult = d->emissiveIntensityMultiplier;
+    for (int m = 0; m < d->numMaterials; m++) {
+        const vrestir_material& M = d->materials[m];
+        S.surfMats.push_back({{M.baseColor[0], M.baseColor[1], M.baseColor[2]}, M.roughness, M.metallic, M.IoR,
+                              {M.emissive[0], M.emissive[1], M.emissive[2]}, M.doubleSided != 0});
+    }
+    for (int t = 0; t < d->numTriangles; t++) {
+        SurfTri T;
+        for (int k = 0; k < 3; k++) {
+            const int v = d->indices[3 * t + k];
+            T.p[k] = {d->positions[3 * v], d->positions[3 * v + 1], d->positions[3 * v + 2]};
+        }
+        f3 fn = cross(T.p[1] - T.p[0], T.p[2] - T.p[0]);
+        fn = dot(fn, fn) > 0.f ? normalize(fn) : f3{0.f, 0.f, 1.f};
+        for (int k = 0; k < 3; k++) {
+            const int v = d->indices[3 * t + k];
+            T.n[k] = d->normals ? f3{d->normals[3 * v], d->normals[3 * v + 1], d->normals[3 * v + 2]} : fn;
+        }
+        T.material = d->materialIds[t];
+        S.surfTris.push_back(T);
+    }
     auto P = [&](int v) { return f3{d->positions[3 * v], d->positions[3 * v + 1], d->positions[3 * v + 2]}; };
     for (int t = 0; t < d->numTriangles; t++) {
         const vrestir_material& m = d->materials[d->materialIds[t]];
@@ -1481,6 +1513,21 @@ int orc_set_surfaces(void* h, const vrestir_surfaces_desc* d) {
     return VRESTIR_OK;
 }
 
+// FindSurfaceHit and FindIfOccluded over [0, tMax] per ray (8 floats: origin, dir,
+// tMax, 0): out[8] = {t, triangle, bary, occluded, 0, 0, 0}.
+void orc_surface_hit(void* h, int n, const float* rays, float* out) {
+    Oracle* O = (Oracle*)h;
+    for (int i = 0; i < n; i++) {
+        const float* r = rays + 8 * i;
+        f3 o{r[0], r[1], r[2]}, d{r[3], r[4], r[5]};
+        HitInfo hi = findSurfaceHit(O->S, o, d, 0.f, r[6]);
+        float* q = out + 8 * i;
+        q[0] = hi.t; q[1] = (float)hi.tri; q[2] = hi.bary.x; q[3] = hi.bary.y;
+        q[4] = findIfOccluded(O->S, o, d, 0.f, r[6]) ? 1.f : 0.f;
+        q[5] = q[6] = q[7] = 0.f;
+    }
+}
+
 // sampleSceneLights at points (light kinds in `kinds` bits: 1 env, 2 point, 4 emissive):
 // out[8] = {Li rgb, pdfArea, light index, uv, ray distance}; index -9 when none.
 void orc_sample_lights(void* h, int kinds, int n, const float* points, const uint32_t* seeds, float* out) {
@@ -1550,6 +1597,7 @@ int orc_set_rows(void* h, int rowBegin, int rowEnd) {
     return VRESTIR_OK;
 }
 int orc_run_stage(void* h, int stage, int round) {
+    if (((Oracle*)h)->P.useSurfaceScene && stage != VRESTIR_STAGE_GENERATE_FEATURES) return VRESTIR_EUNSUPPORTED;
     runStage(*(Oracle*)h, stage, round);
     return VRESTIR_OK;
 }
@@ -1560,6 +1608,7 @@ static void* bufPtr(Oracle* O, int id, size_t& bytes) {
         case VRESTIR_BUF_LEARNED_CDF: bytes = O->learnedCdf.size() * 2; return O->learnedCdf.data();
         case VRESTIR_BUF_TRAIN_RECORDS: bytes = O->trainRecs.size() * 4; return O->trainRecs.data();
         case VRESTIR_BUF_FEATURES: bytes = O->features.size() * 8; return O->features.data();
+        case VRESTIR_BUF_VBUFFER: bytes = O->vbuffer.size() * 16; return O->vbuffer.data();
         case VRESTIR_BUF_TEMPORAL_FEATURES: bytes = O->temporalFeatures.size() * 8; return O->temporalFeatures.data();
         case VRESTIR_BUF_RESERVOIRS0: bytes = O->res[0].size() * 36; return O->res[0].data();
         case VRESTIR_BUF_RESERVOIRS1: bytes = O->res[1].size() * 36; return O->res[1].data();

@@ -62,6 +62,11 @@ struct EnvMapState {
 // (UT/Sampling/AliasTable.slang:34-45).
 struct EmissiveTriangle { f3 p[3]; f3 normal; float area; f3 Le; };
 struct AliasItem { float threshold; uint32_t indexA, indexB; };
+// A scene triangle (positions, vertex shading normals, material) and an untextured
+// metal-rough material.
+struct SurfTri { f3 p[3]; f3 n[3]; int material; };
+struct SurfMaterial { f3 baseColor; float roughness, metallic, IoR; f3 emissive; bool doubleSided; };
+struct VItem4 { int tri; float bx, by, t; };  // the V-buffer entry of the exchange format
 
 struct SceneState {
     Medium med;
@@ -72,6 +77,8 @@ struct SceneState {
     int emActive = 0;                      // triangles with positive flux
     float emWeightSum = 0.f;
     float emMult = 1.f;                    // Scene::emissiveIntensityMultiplier
+    std::vector<SurfTri> surfTris;         // every scene triangle, for ray queries
+    std::vector<SurfMaterial> surfMats;
     f3 env{0, 0, 0};                       // constant environment (envMap.textured == false)
     EnvMapState envMap;
     Camera cam, prevCam;
@@ -349,6 +356,67 @@ static inline f3 velocityWorld(const Medium& m, f3 pW) {
 }
 
 struct SceneLightSample { f3 dir; float distance; f3 Li; float pdf, pdfArea; f3 rayDir; float rayDistance; };
+
+// ---------------------------------------------------------------- ray queries
+// FindSurfaceHit / FindIfOccluded (VR/InlineRayTracingHelpers.slang:42-155) by brute
+// force over every triangle, with the watertight test of Woop, Benthin and Wald
+// (JCGT 2(1), 2013). Closest hit: smallest t, ties to the smallest triangle index.
+struct HitInfo { int tri = -1; float t = kRayTMax; f2 bary{0.f, 0.f}; };
+static inline float axisOf(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+static inline bool watertightHit(f3 o, f3 d, const SurfTri& T, float tMin, float tMax, float& t, f2& bary) {
+    float ad[3] = {std::fabs(d.x), std::fabs(d.y), std::fabs(d.z)};
+    // the axis of the largest |d| component (ties go to the later axis)
+    int kz = (ad[0] > ad[1]) ? (ad[0] > ad[2] ? 0 : 2) : (ad[1] > ad[2] ? 1 : 2);
+    int kx = (kz + 1) % 3, ky = (kx + 1) % 3;
+    if (axisOf(d, kz) < 0.f) std::swap(kx, ky);
+    const float Sx = axisOf(d, kx) / axisOf(d, kz), Sy = axisOf(d, ky) / axisOf(d, kz), Sz = 1.f / axisOf(d, kz);
+    const f3 A = T.p[0] - o, B = T.p[1] - o, C = T.p[2] - o;
+    const float Ax = axisOf(A, kx) - Sx * axisOf(A, kz), Ay = axisOf(A, ky) - Sy * axisOf(A, kz);
+    const float Bx = axisOf(B, kx) - Sx * axisOf(B, kz), By = axisOf(B, ky) - Sy * axisOf(B, kz);
+    const float Cx = axisOf(C, kx) - Sx * axisOf(C, kz), Cy = axisOf(C, ky) - Sy * axisOf(C, kz);
+    float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    if (U == 0.f || V == 0.f || W == 0.f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
+    const float det = U + V + W;
+    if (det == 0.f) return false;
+    const float Az = Sz * axisOf(A, kz), Bz = Sz * axisOf(B, kz), Cz = Sz * axisOf(C, kz);
+    const float Tn = U * Az + V * Bz + W * Cz;
+    const float inv = 1.f / det;
+    t = Tn * inv;
+    if (!(t >= tMin && t <= tMax)) return false;
+    bary = {V * inv, W * inv};
+    return true;
+}
+static inline HitInfo findSurfaceHit(const SceneState& S, f3 o, f3 d, float tMin, float tMax) {
+    HitInfo h;
+    for (size_t i = 0; i < S.surfTris.size(); i++) {
+        float t;
+        f2 b;
+        if (!watertightHit(o, d, S.surfTris[i], std::max(0.f, tMin), tMax, t, b)) continue;
+        if (h.tri < 0 || t < h.t) { h.tri = (int)i; h.t = t; h.bary = b; }
+    }
+    return h;
+}
+static inline bool findIfOccluded(const SceneState& S, f3 o, f3 d, float tMin, float tMax) {
+    for (size_t i = 0; i < S.surfTris.size(); i++) {
+        float t;
+        f2 b;
+        if (watertightHit(o, d, S.surfTris[i], std::max(0.f, tMin), tMax, t, b)) return true;
+    }
+    return false;
+}
+// _prepareShadingData's frontFacing and front-only emission (SC/Shading.slang:172, 229-232).
+static inline bool emitsTowards(const SceneState& S, int tri, f3 V) {
+    const SurfTri& T = S.surfTris[tri];
+    f3 faceN = cross(T.p[1] - T.p[0], T.p[2] - T.p[0]);
+    if (!(dot(V, faceN) >= 0.f)) return false;
+    f3 e = S.surfMats[T.material].emissive;
+    return e.x > 0.f || e.y > 0.f || e.z > 0.f;
+}
 
 // computeRayOrigin (UT/Helpers.slang:92-105).
 static inline f3 computeRayOrigin(f3 pos, f3 normal) {

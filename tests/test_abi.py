@@ -131,8 +131,7 @@ def test_create_without_device_fails_cleanly_or_succeeds_with_one():
 def test_invalid_params_rejected_without_device():
     lib = abi.load_library()
     h = ctypes.c_void_p()
-    for field, val, code in [("useSurfaceScene", 1, abi.VRESTIR_EUNSUPPORTED),
-                             ("vertexReuse", 1, abi.VRESTIR_EUNSUPPORTED),
+    for field, val, code in [("vertexReuse", 1, abi.VRESTIR_EUNSUPPORTED),
                              ("maxBounces", 0, abi.VRESTIR_EINVAL),
                              ("maxBounces", 9, abi.VRESTIR_EINVAL),
                              ("spatialSampleCount", 33, abi.VRESTIR_EINVAL),

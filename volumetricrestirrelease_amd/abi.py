@@ -38,13 +38,13 @@ STAGE_GENERATE_FEATURES, STAGE_TRACE_RAYS, STAGE_TEMPORAL_REUSE, STAGE_SPATIAL_R
 STAGE_LEARNED_PROPOSAL, STAGE_LEARNED_TRAIN = 6, 7
 BUF_FEATURES, BUF_TEMPORAL_FEATURES, BUF_RESERVOIRS0, BUF_RESERVOIRS1, BUF_TEMPORAL_RESERVOIRS, \
     BUF_EXTRA0, BUF_EXTRA1, BUF_TEMPORAL_EXTRA, BUF_COLOR, BUF_LEARNED_CDF, BUF_TRAIN_RECORDS, \
-    BUF_TRACE_REDONE = range(12)
+    BUF_TRACE_REDONE, BUF_VBUFFER = range(13)
 
 TRACK_TR_ANALYTIC, TRACK_TR_RAY_MARCHING, TRACK_TR_RATIO, TRACK_TR_RESIDUAL_RATIO, \
     TRACK_TR_ANALOG_RESIDUAL_RATIO, TRACK_SAMPLE_ANALYTIC, TRACK_SAMPLE_DECOMPOSITION, \
     TRACK_SAMPLE_CELL_BY_DENSITY, TRACK_FEATURE_MARCH, TRACK_DENSITY_AT_POINT, TRACK_RNG_STREAM, \
     TRACK_ENV_EVAL, TRACK_ENV_SAMPLE, TRACK_EMISSION, TRACK_VELOCITY, TRACK_DENSITY_INDEX, \
-    TRACK_VISITED_BRICKS, TRACK_LIGHT_SAMPLE, TRACK_EMISSIVE_EVAL = range(19)
+    TRACK_VISITED_BRICKS, TRACK_LIGHT_SAMPLE, TRACK_EMISSIVE_EVAL, TRACK_SURFACE_HIT = range(20)
 
 # (name, ctype) in declaration order of vrestir_params
 PARAM_FIELDS = [
@@ -134,7 +134,7 @@ class Material(ctypes.Structure):
 
 class SurfacesDesc(ctypes.Structure):
     """vrestir_surfaces_desc: world-space triangles and their materials."""
-    _fields_ = [("positions", POINTER(c_float)), ("numVertices", ctypes.c_int32),
+    _fields_ = [("positions", POINTER(c_float)), ("normals", POINTER(c_float)), ("numVertices", ctypes.c_int32),
                 ("indices", POINTER(ctypes.c_int32)), ("materialIds", POINTER(ctypes.c_int32)),
                 ("numTriangles", ctypes.c_int32), ("materials", POINTER(Material)),
                 ("numMaterials", ctypes.c_int32), ("emissiveIntensityMultiplier", c_float)]

@@ -110,6 +110,12 @@ struct vrestir_handle {
     float* emTriDev = nullptr;
     uint32_t* emAliasDev = nullptr;
     float* emWeightDev = nullptr;
+    // their ray-query structures
+    void* bvhDev = nullptr;
+    float* bvhTriDev = nullptr;
+    float* triDataDev = nullptr;
+    float* materialsDev = nullptr;
+    VItem* vbuf = nullptr;  // the V-buffer of a surface scene
     EnvXf envCur{}, envLast{};
     bool haveEnvLast = false;
     // the volume: frames of a sequence (one for a static volume), each resident in HBM
@@ -169,7 +175,7 @@ static void freeSplit(vrestir_handle* h) {
 static void freeFrameBuffers(vrestir_handle* h) {
     freeSplit(h);
     for (int i = 0; i < 2; i++) { dfree(h->res[i]); dfree(h->extra[i]); dfree(h->color[i]); }
-    dfree(h->resT); dfree(h->extraT); dfree(h->features); dfree(h->featuresT); dfree(h->outInternal);
+    dfree(h->resT); dfree(h->extraT); dfree(h->features); dfree(h->featuresT); dfree(h->vbuf); dfree(h->outInternal);
     h->W = h->H = 0;
 }
 static void freeVolume(vrestir_handle* h) {
@@ -191,7 +197,14 @@ static void freeSurfaces(vrestir_handle* h) {
     dfree(h->emTriDev);
     dfree(h->emAliasDev);
     dfree(h->emWeightDev);
+    dfree(h->bvhDev);
+    dfree(h->bvhTriDev);
+    dfree(h->triDataDev);
+    dfree(h->materialsDev);
     Scene& S = h->sceneHost;
+    S.bvh = nullptr;
+    S.bvhTri = S.triData = S.materials = nullptr;
+    S.numTris = S.numMaterials = 0;
     S.emTri = nullptr;
     S.emAlias = nullptr;
     S.emWeight = nullptr;
@@ -275,7 +288,6 @@ extern "C" void vrestir_default_params(vrestir_params* p) {
 
 static int checkParams(vrestir_handle* h, const vrestir_params* p) {
     if (!p) return fail(h, VRESTIR_EINVAL, "params is NULL");
-    if (p->useSurfaceScene) return fail(h, VRESTIR_EUNSUPPORTED, "surface scenes are not part of this pass build");
     if (p->vertexReuse) return fail(h, VRESTIR_EUNSUPPORTED, "vertex reuse is not implemented");
     if (p->maxBounces < 1 || p->maxBounces > 4) return fail(h, VRESTIR_EINVAL, "maxBounces must be in [1, 4]");
     if (p->initialM < 1) return fail(h, VRESTIR_EINVAL, "initialM must be >= 1");
@@ -940,6 +952,25 @@ extern "C" int32_t vrestir_set_surfaces(vrestir_handle* h, const vrestir_surface
         S.numEmActive = lc.numActive;
         S.emWeightSum = (float)lc.weightSum;
     }
+    SurfaceGeometryHost sg;
+    buildSurfaceGeometry(*d, sg);
+    if (d->numTriangles > 0) {
+        auto up = [&](const void* src, size_t bytes, void** dst) -> hipError_t {
+            hipError_t e = hipMalloc(dst, bytes);
+            if (e != hipSuccess) return e;
+            return hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+        };
+        HIPCHK(h, up(sg.nodes.data(), sg.nodes.size() * sizeof(float), &h->bvhDev));
+        HIPCHK(h, up(sg.tris.data(), sg.tris.size() * sizeof(float), (void**)&h->bvhTriDev));
+        HIPCHK(h, up(sg.triData.data(), sg.triData.size() * sizeof(float), (void**)&h->triDataDev));
+        HIPCHK(h, up(sg.materials.data(), sg.materials.size() * sizeof(float), (void**)&h->materialsDev));
+        S.bvh = (const BvhNode*)h->bvhDev;
+        S.bvhTri = h->bvhTriDev;
+        S.triData = h->triDataDev;
+        S.materials = h->materialsDev;
+        S.numTris = d->numTriangles;
+        S.numMaterials = d->numMaterials;
+    }
     h->optionsChanged = true;
     return uploadScene(h);
 }
@@ -1047,6 +1078,7 @@ static int allocFrameBuffers(vrestir_handle* h, int W, int H, int B) {
     if ((rc = al((void**)&h->extraT, ne * 12))) return rc;
     if ((rc = al((void**)&h->features, n * 8))) return rc;
     if ((rc = al((void**)&h->featuresT, n * 8))) return rc;
+    if ((rc = al((void**)&h->vbuf, n * sizeof(VItem)))) return rc;
     if ((rc = al((void**)&h->outInternal, n * 16))) return rc;
     h->W = W; h->H = H; h->allocB = B;
     return VRESTIR_OK;
@@ -1308,6 +1340,7 @@ extern "C" int32_t vrestir_prepare_frame(vrestir_handle* h, int32_t W, int32_t H
     F.noReuse = !p.enableSpatialReuse && !p.enableTemporalReuse;
     F.coarserIndirect = p.initialUseCoarserGridForIndirectBounce;
     F.useReference = p.useReference;
+    F.useSurfaceScene = p.useSurfaceScene;
     F.baselineSpp = p.baselineSamplePerPixel;
     F.visualizeTransmittance = p.visualizeTotalTransmittance;
     F.isFirstFrame = h->temporalAccumulated ? 0 : 1;
@@ -1532,6 +1565,7 @@ static int stageArgs(vrestir_handle* h, hipStream_t st, int stage, int round, fl
     a.offsets = h->offsetsDev;
     a.features = h->features;
     a.featuresT = h->featuresT;
+    a.vbuf = h->vbuf;
     a.resT = h->resT;
     a.extraT = h->extraT;
     const int t = h->totalRoundId;
@@ -1594,6 +1628,8 @@ extern "C" int32_t vrestir_run_stage(vrestir_handle* h, void* stream, int32_t st
     if (!h->frameOpen) return fail(h, VRESTIR_ESTATE, "vrestir_prepare_frame must precede stages");
     const vrestir_params& p = h->P;
     hipStream_t st = (hipStream_t)stream;
+    if (p.useSurfaceScene && stage != VRESTIR_STAGE_GENERATE_FEATURES)
+        return fail(h, VRESTIR_EUNSUPPORTED, "surface scenes: only GenerateFeatures (V-buffer) is built");
     if (stage == VRESTIR_STAGE_LEARNED_PROPOSAL) return learnedProposal(h, st);
     if (stage == VRESTIR_STAGE_LEARNED_TRAIN) return learnedTrainFrame(h, st);
     LaunchArgs a;
@@ -1631,6 +1667,8 @@ extern "C" int32_t vrestir_execute(vrestir_handle* h, void* stream, int32_t W, i
                                    float* outMotionVec) {
     if (!h) return VRESTIR_EINVAL;
     if (!outRGBA) return fail(h, VRESTIR_EINVAL, "outRGBA is NULL");
+    if (h->P.useSurfaceScene)
+        return fail(h, VRESTIR_EUNSUPPORTED, "surface scenes: only GenerateFeatures (V-buffer) is built");
     int rc = vrestir_prepare_frame(h, W, H);
     if (rc) return rc;
     const vrestir_params& p = h->P;
@@ -1686,6 +1724,7 @@ static int bufInfo(vrestir_handle* h, int id, void** dev, size_t& elems, int& ki
         case VRESTIR_BUF_LEARNED_CDF: *dev = h->lCdf; elems = h->lCdfPixels * 64; return 0;
         case VRESTIR_BUF_TRAIN_RECORDS: *dev = h->lRecs; elems = (size_t)h->lRecSlots * kLearnedRecFloats * 4; return 0;
         case VRESTIR_BUF_TRACE_REDONE: *dev = h->split.fix; elems = 4; return 0;
+        case VRESTIR_BUF_VBUFFER: *dev = h->vbuf; elems = n * sizeof(VItem); return 0;
     }
     return -1;
 }
@@ -1818,7 +1857,7 @@ extern "C" int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t 
                                        const float* rays, const uint32_t* seeds, float* out) {
     if (!h) return VRESTIR_EINVAL;
     if (!h->volumeSet) return fail(h, VRESTIR_ESTATE, "no volume bound");
-    if (mode < 0 || mode > VRESTIR_TRACK_EMISSIVE_EVAL) return fail(h, VRESTIR_EINVAL, "unknown tracking mode");
+    if (mode < 0 || mode > VRESTIR_TRACK_SURFACE_HIT) return fail(h, VRESTIR_EINVAL, "unknown tracking mode");
     if (mode == VRESTIR_TRACK_EMISSION && h->sceneHost.hasEmission && h->sceneHost.lutRows == 0)
         return fail(h, VRESTIR_ESTATE, "no blackbody table bound");
     if (mip < 0 || mip >= kMaxSlots || !h->meta.slots[mip].valid) return fail(h, VRESTIR_EINVAL, "grid slot not bound");
@@ -1971,7 +2010,7 @@ static int batchFrame(vrestir_batch* b, hipStream_t user, const int32_t* widths,
         if (learnedActive(h) != learned)
             return bfail(b, VRESTIR_EUNSUPPORTED, "passes differ in whether the learned proposal is on");
         if (!h->P.enableTemporalReuse || !h->P.enableSpatialReuse || h->P.useReference || h->freezeFrame ||
-            h->splitReuse != 2)
+            h->splitReuse != 2 || h->P.useSurfaceScene)
             return bfail(b, VRESTIR_EUNSUPPORTED, "batches need temporal and spatial reuse, split stages, no reference");
     }
     // the learned proposal's Adam step takes host-computed bias corrections, so

@@ -19,7 +19,7 @@ VR_DEV void copyFeaturesBody(const Frame& F, const float2* __restrict__ features
 
 template <int OP, int B>
 VR_DEV void runOp(const LaunchArgs& a, VBlock vb) {
-    if constexpr (OP == kOpFeatures) featuresBody(a.scene, a.frame, a.features, vb);
+    if constexpr (OP == kOpFeatures) featuresBody(a.scene, a.frame, a.features, a.vbuf, vb);
     else if constexpr (OP == kOpTrace) traceBody<B, 2, 2, 2, false>(a.scene, a.frame, a.res0, a.extra0, a.learned, vb);
     else if constexpr (OP == kOpTraceLearned)
         traceBody<B, 2, 2, 2, true>(a.scene, a.frame, a.res0, a.extra0, a.learned, vb);

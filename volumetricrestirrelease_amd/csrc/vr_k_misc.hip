@@ -6,8 +6,8 @@ namespace vr {
 
 // GenerateFeatures (VR/GenerateFeatures.cs.slang:57-102); the body is in vr_stages.hpp.
 __global__ __launch_bounds__(256) void kFeatures(const Scene* __restrict__ sc, const Frame fr,
-                                                 float2* __restrict__ features) {
-    featuresBody(sc, fr, features, launchBlock());
+                                                 float2* __restrict__ features, VItem* __restrict__ vbuf) {
+    featuresBody(sc, fr, features, vbuf, launchBlock());
 }
 
 
@@ -157,6 +157,14 @@ __global__ __launch_bounds__(256) void kTrack(const Scene* __restrict__ sc, cons
             }
             break;
         }
+        case 19: {  // closest and any hit over [0, tMax]
+            SurfHit h;
+            surfaceQuery(S, ray.o, ray.d, 0.f, ray.tMax, false, h);
+            o[0] = h.t; o[1] = (float)h.tri; o[2] = h.bx; o[3] = h.by;
+            SurfHit a;
+            o[4] = surfaceQuery(S, ray.o, ray.d, 0.f, ray.tMax, true, a) ? 1.f : 0.f;
+            break;
+        }
         default: break;
     }
     float* d = out + 8 * (size_t)i;
@@ -168,7 +176,7 @@ __global__ __launch_bounds__(256) void kTrack(const Scene* __restrict__ sc, cons
 hipError_t launchStage(const LaunchArgs& a) {
     const dim3 g = tiles(a.width, a.height), blk(256, 1, 1);
     switch (a.stage) {
-        case 0: hipLaunchKernelGGL(kFeatures, g, blk, a.occBytes, a.stream, a.scene, a.frame, a.features); break;
+        case 0: hipLaunchKernelGGL(kFeatures, g, blk, a.occBytes, a.stream, a.scene, a.frame, a.features, a.vbuf); break;
         case 1: return launchTraceRays(a);
         case 2: return launchTemporalReuse(a);
         case 3: return launchSpatialReuse(a);

@@ -32,6 +32,7 @@ struct LaunchArgs {
     float* extraT;
     float2* features;
     float2* featuresT;
+    VItem* vbuf;             // the V-buffer (surface scenes)
     float4* color0;
     float4* out;
     float2* mvec;

@@ -3,6 +3,7 @@
 // (MAX_BOUNCES of the reference) so per-bounce arrays live in registers.
 #pragma once
 #include "vr_device.hpp"
+#include "vr_rayquery.hpp"
 
 namespace vr {
 

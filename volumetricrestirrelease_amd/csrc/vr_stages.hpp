@@ -57,16 +57,28 @@ static inline dim3 tiles(int W, int H) { return dim3((W + 15) / 16, (H + 15) / 1
 
 // GenerateFeatures (VR/GenerateFeatures.cs.slang:57-102): the camera ray's
 // transmittance (feature march) and the "no reflective surface" flag.
-VR_DEV void featuresBody(const Scene* __restrict__ sc, const Frame& fr, float2* __restrict__ features, VBlock vb) {
+// With a surface scene the camera ray's closest triangle hit goes to the V-buffer,
+// ends the march, and clears the flag unless the hit is an emitting front face.
+VR_DEV void featuresBody(const Scene* __restrict__ sc, const Frame& fr, float2* __restrict__ features,
+                         VItem* __restrict__ vbuf, VBlock vb) {
     const Scene& S = *sc;
     const Frame& F = fr;
     occCacheLoad(S, F.occSlot[0], F.occSlot[1]);
     PixelIdx px = pixelOf(F, vb);
-    if (!px.ok || F.useReference) return;
+    if (!px.ok) return;
     Rng rng{0, 0, 0, 0};
     Ray r{ld3(F.cam.pos), rayDir(F.cam, px.x, px.y, F.width, F.height), 0.f, kTMax};
+    int noSurface = 1;
+    if (F.useSurfaceScene) {
+        SurfHit h;
+        surfaceQuery(S, r.o, r.d, 0.f, kTMax, false, h);
+        vbuf[px.id] = VItem{h.tri, h.bx, h.by, h.t};
+        r.tMax = h.t;
+        if (h.tri >= 0) noSurface = surfaceEmitsTowards(S, h.tri, -r.d) ? 1 : 0;
+    }
+    if (F.useReference) return;
     float accT = featureMarch(S, r, 0, true, F.initialOpt.visibilityTStepScale, rng);
-    features[px.id] = make_float2(__int_as_float(1), accT);
+    features[px.id] = make_float2(__int_as_float(noSurface), accT);
 }
 
 // IntegrateByVolumePathTracing (VR/VolumePathTracingFunctions.slang:3-132), NEE on.

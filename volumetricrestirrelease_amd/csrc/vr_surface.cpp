@@ -20,6 +20,9 @@ std::string checkSurfacesDesc(const vrestir_surfaces_desc& d) {
         if (d.materialIds[i] < 0 || d.materialIds[i] >= d.numMaterials) return "material id out of range";
     for (int64_t i = 0; i < 3 * (int64_t)d.numVertices; i++)
         if (!std::isfinite(d.positions[i])) return "non-finite vertex position";
+    if (d.normals)
+        for (int64_t i = 0; i < 3 * (int64_t)d.numVertices; i++)
+            if (!std::isfinite(d.normals[i])) return "non-finite vertex normal";
     for (int m = 0; m < d.numMaterials; m++)
         for (int k = 0; k < 3; k++)
             if (!(d.materials[m].emissive[k] >= 0.f) || !std::isfinite(d.materials[m].emissive[k]))
@@ -133,6 +136,154 @@ void buildLightCollection(const vrestir_surfaces_desc& d, LightCollectionHost& o
         out.alias[4 * i + 1] = redirect[i];
         out.alias[4 * i + 2] = i;
         out.alias[4 * i + 3] = 0u;
+    }
+}
+
+namespace {
+
+struct Box {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    void grow(const float* p) {
+        for (int k = 0; k < 3; k++) { lo[k] = std::min(lo[k], p[k]); hi[k] = std::max(hi[k], p[k]); }
+    }
+    void grow(const Box& b) {
+        for (int k = 0; k < 3; k++) { lo[k] = std::min(lo[k], b.lo[k]); hi[k] = std::max(hi[k], b.hi[k]); }
+    }
+    float area() const {
+        if (lo[0] > hi[0]) return 0.f;
+        const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        return 2.f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+float asFloatBits(int32_t v) {
+    float f;
+    std::memcpy(&f, &v, 4);
+    return f;
+}
+
+}  // namespace
+
+void buildSurfaceGeometry(const vrestir_surfaces_desc& d, SurfaceGeometryHost& out) {
+    out = SurfaceGeometryHost();
+    const int nt = d.numTriangles;
+    for (int m = 0; m < d.numMaterials; m++) {
+        const vrestir_material& M = d.materials[m];
+        const float e[kMaterialFloats] = {M.baseColor[0], M.baseColor[1], M.baseColor[2], M.roughness, M.metallic,
+                                          M.IoR, M.emissive[0], M.emissive[1], M.emissive[2],
+                                          (float)(M.doubleSided != 0), 0.f, 0.f};
+        out.materials.insert(out.materials.end(), e, e + kMaterialFloats);
+    }
+    if (nt == 0) return;
+    auto P = [&](int t, int k) { return d.positions + 3 * (size_t)d.indices[3 * t + k]; };
+    // per scene triangle: positions, shading normals (the vertices' own, else the
+    // face normal at each vertex), material index
+    out.triData.assign((size_t)kTriDataFloats * nt, 0.f);
+    for (int t = 0; t < nt; t++) {
+        float* o = &out.triData[(size_t)kTriDataFloats * t];
+        for (int k = 0; k < 3; k++)
+            for (int c = 0; c < 3; c++) o[3 * k + c] = P(t, k)[c];
+        if (d.normals) {
+            for (int k = 0; k < 3; k++)
+                for (int c = 0; c < 3; c++) o[9 + 3 * k + c] = d.normals[3 * (size_t)d.indices[3 * t + k] + c];
+        } else {
+            const F3 p0{o[0], o[1], o[2]}, p1{o[3], o[4], o[5]}, p2{o[6], o[7], o[8]};
+            const F3 N = crossf(sub(p1, p0), sub(p2, p0));
+            const F3 n = dotf(N, N) > 0.f ? normalizef(N) : F3{0.f, 0.f, 1.f};
+            for (int k = 0; k < 3; k++) { o[9 + 3 * k] = n.x; o[10 + 3 * k] = n.y; o[11 + 3 * k] = n.z; }
+        }
+        o[18] = asFloatBits(d.materialIds[t]);
+    }
+    // per-triangle boxes and centroids
+    std::vector<Box> tb(nt);
+    std::vector<float> cen(3 * (size_t)nt);
+    for (int t = 0; t < nt; t++) {
+        for (int k = 0; k < 3; k++) tb[t].grow(P(t, k));
+        for (int c = 0; c < 3; c++) cen[3 * t + c] = 0.5f * (tb[t].lo[c] + tb[t].hi[c]);
+    }
+    std::vector<int> order(nt);
+    for (int t = 0; t < nt; t++) order[t] = t;
+    struct Node { Box b; int first = 0, count = 0; };
+    std::vector<Node> nodes(1);
+    struct Item { int node, begin, end, depth; };
+    std::vector<Item> work{{0, 0, nt, 0}};
+    constexpr int kBins = 16, kLeaf = 4;
+    while (!work.empty()) {
+        const Item it = work.back();
+        work.pop_back();
+        Box b, cb;
+        for (int i = it.begin; i < it.end; i++) {
+            b.grow(tb[order[i]]);
+            cb.grow(&cen[3 * (size_t)order[i]]);
+        }
+        nodes[it.node].b = b;
+        out.depth = std::max(out.depth, it.depth);
+        const int n = it.end - it.begin;
+        auto makeLeaf = [&]() { nodes[it.node].first = it.begin; nodes[it.node].count = n; };
+        if (n <= kLeaf || it.depth >= kBvhMaxDepth) { makeLeaf(); continue; }
+        // binned SAH over the widest centroid axis
+        int axis = 0;
+        for (int k = 1; k < 3; k++)
+            if (cb.hi[k] - cb.lo[k] > cb.hi[axis] - cb.lo[axis]) axis = k;
+        const float ext = cb.hi[axis] - cb.lo[axis];
+        int mid = it.begin + n / 2;
+        if (ext > 0.f) {
+            Box bins[kBins];
+            int cnt[kBins] = {0};
+            auto binOf = [&](int t) {
+                const int k = (int)((cen[3 * (size_t)t + axis] - cb.lo[axis]) / ext * kBins);
+                return std::min(kBins - 1, std::max(0, k));
+            };
+            for (int i = it.begin; i < it.end; i++) {
+                const int k = binOf(order[i]);
+                bins[k].grow(tb[order[i]]);
+                cnt[k]++;
+            }
+            float best = INFINITY;
+            int split = -1;
+            for (int s = 1; s < kBins; s++) {
+                Box l, r;
+                int nl = 0, nr = 0;
+                for (int k = 0; k < s; k++) { l.grow(bins[k]); nl += cnt[k]; }
+                for (int k = s; k < kBins; k++) { r.grow(bins[k]); nr += cnt[k]; }
+                if (nl == 0 || nr == 0) continue;
+                const float c = l.area() * nl + r.area() * nr;
+                if (c < best) { best = c; split = s; }
+            }
+            if (split > 0) {
+                auto pm = std::stable_partition(order.begin() + it.begin, order.begin() + it.end,
+                                                [&](int t) { return binOf(t) < split; });
+                mid = (int)(pm - order.begin());
+            } else {
+                std::stable_sort(order.begin() + it.begin, order.begin() + it.end,
+                                 [&](int a, int c) { return cen[3 * (size_t)a + axis] < cen[3 * (size_t)c + axis]; });
+            }
+        } else if (n <= 16) {
+            makeLeaf();
+            continue;
+        }
+        const int left = (int)nodes.size();
+        nodes.resize(nodes.size() + 2);
+        nodes[it.node].first = left;
+        nodes[it.node].count = 0;
+        work.push_back({left + 1, mid, it.end, it.depth + 1});
+        work.push_back({left, it.begin, mid, it.depth + 1});
+    }
+    out.nodes.resize(8 * nodes.size());
+    for (size_t i = 0; i < nodes.size(); i++) {
+        float* o = &out.nodes[8 * i];
+        for (int k = 0; k < 3; k++) { o[k] = nodes[i].b.lo[k]; o[4 + k] = nodes[i].b.hi[k]; }
+        o[3] = asFloatBits(nodes[i].first);
+        o[7] = asFloatBits(nodes[i].count);
+    }
+    out.tris.resize((size_t)kBvhTriFloats * nt);
+    for (int i = 0; i < nt; i++) {
+        const int t = order[i];
+        float* o = &out.tris[(size_t)kBvhTriFloats * i];
+        for (int k = 0; k < 3; k++)
+            for (int c = 0; c < 3; c++) o[4 * k + c] = P(t, k)[c];
+        o[3] = asFloatBits(t);
+        o[7] = asFloatBits(d.materialIds[t]);
+        o[11] = 0.f;
     }
 }
 

@@ -93,8 +93,39 @@ struct Scene {
     const float* emWeight;
     int32_t numEmTris, numEmActive;
     float emWeightSum, emMult;
+    // Triangle geometry for ray queries (the acceleration structure of
+    // VR/InlineRayTracingHelpers.slang:8): a BVH over the triangles, the triangles in
+    // leaf order (kBvhTriFloats floats: v0, scene triangle index bits, v1, material
+    // index bits, v2, 0), per scene triangle kTriDataFloats floats (vertex
+    // positions, vertex shading normals, material index bits, 0), and per material
+    // kMaterialFloats floats (base colour, roughness, metallic, IoR, emissive,
+    // double-sided, 0, 0).
+    const struct BvhNode* bvh;
+    const float* bvhTri;
+    const float* triData;
+    const float* materials;
+    int32_t numTris, numMaterials;
 };
 constexpr int kEmTriFloats = 16;
+constexpr int kBvhTriFloats = 12;
+constexpr int kTriDataFloats = 20;
+constexpr int kMaterialFloats = 12;
+constexpr int kBvhMaxDepth = 30;  // the builder's limit; traversal stacks hold kBvhMaxDepth + 2 entries
+// A BVH node: count > 0 is a leaf of count triangles from bvhTri entry first;
+// count == 0 is an interior node whose children are nodes first and first + 1.
+struct BvhNode {
+    float bmin[3];
+    int32_t first;
+    float bmax[3];
+    int32_t count;
+};
+// A V-buffer entry (VBufferItem, VR/InlineRayTracingHelpers.slang:17-39): scene
+// triangle index (-1: no hit), the hit's barycentric weights of vertices 1 and 2,
+// and its ray distance (FLT_MAX: no hit).
+struct VItem {
+    int32_t tri;
+    float bx, by, t;
+};
 
 // Orientation of the environment map for one frame (EnvMapData::transform and
 // invTransform, LT/EnvMapData.slang:35-41): world = toWorld * local, row-major.
@@ -141,7 +172,7 @@ struct Frame {
     int32_t enableSpatial;
     int32_t spatialRounds;
     int32_t outputMotionVec;
-    int32_t pad_;
+    int32_t useSurfaceScene;      // GenerateFeatures intersects the triangles and writes the V-buffer
     int32_t occSlot[2];           // grid slots whose occupancy tables the kernel keeps in LDS (-1: none)
     CameraFrame cam, prevCam;
     float prevView[16], prevProj[16];

@@ -361,6 +361,7 @@ RESERVOIR_DTYPE = np.dtype([("runningSum", "<f4"), ("M", "<f4"), ("depth", "<f4"
                             ("lightUV", "<f4", 2), ("lightID", "<i4"), ("sampledPixel", "<i4"),
                             ("extraBounceStartId", "<i4")])
 FEATURES_DTYPE = np.dtype([("noReflectiveSurface", "<i4"), ("transmittance", "<f4")])
+VBUFFER_DTYPE = np.dtype([("tri", "<i4"), ("bary", "<f4", 2), ("t", "<f4")])
 
 
 class VolumetricReSTIRBatch:
