@@ -95,27 +95,43 @@ __global__ __launch_bounds__(kBuildThreads) void kClamp(const float* __restrict_
     waveMax(m, maxBits);
 }
 // Conservative grid: an empty voxel takes the mean of the positive values of its
-// 27-neighbourhood (summed x outermost, as the host build does).
+// 27-neighbourhood (summed x outermost, as the host build does). A workgroup takes
+// a 64 x 4 x 4 tile: the tile and its one-voxel halo are read once into LDS, a tile
+// with no positive value anywhere in it is copied through, and every other empty
+// voxel sums its neighbourhood from LDS in the host build's order.
+constexpr int kDilX = 64, kDilY = 4, kDilZ = 4;
 __global__ __launch_bounds__(kBuildThreads) void kDilate(const float* __restrict__ s, float* __restrict__ d, int nx,
                                                          int ny, int nz, unsigned* __restrict__ maxBits) {
-    const size_t n = (size_t)nx * ny * nz;
+    constexpr int TX = kDilX + 2, TY = kDilY + 2, TZ = kDilZ + 2;
+    __shared__ float t[TZ][TY][TX];
+    const int x0 = (int)blockIdx.x * kDilX, y0 = (int)blockIdx.y * kDilY, z0 = (int)blockIdx.z * kDilZ;
+    int positive = 0;
+    for (int i = threadIdx.x; i < TX * TY * TZ; i += blockDim.x) {
+        const int lx = i % TX, ly = (i / TX) % TY, lz = i / (TX * TY);
+        const float v = at(s, x0 + lx - 1, y0 + ly - 1, z0 + lz - 1, nx, ny, nz);
+        t[lz][ly][lx] = v;
+        positive |= v > 0.f ? 1 : 0;
+    }
+    const int any = __syncthreads_or(positive);
     unsigned m = 0u;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((size_t)nx * ny));
-        const float v = s[i];
+    for (int i = threadIdx.x; i < kDilX * kDilY * kDilZ; i += blockDim.x) {
+        const int lx = i % kDilX, ly = (i / kDilX) % kDilY, lz = i / (kDilX * kDilY);
+        const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
+        if (x >= nx || y >= ny || z >= nz) continue;
+        const float v = t[lz + 1][ly + 1][lx + 1];
         float o = v;
-        if (v == 0.f) {
+        if (v == 0.f && any) {
             float avg = 0.f;
             for (int ii = -1; ii <= 1; ii++)
                 for (int jj = -1; jj <= 1; jj++)
                     for (int kk = -1; kk <= 1; kk++) {
-                        const float tv = at(s, x + ii, y + jj, z + kk, nx, ny, nz);
+                        const float tv = t[lz + 1 + kk][ly + 1 + jj][lx + 1 + ii];
                         avg += tv > 0.f ? tv : 0.f;
                     }
             avg /= 27.f;
             if (avg > 0.f) o = avg;
         }
-        d[i] = o;
+        d[lin(x, y, z, nx, ny)] = o;
         m = max(m, __float_as_uint(o));
     }
     waveMax(m, maxBits);
@@ -148,10 +164,16 @@ __global__ __launch_bounds__(kBuildThreads) void kBrickOcc(const float* __restri
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < nb; c += (size_t)gridDim.x * blockDim.x) {
         const int bx = (int)(c % nbx), by = (int)((c / nbx) % nby), bz = (int)(c / ((size_t)nbx * nby));
         bool any = false;
+        // a row's 8 loads are issued together, then tested
         for (int z = 0; z < 8 && !any; z++)
-            for (int y = 0; y < 8 && !any; y++)
-                for (int x = 0; x < 8 && !any; x++)
-                    if (stored(s[lin(bx * 8 + x, by * 8 + y, bz * 8 + z, nx, ny)], gmax, fp16) > 0.f) any = true;
+            for (int y = 0; y < 8 && !any; y++) {
+                float row[8];
+                const float* src = s + lin(bx * 8, by * 8 + y, bz * 8 + z, nx, ny);
+#pragma unroll
+                for (int x = 0; x < 8; x++) row[x] = src[x];
+#pragma unroll
+                for (int x = 0; x < 8; x++) any |= stored(row[x], gmax, fp16) > 0.f;
+            }
         occ[c] = any ? 1 : 0;
     }
 }
@@ -225,9 +247,14 @@ __global__ __launch_bounds__(kBuildThreads) void kBrickFill(const float* __restr
         const int x0 = (int)(c % nbx) * 8, y0 = (int)((c / nbx) % nby) * 8, z0 = (int)(c / ((size_t)nbx * nby)) * 8;
         float mn = 3.402823466e+38f, mx = 0.f, sum = 0.f;
         for (int i = -1; i <= 8; i++)
-            for (int j = -1; j <= 8; j++)
+            for (int j = -1; j <= 8; j++) {
+                // the 10 loads along z are issued together, then consumed in order
+                float raw[kApron];
+#pragma unroll
+                for (int k = -1; k <= 8; k++) raw[k + 1] = at(s, x0 + i, y0 + j, z0 + k, nx, ny, nz);
+#pragma unroll
                 for (int k = -1; k <= 8; k++) {
-                    const float v = stored(at(s, x0 + i, y0 + j, z0 + k, nx, ny, nz), gmax, FP16);
+                    const float v = stored(raw[k + 1], gmax, FP16);
                     const size_t o = id * kBrickVals + (size_t)(i + 1) + kApron * ((size_t)(j + 1) + kApron * (size_t)(k + 1));
                     if (FP16) ((uint16_t*)pool)[o] = halfBits(v);
                     else ((float*)pool)[o] = v;
@@ -235,6 +262,7 @@ __global__ __launch_bounds__(kBuildThreads) void kBrickFill(const float* __restr
                     mx = fmaxf(mx, v);
                     sum = __fadd_rn(sum, v);
                 }
+            }
         bounds[4 * id + 0] = mn;
         bounds[4 * id + 1] = mx;
         bounds[4 * id + 2] = sum / 512.f;
@@ -251,8 +279,9 @@ hipError_t buildClamp(hipStream_t st, const float* in, float* out, size_t n, int
     return hipGetLastError();
 }
 hipError_t buildDilate(hipStream_t st, const float* s, float* d, int nx, int ny, int nz, unsigned* maxBits) {
-    hipLaunchKernelGGL(kDilate, gridFor((size_t)nx * ny * nz), dim3(kBuildThreads), 0, st, s, d, nx, ny, nz,
-                       maxBits);
+    const dim3 g((unsigned)((nx + kDilX - 1) / kDilX), (unsigned)((ny + kDilY - 1) / kDilY),
+                 (unsigned)((nz + kDilZ - 1) / kDilZ));
+    hipLaunchKernelGGL(kDilate, g, dim3(kBuildThreads), 0, st, s, d, nx, ny, nz, maxBits);
     return hipGetLastError();
 }
 hipError_t buildReduce2(hipStream_t st, const float* s, float* d, int nx, int ny, int nz, unsigned* maxBits) {
