@@ -367,7 +367,13 @@ enum {
     /* FindSurfaceHit and FindIfOccluded (VR/InlineRayTracingHelpers.slang:42-155) over
      * [0, tMax]: out = {closest t (FLT_MAX: none), its triangle, barycentric weights of
      * vertices 1 and 2, occluded (any hit), 0, 0, 0}. */
-    VRESTIR_TRACK_SURFACE_HIT = 19
+    VRESTIR_TRACK_SURFACE_HIT = 19,
+    /* Shading at the closest hit over [0, tMax] (computeSurfaceShadingInfo and the
+     * legacy standard BSDF, MaterialShading.slang:167-631): numSamples 0 gives
+     * sampleBSDF's {wi xyz, pdf, weight rgb, lobe (1 diffuse, 2 specular; -1 no
+     * sample)}, numSamples 1 the evaluation at that wi {evalBSDFCosine rgb,
+     * evalPdfBSDF, emissive rgb, NdotV}; all zero without a hit. */
+    VRESTIR_TRACK_SURFACE_SHADE = 20
 };
 int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t mode, int32_t mipLevel,
                             int32_t useLinearSampler, float tStepScale, int32_t numSamples, int32_t n,

@@ -38,6 +38,7 @@ def lib():
         "orc_sample_lights": (None, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
         "orc_emissive_eval": (None, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
         "orc_surface_hit": (None, [c_void_p, c_int, c_void_p, c_void_p]),
+        "orc_surface_shade": (None, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
         "orc_set_camera": (c_int, [c_void_p, POINTER(abi.Camera), c_int, c_int]),
         "orc_execute": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_int]),
         "orc_prepare_frame": (c_int, [c_void_p, c_int, c_int, c_int]),
@@ -169,6 +170,14 @@ class Oracle:
         r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
         out = np.zeros((r.shape[0], 8), np.float32)
         self.L.orc_surface_hit(self.h, r.shape[0], r.ctypes.data, out.ctypes.data)
+        return out
+
+    def surface_shade(self, variant, rays: np.ndarray, seeds: np.ndarray) -> np.ndarray:
+        """Shading at each ray's closest hit: variant 0 a BSDF sample, 1 its evaluation: (n, 8)."""
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        sd = np.ascontiguousarray(seeds, dtype=np.uint32).reshape(-1, 3)
+        out = np.zeros((r.shape[0], 8), np.float32)
+        self.L.orc_surface_shade(self.h, variant, r.shape[0], r.ctypes.data, sd.ctypes.data, out.ctypes.data)
         return out
 
     def emissive_eval(self, points: np.ndarray, tris: np.ndarray, uvs: np.ndarray) -> np.ndarray:

@@ -1528,6 +1528,33 @@ void orc_surface_hit(void* h, int n, const float* rays, float* out) {
     }
 }
 
+// Shading at each ray's closest hit (rays of 8 floats, seeds of 3 words): variant 0
+// out = sampleBSDF {wi, pdf, weight, lobe or -1}; variant 1 = the evaluation at that wi
+// {evalBSDFCosine, evalPdfBSDF, emissive, NdotV}; zeros without a hit.
+void orc_surface_shade(void* h, int variant, int n, const float* rays, const uint32_t* seeds, float* out) {
+    Oracle* O = (Oracle*)h;
+    for (int i = 0; i < n; i++) {
+        const float* r = rays + 8 * i;
+        float* q = out + 8 * i;
+        for (int k = 0; k < 8; k++) q[k] = 0.f;
+        f3 o{r[0], r[1], r[2]}, d{r[3], r[4], r[5]};
+        HitInfo hi = findSurfaceHit(O->S, o, d, 0.f, r[6]);
+        if (hi.tri < 0) continue;
+        ShadingData sd = computeSurfaceShadingInfo(O->S, hi.tri, hi.bary, -d);
+        SampleGen sg = makeSampleGen(seeds[3 * i], seeds[3 * i + 1], seeds[3 * i + 2]);
+        BSDFSample bs;
+        bool ok = sampleBSDF(sd, sg, bs);
+        if (variant == 0) {
+            q[0] = bs.wi.x; q[1] = bs.wi.y; q[2] = bs.wi.z; q[3] = bs.pdf;
+            q[4] = bs.weight.x; q[5] = bs.weight.y; q[6] = bs.weight.z; q[7] = ok ? (float)bs.lobe : -1.f;
+        } else {
+            f3 f = evalBSDFCosine(sd, bs.wi);
+            q[0] = f.x; q[1] = f.y; q[2] = f.z; q[3] = evalPdfBSDF(sd, bs.wi);
+            q[4] = sd.emissive.x; q[5] = sd.emissive.y; q[6] = sd.emissive.z; q[7] = sd.NdotV;
+        }
+    }
+}
+
 // sampleSceneLights at points (light kinds in `kinds` bits: 1 env, 2 point, 4 emissive):
 // out[8] = {Li rgb, pdfArea, light index, uv, ray distance}; index -9 when none.
 void orc_sample_lights(void* h, int kinds, int n, const float* points, const uint32_t* seeds, float* out) {

@@ -409,6 +409,178 @@ static inline bool findIfOccluded(const SceneState& S, f3 o, f3 d, float tMin, f
     }
     return false;
 }
+// ---------------------------------------------------------------- surface shading
+// computeSurfaceShadingInfo (VR/InlineRayTracingHelpers.slang:157-173) with the
+// metal-rough branch of _prepareShadingData (SC/Shading.slang:155-245), and the
+// legacy standard BSDF (Experimental/Scene/Material/MaterialShading.slang:167-631:
+// Frostbite diffuse, height-correlated Smith-GGX, Schlick Fresnel). The tangent
+// frame is Duff et al.'s orthonormal basis around N (the triangles carry no tangents).
+static inline f3 computeRayOrigin(f3 pos, f3 normal);
+struct ShadingData {
+    f3 posW, V, N, faceN, T, B;
+    bool frontFacing;
+    float NdotV;
+    f3 diffuse, specular, emissive;
+    float linearRoughness, ggxAlpha, metallic;
+};
+static const float kMinGGXAlpha = 0.0064f, kMinCosTheta = 1e-6f;
+static inline float saturate(float x) { return std::min(std::max(x, 0.f), 1.f); }
+static inline f3 lerp3(f3 a, f3 b, float t) { return a + (b - a) * t; }
+static inline ShadingData computeSurfaceShadingInfo(const SceneState& S, int tri, f2 bary, f3 viewDir) {
+    const SurfTri& T = S.surfTris[tri];
+    const SurfMaterial& M = S.surfMats[T.material];
+    float b0 = 1.f - bary.x - bary.y;
+    ShadingData sd;
+    sd.posW = T.p[0] * b0 + T.p[1] * bary.x + T.p[2] * bary.y;
+    sd.faceN = normalize(cross(T.p[1] - T.p[0], T.p[2] - T.p[0]));
+    sd.N = normalize(T.n[0] * b0 + T.n[1] * bary.x + T.n[2] * bary.y);
+    sd.V = viewDir;
+    sd.frontFacing = dot(sd.V, sd.faceN) >= 0.f;
+    sd.diffuse = lerp3(M.baseColor, mk3(0.f), M.metallic);
+    float f = (M.IoR - 1.f) / (M.IoR + 1.f), F0 = f * f;
+    sd.specular = lerp3(mk3(F0), M.baseColor, M.metallic);
+    sd.linearRoughness = M.roughness;
+    sd.metallic = M.metallic;
+    sd.ggxAlpha = std::max(0.0064f, M.roughness * M.roughness);
+    sd.emissive = sd.frontFacing ? M.emissive : mk3(0.f);
+    sd.NdotV = dot(sd.N, sd.V);
+    if (!sd.frontFacing && M.doubleSided) {
+        sd.N = -sd.N;
+        sd.NdotV = -sd.NdotV;
+    }
+    // Duff, Burgess, Christensen, Hery, Kensler, Liani, Villemin (JCGT 2017)
+    float sign = std::copysign(1.f, sd.N.z);
+    float a = -1.f / (sign + sd.N.z), c = sd.N.x * sd.N.y * a;
+    sd.T = {1.f + sign * sd.N.x * sd.N.x * a, sign * c, -sign * sd.N.x};
+    sd.B = {c, sign + sd.N.y * sd.N.y * a, -sd.N.y};
+    sd.posW = computeRayOrigin(sd.posW, sd.frontFacing ? sd.faceN : -sd.faceN);
+    return sd;
+}
+static inline f3 fromLocal(f3 v, const ShadingData& sd) { return sd.T * v.x + sd.B * v.y + sd.N * v.z; }
+static inline f3 evalFresnelSchlick(f3 f0, f3 f90, float cosTheta) {
+    return f0 + (f90 - f0) * std::pow(std::max(1.f - cosTheta, 0.f), 5.f);
+}
+static inline float evalNdfGGX(float alpha, float cosTheta) {
+    float a2 = alpha * alpha;
+    float d = ((cosTheta * a2 - cosTheta) * cosTheta + 1.f);
+    return a2 / (d * d * (float)M_PI);
+}
+static inline float evalMaskingSmithGGXCorrelated(float alpha, float NdotL, float NdotV) {
+    float a2 = alpha * alpha;
+    float lambdaV = NdotL * std::sqrt((-NdotV * a2 + NdotV) * NdotV + a2);
+    float lambdaL = NdotV * std::sqrt((-NdotL * a2 + NdotL) * NdotL + a2);
+    return 0.5f / (lambdaV + lambdaL);
+}
+static inline f3 evalDiffuseFrostbite(const ShadingData& sd, float NdotL, float NdotV, float LdotH) {
+    float energyBias = 0.f + (0.5f - 0.f) * sd.linearRoughness;
+    float energyFactor = 1.f + (1.f / 1.51f - 1.f) * sd.linearRoughness;
+    float fd90 = energyBias + 2.f * LdotH * LdotH * sd.linearRoughness;
+    float lightScatter = evalFresnelSchlick(mk3(1.f), mk3(fd90), NdotL).x;
+    float viewScatter = evalFresnelSchlick(mk3(1.f), mk3(fd90), NdotV).x;
+    return sd.diffuse * (viewScatter * lightScatter * energyFactor * (1.f / (float)M_PI));
+}
+enum { kLobeDiffuse = 1, kLobeSpecular = 2 };
+static inline f3 evalBSDFLobes(const ShadingData& sd, f3 L, int lobes) {
+    float NdotL = dot(sd.N, L);
+    if (std::min(sd.NdotV, NdotL) < kMinCosTheta) return mk3(0.f);
+    f3 H = normalize(sd.V + L);
+    float NdotH = saturate(dot(sd.N, H)), LdotH = saturate(dot(L, H)), NdotV = saturate(sd.NdotV);
+    NdotL = saturate(NdotL);
+    f3 thp = mk3(0.f);
+    if (lobes & kLobeDiffuse) thp = thp + evalDiffuseFrostbite(sd, NdotL, NdotV, LdotH);
+    if (lobes & kLobeSpecular) {
+        float alpha = std::max(kMinGGXAlpha, sd.ggxAlpha);
+        float D = evalNdfGGX(alpha, NdotH);
+        float G = evalMaskingSmithGGXCorrelated(alpha, NdotL, NdotV);
+        thp = thp + evalFresnelSchlick(sd.specular, mk3(1.f), LdotH) * (D * G);
+    }
+    return thp;
+}
+static inline f3 evalBSDFCosine(const ShadingData& sd, f3 L) {
+    return evalBSDFLobes(sd, L, 3) * saturate(dot(sd.N, L));
+}
+static inline float evalPdfDiffuse(const ShadingData& sd, f3 L) {
+    float NdotL = dot(sd.N, L);
+    return NdotL < kMinCosTheta ? 0.f : NdotL * (float)M_1_PI;
+}
+static inline float evalPdfSpecular(const ShadingData& sd, f3 L) {
+    float NdotL = dot(sd.N, L);
+    if (std::min(sd.NdotV, NdotL) < kMinCosTheta) return 0.f;
+    f3 H = normalize(sd.V + L);
+    float NdotH = saturate(dot(sd.N, H)), VdotH = saturate(dot(sd.V, H));
+    float alpha = std::max(kMinGGXAlpha, sd.ggxAlpha);
+    return evalNdfGGX(alpha, NdotH) * NdotH / (4.f * VdotH);
+}
+static inline float getDiffuseProbability(const ShadingData& sd) {
+    float dw = (1.f - sd.metallic) * luminance(sd.diffuse);
+    float sw = luminance(evalFresnelSchlick(sd.specular, mk3(1.f), dot(sd.V, sd.N)));
+    return (dw + sw) > 0.f ? dw / (dw + sw) : 1.f;
+}
+static inline float evalPdfBSDF(const ShadingData& sd, f3 L) {
+    float pd = getDiffuseProbability(sd);
+    return pd * evalPdfDiffuse(sd, L) + (1.f - pd) * evalPdfSpecular(sd, L);
+}
+struct BSDFSample { f3 wi, weight; float pdf; int lobe; };
+static inline bool sampleBSDF(const ShadingData& sd, SampleGen& sg, BSDFSample& r) {
+    f2 u;
+    u.x = next1D(sg);
+    u.y = next1D(sg);
+    float uSelect = next1D(sg);
+    float pDiffuse = getDiffuseProbability(sd);
+    float pmf, pdfOther;
+    r.weight = mk3(0.f);
+    if (uSelect < pDiffuse) {
+        // sampleDiffuse: sample_cosine_hemisphere_concentric (UT/Math/MathHelpers.slang:293-322)
+        float x = 2.f * u.x - 1.f, y = 2.f * u.y - 1.f, dx = 0.f, dy = 0.f;
+        if (!(x == 0.f && y == 0.f)) {
+            float rr, phi;
+            if (std::fabs(x) > std::fabs(y)) { rr = x; phi = (y / x) * ((float)M_PI / 4.f); }
+            else { rr = y; phi = (float)M_PI / 2.f - (x / y) * ((float)M_PI / 4.f); }
+            dx = rr * std::cos(phi);
+            dy = rr * std::sin(phi);
+        }
+        float z = std::sqrt(std::max(0.f, 1.f - (dx * dx + dy * dy)));
+        r.pdf = z * (float)M_1_PI;
+        r.wi = fromLocal({dx, dy, z}, sd);
+        r.lobe = kLobeDiffuse;
+        float NdotL = dot(sd.N, r.wi);
+        if (std::min(sd.NdotV, NdotL) < kMinCosTheta) r.pdf = 0.f;
+        else r.weight = evalBSDFLobes(sd, r.wi, kLobeDiffuse) * (float)M_PI;
+        pmf = pDiffuse;
+        pdfOther = evalPdfSpecular(sd, r.wi);
+    } else {
+        // sampleSpecular with sampleNdfGGX_Walter
+        float alpha = std::max(kMinGGXAlpha, sd.ggxAlpha);
+        float a2 = alpha * alpha;
+        float cos2 = std::min((1.f - u.x) / ((a2 - 1.f) * u.x + 1.f), 1.0f);
+        float cosH = std::sqrt(cos2), sinH = std::sqrt(1.f - cos2);
+        float phi = u.y * 2.f * (float)M_PI;
+        f3 H = fromLocal({sinH * std::cos(phi), sinH * std::sin(phi), cosH}, sd);
+        float VdotH = dot(sd.V, H);
+        r.wi = normalize(H * (2.f * VdotH) - sd.V);
+        float d = (a2 - 1.f) * cos2 + 1.f;
+        r.pdf = (a2 * cosH) / (d * d * VdotH * 4.f * (float)M_PI);
+        r.lobe = kLobeSpecular;
+        float NdotL = dot(sd.N, r.wi);
+        if (NdotL < kMinCosTheta) r.pdf = 0.f;
+        if (std::min(sd.NdotV, NdotL) < kMinCosTheta) {
+            r.pdf = 0.f;
+        } else {
+            float NdotV = saturate(sd.NdotV);
+            NdotL = saturate(NdotL);
+            float NdotH = saturate(cosH);
+            VdotH = saturate(VdotH);
+            float G = evalMaskingSmithGGXCorrelated(alpha, NdotL, NdotV);
+            r.weight = evalFresnelSchlick(sd.specular, mk3(1.f), VdotH) * ((G * NdotL * VdotH * 4.f) / NdotH);
+        }
+        pmf = 1.f - pDiffuse;
+        pdfOther = evalPdfDiffuse(sd, r.wi);
+    }
+    r.pdf = pmf * r.pdf + (1.f - pmf) * pdfOther;
+    r.weight = r.weight / pmf;
+    return r.pdf > 0.f;
+}
+
 // _prepareShadingData's frontFacing and front-only emission (SC/Shading.slang:172, 229-232).
 static inline bool emitsTowards(const SceneState& S, int tri, f3 V) {
     const SurfTri& T = S.surfTris[tri];

@@ -17,6 +17,7 @@ int orc_set_volume(void* h, const vrestir_volume_desc* d);
 int orc_set_lights(void* h, const vrestir_point_light* lights, int n, const float env[3]);
 int orc_set_surfaces(void* h, const vrestir_surfaces_desc* desc);
 void orc_surface_hit(void* h, int n, const float* rays, float* out);
+void orc_surface_shade(void* h, int variant, int n, const float* rays, const uint32_t* seeds, float* out);
 void orc_sample_lights(void* h, int kinds, int n, const float* points, const uint32_t* seeds, float* out);
 void orc_emissive_eval(void* h, int n, const float* points, const int* tris, const float* uvs, float* out);
 int orc_set_camera(void* h, const vrestir_camera* c, int width, int height);

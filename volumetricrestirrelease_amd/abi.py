@@ -44,7 +44,8 @@ TRACK_TR_ANALYTIC, TRACK_TR_RAY_MARCHING, TRACK_TR_RATIO, TRACK_TR_RESIDUAL_RATI
     TRACK_TR_ANALOG_RESIDUAL_RATIO, TRACK_SAMPLE_ANALYTIC, TRACK_SAMPLE_DECOMPOSITION, \
     TRACK_SAMPLE_CELL_BY_DENSITY, TRACK_FEATURE_MARCH, TRACK_DENSITY_AT_POINT, TRACK_RNG_STREAM, \
     TRACK_ENV_EVAL, TRACK_ENV_SAMPLE, TRACK_EMISSION, TRACK_VELOCITY, TRACK_DENSITY_INDEX, \
-    TRACK_VISITED_BRICKS, TRACK_LIGHT_SAMPLE, TRACK_EMISSIVE_EVAL, TRACK_SURFACE_HIT = range(20)
+    TRACK_VISITED_BRICKS, TRACK_LIGHT_SAMPLE, TRACK_EMISSIVE_EVAL, TRACK_SURFACE_HIT, \
+    TRACK_SURFACE_SHADE = range(21)
 
 # (name, ctype) in declaration order of vrestir_params
 PARAM_FIELDS = [

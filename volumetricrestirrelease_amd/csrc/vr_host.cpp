@@ -1857,7 +1857,7 @@ extern "C" int32_t vrestir_track_batch(vrestir_handle* h, void* stream, int32_t 
                                        const float* rays, const uint32_t* seeds, float* out) {
     if (!h) return VRESTIR_EINVAL;
     if (!h->volumeSet) return fail(h, VRESTIR_ESTATE, "no volume bound");
-    if (mode < 0 || mode > VRESTIR_TRACK_SURFACE_HIT) return fail(h, VRESTIR_EINVAL, "unknown tracking mode");
+    if (mode < 0 || mode > VRESTIR_TRACK_SURFACE_SHADE) return fail(h, VRESTIR_EINVAL, "unknown tracking mode");
     if (mode == VRESTIR_TRACK_EMISSION && h->sceneHost.hasEmission && h->sceneHost.lutRows == 0)
         return fail(h, VRESTIR_ESTATE, "no blackbody table bound");
     if (mip < 0 || mip >= kMaxSlots || !h->meta.slots[mip].valid) return fail(h, VRESTIR_EINVAL, "grid slot not bound");

@@ -165,6 +165,22 @@ __global__ __launch_bounds__(256) void kTrack(const Scene* __restrict__ sc, cons
             o[4] = surfaceQuery(S, ray.o, ray.d, 0.f, ray.tMax, true, a) ? 1.f : 0.f;
             break;
         }
+        case 20: {  // shading at the closest hit: a BSDF sample (numSamples 0) or its evaluation (1)
+            SurfHit h;
+            if (!surfaceQuery(S, ray.o, ray.d, 0.f, ray.tMax, false, h)) break;
+            const ShadingData sd = surfaceShading(S, h.tri, h.bx, h.by, -ray.d);
+            BSDFSample bs;
+            const bool ok = sampleBSDF(sd, rng, bs);
+            if (numSamples == 0) {
+                o[0] = bs.wi.x; o[1] = bs.wi.y; o[2] = bs.wi.z; o[3] = bs.pdf;
+                o[4] = bs.weight.x; o[5] = bs.weight.y; o[6] = bs.weight.z; o[7] = ok ? (float)bs.lobe : -1.f;
+            } else {
+                const v3 f = evalBSDFCosine(sd, bs.wi);
+                o[0] = f.x; o[1] = f.y; o[2] = f.z; o[3] = evalPdfBSDF(sd, bs.wi);
+                o[4] = sd.emissive.x; o[5] = sd.emissive.y; o[6] = sd.emissive.z; o[7] = sd.NdotV;
+            }
+            break;
+        }
         default: break;
     }
     float* d = out + 8 * (size_t)i;

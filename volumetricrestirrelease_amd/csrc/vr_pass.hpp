@@ -4,6 +4,7 @@
 #pragma once
 #include "vr_device.hpp"
 #include "vr_rayquery.hpp"
+#include "vr_bsdf.hpp"
 
 namespace vr {
 
@@ -199,17 +200,6 @@ VR_DEV v3 envSample(const Scene& S, const EnvXf& X, float r0, float r1, float& p
 struct LightSample { v3 dir, Li, rayDir; float pdf, pdfArea, rayDistance; };
 
 // ------------------------------------------------------------------ emissive triangles
-// computeRayOrigin (UT/Helpers.slang:92-105): a point moved off its surface along
-// the normal, by a few ulps of each coordinate (by a fixed 2^-16 near the origin).
-VR_DEV v3 rayOriginOffset(v3 p, v3 n) {
-    const float origin = 1.f / 32.f, fScale = 1.f / 65536.f, iScale = 256.f;
-    const int ix = (int)(n.x * iScale), iy = (int)(n.y * iScale), iz = (int)(n.z * iScale);
-    const float px = __int_as_float(__float_as_int(p.x) + (p.x < 0.f ? -ix : ix));
-    const float py = __int_as_float(__float_as_int(p.y) + (p.y < 0.f ? -iy : iy));
-    const float pz = __int_as_float(__float_as_int(p.z) + (p.z < 0.f ? -iz : iz));
-    return V3(fabsf(p.x) < origin ? p.x + n.x * fScale : px, fabsf(p.y) < origin ? p.y + n.y * fScale : py,
-              fabsf(p.z) < origin ? p.z + n.z * fScale : pz);
-}
 struct TriLightSample { v3 posW, normal, dir, Le; float distance, pdf, pdfArea, cosTheta; };
 // sampleTriangle (LT/EmissiveLightSamplerHelpers.slang:56-94): the point of
 // sample_triangle(u) (UT/Math/MathHelpers.slang:356-361) on collection triangle
