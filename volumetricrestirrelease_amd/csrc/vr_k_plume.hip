@@ -13,7 +13,12 @@
 //     voxel centre through the trilinearly interpolated velocity, sample the old
 //     field trilinearly there, then decay (density) and cool (temperature), and
 //     add the source: a noisy sphere at the base of the axis, hot at its core.
-// Deterministic: no atomics, every value a function of the seed and the step.
+// Empty space is skipped: each step marks the 8^3 blocks where the new density is
+// non-zero, the marks are dilated by the farthest a back-trace can reach (from the
+// largest lattice speed), and a voxel outside the dilated marks and the source
+// writes zeros without tracing, which is what tracing would give it.
+// Deterministic: every value a function of the seed and the step (the only atomic
+// is a maximum).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -73,9 +78,10 @@ struct PlumeParams {
 };
 
 // velocity on the V^3 lattice (lattice point i at (i + 0.5) / V), unit cube per frame
-__global__ __launch_bounds__(kThreads) void kVelocity(PlumeParams P, float4* __restrict__ vel) {
+__global__ __launch_bounds__(kThreads) void kVelocity(PlumeParams P, float4* __restrict__ vel,
+                                                      unsigned* __restrict__ vmaxBits) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= kVel * kVel * kVel) return;
+    if (i >= kVel * kVel * kVel) return;  // kVel^3 is a multiple of the block size
     const int x = i % kVel, y = (i / kVel) % kVel, z = i / (kVel * kVel);
     const float px = (x + 0.5f) / kVel, py = (y + 0.5f) / kVel, pz = (z + 0.5f) / kVel;
     const float h = 0.5f / kVel;
@@ -88,7 +94,34 @@ __global__ __launch_bounds__(kThreads) void kVelocity(PlumeParams P, float4* __r
     const float cx = d(2, 1) - d(1, 2), cy = d(0, 2) - d(2, 0), cz = d(1, 0) - d(0, 1);
     const float rx = px - 0.5f, rz = pz - 0.5f;
     const float up = P.updraft * expf(-(rx * rx + rz * rz) / (P.width * P.width));
-    vel[i] = make_float4(P.curl * cx, up + P.curl * cy, P.curl * cz, 0.f);
+    const float4 v = make_float4(P.curl * cx, up + P.curl * cy, P.curl * cz, 0.f);
+    vel[i] = v;
+    unsigned m = __float_as_uint(sqrtf(v.x * v.x + v.y * v.y + v.z * v.z));
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(vmaxBits, m);
+}
+// One axis of the block-mask dilation: dst[b] = OR of src over blocks b +- k along
+// the axis, k up to the reach of a back-trace in blocks: ceil((dt vmax n + 2.5) / 8)
+// (a trilinear sample reaches 1.5 voxels past the back-traced point; 1 more covers
+// rounding). A reach beyond 8 blocks marks everything.
+__global__ __launch_bounds__(kThreads) void kMaskDilate(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
+                                                        int nb, int axis, float dtn,
+                                                        const unsigned* __restrict__ vmaxBits) {
+    const size_t total = (size_t)nb * nb * nb;
+    const float reach = dtn * __uint_as_float(*vmaxBits) + 2.5f;
+    const int r = (int)ceilf(reach / 8.f);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        if (!(reach <= 64.f)) { dst[i] = 1; continue; }
+        const int c[3] = {(int)(i % nb), (int)((i / nb) % nb), (int)(i / ((size_t)nb * nb))};
+        const size_t stride = axis == 0 ? 1 : (axis == 1 ? (size_t)nb : (size_t)nb * nb);
+        uint8_t any = 0;
+        for (int k = -r; k <= r; k++) {
+            const int q = c[axis] + k;
+            if (q < 0 || q >= nb) continue;
+            any |= src[(size_t)((ptrdiff_t)i + (ptrdiff_t)k * (ptrdiff_t)stride)];
+        }
+        dst[i] = any ? 1 : 0;
+    }
 }
 
 __device__ __forceinline__ float4 velAt(const float4* __restrict__ vel, float px, float py, float pz) {
@@ -127,12 +160,23 @@ __device__ __forceinline__ float sampleField(const float* __restrict__ f, int n,
 
 __global__ __launch_bounds__(kThreads) void kAdvect(PlumeParams P, const float4* __restrict__ vel,
                                                     const float* __restrict__ d0, const float* __restrict__ t0,
-                                                    float* __restrict__ d1, float* __restrict__ t1) {
-    const int n = P.n;
+                                                    float* __restrict__ d1, float* __restrict__ t1,
+                                                    const uint8_t* __restrict__ reachable,
+                                                    uint8_t* __restrict__ marks) {
+    const int n = P.n, nb = n / 8;
     const size_t total = (size_t)n * n * n;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
         const int x = (int)(i % n), y = (int)((i / n) % n), z = (int)(i / ((size_t)n * n));
         const float px = (x + 0.5f) / n, py = (y + 0.5f) / n, pz = (z + 0.5f) / n;
+        const size_t blk = (size_t)(x / 8) + (size_t)nb * ((size_t)(y / 8) + (size_t)nb * (size_t)(z / 8));
+        {
+            const float sx = px - 0.5f, sy = py - P.srcY, sz = pz - 0.5f;
+            if (!reachable[blk] && !(sqrtf(sx * sx + sy * sy + sz * sz) / P.srcR < 1.f)) {
+                d1[i] = 0.f;
+                t1[i] = 0.f;
+                continue;
+            }
+        }
         // midpoint back-trace
         const float4 v1 = velAt(vel, px, py, pz);
         const float4 v2 = velAt(vel, px - 0.5f * P.dt * v1.x, py - 0.5f * P.dt * v1.y, pz - 0.5f * P.dt * v1.z);
@@ -150,8 +194,10 @@ __global__ __launch_bounds__(kThreads) void kAdvect(PlumeParams P, const float4*
         }
         // the grid's outer voxels stay empty, so the plume never touches the box
         const bool edge = x < 2 || y < 2 || z < 2 || x >= n - 2 || y >= n - 2 || z >= n - 2;
-        d1[i] = edge || dens < 1e-3f ? 0.f : dens;
-        t1[i] = edge || dens < 1e-3f ? 0.f : temp;
+        const bool zero = edge || dens < 1e-3f;
+        d1[i] = zero ? 0.f : dens;
+        t1[i] = zero ? 0.f : temp;
+        if (!zero) marks[blk] = 1;
     }
 }
 
@@ -165,6 +211,9 @@ struct vrestir_plume {
     float* d[2] = {nullptr, nullptr};
     float* t[2] = {nullptr, nullptr};
     float4* vel = nullptr;
+    uint8_t* marks[2] = {nullptr, nullptr};  // per 8^3 block: non-zero density in d[k]
+    uint8_t* reach[2] = {nullptr, nullptr};  // dilation ping-pong; reach[0] ends as the reachable blocks
+    unsigned* vmaxBits = nullptr;             // the largest lattice speed of the step
     int cur = 0;
 };
 
@@ -183,6 +232,12 @@ extern "C" int32_t vrestir_plume_create(int32_t n, uint64_t seed, vrestir_plume*
         ok = ok && hipMalloc((void**)&p->t[k], bytes) == hipSuccess;
     }
     ok = ok && hipMalloc((void**)&p->vel, (size_t)kVel * kVel * kVel * sizeof(float4)) == hipSuccess;
+    const size_t nb3 = (size_t)(n / 8) * (n / 8) * (n / 8);
+    for (int k = 0; k < 2 && ok; k++) {
+        ok = ok && hipMalloc((void**)&p->marks[k], nb3) == hipSuccess && hipMemset(p->marks[k], 0, nb3) == hipSuccess;
+        ok = ok && hipMalloc((void**)&p->reach[k], nb3) == hipSuccess;
+    }
+    ok = ok && hipMalloc((void**)&p->vmaxBits, sizeof(unsigned)) == hipSuccess;
     ok = ok && hipMemset(p->d[0], 0, bytes) == hipSuccess && hipMemset(p->t[0], 0, bytes) == hipSuccess;
     if (!ok) {
         vrestir_plume_destroy(p);
@@ -200,6 +255,11 @@ extern "C" int32_t vrestir_plume_destroy(vrestir_plume* p) {
         if (p->t[k]) (void)hipFree(p->t[k]);
     }
     if (p->vel) (void)hipFree(p->vel);
+    for (int k = 0; k < 2; k++) {
+        if (p->marks[k]) (void)hipFree(p->marks[k]);
+        if (p->reach[k]) (void)hipFree(p->reach[k]);
+    }
+    if (p->vmaxBits) (void)hipFree(p->vmaxBits);
     delete p;
     return VRESTIR_OK;
 }
@@ -223,13 +283,26 @@ extern "C" int32_t vrestir_plume_advance(vrestir_plume* p, void* stream, int32_t
         P.srcY = 0.14f;
         P.srcDensity = 1.f;
         P.srcTemperature = 8.f;
+        if (hipMemsetAsync(p->vmaxBits, 0, sizeof(unsigned), st) != hipSuccess) return VRESTIR_EHIP;
         hipLaunchKernelGGL(kVelocity, dim3((kVel * kVel * kVel + kThreads - 1) / kThreads), dim3(kThreads), 0, st, P,
-                           p->vel);
+                           p->vel, p->vmaxBits);
+        // the blocks a back-trace can reach from marked ones: x, y, z dilations
+        const int nb = p->n / 8;
+        const size_t nb3 = (size_t)nb * nb * nb;
+        const unsigned mblocks = (unsigned)((nb3 + kThreads - 1) / kThreads);
+        const float dtn = P.dt * (float)p->n;
+        hipLaunchKernelGGL(kMaskDilate, dim3(mblocks), dim3(kThreads), 0, st, p->marks[p->cur], p->reach[1], nb, 0,
+                           dtn, p->vmaxBits);
+        hipLaunchKernelGGL(kMaskDilate, dim3(mblocks), dim3(kThreads), 0, st, p->reach[1], p->reach[0], nb, 1, dtn,
+                           p->vmaxBits);
+        hipLaunchKernelGGL(kMaskDilate, dim3(mblocks), dim3(kThreads), 0, st, p->reach[0], p->reach[1], nb, 2, dtn,
+                           p->vmaxBits);
+        if (hipMemsetAsync(p->marks[1 - p->cur], 0, nb3, st) != hipSuccess) return VRESTIR_EHIP;
         const size_t total = (size_t)p->n * p->n * p->n;
         const unsigned blocks = (unsigned)((total + kThreads - 1) / kThreads < 262144 ? (total + kThreads - 1) / kThreads
                                                                                       : 262144);
         hipLaunchKernelGGL(kAdvect, dim3(blocks), dim3(kThreads), 0, st, P, p->vel, p->d[p->cur], p->t[p->cur],
-                           p->d[1 - p->cur], p->t[1 - p->cur]);
+                           p->d[1 - p->cur], p->t[1 - p->cur], p->reach[1], p->marks[1 - p->cur]);
         if (hipGetLastError() != hipSuccess) return VRESTIR_EHIP;
         p->cur = 1 - p->cur;
         p->steps++;
