@@ -667,25 +667,14 @@ VR_DEV void traceSegmentsBody(const Scene* __restrict__ sc, const Frame& fr, con
             if (head >= total && !__ballot(w.active) && !__ballot(waitMask != 0)) break;
         }
     };
+    // TK = 1 walks flat only: launchTraceCells refuses a grid it could not walk
+    // that way, so no launch leaves a transmittance unwritten
     if (flat) {
         FlatWalk w;
         trace(w);
     } else if constexpr (TK == 2) {
         Walker<2> w;
         trace(w);
-    } else {
-        // analytic rays on a grid whose block bits do not fit LDS (the launch gate
-        // sends such grids to the one-kernel form; this keeps every transmittance
-        // written should a launch pass it anyway): each lane traces its own pixel's
-        // rays with the one-kernel tracker
-        Rng rng{0, 0, 0, 0};
-        for (int r = 0; r < cnt; r++) {
-            const TraceRec R = recs[(size_t)(p0 + lane) * cap + r];
-            const bool linear = R.kind == 0 ? opt.visibilityUseLinearSampler != 0 : opt.lightingUseLinearSampler != 0;
-            trs[(size_t)r * numPixels + p0 + lane] =
-                trAnalytic(S, Ray{V3(R.ox, R.oy, R.oz), V3(R.dx, R.dy, R.dz), 0.f, R.tMax}, R.kind == 0 ? vs : ls, rng,
-                           linear);
-        }
     }
 }
 template <int TK>
