@@ -31,7 +31,7 @@ typedef enum vrestir_status {
     VRESTIR_EHIP = 3,         /* a HIP runtime call failed */
     VRESTIR_ENOMEM = 4,       /* device or host allocation failed */
     VRESTIR_ESTATE = 5,       /* call out of order (e.g. execute before set_volume) */
-    VRESTIR_EUNSUPPORTED = 6  /* option the pass does not implement (surface scenes) */
+    VRESTIR_EUNSUPPORTED = 6  /* option the pass does not implement (vertex reuse; surface vertices in the reservoir stages) */
 } vrestir_status;
 
 /* Tracking methods, MIS, neighbour sampler, reprojection modes:

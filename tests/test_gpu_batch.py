@@ -106,6 +106,15 @@ def test_batch_matches_passes_rendered_alone(use_graph):
     compare(singles(specs), batched(specs, tuning=(64, 2, use_graph)), specs)
 
 
+def test_batch_with_emissive_triangles_matches_passes_rendered_alone():
+    specs = batch_specs()[:3]
+    for s in specs:
+        s.surfaces = scenes.emissive_panels()
+        s.params.update(useEmissiveLights=1)
+    say("case: batch of 3 passes lit by emissive triangles too, graph on, " + f"{FRAMES} frames")
+    compare(singles(specs), batched(specs, tuning=(64, 2, True)), specs)
+
+
 def test_batch_after_autotune_matches_passes_rendered_alone():
     specs = batch_specs()
     say("case: autotuned batch, then 3 frames from a reset history")
