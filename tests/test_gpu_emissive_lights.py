@@ -113,3 +113,24 @@ def test_whole_frames_with_emissive_triangles():
         frac = float((r <= 1e-3).mean())
         say(f"   frame {f}: frac within 1e-3 {frac:.4f}, mean radiance oracle {co[:, :3].mean():.5f} gpu {cg[:, :3].mean():.5f}")
         assert frac >= 0.97
+
+
+def test_clearing_surfaces_restores_the_plain_scene():
+    import torch
+    from volumetricrestirrelease_amd.volumetric_restir import VolumetricReSTIR
+    say("case: a pass whose triangles are set and then removed renders as one that never had them")
+    plain = small_scene(res=32, width=40, height=32, useEmissiveLights=1)
+    lit = emissive_scene()
+    imgs = []
+    for cleared in (False, True):
+        g = VolumetricReSTIR()
+        g.setScene(lit if cleared else plain)
+        if cleared:
+            g.setSurfaces(plain)  # spec.surfaces is None: vrestir_set_surfaces(NULL)
+        out = torch.zeros((plain.height, plain.width, 4), device="cuda")
+        g.setCamera(plain, 0)
+        g.execute(out)
+        torch.cuda.synchronize()
+        imgs.append(out.cpu().numpy().copy())
+        g.close()
+    assert np.array_equal(imgs[0], imgs[1])
